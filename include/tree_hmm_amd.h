@@ -1,0 +1,336 @@
+/*
+ * tree_hmm_amd.h: the C-ABI of the MI355X-native batched tree-HMM inference
+ * library (libtree_hmm_amd.so).
+ *
+ * Every entry point takes plain pointers and sizes and returns a
+ * thmm_status. The status codes map one to one onto the exception classes
+ * the reference C++ API throws (invalid_argument, logic_error, length_error,
+ * domain_error, runtime_error); thmm_last_error() returns the message of the
+ * most recent failure on the calling thread. include/tree_hmm_amd/rocm.h
+ * rebuilds the reference's tree_hmm::rocm C++ surface on top of this file.
+ *
+ * Citations name files of joaospinto/parallel_tree_hmm.
+ */
+#ifndef TREE_HMM_AMD_H_
+#define TREE_HMM_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Status                                                                   */
+/* ------------------------------------------------------------------------ */
+
+typedef enum thmm_status {
+  THMM_OK = 0,
+  THMM_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+  THMM_LOGIC_ERROR = 2,      /* std::logic_error */
+  THMM_LENGTH_ERROR = 3,     /* std::length_error */
+  THMM_DOMAIN_ERROR = 4,     /* std::domain_error */
+  THMM_RUNTIME_ERROR = 5     /* std::runtime_error (HIP failures) */
+} thmm_status;
+
+/* Message of the last failing call on this thread ("" if none). Replaces the
+ * what() strings of the reference's exceptions
+ * (src/gpu_backend_impl.inc:45-50,52-67). */
+const char *thmm_last_error(void);
+
+/* ------------------------------------------------------------------------ */
+/* Rake-compress plan                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* The op records carry the field meaning of the planner structs the
+ * reference consumes (btrc::Rake, BranchCombination, BranchAbsorption,
+ * Compression; used at src/inference.cc:155-214,363-412 and sized at
+ * src/gpu_backend_impl.inc:39-42). Edge fields name edge slots in the
+ * reference convention: a compression overwrites its left_edge slot with the
+ * composed parent->child path. */
+typedef struct thmm_rake {
+  uint32_t leaf, parent, edge, branch;
+} thmm_rake;
+typedef struct thmm_branch_combination {
+  uint32_t destination, source, stage;
+} thmm_branch_combination;
+typedef struct thmm_branch_absorption {
+  uint32_t parent, branch, round;
+} thmm_branch_absorption;
+typedef struct thmm_compression {
+  uint32_t left_edge, middle, right_edge, parent, child, tape;
+} thmm_compression;
+
+typedef enum thmm_primitive {
+  THMM_RAKE = 0,
+  THMM_BRANCH_COMBINATION = 1,
+  THMM_BRANCH_ABSORPTION = 2,
+  THMM_COMPRESSION = 3
+} thmm_primitive;
+
+/* One level of the contraction: `count` write-disjoint ops of one kind,
+ * starting at `offset` in that kind's op array (btrc::PrimitiveBatch). */
+typedef struct thmm_primitive_batch {
+  uint32_t primitive, offset, count;
+} thmm_primitive_batch;
+
+/* Flat view of a plan. Arrays are owned by the plan object. */
+typedef struct thmm_plan_arrays {
+  size_t num_nodes, num_edges, num_branches, num_compressions, num_roots;
+  size_t num_rakes, num_combinations, num_absorptions, num_batches;
+  size_t num_rounds;
+  const uint32_t *roots;         /* [num_roots], ascending */
+  const uint32_t *edge_parents;  /* [num_edges] */
+  const uint32_t *edge_children; /* [num_edges] */
+  const thmm_rake *rakes;
+  const thmm_branch_combination *combinations;
+  const thmm_branch_absorption *absorptions;
+  const thmm_compression *compressions; /* [num_compressions] */
+  const thmm_primitive_batch *batches;
+  /* Per round: index of the round's first batch, [num_rounds + 1]. */
+  const uint32_t *round_batch_offsets;
+} thmm_plan_arrays;
+
+typedef struct thmm_plan thmm_plan;
+
+/* Builds a rake-compress plan from a parent array (parents[i] == -1 marks a
+ * root; several roots make a forest whose trees are contracted together).
+ * Replaces btrc::MakePlan(std::vector<int64_t>) as called at
+ * tests/inference_test.cc:159-160 and tests/accelerator_test.h:19-20. */
+thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes,
+                             thmm_plan **out);
+
+/* Wraps an externally built plan (e.g. a btrc::Plan flattened by the C++
+ * layer). The arrays are copied and checked: every non-root node must be
+ * eliminated exactly once and every batch must be write-disjoint. */
+thmm_status thmm_plan_create_from_arrays(const thmm_plan_arrays *arrays,
+                                         thmm_plan **out);
+
+thmm_status thmm_plan_get_arrays(const thmm_plan *plan, thmm_plan_arrays *out);
+void thmm_plan_destroy(thmm_plan *plan);
+
+/* ------------------------------------------------------------------------ */
+/* Device + workspace                                                       */
+/* ------------------------------------------------------------------------ */
+
+/* 1 if a HIP device is present (src/gpu_backend_impl.inc:1782-1785). */
+int thmm_available(void);
+/* "<name> (<gcnArchName>)" (src/gpu_backend_impl.inc:1787-1799). */
+thmm_status thmm_device_description(int device, char *buffer, size_t size);
+
+typedef enum thmm_precision {
+  THMM_F32 = 0, /* Scalar = float: FP32 values, FP64 running log scales */
+  THMM_F64 = 1  /* Scalar = double: plain FP64 everywhere */
+} thmm_precision;
+
+enum {
+  THMM_RESERVE_MAXIMUM = 1u,   /* Workspace::ReserveMaximum */
+  THMM_RESERVE_SAMPLING = 2u,  /* Workspace::ReserveSampling */
+  THMM_RESERVE_MARGINALS = 4u, /* Workspace::ReserveMarginals */
+  /* No reference counterpart: allocate no pinned staging and no
+   * workspace-owned outputs; only the *_device entry points may be used.
+   * Keeps multi-GB edge-marginal outputs single-copy in HBM. */
+  THMM_RESERVE_DEVICE_RESIDENT = 8u
+};
+
+/* tree_hmm::CategoricalInputUpdate (include/tree_hmm/accelerator.h:92-96). */
+typedef enum thmm_categorical_update {
+  THMM_UPDATE_ALL = 0,
+  THMM_UPDATE_FACTORS = 1,
+  THMM_UPDATE_NONE = 2
+} thmm_categorical_update;
+
+/* tree_hmm::AcceleratorTimings (include/tree_hmm/accelerator.h:98-111). */
+typedef struct thmm_timings {
+  double upload_ms, kernel_ms, download_ms, wall_ms;
+} thmm_timings;
+
+typedef struct thmm_workspace thmm_workspace;
+
+thmm_status thmm_workspace_create(thmm_workspace **out);
+void thmm_workspace_destroy(thmm_workspace *workspace);
+
+/* Workspace::Reserve / ReserveMaximum / ReserveSampling / ReserveMarginals
+ * (include/tree_hmm/detail/gpu_backend_api.inc:19-51 and
+ * src/gpu_backend_impl.inc:1803-2076). `identity` is the address the caller
+ * uses to name its plan object; later calls must present the same identity
+ * (src/gpu_backend_impl.inc:2162-2166). The plan must outlive the
+ * reservation. `flags` is a bitwise OR of THMM_RESERVE_*. */
+thmm_status thmm_reserve(thmm_workspace *workspace, const thmm_plan *plan,
+                         const void *identity, size_t states,
+                         size_t batch_capacity, thmm_precision precision,
+                         unsigned flags, int device);
+
+/* Workspace::ReserveCategorical{,Maximum,Sampling,Marginals}
+ * (src/gpu_backend_impl.inc:1981-2031). */
+thmm_status thmm_reserve_categorical(
+    thmm_workspace *workspace, const thmm_plan *plan, const void *identity,
+    size_t states, size_t batch_capacity, size_t categories,
+    const uint32_t *observation_nodes, size_t num_observation_nodes,
+    thmm_precision precision, unsigned flags, int device);
+
+/* Workspace::Inputs(batch): pinned host storage laid out as
+ * BatchedModelView ([batch, node, state] and [edge, parent, child]).
+ * (src/gpu_backend_impl.inc:2078-2102). Element type follows the
+ * workspace precision. */
+thmm_status thmm_inputs(thmm_workspace *workspace, size_t batch,
+                        void **node_potentials, size_t *node_count,
+                        void **edge_potentials, size_t *edge_count);
+
+/* Workspace::CategoricalInputs(batch) (src/gpu_backend_impl.inc:2104-2134). */
+thmm_status thmm_categorical_inputs(thmm_workspace *workspace, size_t batch,
+                                    uint8_t **observations,
+                                    size_t *observation_count,
+                                    void **root_potential,
+                                    void **emission_potentials,
+                                    size_t *emission_count,
+                                    void **edge_potentials, size_t *edge_count);
+
+/* Workspace::Uniforms(batch) (src/gpu_backend_impl.inc:2136-2150). */
+thmm_status thmm_uniforms(thmm_workspace *workspace, size_t batch,
+                          void **uniforms, size_t *count);
+
+/* BatchedModelView (include/tree_hmm/accelerator.h:15-23). */
+typedef struct thmm_dense_model {
+  const void *identity;
+  size_t states, batch;
+  const void *node_potentials; /* [batch, node, state] */
+  size_t node_count;
+  const void *edge_potentials; /* [edge, parent state, child state] */
+  size_t edge_count;
+} thmm_dense_model;
+
+/* BatchedCategoricalModelView (include/tree_hmm/accelerator.h:45-60). */
+typedef struct thmm_categorical_model {
+  const void *identity;
+  size_t states, batch, categories;
+  const uint32_t *observation_nodes;
+  size_t num_observation_nodes;
+  const uint8_t *observations; /* [batch, observation node] */
+  size_t observation_count;
+  const void *root_potential; /* [state] */
+  size_t root_count;
+  const void *emission_potentials; /* [category, state] */
+  size_t emission_count;
+  const void *edge_potentials; /* [edge, parent state, child state] */
+  size_t edge_count;
+} thmm_categorical_model;
+
+/* Result spans point into workspace-owned pinned memory and stay valid until
+ * the next call, Reserve or destruction (include/tree_hmm/accelerator.h:
+ * 113-146). Value element types follow the workspace precision. Partition
+ * outputs hold batch * num_roots values, batch-major. */
+
+/* PartitionFunctionPrepared (log_domain = 0) and
+ * LogPartitionFunctionPrepared (log_domain = 1)
+ * (src/gpu_backend_impl.inc:3021-3031). */
+thmm_status thmm_partition(thmm_workspace *workspace,
+                           const thmm_dense_model *model, int log_domain,
+                           const void **values, size_t *count,
+                           thmm_timings *timings);
+/* Categorical overloads with CategoricalInputUpdate
+ * (src/gpu_backend_impl.inc:3033-3059). */
+thmm_status thmm_partition_categorical(thmm_workspace *workspace,
+                                       const thmm_categorical_model *model,
+                                       thmm_categorical_update update,
+                                       int log_domain, const void **values,
+                                       size_t *count, thmm_timings *timings);
+
+/* MaximumAPosterioriPrepared (src/gpu_backend_impl.inc:3061-3079). */
+thmm_status thmm_maximum(thmm_workspace *workspace,
+                         const thmm_dense_model *model,
+                         const void **log_weights, const uint32_t **states,
+                         thmm_timings *timings);
+thmm_status thmm_maximum_categorical(thmm_workspace *workspace,
+                                     const thmm_categorical_model *model,
+                                     thmm_categorical_update update,
+                                     const void **log_weights,
+                                     const uint32_t **states,
+                                     thmm_timings *timings);
+
+/* PosteriorSamplePrepared (src/gpu_backend_impl.inc:3081-3110). `uniforms`
+ * holds one U[0,1) variate per batch item and node, [batch, node]. */
+thmm_status thmm_sample(thmm_workspace *workspace,
+                        const thmm_dense_model *model, const void *uniforms,
+                        size_t uniform_count, const uint32_t **states,
+                        thmm_timings *timings);
+thmm_status thmm_sample_categorical(thmm_workspace *workspace,
+                                    const thmm_categorical_model *model,
+                                    const void *uniforms, size_t uniform_count,
+                                    thmm_categorical_update update,
+                                    const uint32_t **states,
+                                    thmm_timings *timings);
+
+/* PosteriorMarginalsPrepared (src/gpu_backend_impl.inc:3112-3130): log Z
+ * [batch * roots], node marginals [batch, node, state] and edge marginals
+ * [batch, edge, parent state, child state]. */
+thmm_status thmm_marginals(thmm_workspace *workspace,
+                           const thmm_dense_model *model,
+                           const void **log_partitions, const void **nodes,
+                           const void **edges, thmm_timings *timings);
+thmm_status thmm_marginals_categorical(thmm_workspace *workspace,
+                                       const thmm_categorical_model *model,
+                                       thmm_categorical_update update,
+                                       const void **log_partitions,
+                                       const void **nodes, const void **edges,
+                                       thmm_timings *timings);
+
+/* The FP64 log partitions of the latest partition/marginal call, before
+ * rounding to the workspace Scalar ([batch * roots]). No reference
+ * counterpart: the reference returns Scalar only. */
+thmm_status thmm_result_log_partitions_f64(thmm_workspace *workspace,
+                                           const double **values,
+                                           size_t *count);
+
+/* ------------------------------------------------------------------------ */
+/* Device-resident entry points                                             */
+/* ------------------------------------------------------------------------ */
+/* Inputs already in HBM, outputs written to caller device buffers, work
+ * enqueued on `stream` (a hipStream_t; NULL = the workspace stream) without
+ * a host synchronisation. Element type follows the workspace precision.
+ * These have no reference counterpart; they serve callers that keep
+ * potentials resident (the reference uploads every call,
+ * src/gpu_backend_impl.inc:2197-2213). */
+
+/* Scaled log partitions: d_log_z is double [batch * roots]. */
+thmm_status thmm_log_partition_device(thmm_workspace *workspace,
+                                      const void *d_node_potentials,
+                                      const void *d_edge_potentials,
+                                      size_t batch, double *d_log_z,
+                                      void *stream);
+
+/* log Z + adjoint: node marginals [batch, node, state], edge marginals
+ * [batch, edge, parent, child] (= d log Z / d log potentials). */
+thmm_status thmm_marginals_device(thmm_workspace *workspace,
+                                  const void *d_node_potentials,
+                                  const void *d_edge_potentials, size_t batch,
+                                  double *d_log_z, void *d_node_marginals,
+                                  void *d_edge_marginals, void *stream);
+
+/* Forward half of the adjoint pair: log potentials in, log Z out. Keeps the
+ * contraction state in the workspace for thmm_logz_backward_device. */
+thmm_status thmm_logz_forward_device(thmm_workspace *workspace,
+                                     const void *d_log_node_potentials,
+                                     const void *d_log_edge_potentials,
+                                     size_t batch, double *d_log_z,
+                                     void *stream);
+
+/* Backward half: d_grad_log_z (double [batch * roots], NULL = all ones)
+ * scales the adjoint. d_grad_nodes is [batch, node, state]. d_grad_edges is
+ * [batch, edge, parent, child], or [edge, parent, child] summed over the
+ * batch when reduce_edges_over_batch != 0 (the edges are broadcast). */
+thmm_status thmm_logz_backward_device(thmm_workspace *workspace,
+                                      const double *d_grad_log_z,
+                                      void *d_grad_nodes, void *d_grad_edges,
+                                      int reduce_edges_over_batch,
+                                      void *stream);
+
+/* Blocks until the workspace stream has drained. */
+thmm_status thmm_synchronize(thmm_workspace *workspace);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* TREE_HMM_AMD_H_ */

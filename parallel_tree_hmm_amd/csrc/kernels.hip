@@ -1,0 +1,1017 @@
+// gfx950 kernels of the batched tree-HMM contraction.
+//
+// Work decomposition: one "item" is one op of a plan level applied to one
+// batch item. A group of G lanes (G = next power of two >= K, at most 64)
+// owns an item, lane s owning state s (the parent state of a rake, the
+// output row of a compression). Items are numbered op-major, batch-minor, so
+// the lanes of a wavefront share the op and read the batch-shared edge tile
+// once while their node/branch vectors are contiguous in [node][batch][state]
+// storage. Group reductions use cross-lane shuffles within the group.
+//
+// Algebra (scaled linear domain; each output divided by its maximum and the
+// log of that maximum added to an FP64 running log scale) follows
+// src/inference.cc:697-836 and src/gpu_device_algebra.h:41-52. The reverse
+// pass computes the log-domain adjoint of src/inference.cc:416-518 with
+// ratios of normalised values (term / output instead of exp(term - output)).
+// Max-product follows src/inference.cc:564-695 in FP64; sampling follows the
+// inverse-CDF rule of src/inference.cc:128-146 in FP64.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "launch.h"
+
+namespace thmm {
+namespace {
+
+thread_local std::string g_launch_error;
+
+void Check(hipError_t status, const char *what) {
+  if (status != hipSuccess && g_launch_error.empty())
+    g_launch_error = std::string(what) + ": " + hipGetErrorString(status);
+}
+
+__device__ inline float Max(float a, float b) { return fmaxf(a, b); }
+__device__ inline double Max(double a, double b) { return fmax(a, b); }
+
+template <int G, typename V> __device__ inline V GroupMax(V v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v = Max(v, __shfl_xor(v, o, G));
+  return v;
+}
+
+constexpr double kNegInf = -__builtin_huge_val();
+
+template <typename T> __device__ inline double UpdatedScale(double input_scale, T top) {
+  return top > T(0) ? input_scale + log(static_cast<double>(top)) : kNegInf;
+}
+
+template <typename T>
+__device__ inline const T *Slot(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
+                                std::uint32_t b) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  return slot < d.E ? static_cast<const T *>(buf.edges) + slot * kk
+                    : static_cast<const T *>(buf.paths) +
+                          (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+}
+
+__device__ inline double SlotScale(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
+                                   std::uint32_t b) {
+  return slot < d.E ? buf.edge_scale[slot]
+                    : buf.path_scale[static_cast<std::uint64_t>(slot - d.E) * d.B + b];
+}
+
+// Adjoint storage of a slot: edge marginals are written batch-major
+// [B][E][K][K] straight into the output; composed slots are internal.
+template <typename T>
+__device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
+                                 std::uint32_t b) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  return slot < d.E ? static_cast<T *>(buf.edge_out) + (static_cast<std::uint64_t>(b) * d.E + slot) * kk
+                    : static_cast<T *>(buf.path_adj) +
+                          (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+}
+
+struct Lane {
+  std::uint64_t item;
+  std::uint32_t s;
+};
+
+template <int G> __device__ inline Lane LaneOf() {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  return {tid / G, static_cast<std::uint32_t>(tid % G)};
+}
+
+// ------------------------------------------------------------------------
+// Scaled forward
+// ------------------------------------------------------------------------
+
+// branch = path . leaf (src/inference.cc:704-723)
+template <typename T, int G>
+__global__ void RakeForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                            LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceRake op = ops.rakes[offset + lane.item / d.B];
+  const T *v = static_cast<const T *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
+  const T *P = Slot<T>(buf, d, op.slot, b);
+  T m = T(0);
+  if (lane.s < d.K) {
+    const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
+    for (std::uint32_t c = 0; c < d.K; ++c) m += row[c] * v[c];
+  }
+  const T top = GroupMax<G>(m);
+  const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
+  if (lane.s < d.K) static_cast<T *>(buf.branches)[at * d.K + lane.s] = top > T(0) ? m * (T(1) / top) : m;
+  if (lane.s == 0)
+    buf.branch_scale[at] =
+        UpdatedScale(SlotScale(buf, d, op.slot, b) + buf.node_scale[static_cast<std::uint64_t>(op.leaf) * d.B + b], top);
+}
+
+// branch[dst] *= branch[src] (src/inference.cc:725-735)
+template <typename T, int G>
+__global__ void CombineForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                               LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceCombine op = ops.combines[offset + lane.item / d.B];
+  const std::uint64_t dst = static_cast<std::uint64_t>(op.destination) * d.B + b;
+  const std::uint64_t src = static_cast<std::uint64_t>(op.source) * d.B + b;
+  T *branches = static_cast<T *>(buf.branches);
+  T x = T(0);
+  if (lane.s < d.K) x = branches[dst * d.K + lane.s] * branches[src * d.K + lane.s];
+  const T top = GroupMax<G>(x);
+  if (lane.s < d.K) branches[dst * d.K + lane.s] = top > T(0) ? x * (T(1) / top) : x;
+  if (lane.s == 0) buf.branch_scale[dst] = UpdatedScale(buf.branch_scale[dst] + buf.branch_scale[src], top);
+}
+
+// node[parent] *= branch (src/inference.cc:737-747)
+template <typename T, int G>
+__global__ void AbsorbForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                              LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceAbsorb op = ops.absorbs[offset + lane.item / d.B];
+  const std::uint64_t node = static_cast<std::uint64_t>(op.parent) * d.B + b;
+  const std::uint64_t branch = static_cast<std::uint64_t>(op.branch) * d.B + b;
+  T *nodes = static_cast<T *>(buf.nodes);
+  T x = T(0);
+  if (lane.s < d.K) x = nodes[node * d.K + lane.s] * static_cast<const T *>(buf.branches)[branch * d.K + lane.s];
+  const T top = GroupMax<G>(x);
+  if (lane.s < d.K) nodes[node * d.K + lane.s] = top > T(0) ? x * (T(1) / top) : x;
+  if (lane.s == 0) buf.node_scale[node] = UpdatedScale(buf.node_scale[node] + buf.branch_scale[branch], top);
+}
+
+// out = L diag(mid) R into the fresh slot (src/inference.cc:749-776)
+template <typename T, int G>
+__global__ void CompressForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceCompress op = ops.compresses[offset + lane.item / d.B];
+  const T *L = Slot<T>(buf, d, op.left, b);
+  const T *R = Slot<T>(buf, d, op.right, b);
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+  const T *mid = static_cast<const T *>(buf.nodes) + mid_at * d.K;
+  T acc[G];
+#pragma unroll
+  for (int c = 0; c < G; ++c) acc[c] = T(0);
+  T top = T(0);
+  if (lane.s < d.K) {
+    for (std::uint32_t m = 0; m < d.K; ++m) {
+      const T lm = L[static_cast<std::uint64_t>(lane.s) * d.K + m] * mid[m];
+      const T *row = R + static_cast<std::uint64_t>(m) * d.K;
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < static_cast<int>(d.K)) acc[c] += lm * row[c];
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < static_cast<int>(d.K)) top = Max(top, acc[c]);
+  }
+  top = GroupMax<G>(top);
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  if (lane.s < d.K) {
+    T *out = static_cast<T *>(buf.paths) + out_at * d.K * d.K + static_cast<std::uint64_t>(lane.s) * d.K;
+    const T inverse = top > T(0) ? T(1) / top : T(1);
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < static_cast<int>(d.K)) out[c] = acc[c] * inverse;
+  }
+  if (lane.s == 0)
+    buf.path_scale[out_at] = UpdatedScale(
+        SlotScale(buf, d, op.left, b) + buf.node_scale[mid_at] + SlotScale(buf, d, op.right, b), top);
+}
+
+// log Z = scale(root) + log sum(root), summed in FP64
+// (src/inference.cc:778-784).
+template <typename T>
+__global__ void Finish(DeviceOps ops, Dims d, LinearBuffers buf, double *log_z) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.roots) * d.B) return;
+  const std::uint32_t r = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const std::uint64_t at = static_cast<std::uint64_t>(ops.roots[r]) * d.B + b;
+  const T *x = static_cast<const T *>(buf.nodes) + at * d.K;
+  double sum = 0.0;
+  for (std::uint32_t s = 0; s < d.K; ++s) sum += static_cast<double>(x[s]);
+  log_z[static_cast<std::uint64_t>(b) * d.roots + r] = sum > 0.0 ? buf.node_scale[at] + log(sum) : kNegInf;
+}
+
+// ------------------------------------------------------------------------
+// Reverse (adjoint)
+// ------------------------------------------------------------------------
+
+// root adjoint = root marginal (src/inference.cc:338-343), times grad.
+template <typename T>
+__global__ void SeedAdjoint(DeviceOps ops, Dims d, LinearBuffers buf, const double *grad) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.roots) * d.B) return;
+  const std::uint32_t r = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const std::uint64_t at = static_cast<std::uint64_t>(ops.roots[r]) * d.B + b;
+  const T *x = static_cast<const T *>(buf.nodes) + at * d.K;
+  T *adj = static_cast<T *>(buf.node_adj) + at * d.K;
+  double sum = 0.0;
+  for (std::uint32_t s = 0; s < d.K; ++s) sum += static_cast<double>(x[s]);
+  const double g = grad ? grad[static_cast<std::uint64_t>(b) * d.roots + r] : 1.0;
+  const double factor = sum > 0.0 ? g / sum : 0.0;
+  for (std::uint32_t s = 0; s < d.K; ++s) adj[s] = static_cast<T>(static_cast<double>(x[s]) * factor);
+}
+
+// Rake adjoint (src/inference.cc:488-518): with m = P v,
+// adj_P[p][c] = adj_m[p] P[p][c] v[c] / m[p], adj_v[c] = sum_p adj_P[p][c].
+template <typename T, int G>
+__global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                            LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceRake op = ops.rakes[offset + lane.item / d.B];
+  const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
+  const T *v = static_cast<const T *>(buf.nodes) + leaf_at * d.K;
+  const T *P = Slot<T>(buf, d, op.slot, b);
+  T g = T(0);
+  if (lane.s < d.K) {
+    const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
+    T m = T(0);
+    for (std::uint32_t c = 0; c < d.K; ++c) m += row[c] * v[c];
+    const T a = static_cast<const T *>(buf.branch_adj)[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
+    g = (a != T(0) && m > T(0)) ? a / m : T(0);
+    T *out = SlotAdjoint<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
+    for (std::uint32_t c = 0; c < d.K; ++c) out[c] = g * row[c] * v[c];
+  }
+  T acc = T(0);
+  for (std::uint32_t p = 0; p < d.K; ++p) {
+    const T gp = __shfl(g, static_cast<int>(p), G);
+    if (lane.s < d.K) acc += gp * P[static_cast<std::uint64_t>(p) * d.K + lane.s];
+  }
+  if (lane.s < d.K) static_cast<T *>(buf.node_adj)[leaf_at * d.K + lane.s] = acc * v[lane.s];
+}
+
+// adj(branch) = adj(parent) (src/inference.cc:466-475)
+template <typename T, int G>
+__global__ void AbsorbReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                              LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceAbsorb op = ops.absorbs[offset + lane.item / d.B];
+  static_cast<T *>(buf.branch_adj)[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s] =
+      static_cast<const T *>(buf.node_adj)[(static_cast<std::uint64_t>(op.parent) * d.B + b) * d.K + lane.s];
+}
+
+// adj(source) = adj(destination) (src/inference.cc:477-486)
+template <typename T, int G>
+__global__ void CombineReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                               LinearBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceCombine op = ops.combines[offset + lane.item / d.B];
+  T *adj = static_cast<T *>(buf.branch_adj);
+  adj[(static_cast<std::uint64_t>(op.source) * d.B + b) * d.K + lane.s] =
+      adj[(static_cast<std::uint64_t>(op.destination) * d.B + b) * d.K + lane.s];
+}
+
+// Compression adjoint (src/inference.cc:416-464). With o = L diag(mid) R and
+// Gr = adj_o / o (0 where adj_o or o is 0):
+//   adj_L = L diag(mid) .* (Gr R^T),  adj_R = diag(mid) R .* (L^T Gr),
+//   adj_mid[m] = sum_c adj_R[m][c].
+// Lane p forms row p of o, Gr and adj_L; Gr goes through LDS so that lane m
+// can form row m of L^T Gr.
+template <typename T, int G>
+__global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                LinearBuffers buf) {
+  extern __shared__ unsigned char shared_bytes[];
+  T *gr = reinterpret_cast<T *>(shared_bytes) + (threadIdx.x / G) * G * G;
+  const Lane lane = LaneOf<G>();
+  const bool active = lane.item < static_cast<std::uint64_t>(count) * d.B && lane.s < d.K;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  DeviceCompress op{};
+  const T *L = nullptr;
+  const T *R = nullptr;
+  const T *mid = nullptr;
+  std::uint64_t mid_at = 0;
+  if (lane.item < static_cast<std::uint64_t>(count) * d.B) {
+    op = ops.compresses[offset + lane.item / d.B];
+    L = Slot<T>(buf, d, op.left, b);
+    R = Slot<T>(buf, d, op.right, b);
+    mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+    mid = static_cast<const T *>(buf.nodes) + mid_at * d.K;
+  }
+  if (active) {
+    const std::uint64_t p = lane.s;
+    T acc[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) acc[c] = T(0);
+    for (std::uint32_t m = 0; m < d.K; ++m) {
+      const T lm = L[p * d.K + m] * mid[m];
+      const T *row = R + static_cast<std::uint64_t>(m) * d.K;
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < static_cast<int>(d.K)) acc[c] += lm * row[c];
+    }
+    const T *adj_out = static_cast<const T *>(buf.path_adj) +
+                       (static_cast<std::uint64_t>(op.out - d.E) * d.B + b) * d.K * d.K + p * d.K;
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      if (c < static_cast<int>(d.K)) {
+        const T a = adj_out[c];
+        acc[c] = (a != T(0) && acc[c] > T(0)) ? a / acc[c] : T(0);
+        gr[p * G + c] = acc[c];
+      }
+    }
+    T *adj_left = SlotAdjoint<T>(buf, d, op.left, b) + p * d.K;
+    for (std::uint32_t m = 0; m < d.K; ++m) {
+      const T *row = R + static_cast<std::uint64_t>(m) * d.K;
+      T h = T(0);
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < static_cast<int>(d.K)) h += acc[c] * row[c];
+      adj_left[m] = L[p * d.K + m] * mid[m] * h;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const std::uint64_t m = lane.s;
+    T r[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) r[c] = T(0);
+    for (std::uint32_t p = 0; p < d.K; ++p) {
+      const T lp = L[static_cast<std::uint64_t>(p) * d.K + m];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < static_cast<int>(d.K)) r[c] += lp * gr[p * G + c];
+    }
+    T *adj_right = SlotAdjoint<T>(buf, d, op.right, b) + m * d.K;
+    const T *row = R + m * d.K;
+    T total = T(0);
+#pragma unroll
+    for (int c = 0; c < G; ++c) {
+      if (c < static_cast<int>(d.K)) {
+        const T value = row[c] * mid[m] * r[c];
+        adj_right[c] = value;
+        total += value;
+      }
+    }
+    static_cast<T *>(buf.node_adj)[mid_at * d.K + m] = total;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Staging
+// ------------------------------------------------------------------------
+
+// in [X][Y][K] -> out [Y][X][K] through a 64 x 64-element LDS tile.
+// The grid is one-dimensional: block i covers x-tile i % x_tiles and y-tile
+// i / x_tiles. Requires K <= 64.
+template <typename In, typename Out>
+__global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
+                                 std::uint32_t K, double *scale_zero, std::uint32_t x_tiles) {
+  __shared__ Out tile[64][65];
+  const std::uint32_t ty = 64 / K;  // y-vectors per tile row
+  const std::uint32_t row = ty * K; // elements per tile row
+  const std::uint64_t x0 = static_cast<std::uint64_t>(blockIdx.x % x_tiles) * 64;
+  const std::uint64_t y0 = static_cast<std::uint64_t>(blockIdx.x / x_tiles) * ty;
+  for (std::uint32_t i = threadIdx.x; i < 64 * row; i += blockDim.x) {
+    const std::uint32_t x = i / row, j = i % row;
+    const std::uint64_t gx = x0 + x;
+    if (gx < X && y0 * K + j < static_cast<std::uint64_t>(Y) * K)
+      tile[x][j] = static_cast<Out>(in[(gx * Y + y0) * K + j]);
+  }
+  __syncthreads();
+  const std::uint32_t xk = 64 * K;
+  for (std::uint32_t i = threadIdx.x; i < ty * xk; i += blockDim.x) {
+    const std::uint32_t y = i / xk, r = i % xk;
+    const std::uint32_t x = r / K, k = r % K;
+    const std::uint64_t gy = y0 + y, gx = x0 + x;
+    if (gy < Y && gx < X) out[(gy * X + gx) * K + k] = tile[x][y * K + k];
+  }
+  if (scale_zero != nullptr) {
+    for (std::uint32_t i = threadIdx.x; i < 64 * ty; i += blockDim.x) {
+      const std::uint64_t gy = y0 + i / 64, gx = x0 + i % 64;
+      if (gy < Y && gx < X) scale_zero[gy * X + gx] = 0.0;
+    }
+  }
+}
+
+template <typename In, typename Out>
+__global__ void Convert(const In *in, Out *out, std::uint64_t count) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid < count) out[tid] = static_cast<Out>(in[tid]);
+}
+
+// log potentials [B][N][K] -> normalised linear [N][B][K] + FP64 scales.
+template <typename T>
+__global__ void InitNodesFromLogs(const T *in, T *nodes, double *scale, Dims d) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.N) * d.B) return;
+  const std::uint32_t n = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const T *x = in + (static_cast<std::uint64_t>(b) * d.N + n) * d.K;
+  T top = static_cast<T>(kNegInf);
+  for (std::uint32_t s = 0; s < d.K; ++s) top = Max(top, x[s]);
+  T *out = nodes + tid * d.K;
+  const bool finite = top > static_cast<T>(kNegInf);
+  for (std::uint32_t s = 0; s < d.K; ++s) out[s] = finite ? static_cast<T>(exp(x[s] - top)) : T(0);
+  scale[tid] = finite ? static_cast<double>(top) : kNegInf;
+}
+
+template <typename T>
+__global__ void InitEdgesFromLogs(const T *in, T *edges, double *scale, Dims d) {
+  const std::uint64_t e = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= d.E) return;
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  const T *x = in + e * kk;
+  T top = static_cast<T>(kNegInf);
+  for (std::uint64_t i = 0; i < kk; ++i) top = Max(top, x[i]);
+  const bool finite = top > static_cast<T>(kNegInf);
+  for (std::uint64_t i = 0; i < kk; ++i) edges[e * kk + i] = finite ? static_cast<T>(exp(x[i] - top)) : T(0);
+  scale[e] = finite ? static_cast<double>(top) : kNegInf;
+}
+
+// Categorical node factors (src/gpu_backend_impl.inc:326-382 semantics):
+// (root ? root[s] : 1) * (observed ? (cat < C ? emission[cat][s] : 0) : 1),
+// with the unobserved case not multiplied at all.
+template <typename T>
+__global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *obs_index,
+                                std::uint32_t obs_count, std::uint32_t categories,
+                                const std::uint8_t *is_root, const T *root, const T *emissions,
+                                T *nodes, double *scale, Dims d) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.N) * d.B) return;
+  const std::uint32_t n = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  T *out = nodes + tid * d.K;
+  const bool rooted = is_root[n] != 0;
+  const std::uint32_t o = obs_index[n];
+  if (o == 0xFFFFFFFFu) {
+    for (std::uint32_t s = 0; s < d.K; ++s) out[s] = rooted ? root[s] : T(1);
+  } else {
+    const std::uint32_t category = obs[static_cast<std::uint64_t>(b) * obs_count + o];
+    for (std::uint32_t s = 0; s < d.K; ++s) {
+      const T e = category < categories ? emissions[static_cast<std::uint64_t>(category) * d.K + s] : T(0);
+      out[s] = (rooted ? root[s] : T(1)) * e;
+    }
+  }
+  scale[tid] = 0.0;
+}
+
+template <typename T>
+__global__ void ReduceEdges(const T *edge_out, T *out, Dims d) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t total = static_cast<std::uint64_t>(d.E) * kk;
+  if (tid >= total) return;
+  double sum = 0.0;
+  for (std::uint32_t b = 0; b < d.B; ++b) sum += static_cast<double>(edge_out[b * total + tid]);
+  out[tid] = static_cast<T>(sum);
+}
+
+template <typename T>
+__global__ void CastF64(const double *in, T *out, std::uint64_t count) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid < count) out[tid] = static_cast<T>(in[tid]);
+}
+
+// ------------------------------------------------------------------------
+// Max-product (FP64 log domain)
+// ------------------------------------------------------------------------
+
+template <typename In>
+__global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
+  const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
+  if (tid < node_values) {
+    // internal [N][B][K] from public [B][N][K]
+    const std::uint64_t s = tid % d.K, nb = tid / d.K;
+    const std::uint64_t b = nb % d.B, n = nb / d.B;
+    const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
+    nodes[tid] = x > 0.0 ? log(x) : kNegInf;  // src/inference.cc:95-98
+  } else if (tid < node_values + edge_values) {
+    const std::uint64_t i = tid - node_values;
+    const double x = static_cast<double>(edges_in[i]);
+    edges[i] = x > 0.0 ? log(x) : kNegInf;
+  }
+}
+
+__device__ inline const double *MaxSlot(const MaxBuffers &buf, const Dims &d, std::uint32_t slot,
+                                        std::uint32_t b) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  return slot < d.E ? buf.edges + slot * kk : buf.paths + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+}
+
+// src/inference.cc:576-594: strict '>' argmax of path + leaf.
+template <int G>
+__global__ void MaxRake(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceRake op = ops.rakes[offset + lane.item / d.B];
+  const double *v = buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
+  const double *row = MaxSlot(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
+  double best = row[0] + v[0];
+  std::uint32_t arg = 0;
+  for (std::uint32_t c = 1; c < d.K; ++c) {
+    const double candidate = row[c] + v[c];
+    if (candidate > best) {
+      best = candidate;
+      arg = c;
+    }
+  }
+  const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s;
+  buf.branches[at] = best;
+  buf.rake_choice[at] = static_cast<std::uint8_t>(arg);
+}
+
+template <int G>
+__global__ void MaxCombine(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceCombine op = ops.combines[offset + lane.item / d.B];
+  buf.branches[(static_cast<std::uint64_t>(op.destination) * d.B + b) * d.K + lane.s] +=
+      buf.branches[(static_cast<std::uint64_t>(op.source) * d.B + b) * d.K + lane.s];
+}
+
+template <int G>
+__global__ void MaxAbsorb(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceAbsorb op = ops.absorbs[offset + lane.item / d.B];
+  buf.nodes[(static_cast<std::uint64_t>(op.parent) * d.B + b) * d.K + lane.s] +=
+      buf.branches[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
+}
+
+// src/inference.cc:614-639: candidate (L + mid) + R, strict '>'.
+template <int G>
+__global__ void MaxCompress(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf) {
+  const Lane lane = LaneOf<G>();
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
+  const DeviceCompress op = ops.compresses[offset + lane.item / d.B];
+  const double *L = MaxSlot(buf, d, op.left, b) + static_cast<std::uint64_t>(lane.s) * d.K;
+  const double *R = MaxSlot(buf, d, op.right, b);
+  const double *mid = buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * d.K;
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  double *out = buf.paths + out_at * d.K * d.K + static_cast<std::uint64_t>(lane.s) * d.K;
+  std::uint8_t *choice = buf.compress_choice + out_at * d.K * d.K + static_cast<std::uint64_t>(lane.s) * d.K;
+  for (std::uint32_t c = 0; c < d.K; ++c) {
+    double best = (L[0] + mid[0]) + R[c];
+    std::uint32_t arg = 0;
+    for (std::uint32_t m = 1; m < d.K; ++m) {
+      const double candidate = (L[m] + mid[m]) + R[static_cast<std::uint64_t>(m) * d.K + c];
+      if (candidate > best) {
+        best = candidate;
+        arg = m;
+      }
+    }
+    out[c] = best;
+    choice[c] = static_cast<std::uint8_t>(arg);
+  }
+}
+
+// src/inference.cc:641-649
+__global__ void MaxFinish(DeviceOps ops, Dims d, MaxBuffers buf, double *log_weight, std::uint32_t *assignment) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.roots) * d.B) return;
+  const std::uint32_t r = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const std::uint32_t root = ops.roots[r];
+  const double *x = buf.nodes + (static_cast<std::uint64_t>(root) * d.B + b) * d.K;
+  double best = x[0];
+  std::uint32_t arg = 0;
+  for (std::uint32_t s = 1; s < d.K; ++s) {
+    if (x[s] > best) {
+      best = x[s];
+      arg = s;
+    }
+  }
+  log_weight[static_cast<std::uint64_t>(b) * d.roots + r] = best;
+  assignment[static_cast<std::uint64_t>(b) * d.N + root] = arg;
+}
+
+// src/inference.cc:651-667
+__global__ void MaxExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                               MaxBuffers buf, std::uint32_t *assignment) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const DeviceRake op = ops.rakes[offset + tid / d.B];
+  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+  a[op.leaf] = buf.rake_choice[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + a[op.parent]];
+}
+
+__global__ void MaxExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                      MaxBuffers buf, std::uint32_t *assignment) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const DeviceCompress op = ops.compresses[offset + tid / d.B];
+  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+  const std::uint64_t at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  a[op.middle] = buf.compress_choice[at * d.K * d.K + static_cast<std::uint64_t>(a[op.parent]) * d.K + a[op.child]];
+}
+
+// ------------------------------------------------------------------------
+// Posterior sampling (FP64 scaled state)
+// ------------------------------------------------------------------------
+
+// Ascending inverse CDF over w / sum(w), skipping zero weights, falling back
+// to the last supported state (src/inference.cc:128-146). Returns K when no
+// state is supported.
+template <class W>
+__device__ inline std::uint32_t InverseCdf(std::uint32_t K, W weight, double u) {
+  double total = 0.0;
+  for (std::uint32_t s = 0; s < K; ++s) total += weight(s);
+  double cumulative = 0.0;
+  std::uint32_t last = K;
+  for (std::uint32_t s = 0; s < K; ++s) {
+    const double w = weight(s);
+    if (!(w > 0.0)) continue;
+    last = s;
+    cumulative += w / total;
+    if (u < cumulative) return s;
+  }
+  return last;
+}
+
+template <typename U>
+__global__ void SampleSeed(DeviceOps ops, Dims d, LinearBuffers buf, const U *uniforms,
+                           std::uint32_t *assignment, std::uint32_t *failure) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.roots) * d.B) return;
+  const std::uint32_t r = static_cast<std::uint32_t>(tid / d.B);
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const std::uint32_t root = ops.roots[r];
+  const double *x = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(root) * d.B + b) * d.K;
+  const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + root;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return x[i]; }, static_cast<double>(uniforms[at]));
+  if (s >= d.K) {
+    *failure = 1;
+    assignment[at] = 0;
+  } else {
+    assignment[at] = s;
+  }
+}
+
+template <typename U>
+__global__ void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                  LinearBuffers buf, const U *uniforms, std::uint32_t *assignment,
+                                  std::uint32_t *failure) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const DeviceRake op = ops.rakes[offset + tid / d.B];
+  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+  const double *row = Slot<double>(buf, d, op.slot, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
+  const double *v = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t c) { return row[c] * v[c]; },
+                                     static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.leaf]));
+  if (s >= d.K) {
+    *failure = 1;
+    a[op.leaf] = 0;
+  } else {
+    a[op.leaf] = s;
+  }
+}
+
+template <typename U>
+__global__ void SampleExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                         LinearBuffers buf, const U *uniforms, std::uint32_t *assignment,
+                                         std::uint32_t *failure) {
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  const DeviceCompress op = ops.compresses[offset + tid / d.B];
+  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+  const std::uint32_t cs = a[op.child];
+  const double *L = Slot<double>(buf, d, op.left, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
+  const double *R = Slot<double>(buf, d, op.right, b);
+  const double *mid = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.middle) * d.B + b) * d.K;
+  const std::uint32_t s = InverseCdf(
+      d.K, [&](std::uint32_t m) { return L[m] * mid[m] * R[static_cast<std::uint64_t>(m) * d.K + cs]; },
+      static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.middle]));
+  if (s >= d.K) {
+    *failure = 1;
+    a[op.middle] = 0;
+  } else {
+    a[op.middle] = s;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Launch helpers
+// ------------------------------------------------------------------------
+
+constexpr unsigned kThreads = 256;
+
+unsigned BlocksFor(std::uint64_t threads, unsigned block) {
+  return static_cast<unsigned>((threads + block - 1) / block);
+}
+
+unsigned GroupBlock(int G) { return G <= 8 ? 256u : (G == 16 ? 128u : 64u); }
+
+hipStream_t StreamOf(const Launch &l) { return static_cast<hipStream_t>(l.stream); }
+
+template <typename T, int G>
+void ForwardLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                   const LinearBuffers &buf) {
+  const unsigned block = GroupBlock(G);
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
+  switch (primitive) {
+    case 0: RakeForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 1: CombineForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 2: AbsorbForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 3: CompressForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+  }
+  Check(hipGetLastError(), "forward level launch");
+}
+
+template <typename T, int G>
+void ReverseLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                   const LinearBuffers &buf) {
+  const unsigned block = GroupBlock(G);
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
+  switch (primitive) {
+    case 0: RakeReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 1: CombineReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 2: AbsorbReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 3: {
+      const std::size_t shared = static_cast<std::size_t>(block / G) * G * G * sizeof(T);
+      CompressReverse<T, G><<<grid, block, shared, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      break;
+    }
+  }
+  Check(hipGetLastError(), "reverse level launch");
+}
+
+template <int G>
+void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+               const MaxBuffers &buf) {
+  const unsigned block = GroupBlock(G);
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
+  switch (primitive) {
+    case 0: MaxRake<G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 1: MaxCombine<G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 2: MaxAbsorb<G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 3: MaxCompress<G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+  }
+  Check(hipGetLastError(), "max-product level launch");
+}
+
+// Dispatches a runtime group width onto the template instantiations: runs
+// CALL with the constant G bound.
+#define THMM_WITH_GROUP(K, CALL)                  \
+  switch (GroupWidth(K)) {                        \
+    case 1: { constexpr int G = 1; CALL; } break;   \
+    case 2: { constexpr int G = 2; CALL; } break;   \
+    case 4: { constexpr int G = 4; CALL; } break;   \
+    case 8: { constexpr int G = 8; CALL; } break;   \
+    case 16: { constexpr int G = 16; CALL; } break; \
+    case 32: { constexpr int G = 32; CALL; } break; \
+    default: { constexpr int G = 64; CALL; } break; \
+  }
+
+}  // namespace
+
+std::uint32_t GroupWidth(std::uint32_t states) {
+  std::uint32_t g = 1;
+  while (g < states) g *= 2;
+  return g;
+}
+
+const char *LastLaunchError() {
+  static thread_local std::string reported;
+  reported = g_launch_error;
+  g_launch_error.clear();
+  return reported.c_str();
+}
+
+void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes, double *node_scale) {
+  const Dims &d = l.dims;
+  const std::uint32_t ty = 64 / d.K;
+  const std::uint32_t x_tiles = (d.B + 63) / 64;
+  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((d.N + ty - 1) / ty));
+  if (l.real == Real::kF32) {
+    if (in_real == Real::kF32)
+      TransposeVectors<float, float><<<grid, kThreads, 0, StreamOf(l)>>>(
+          static_cast<const float *>(input), static_cast<float *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+    else
+      TransposeVectors<double, float><<<grid, kThreads, 0, StreamOf(l)>>>(
+          static_cast<const double *>(input), static_cast<float *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+  } else {
+    if (in_real == Real::kF32)
+      TransposeVectors<float, double><<<grid, kThreads, 0, StreamOf(l)>>>(
+          static_cast<const float *>(input), static_cast<double *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+    else
+      TransposeVectors<double, double><<<grid, kThreads, 0, StreamOf(l)>>>(
+          static_cast<const double *>(input), static_cast<double *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+  }
+  Check(hipGetLastError(), "node initialisation launch");
+}
+
+void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes, double *node_scale) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.N) * l.dims.B, kThreads);
+  if (l.real == Real::kF32)
+    InitNodesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const float *>(log_input), static_cast<float *>(nodes), node_scale, l.dims);
+  else
+    InitNodesFromLogs<double><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const double *>(log_input), static_cast<double *>(nodes), node_scale, l.dims);
+  Check(hipGetLastError(), "log node initialisation launch");
+}
+
+void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges, double *edge_scale) {
+  if (l.dims.E == 0) return;
+  const unsigned grid = BlocksFor(l.dims.E, kThreads);
+  if (l.real == Real::kF32)
+    InitEdgesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const float *>(log_input), static_cast<float *>(edges), edge_scale, l.dims);
+  else
+    InitEdgesFromLogs<double><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const double *>(log_input), static_cast<double *>(edges), edge_scale, l.dims);
+  Check(hipGetLastError(), "log edge initialisation launch");
+}
+
+void LaunchInitCategorical(const Launch &l, const std::uint8_t *observations,
+                           const std::uint32_t *observation_index_by_node, std::uint32_t observation_count,
+                           std::uint32_t categories, const std::uint8_t *is_root, const void *root_potential,
+                           const void *emissions, void *nodes, double *node_scale) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.N) * l.dims.B, kThreads);
+  if (l.real == Real::kF32)
+    InitCategorical<float><<<grid, kThreads, 0, StreamOf(l)>>>(
+        observations, observation_index_by_node, observation_count, categories, is_root,
+        static_cast<const float *>(root_potential), static_cast<const float *>(emissions),
+        static_cast<float *>(nodes), node_scale, l.dims);
+  else
+    InitCategorical<double><<<grid, kThreads, 0, StreamOf(l)>>>(
+        observations, observation_index_by_node, observation_count, categories, is_root,
+        static_cast<const double *>(root_potential), static_cast<const double *>(emissions),
+        static_cast<double *>(nodes), node_scale, l.dims);
+  Check(hipGetLastError(), "categorical initialisation launch");
+}
+
+void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                        const LinearBuffers &buf) {
+  if (count == 0) return;
+  if (l.real == Real::kF32) {
+    THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, offset, count, buf)));
+  } else {
+    THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<double, G>(l, primitive, offset, count, buf)));
+  }
+}
+
+void LaunchFinish(const Launch &l, const LinearBuffers &buf, double *log_z) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.roots) * l.dims.B, kThreads);
+  if (l.real == Real::kF32)
+    Finish<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, log_z);
+  else
+    Finish<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, log_z);
+  Check(hipGetLastError(), "finish launch");
+}
+
+void LaunchSeedAdjoint(const Launch &l, const LinearBuffers &buf, const double *grad) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.roots) * l.dims.B, kThreads);
+  if (l.real == Real::kF32)
+    SeedAdjoint<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, grad);
+  else
+    SeedAdjoint<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, grad);
+  Check(hipGetLastError(), "adjoint seed launch");
+}
+
+void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                        const LinearBuffers &buf) {
+  if (count == 0) return;
+  if (l.real == Real::kF32) {
+    THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<float, G>(l, primitive, offset, count, buf)));
+  } else {
+    THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<double, G>(l, primitive, offset, count, buf)));
+  }
+}
+
+void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out) {
+  const Dims &d = l.dims;
+  const std::uint32_t ty = 64 / d.K;
+  const std::uint32_t x_tiles = (d.N + 63) / 64;
+  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((d.B + ty - 1) / ty));
+  if (l.real == Real::kF32)
+    TransposeVectors<float, float><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const float *>(node_adj), static_cast<float *>(out), d.N, d.B, d.K, nullptr, x_tiles);
+  else
+    TransposeVectors<double, double><<<grid, kThreads, 0, StreamOf(l)>>>(
+        static_cast<const double *>(node_adj), static_cast<double *>(out), d.N, d.B, d.K, nullptr, x_tiles);
+  Check(hipGetLastError(), "node marginal transpose launch");
+}
+
+void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out) {
+  const std::uint64_t total = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  if (total == 0) return;
+  const unsigned grid = BlocksFor(total, kThreads);
+  if (l.real == Real::kF32)
+    ReduceEdges<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(edge_out),
+                                                          static_cast<float *>(out), l.dims);
+  else
+    ReduceEdges<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(edge_out),
+                                                           static_cast<double *>(out), l.dims);
+  Check(hipGetLastError(), "edge reduction launch");
+}
+
+void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t count) {
+  if (count == 0) return;
+  Convert<float, double><<<BlocksFor(count, kThreads), kThreads, 0, StreamOf(l)>>>(in, out, count);
+  Check(hipGetLastError(), "widen launch");
+}
+
+void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t count) {
+  if (count == 0) return;
+  const unsigned grid = BlocksFor(count, kThreads);
+  if (l.real == Real::kF32)
+    CastF64<float><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<float *>(out), count);
+  else
+    CastF64<double><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<double *>(out), count);
+  Check(hipGetLastError(), "cast launch");
+}
+
+void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in, double *nodes,
+                   double *edges) {
+  const std::uint64_t total = static_cast<std::uint64_t>(l.dims.N) * l.dims.B * l.dims.K +
+                              static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  const unsigned grid = BlocksFor(total, kThreads);
+  if (in_real == Real::kF32)
+    MaxInit<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
+                                                      static_cast<const float *>(edges_in), nodes, edges, l.dims);
+  else
+    MaxInit<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(nodes_in),
+                                                       static_cast<const double *>(edges_in), nodes, edges, l.dims);
+  Check(hipGetLastError(), "max-product initialisation launch");
+}
+
+void LaunchMaxLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                    const MaxBuffers &buf) {
+  if (count == 0) return;
+  THMM_WITH_GROUP(l.dims.K, (MaxLevelT<G>(l, primitive, offset, count, buf)));
+}
+
+void LaunchMaxFinish(const Launch &l, const MaxBuffers &buf, double *log_weight, std::uint32_t *assignment) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.roots) * l.dims.B, kThreads);
+  MaxFinish<<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, log_weight, assignment);
+  Check(hipGetLastError(), "max-product finish launch");
+}
+
+void LaunchMaxExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                          const MaxBuffers &buf, std::uint32_t *assignment) {
+  if (count == 0) return;
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, kThreads);
+  if (primitive == 0)
+    MaxExpandRakes<<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, assignment);
+  else if (primitive == 3)
+    MaxExpandCompressions<<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, assignment);
+  Check(hipGetLastError(), "MAP expansion launch");
+}
+
+void LaunchSampleSeed(const Launch &l, const LinearBuffers &buf, Real in_real, const void *uniforms,
+                      std::uint32_t *assignment, std::uint32_t *failure) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.roots) * l.dims.B, kThreads);
+  if (in_real == Real::kF32)
+    SampleSeed<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, static_cast<const float *>(uniforms),
+                                                         assignment, failure);
+  else
+    SampleSeed<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf,
+                                                          static_cast<const double *>(uniforms), assignment, failure);
+  Check(hipGetLastError(), "sample seed launch");
+}
+
+void LaunchSampleExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                             const LinearBuffers &buf, Real in_real, const void *uniforms,
+                             std::uint32_t *assignment, std::uint32_t *failure) {
+  if (count == 0 || (primitive != 0 && primitive != 3)) return;
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, kThreads);
+  if (in_real == Real::kF32) {
+    const float *u = static_cast<const float *>(uniforms);
+    if (primitive == 0)
+      SampleExpandRakes<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment, failure);
+    else
+      SampleExpandCompressions<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u,
+                                                                          assignment, failure);
+  } else {
+    const double *u = static_cast<const double *>(uniforms);
+    if (primitive == 0)
+      SampleExpandRakes<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment,
+                                                                    failure);
+    else
+      SampleExpandCompressions<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u,
+                                                                           assignment, failure);
+  }
+  Check(hipGetLastError(), "sample expansion launch");
+}
+
+}  // namespace thmm

@@ -1,0 +1,157 @@
+// Interface between the host executor (workspace.cc) and the gfx950 kernels
+// (kernels.hip). Plain structs of device pointers and sizes; no HIP types.
+#ifndef TREE_HMM_AMD_LAUNCH_H_
+#define TREE_HMM_AMD_LAUNCH_H_
+
+#include <cstddef>
+#include <cstdint>
+
+namespace thmm {
+
+// Ops after slot renaming. Slots 0..E-1 hold the (batch-shared) edge
+// potentials; compression c writes the fresh per-item slot E + c instead of
+// overwriting its left edge, so nothing the reverse pass needs is ever
+// overwritten and every slot, node and branch is consumed exactly once.
+struct DeviceRake {
+  std::uint32_t leaf, slot, branch, parent;
+};
+struct DeviceCombine {
+  std::uint32_t destination, source;
+};
+struct DeviceAbsorb {
+  std::uint32_t parent, branch;
+};
+struct DeviceCompress {
+  std::uint32_t left, middle, right, out;  // out == E + compression index
+  std::uint32_t parent, child, pad0, pad1;
+};
+
+struct DeviceOps {
+  const DeviceRake *rakes;
+  const DeviceCombine *combines;
+  const DeviceAbsorb *absorbs;
+  const DeviceCompress *compresses;
+  const std::uint32_t *roots;
+};
+
+struct Dims {
+  std::uint32_t K;      // states
+  std::uint32_t B;      // batch items in this call
+  std::uint32_t E;      // edges (= shared slots)
+  std::uint32_t N;      // nodes
+  std::uint32_t roots;  // number of roots
+};
+
+// Scaled linear-domain state (element type float or double, erased).
+//   nodes      [N][B][K]        node_scale   [N][B]   (FP64 log scales)
+//   edges      [E][K][K]        edge_scale   [E]      (shared slots)
+//   paths      [C][B][K][K]     path_scale   [C][B]   (composed slots)
+//   branches   [R][B][K]        branch_scale [R][B]
+//   node_adj   [N][B][K]        branch_adj   [R][B][K]
+//   path_adj   [C][B][K][K]     edge_out     [B][E][K][K] (edge marginals)
+struct LinearBuffers {
+  void *nodes;
+  double *node_scale;
+  const void *edges;
+  const double *edge_scale;
+  void *paths;
+  double *path_scale;
+  void *branches;
+  double *branch_scale;
+  void *node_adj;
+  void *branch_adj;
+  void *path_adj;
+  void *edge_out;
+};
+
+// Log-domain max-product state (FP64):
+//   nodes [N][B][K], edges [E][K][K], paths [C][B][K][K], branches [R][B][K],
+//   rake_choice [R][B][K] u8, compress_choice [C][B][K][K] u8.
+struct MaxBuffers {
+  double *nodes;
+  const double *edges;
+  double *paths;
+  double *branches;
+  std::uint8_t *rake_choice;
+  std::uint8_t *compress_choice;
+};
+
+enum class Real { kF32 = 0, kF64 = 1 };
+
+struct Launch {
+  Real real;
+  Dims dims;
+  DeviceOps ops;
+  void *stream;  // hipStream_t
+};
+
+// ---- input staging ----
+// public [B][N][K] -> internal [N][B][K]; node_scale = 0. `in_real` is the
+// element type of `input`; `out` has element type launch.real.
+void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes,
+                     double *node_scale);
+// Log potentials [B][N][K] -> max-shifted linear values + FP64 scales.
+void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes,
+                             double *node_scale);
+// Log edge potentials [E][K][K] -> max-shifted linear values + scales.
+void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges,
+                             double *edge_scale);
+// Categorical observations -> internal [N][B][K] nodes, bitwise equal to the
+// dense construction root-or-one times emission.
+void LaunchInitCategorical(const Launch &l, const std::uint8_t *observations,
+                           const std::uint32_t *observation_index_by_node,
+                           std::uint32_t observation_count, std::uint32_t categories,
+                           const std::uint8_t *is_root, const void *root_potential,
+                           const void *emissions, void *nodes, double *node_scale);
+
+// ---- scaled forward (log Z) ----
+void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
+                        std::uint32_t count, const LinearBuffers &buf);
+// log_z[b * roots + r] (FP64).
+void LaunchFinish(const Launch &l, const LinearBuffers &buf, double *log_z);
+
+// ---- reverse (adjoint = marginals) ----
+// node_adj(root) = root / sum(root), scaled by grad[b * roots + r] if given.
+void LaunchSeedAdjoint(const Launch &l, const LinearBuffers &buf, const double *grad);
+void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
+                        std::uint32_t count, const LinearBuffers &buf);
+// node_adj [N][B][K] -> out [B][N][K].
+void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out);
+// out[e][.] = sum_b edge_out[b][e][.]
+void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out);
+// Widens `count` FP32 values to FP64.
+void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t count);
+// Casts `count` FP64 values to the launch element type.
+void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t count);
+
+// ---- max-product (FP64) ----
+void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in,
+                   double *nodes, double *edges);
+void LaunchMaxLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
+                    std::uint32_t count, const MaxBuffers &buf);
+void LaunchMaxFinish(const Launch &l, const MaxBuffers &buf, double *log_weight,
+                     std::uint32_t *assignment);
+void LaunchMaxExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
+                          std::uint32_t count, const MaxBuffers &buf,
+                          std::uint32_t *assignment);
+
+// ---- posterior sampling (FP64 scaled state) ----
+// uniforms [B][N] (element type in_real), assignment [B][N]. A root with no
+// supported state sets *failure.
+void LaunchSampleSeed(const Launch &l, const LinearBuffers &buf, Real in_real,
+                      const void *uniforms, std::uint32_t *assignment, std::uint32_t *failure);
+void LaunchSampleExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
+                             std::uint32_t count, const LinearBuffers &buf, Real in_real,
+                             const void *uniforms, std::uint32_t *assignment,
+                             std::uint32_t *failure);
+
+// Group width (lanes per batch item) used for a state count.
+std::uint32_t GroupWidth(std::uint32_t states);
+
+// Reports the first launch error of the calling thread and clears it; empty
+// when none. The executor turns a non-empty result into runtime_error.
+const char *LastLaunchError();
+
+}  // namespace thmm
+
+#endif  // TREE_HMM_AMD_LAUNCH_H_

@@ -1,0 +1,1054 @@
+// Host executor: see workspace.h.
+#include "workspace.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+namespace thmm {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// HIP failures become runtime_error("<op>: <hipGetErrorString>")
+// (src/gpu_backend_impl.inc:45-50).
+void Hip(hipError_t status, const char *what) {
+  if (status != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(status));
+}
+
+void CheckLaunches() {
+  const char *error = LastLaunchError();
+  if (error != nullptr && error[0] != '\0') throw std::runtime_error(error);
+}
+
+// src/gpu_backend_impl.inc:52-67
+std::uint32_t CheckedU32(std::size_t value, const char *what) {
+  if (value > std::numeric_limits<std::uint32_t>::max()) throw std::length_error(std::string(what) + " exceeds uint32_t");
+  return static_cast<std::uint32_t>(value);
+}
+
+std::size_t Product(std::initializer_list<std::size_t> values, const char *what) {
+  std::size_t result = 1;
+  for (std::size_t v : values) {
+    if (v != 0 && result > std::numeric_limits<std::size_t>::max() / v)
+      throw std::length_error(std::string(what) + " overflows size_t");
+    result *= v;
+  }
+  return result;
+}
+
+std::size_t RealSize(Real real) { return real == Real::kF32 ? sizeof(float) : sizeof(double); }
+
+// Owning device allocation (at least one byte so that every pointer is
+// valid, as src/gpu_backend_impl.inc:69-75 does).
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  std::size_t bytes = 0;
+  void Allocate(std::size_t size, const char *what) {
+    Release();
+    Hip(hipMalloc(&ptr, std::max<std::size_t>(size, 1)), what);
+    bytes = size;
+  }
+  void Release() noexcept {
+    if (ptr != nullptr) static_cast<void>(hipFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename V> V *as() const { return static_cast<V *>(ptr); }
+  ~DeviceBuffer() { Release(); }
+};
+
+struct PinnedBuffer {
+  void *ptr = nullptr;
+  std::size_t bytes = 0;
+  void Allocate(std::size_t size, const char *what) {
+    Release();
+    Hip(hipHostMalloc(&ptr, std::max<std::size_t>(size, 1), hipHostMallocDefault), what);
+    bytes = size;
+  }
+  void Release() noexcept {
+    if (ptr != nullptr) static_cast<void>(hipHostFree(ptr));
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename V> V *as() const { return static_cast<V *>(ptr); }
+  ~PinnedBuffer() { Release(); }
+};
+
+struct Level {
+  std::uint32_t primitive, offset, count;
+};
+
+}  // namespace
+
+struct Workspace::Impl {
+  const Plan *plan = nullptr;
+  const void *identity = nullptr;
+  std::size_t states = 0, capacity = 0, categories = 0;
+  Real real = Real::kF32;
+  unsigned flags = 0;
+  int device = 0;
+  bool categorical = false;
+  std::vector<std::uint32_t> observation_nodes;
+  std::size_t resident_observation_batch = 0;
+  bool resident_factors = false;
+  std::size_t last_batch = 0;
+  bool forward_ready = false;  // logz_forward state present for backward
+
+  hipStream_t stream = nullptr;
+  hipEvent_t events[6] = {};
+  std::vector<Level> levels;
+
+  // topology
+  DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
+  // inputs
+  PinnedBuffer host_nodes, host_edges, host_uniforms, host_observations, host_root, host_emissions;
+  DeviceBuffer input_nodes, uniforms, observations, root_potential, emissions;
+  // scaled linear state (Scalar)
+  DeviceBuffer edges, edge_scale, nodes, node_scale, paths, path_scale, branches, branch_scale, log_z;
+  // adjoint
+  DeviceBuffer node_adj, branch_adj, path_adj, node_out, edge_out;
+  // FP64 scaled state for sampling when Scalar is float
+  DeviceBuffer s_edges, s_edge_scale, s_nodes, s_node_scale, s_paths, s_path_scale, s_branches, s_branch_scale;
+  DeviceBuffer assignment, failure;
+  // max-product
+  DeviceBuffer m_nodes, m_edges, m_paths, m_branches, rake_choice, compress_choice, log_weight;
+  // results
+  PinnedBuffer host_values, host_log_z, host_assignment, host_node_out, host_edge_out, host_failure;
+
+  ~Impl() { Clear(); }
+
+  void Clear() noexcept {
+    if (stream != nullptr) {
+      static_cast<void>(hipSetDevice(device));
+      static_cast<void>(hipStreamSynchronize(stream));
+    }
+    for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
+                            &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
+                            &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
+                            &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
+                            &s_edge_scale, &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
+                            &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
+                            &rake_choice, &compress_choice, &log_weight})
+      b->Release();
+    for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
+                            &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
+                            &host_edge_out, &host_failure})
+      b->Release();
+    for (hipEvent_t &e : events) {
+      if (e != nullptr) static_cast<void>(hipEventDestroy(e));
+      e = nullptr;
+    }
+    if (stream != nullptr) static_cast<void>(hipStreamDestroy(stream));
+    stream = nullptr;
+    plan = nullptr;
+    identity = nullptr;
+    states = capacity = categories = 0;
+    flags = 0;
+    device = 0;
+    categorical = false;
+    observation_nodes.clear();
+    resident_observation_batch = 0;
+    resident_factors = false;
+    last_batch = 0;
+    forward_ready = false;
+    levels.clear();
+  }
+
+  bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
+  std::size_t N() const { return plan->num_nodes; }
+  std::size_t E() const { return plan->num_edges; }
+  std::size_t R() const { return plan->num_branches; }
+  std::size_t C() const { return plan->num_compressions(); }
+  std::size_t Roots() const { return plan->roots.size(); }
+  std::size_t KK() const { return states * states; }
+  std::size_t S() const { return RealSize(real); }
+
+  Launch MakeLaunch(Real r, std::size_t batch, hipStream_t s) const {
+    Launch l{};
+    l.real = r;
+    l.dims = {static_cast<std::uint32_t>(states), static_cast<std::uint32_t>(batch), static_cast<std::uint32_t>(E()),
+              static_cast<std::uint32_t>(N()), static_cast<std::uint32_t>(Roots())};
+    l.ops = {rakes.as<DeviceRake>(), combines.as<DeviceCombine>(), absorbs.as<DeviceAbsorb>(),
+             compresses.as<DeviceCompress>(), roots.as<std::uint32_t>()};
+    l.stream = s;
+    return l;
+  }
+
+  LinearBuffers Linear(const void *edge_values) const {
+    return {nodes.ptr, node_scale.as<double>(), edge_values, edge_scale.as<double>(), paths.ptr,
+            path_scale.as<double>(), branches.ptr, branch_scale.as<double>(), node_adj.ptr, branch_adj.ptr,
+            path_adj.ptr, edge_out.ptr};
+  }
+
+  LinearBuffers SamplingLinear() const {
+    if (real == Real::kF64) return Linear(edges.ptr);
+    return {s_nodes.ptr, s_node_scale.as<double>(), s_edges.ptr, s_edge_scale.as<double>(), s_paths.ptr,
+            s_path_scale.as<double>(), s_branches.ptr, s_branch_scale.as<double>(), nullptr, nullptr, nullptr,
+            nullptr};
+  }
+
+  MaxBuffers Max() const {
+    return {m_nodes.as<double>(), m_edges.as<double>(), m_paths.as<double>(), m_branches.as<double>(),
+            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>()};
+  }
+};
+
+namespace {
+
+using Impl = Workspace::Impl;
+
+template <typename V> void Upload(DeviceBuffer &dst, const std::vector<V> &src, hipStream_t stream, const char *what) {
+  dst.Allocate(src.size() * sizeof(V), what);
+  if (!src.empty()) Hip(hipMemcpyAsync(dst.ptr, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, stream), what);
+}
+
+void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
+                   Real real, unsigned flags, int device) {
+  if (states == 0 || batch == 0) throw std::invalid_argument("accelerator state and batch counts must be nonzero");
+  if (states > 64) throw std::invalid_argument("state count exceeds the accelerator limit of 64");
+  if (real != Real::kF32 && real != Real::kF64) throw std::invalid_argument("unknown precision");
+  w.Clear();
+  Hip(hipSetDevice(device), "GPU runtime SetDevice");
+  w.plan = &plan;
+  w.identity = identity;
+  w.states = states;
+  w.capacity = batch;
+  w.real = real;
+  w.flags = flags;
+  w.device = device;
+  CheckedU32(batch, "batch count");
+  CheckedU32(Product({plan.num_nodes, batch}, "node-batch count"), "node-batch count");
+  Hip(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking), "GPU runtime StreamCreateWithFlags");
+  for (hipEvent_t &e : w.events) Hip(hipEventCreate(&e), "GPU runtime EventCreate");
+
+  // Slot renaming: replay the plan's edge slots and give compression c the
+  // fresh slot E + c.
+  const std::size_t E = plan.num_edges;
+  std::vector<std::uint32_t> current(E);
+  for (std::size_t e = 0; e < E; ++e) current[e] = static_cast<std::uint32_t>(e);
+  std::vector<DeviceRake> rakes(plan.rakes.size());
+  std::vector<DeviceCombine> combines(plan.combinations.size());
+  std::vector<DeviceAbsorb> absorbs(plan.absorptions.size());
+  std::vector<DeviceCompress> compresses(plan.compressions.size());
+  for (const thmm_primitive_batch &batch_desc : plan.batches) {
+    for (std::uint32_t i = batch_desc.offset; i < batch_desc.offset + batch_desc.count; ++i) {
+      switch (batch_desc.primitive) {
+        case THMM_RAKE: {
+          const thmm_rake &op = plan.rakes[i];
+          rakes[i] = {op.leaf, current[op.edge], op.branch, op.parent};
+          break;
+        }
+        case THMM_BRANCH_COMBINATION:
+          combines[i] = {plan.combinations[i].destination, plan.combinations[i].source};
+          break;
+        case THMM_BRANCH_ABSORPTION:
+          absorbs[i] = {plan.absorptions[i].parent, plan.absorptions[i].branch};
+          break;
+        case THMM_COMPRESSION: {
+          const thmm_compression &op = plan.compressions[i];
+          const std::uint32_t out = static_cast<std::uint32_t>(E + i);
+          compresses[i] = {current[op.left_edge], op.middle, current[op.right_edge], out, op.parent, op.child, 0, 0};
+          break;
+        }
+      }
+    }
+    // Renaming happens after the whole batch: ops of one batch never read a
+    // slot another op of the same batch writes.
+    if (batch_desc.primitive == THMM_COMPRESSION)
+      for (std::uint32_t i = batch_desc.offset; i < batch_desc.offset + batch_desc.count; ++i)
+        current[plan.compressions[i].left_edge] = static_cast<std::uint32_t>(E + i);
+  }
+  Upload(w.rakes, rakes, w.stream, "topology upload");
+  Upload(w.combines, combines, w.stream, "topology upload");
+  Upload(w.absorbs, absorbs, w.stream, "topology upload");
+  Upload(w.compresses, compresses, w.stream, "topology upload");
+  Upload(w.roots, plan.roots, w.stream, "topology upload");
+  std::vector<std::uint8_t> is_root(plan.num_nodes, 0);
+  for (std::uint32_t r : plan.roots) is_root[r] = 1;
+  Upload(w.is_root, is_root, w.stream, "topology upload");
+  for (const thmm_primitive_batch &b : plan.batches) w.levels.push_back({b.primitive, b.offset, b.count});
+
+  const std::size_t K = states, KK = K * K, N = plan.num_nodes, R = plan.num_branches, C = plan.num_compressions();
+  const std::size_t S = RealSize(real);
+  const std::size_t roots = plan.roots.size();
+  const bool resident = (flags & kReserveDeviceResident) != 0;
+  const std::size_t node_values = Product({batch, N, K}, "node workspace");
+  const std::size_t edge_values = Product({E, KK}, "edge inputs");
+  const std::size_t path_values = Product({C, batch, KK}, "path workspace");
+  const std::size_t branch_values = Product({R, batch, K}, "branch workspace");
+
+  w.edges.Allocate(edge_values * S, "edge storage");
+  w.edge_scale.Allocate(E * sizeof(double), "edge scales");
+  w.nodes.Allocate(node_values * S, "node storage");
+  w.node_scale.Allocate(N * batch * sizeof(double), "node scales");
+  w.paths.Allocate(path_values * S, "path storage");
+  w.path_scale.Allocate(C * batch * sizeof(double), "path scales");
+  w.branches.Allocate(branch_values * S, "branch storage");
+  w.branch_scale.Allocate(R * batch * sizeof(double), "branch scales");
+  w.log_z.Allocate(batch * roots * sizeof(double), "log partitions");
+  if (!resident) {
+    w.host_edges.Allocate(edge_values * S, "pinned edge inputs");
+    w.host_values.Allocate(batch * roots * S, "pinned outputs");
+    w.host_log_z.Allocate(batch * roots * sizeof(double), "pinned log partitions");
+  }
+  if (flags & THMM_RESERVE_MAXIMUM) {
+    w.m_nodes.Allocate(node_values * sizeof(double), "MAP nodes");
+    w.m_edges.Allocate(edge_values * sizeof(double), "MAP edges");
+    w.m_paths.Allocate(path_values * sizeof(double), "MAP paths");
+    w.m_branches.Allocate(branch_values * sizeof(double), "MAP branches");
+    w.rake_choice.Allocate(branch_values, "MAP rake choices");
+    w.compress_choice.Allocate(path_values, "MAP compression choices");
+    w.log_weight.Allocate(batch * roots * sizeof(double), "MAP log weights");
+    w.assignment.Allocate(N * batch * sizeof(std::uint32_t), "assignments");
+    if (!resident) w.host_assignment.Allocate(N * batch * sizeof(std::uint32_t), "pinned assignments");
+  }
+  if (flags & THMM_RESERVE_SAMPLING) {
+    if (real == Real::kF32) {
+      w.s_edges.Allocate(edge_values * sizeof(double), "sampling edges");
+      w.s_edge_scale.Allocate(E * sizeof(double), "sampling edge scales");
+      w.s_nodes.Allocate(node_values * sizeof(double), "sampling nodes");
+      w.s_node_scale.Allocate(N * batch * sizeof(double), "sampling node scales");
+      w.s_paths.Allocate(path_values * sizeof(double), "sampling paths");
+      w.s_path_scale.Allocate(C * batch * sizeof(double), "sampling path scales");
+      w.s_branches.Allocate(branch_values * sizeof(double), "sampling branches");
+      w.s_branch_scale.Allocate(R * batch * sizeof(double), "sampling branch scales");
+    }
+    w.uniforms.Allocate(N * batch * S, "posterior uniforms");
+    w.failure.Allocate(sizeof(std::uint32_t), "sampling status");
+    if (!w.assignment.ptr) w.assignment.Allocate(N * batch * sizeof(std::uint32_t), "assignments");
+    if (!resident) {
+      w.host_uniforms.Allocate(N * batch * S, "pinned uniforms");
+      if (!w.host_assignment.ptr) w.host_assignment.Allocate(N * batch * sizeof(std::uint32_t), "pinned assignments");
+      w.host_failure.Allocate(sizeof(std::uint32_t), "pinned sampling status");
+    }
+  }
+  if (flags & THMM_RESERVE_MARGINALS) {
+    w.node_adj.Allocate(node_values * S, "node adjoints");
+    w.branch_adj.Allocate(branch_values * S, "branch adjoints");
+    w.path_adj.Allocate(path_values * S, "path adjoints");
+    if (!resident) {
+      w.node_out.Allocate(node_values * S, "node marginals");
+      w.edge_out.Allocate(Product({batch, E, KK}, "edge marginals") * S, "edge marginals");
+      w.host_node_out.Allocate(node_values * S, "pinned node marginals");
+      w.host_edge_out.Allocate(Product({batch, E, KK}, "edge marginals") * S, "pinned edge marginals");
+    }
+  }
+  Hip(hipStreamSynchronize(w.stream), "topology upload");
+}
+
+void RequireHost(const Impl &w) {
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
+  if (w.device_resident())
+    throw std::logic_error("this workspace was reserved for device-resident calls only");
+}
+
+struct Timer {
+  Clock::time_point wall = Clock::now();
+};
+
+void FillTimings(Impl &w, const Timer &timer, thmm_timings *t) {
+  if (t == nullptr) return;
+  float upload = 0.f, kernel = 0.f, download = 0.f;
+  Hip(hipEventElapsedTime(&upload, w.events[0], w.events[1]), "GPU runtime EventElapsedTime upload");
+  Hip(hipEventElapsedTime(&kernel, w.events[2], w.events[3]), "GPU runtime EventElapsedTime kernels");
+  Hip(hipEventElapsedTime(&download, w.events[4], w.events[5]), "GPU runtime EventElapsedTime download");
+  t->upload_ms = upload;
+  t->kernel_ms = kernel;
+  t->download_ms = download;
+  t->wall_ms = std::chrono::duration<double, std::milli>(Clock::now() - timer.wall).count();
+}
+
+void Record(Impl &w, int index) { Hip(hipEventRecord(w.events[index], w.stream), "GPU runtime EventRecord"); }
+
+// Validation shared by every prepared call (src/gpu_backend_impl.inc:2162-2173).
+void CheckCall(const Impl &w, const void *identity, std::size_t states, std::size_t batch, std::size_t edge_count,
+               const char *needs) {
+  if (w.plan == nullptr || w.identity != identity || w.states != states || batch == 0 || batch > w.capacity)
+    throw std::invalid_argument(std::string("prepared accelerator inference requires ") + needs +
+                                " for this plan, state count, and batch capacity");
+  if (edge_count != w.E() * w.KK()) throw std::invalid_argument("accelerator edge input shape is wrong");
+}
+
+// Stages the dense model into pinned memory (skipped for workspace-backed
+// spans, src/gpu_backend_impl.inc:2820-2823) and enqueues the upload.
+void StageDense(Impl &w, const thmm_dense_model &model) {
+  if (w.categorical) throw std::invalid_argument("dense accelerator inference cannot use a categorical workspace");
+  const std::size_t node_values = Product({model.batch, w.N(), w.states}, "node inputs");
+  if (model.node_count != node_values) throw std::invalid_argument("accelerator node input shape is wrong");
+  if (node_values != 0 && model.node_potentials == nullptr) throw std::invalid_argument("node potentials are null");
+  if (model.edge_count != 0 && model.edge_potentials == nullptr) throw std::invalid_argument("edge potentials are null");
+  if (model.node_potentials != w.host_nodes.ptr) std::memcpy(w.host_nodes.ptr, model.node_potentials, node_values * w.S());
+  if (model.edge_potentials != w.host_edges.ptr && model.edge_count != 0)
+    std::memcpy(w.host_edges.ptr, model.edge_potentials, model.edge_count * w.S());
+}
+
+void UploadDense(Impl &w, const thmm_dense_model &model) {
+  Hip(hipMemcpyAsync(w.input_nodes.ptr, w.host_nodes.ptr, model.node_count * w.S(), hipMemcpyHostToDevice, w.stream),
+      "GPU runtime MemcpyAsync node upload");
+  if (model.edge_count != 0)
+    Hip(hipMemcpyAsync(w.edges.ptr, w.host_edges.ptr, model.edge_count * w.S(), hipMemcpyHostToDevice, w.stream),
+        "GPU runtime MemcpyAsync edge upload");
+}
+
+void ZeroEdgeScales(Impl &w, hipStream_t stream) {
+  if (w.E() != 0)
+    Hip(hipMemsetAsync(w.edge_scale.ptr, 0, w.E() * sizeof(double), stream), "GPU runtime MemsetAsync edge scales");
+}
+
+void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf) {
+  for (const Level &level : levels) LaunchForwardLevel(l, level.primitive, level.offset, level.count, buf);
+}
+
+void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf) {
+  for (std::size_t i = levels.size(); i-- > 0;)
+    LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
+}
+
+// Categorical staging with residency policies (src/gpu_backend_impl.inc:2841-2947).
+struct CategoricalPolicy {
+  bool update_observations;
+  bool update_factors;
+};
+
+CategoricalPolicy StageCategorical(Impl &w, const thmm_categorical_model &model, thmm_categorical_update update) {
+  if (!w.categorical || model.categories != w.categories || model.num_observation_nodes != w.observation_nodes.size() ||
+      (model.num_observation_nodes != 0 &&
+       (model.observation_nodes == nullptr ||
+        !std::equal(model.observation_nodes, model.observation_nodes + model.num_observation_nodes,
+                    w.observation_nodes.begin()))))
+    throw std::invalid_argument("categorical accelerator model does not match the reserved workspace");
+  const std::size_t observation_values = Product({model.batch, model.num_observation_nodes}, "categorical observations");
+  const std::size_t emission_values = Product({model.categories, model.states}, "categorical emissions");
+  if (model.observation_count != observation_values || model.root_count != model.states ||
+      model.emission_count != emission_values)
+    throw std::invalid_argument("accelerator categorical input shapes are wrong");
+  if (update != THMM_UPDATE_ALL && update != THMM_UPDATE_FACTORS && update != THMM_UPDATE_NONE)
+    throw std::invalid_argument("unknown categorical input update policy");
+  const CategoricalPolicy policy{update == THMM_UPDATE_ALL, update != THMM_UPDATE_NONE};
+  if (!policy.update_observations && w.resident_observation_batch != model.batch)
+    throw std::logic_error("categorical observations are not resident for this batch; use kAll");
+  if (!policy.update_factors && !w.resident_factors)
+    throw std::logic_error("categorical factors are not resident; use kAll or kFactors");
+  if (policy.update_observations && model.observations != w.host_observations.ptr && observation_values != 0)
+    std::memcpy(w.host_observations.ptr, model.observations, observation_values);
+  if (policy.update_factors) {
+    if (model.root_potential != w.host_root.ptr) std::memcpy(w.host_root.ptr, model.root_potential, w.states * w.S());
+    if (model.emission_potentials != w.host_emissions.ptr)
+      std::memcpy(w.host_emissions.ptr, model.emission_potentials, emission_values * w.S());
+    if (model.edge_potentials != w.host_edges.ptr && model.edge_count != 0)
+      std::memcpy(w.host_edges.ptr, model.edge_potentials, model.edge_count * w.S());
+  }
+  return policy;
+}
+
+void UploadCategorical(Impl &w, const thmm_categorical_model &model, const CategoricalPolicy &policy) {
+  if (policy.update_observations && model.observation_count != 0)
+    Hip(hipMemcpyAsync(w.observations.ptr, w.host_observations.ptr, model.observation_count, hipMemcpyHostToDevice,
+                       w.stream),
+        "GPU runtime MemcpyAsync categorical observation upload");
+  if (policy.update_factors) {
+    Hip(hipMemcpyAsync(w.root_potential.ptr, w.host_root.ptr, w.states * w.S(), hipMemcpyHostToDevice, w.stream),
+        "GPU runtime MemcpyAsync root-potential upload");
+    Hip(hipMemcpyAsync(w.emissions.ptr, w.host_emissions.ptr, model.emission_count * w.S(), hipMemcpyHostToDevice,
+                       w.stream),
+        "GPU runtime MemcpyAsync emission-potential upload");
+    if (model.edge_count != 0)
+      Hip(hipMemcpyAsync(w.edges.ptr, w.host_edges.ptr, model.edge_count * w.S(), hipMemcpyHostToDevice, w.stream),
+          "GPU runtime MemcpyAsync edge upload");
+  }
+}
+
+void FinishCategorical(Impl &w, const thmm_categorical_model &model, const CategoricalPolicy &policy) {
+  if (policy.update_observations) w.resident_observation_batch = model.batch;
+  if (policy.update_factors) w.resident_factors = true;
+}
+
+// Writes the internal [N][B][K] Scalar node values for a call.
+void InitNodesDense(Impl &w, std::size_t batch) {
+  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
+  LaunchInitNodes(l, w.real, w.input_nodes.ptr, w.nodes.ptr, w.node_scale.as<double>());
+}
+
+void InitNodesCategorical(Impl &w, const thmm_categorical_model &model) {
+  const Launch l = w.MakeLaunch(w.real, model.batch, w.stream);
+  LaunchInitCategorical(l, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+                        static_cast<std::uint32_t>(model.num_observation_nodes),
+                        static_cast<std::uint32_t>(model.categories), w.is_root.as<std::uint8_t>(),
+                        w.root_potential.ptr, w.emissions.ptr, w.nodes.ptr, w.node_scale.as<double>());
+}
+
+void DownloadLogZ(Impl &w, std::size_t batch) {
+  Hip(hipMemcpyAsync(w.host_log_z.ptr, w.log_z.ptr, batch * w.Roots() * sizeof(double), hipMemcpyDeviceToHost,
+                     w.stream),
+      "GPU runtime MemcpyAsync output download");
+}
+
+void WaitDownload(Impl &w) { Hip(hipEventSynchronize(w.events[5]), "tree-HMM accelerator execution"); }
+
+// Converts the FP64 log partitions to Scalar outputs (exp for Z).
+void PublishValues(Impl &w, std::size_t batch, bool log_domain) {
+  const double *src = w.host_log_z.as<double>();
+  const std::size_t count = batch * w.Roots();
+  for (std::size_t i = 0; i < count; ++i) {
+    const double v = log_domain ? src[i] : std::exp(src[i]);
+    if (w.real == Real::kF32)
+      w.host_values.as<float>()[i] = static_cast<float>(v);
+    else
+      w.host_values.as<double>()[i] = v;
+  }
+}
+
+void RequireFiniteLogZ(const Impl &w, std::size_t batch, const char *message) {
+  const double *src = w.host_log_z.as<double>();
+  for (std::size_t i = 0; i < batch * w.Roots(); ++i)
+    if (!std::isfinite(src[i])) throw std::domain_error(message);
+}
+
+// Scaled forward + log Z for a staged call.
+template <class Init> void RunPartition(Impl &w, std::size_t batch, Init init) {
+  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
+  ZeroEdgeScales(w, w.stream);
+  init();
+  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, w.log_z.as<double>());
+  CheckLaunches();
+}
+
+template <class Init> void RunMarginals(Impl &w, std::size_t batch, Init init) {
+  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
+  ZeroEdgeScales(w, w.stream);
+  init();
+  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, w.log_z.as<double>());
+  LaunchSeedAdjoint(l, buf, nullptr);
+  ReverseLevels(l, w.levels, buf);
+  LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
+  CheckLaunches();
+}
+
+template <class Init> void RunMaximum(Impl &w, std::size_t batch, Init init) {
+  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
+  init();
+  // Logs of the staged factors, internal layout.
+  const Launch internal = w.MakeLaunch(w.real, batch, w.stream);
+  Launch as_public = internal;
+  // nodes are already internal [N][B][K]: present them to MaxInit as a
+  // [1][N*B][K] "public" array so that the transpose is the identity.
+  as_public.dims.B = 1;
+  as_public.dims.N = static_cast<std::uint32_t>(w.N() * batch);
+  LaunchMaxInit(as_public, w.real, w.nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
+  const MaxBuffers buf = w.Max();
+  for (const Level &level : w.levels) LaunchMaxLevel(l, level.primitive, level.offset, level.count, buf);
+  LaunchMaxFinish(l, buf, w.log_weight.as<double>(), w.assignment.as<std::uint32_t>());
+  for (std::size_t i = w.levels.size(); i-- > 0;)
+    LaunchMaxExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf,
+                         w.assignment.as<std::uint32_t>());
+  CheckLaunches();
+}
+
+template <class Init> void RunSample(Impl &w, std::size_t batch, Init init) {
+  const Launch l = w.MakeLaunch(Real::kF64, batch, w.stream);
+  ZeroEdgeScales(w, w.stream);
+  init();
+  if (w.real == Real::kF32) {
+    // Widen the staged float factors to the FP64 sampling state.
+    LaunchWidenF32(l, w.nodes.as<float>(), w.s_nodes.as<double>(), w.N() * batch * w.states);
+    LaunchWidenF32(l, w.edges.as<float>(), w.s_edges.as<double>(), w.E() * w.KK());
+    Hip(hipMemsetAsync(w.s_node_scale.ptr, 0, w.N() * batch * sizeof(double), w.stream),
+        "GPU runtime MemsetAsync sampling node scales");
+    if (w.E() != 0)
+      Hip(hipMemsetAsync(w.s_edge_scale.ptr, 0, w.E() * sizeof(double), w.stream),
+          "GPU runtime MemsetAsync sampling edge scales");
+  }
+  Hip(hipMemsetAsync(w.failure.ptr, 0, sizeof(std::uint32_t), w.stream), "GPU runtime MemsetAsync sampling status");
+  const LinearBuffers buf = w.SamplingLinear();
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, w.log_z.as<double>());
+  LaunchSampleSeed(l, buf, w.real, w.uniforms.ptr, w.assignment.as<std::uint32_t>(), w.failure.as<std::uint32_t>());
+  for (std::size_t i = w.levels.size(); i-- > 0;)
+    LaunchSampleExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf, w.real,
+                            w.uniforms.ptr, w.assignment.as<std::uint32_t>(), w.failure.as<std::uint32_t>());
+  CheckLaunches();
+}
+
+void ValidateUniforms(const Impl &w, const void *uniforms, std::size_t count, std::size_t batch) {
+  if (!(w.flags & THMM_RESERVE_SAMPLING) || count != Product({batch, w.N()}, "posterior assignments"))
+    throw std::invalid_argument(
+        "prepared accelerator posterior sampling requires ReserveSampling and one uniform variate per batch item "
+        "and node");
+  if (count != 0 && uniforms == nullptr) throw std::invalid_argument("uniforms are null");
+  // src/gpu_backend_impl.inc:2182-2187
+  for (std::size_t i = 0; i < count; ++i) {
+    const double u = w.real == Real::kF32 ? static_cast<const float *>(uniforms)[i]
+                                          : static_cast<const double *>(uniforms)[i];
+    if (!std::isfinite(u) || u < 0.0 || u >= 1.0)
+      throw std::invalid_argument("posterior-sampling variates must lie in [0, 1)");
+  }
+}
+
+}  // namespace
+
+Workspace::Workspace() : impl_(std::make_unique<Impl>()) {}
+Workspace::~Workspace() = default;
+
+void Workspace::Reserve(const Plan &plan, const void *identity, std::size_t states, std::size_t batch, Real real,
+                        unsigned flags, int device) {
+  Impl &w = *impl_;
+  try {
+    ReserveCommon(w, plan, identity, states, batch, real, flags, device);
+    const std::size_t node_values = Product({batch, plan.num_nodes, states}, "node inputs");
+    w.input_nodes.Allocate(node_values * w.S(), "node inputs");
+    if (!w.device_resident()) w.host_nodes.Allocate(node_values * w.S(), "pinned node inputs");
+  } catch (...) {
+    w.Clear();
+    throw;
+  }
+}
+
+void Workspace::ReserveCategorical(const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
+                                   std::size_t categories, const std::uint32_t *observation_nodes,
+                                   std::size_t observation_count, Real real, unsigned flags, int device) {
+  // src/gpu_backend_impl.inc:1985-1998
+  if (categories == 0 || categories > 256)
+    throw std::invalid_argument("accelerator categorical model must have between 1 and 256 categories");
+  if (observation_count != 0 && observation_nodes == nullptr) throw std::invalid_argument("observation nodes are null");
+  for (std::size_t i = 0; i < observation_count; ++i)
+    if (observation_nodes[i] >= plan.num_nodes || (i > 0 && observation_nodes[i] <= observation_nodes[i - 1]))
+      throw std::invalid_argument(
+          "accelerator categorical observation nodes must be valid and strictly increasing");
+  Impl &w = *impl_;
+  try {
+    ReserveCommon(w, plan, identity, states, batch, real, flags & ~kReserveDeviceResident, device);
+    w.categorical = true;
+    w.categories = categories;
+    w.observation_nodes.assign(observation_nodes, observation_nodes + observation_count);
+    const std::size_t observation_values = Product({batch, observation_count}, "categorical observations");
+    const std::size_t emission_values = Product({categories, states}, "categorical emissions");
+    w.host_observations.Allocate(observation_values, "pinned observations");
+    w.host_root.Allocate(states * w.S(), "pinned root potential");
+    w.host_emissions.Allocate(emission_values * w.S(), "pinned emissions");
+    w.observations.Allocate(observation_values, "observations");
+    w.root_potential.Allocate(states * w.S(), "root potential");
+    w.emissions.Allocate(emission_values * w.S(), "emissions");
+    std::vector<std::uint32_t> index(plan.num_nodes, 0xFFFFFFFFu);
+    for (std::size_t i = 0; i < observation_count; ++i) index[observation_nodes[i]] = static_cast<std::uint32_t>(i);
+    Upload(w.observation_index, index, w.stream, "categorical topology upload");
+    Hip(hipStreamSynchronize(w.stream), "categorical topology upload");
+  } catch (...) {
+    w.Clear();
+    throw;
+  }
+}
+
+void Workspace::Inputs(std::size_t batch, void **nodes, std::size_t *node_count, void **edges,
+                       std::size_t *edge_count) {
+  Impl &w = *impl_;
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede Inputs");
+  if (w.categorical) throw std::logic_error("accelerator dense Inputs cannot be used after ReserveCategorical");
+  if (w.device_resident()) throw std::logic_error("this workspace has no pinned input storage");
+  if (batch == 0 || batch > w.capacity) throw std::invalid_argument("accelerator input batch exceeds the reserved capacity");
+  *nodes = w.host_nodes.ptr;
+  *node_count = Product({batch, w.N(), w.states}, "accelerator input nodes");
+  *edges = w.host_edges.ptr;
+  *edge_count = w.E() * w.KK();
+}
+
+void Workspace::CategoricalInputs(std::size_t batch, std::uint8_t **observations, std::size_t *observation_count,
+                                  void **root, void **emissions, std::size_t *emission_count, void **edges,
+                                  std::size_t *edge_count) {
+  Impl &w = *impl_;
+  if (w.plan == nullptr || !w.categorical)
+    throw std::logic_error("accelerator Workspace::ReserveCategorical must precede CategoricalInputs");
+  if (batch == 0 || batch > w.capacity)
+    throw std::invalid_argument("accelerator categorical input batch exceeds the reserved capacity");
+  *observations = w.host_observations.as<std::uint8_t>();
+  *observation_count = Product({batch, w.observation_nodes.size()}, "categorical observations");
+  *root = w.host_root.ptr;
+  *emissions = w.host_emissions.ptr;
+  *emission_count = w.categories * w.states;
+  *edges = w.host_edges.ptr;
+  *edge_count = w.E() * w.KK();
+}
+
+void Workspace::Uniforms(std::size_t batch, void **uniforms, std::size_t *count) {
+  Impl &w = *impl_;
+  if (!(w.flags & THMM_RESERVE_SAMPLING) || w.plan == nullptr)
+    throw std::logic_error("accelerator Workspace::ReserveSampling must precede Uniforms");
+  if (w.device_resident()) throw std::logic_error("this workspace has no pinned uniform storage");
+  if (batch == 0 || batch > w.capacity) throw std::invalid_argument("accelerator uniform batch exceeds the reserved capacity");
+  *uniforms = w.host_uniforms.ptr;
+  *count = Product({batch, w.N()}, "accelerator posterior uniforms");
+}
+
+void Workspace::Partition(const thmm_dense_model &model, bool log_domain, const void **values, std::size_t *count,
+                          thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "Workspace::Reserve");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  StageDense(w, model);
+  Record(w, 0);
+  UploadDense(w, model);
+  Record(w, 1);
+  Record(w, 2);
+  RunPartition(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  Record(w, 5);
+  WaitDownload(w);
+  PublishValues(w, model.batch, log_domain);
+  *values = w.host_values.ptr;
+  *count = model.batch * w.Roots();
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::PartitionCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                                     bool log_domain, const void **values, std::size_t *count,
+                                     thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "Workspace::Reserve");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const CategoricalPolicy policy = StageCategorical(w, model, update);
+  Record(w, 0);
+  UploadCategorical(w, model, policy);
+  Record(w, 1);
+  Record(w, 2);
+  RunPartition(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  Record(w, 5);
+  WaitDownload(w);
+  FinishCategorical(w, model, policy);
+  PublishValues(w, model.batch, log_domain);
+  *values = w.host_values.ptr;
+  *count = model.batch * w.Roots();
+  FillTimings(w, timer, timings);
+}
+
+namespace {
+
+void DownloadMaximum(Impl &w, std::size_t batch, const void **log_weights, const std::uint32_t **states) {
+  Hip(hipMemcpyAsync(w.host_log_z.ptr, w.log_weight.ptr, batch * w.Roots() * sizeof(double), hipMemcpyDeviceToHost,
+                     w.stream),
+      "GPU runtime MemcpyAsync MAP-weight download");
+  Hip(hipMemcpyAsync(w.host_assignment.ptr, w.assignment.ptr, batch * w.N() * sizeof(std::uint32_t),
+                     hipMemcpyDeviceToHost, w.stream),
+      "GPU runtime MemcpyAsync MAP-assignment download");
+  Record(w, 5);
+  WaitDownload(w);
+  // src/gpu_backend_impl.inc:2581-2585
+  RequireFiniteLogZ(w, batch, "the tree HMM has no positive-weight assignment");
+  PublishValues(w, batch, true);
+  *log_weights = w.host_values.ptr;
+  *states = w.host_assignment.as<std::uint32_t>();
+}
+
+void RequireMaximum(const Impl &w) {
+  if (!(w.flags & THMM_RESERVE_MAXIMUM))
+    throw std::invalid_argument(
+        "prepared accelerator MAP inference requires ReserveMaximum for this plan, state count, and batch "
+        "capacity");
+}
+
+void RequireMarginals(const Impl &w) {
+  if (!(w.flags & THMM_RESERVE_MARGINALS))
+    throw std::invalid_argument(
+        "prepared accelerator marginal inference requires ReserveMarginals for this plan, state count, and batch "
+        "capacity");
+}
+
+void DownloadSample(Impl &w, std::size_t batch, const std::uint32_t **states) {
+  Hip(hipMemcpyAsync(w.host_assignment.ptr, w.assignment.ptr, batch * w.N() * sizeof(std::uint32_t),
+                     hipMemcpyDeviceToHost, w.stream),
+      "GPU runtime MemcpyAsync posterior-sample download");
+  Hip(hipMemcpyAsync(w.host_failure.ptr, w.failure.ptr, sizeof(std::uint32_t), hipMemcpyDeviceToHost, w.stream),
+      "GPU runtime MemcpyAsync sampling status download");
+  Record(w, 5);
+  WaitDownload(w);
+  // src/gpu_backend_impl.inc:2377-2384
+  RequireFiniteLogZ(w, batch, "the tree HMM has a nonpositive partition function");
+  if (*w.host_failure.as<std::uint32_t>() != 0)
+    throw std::domain_error("cannot sample from an empty conditional distribution");
+  *states = w.host_assignment.as<std::uint32_t>();
+}
+
+void DownloadMarginals(Impl &w, std::size_t batch, const void **log_partitions, const void **nodes,
+                       const void **edges) {
+  const std::size_t node_values = batch * w.N() * w.states;
+  const std::size_t edge_values = batch * w.E() * w.KK();
+  Hip(hipMemcpyAsync(w.host_node_out.ptr, w.node_out.ptr, node_values * w.S(), hipMemcpyDeviceToHost, w.stream),
+      "GPU runtime MemcpyAsync node-marginal download");
+  if (edge_values != 0)
+    Hip(hipMemcpyAsync(w.host_edge_out.ptr, w.edge_out.ptr, edge_values * w.S(), hipMemcpyDeviceToHost, w.stream),
+        "GPU runtime MemcpyAsync edge-marginal download");
+  Record(w, 5);
+  WaitDownload(w);
+  // src/gpu_backend_impl.inc:2780-2785
+  RequireFiniteLogZ(w, batch, "the tree HMM has a nonpositive partition function");
+  PublishValues(w, batch, true);
+  *log_partitions = w.host_values.ptr;
+  *nodes = w.host_node_out.ptr;
+  *edges = w.host_edge_out.ptr;
+}
+
+void UploadUniforms(Impl &w, const void *uniforms, std::size_t count) {
+  if (uniforms != w.host_uniforms.ptr) std::memcpy(w.host_uniforms.ptr, uniforms, count * w.S());
+  Hip(hipMemcpyAsync(w.uniforms.ptr, w.host_uniforms.ptr, count * w.S(), hipMemcpyHostToDevice, w.stream),
+      "GPU runtime MemcpyAsync posterior-uniform upload");
+}
+
+}  // namespace
+
+void Workspace::Maximum(const thmm_dense_model &model, const void **log_weights, const std::uint32_t **states,
+                        thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  RequireMaximum(w);
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveMaximum");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  StageDense(w, model);
+  Record(w, 0);
+  UploadDense(w, model);
+  Record(w, 1);
+  Record(w, 2);
+  RunMaximum(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadMaximum(w, model.batch, log_weights, states);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::MaximumCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                                   const void **log_weights, const std::uint32_t **states, thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  RequireMaximum(w);
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalMaximum");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const CategoricalPolicy policy = StageCategorical(w, model, update);
+  Record(w, 0);
+  UploadCategorical(w, model, policy);
+  Record(w, 1);
+  Record(w, 2);
+  RunMaximum(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadMaximum(w, model.batch, log_weights, states);
+  FinishCategorical(w, model, policy);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::Sample(const thmm_dense_model &model, const void *uniforms, std::size_t uniform_count,
+                       const std::uint32_t **states, thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveSampling");
+  ValidateUniforms(w, uniforms, uniform_count, model.batch);
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  StageDense(w, model);
+  Record(w, 0);
+  UploadDense(w, model);
+  UploadUniforms(w, uniforms, uniform_count);
+  Record(w, 1);
+  Record(w, 2);
+  RunSample(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  DownloadSample(w, model.batch, states);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::SampleCategorical(const thmm_categorical_model &model, const void *uniforms,
+                                  std::size_t uniform_count, thmm_categorical_update update,
+                                  const std::uint32_t **states, thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalSampling");
+  ValidateUniforms(w, uniforms, uniform_count, model.batch);
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const CategoricalPolicy policy = StageCategorical(w, model, update);
+  Record(w, 0);
+  UploadCategorical(w, model, policy);
+  UploadUniforms(w, uniforms, uniform_count);
+  Record(w, 1);
+  Record(w, 2);
+  RunSample(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  DownloadSample(w, model.batch, states);
+  FinishCategorical(w, model, policy);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::Marginals(const thmm_dense_model &model, const void **log_partitions, const void **nodes,
+                          const void **edges, thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  RequireMarginals(w);
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveMarginals");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  StageDense(w, model);
+  Record(w, 0);
+  UploadDense(w, model);
+  Record(w, 1);
+  Record(w, 2);
+  RunMarginals(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  DownloadMarginals(w, model.batch, log_partitions, nodes, edges);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::MarginalsCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                                     const void **log_partitions, const void **nodes, const void **edges,
+                                     thmm_timings *timings) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  const Timer timer;
+  RequireMarginals(w);
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalMarginals");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const CategoricalPolicy policy = StageCategorical(w, model, update);
+  Record(w, 0);
+  UploadCategorical(w, model, policy);
+  Record(w, 1);
+  Record(w, 2);
+  RunMarginals(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  Record(w, 3);
+  Record(w, 4);
+  DownloadLogZ(w, model.batch);
+  DownloadMarginals(w, model.batch, log_partitions, nodes, edges);
+  FinishCategorical(w, model, policy);
+  FillTimings(w, timer, timings);
+}
+
+void Workspace::LogPartitionsF64(const double **values, std::size_t *count) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  *values = w.host_log_z.as<double>();
+  *count = w.capacity * w.Roots();
+}
+
+namespace {
+
+hipStream_t ChooseStream(const Impl &w, void *stream) {
+  return stream != nullptr ? static_cast<hipStream_t>(stream) : w.stream;
+}
+
+void CheckDevice(const Impl &w, std::size_t batch, const void *nodes, const void *edges) {
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
+  if (w.categorical) throw std::invalid_argument("device-resident calls take dense potentials");
+  if (batch == 0 || batch > w.capacity)
+    throw std::invalid_argument("device-resident batch exceeds the reserved capacity");
+  if (nodes == nullptr || (w.E() != 0 && edges == nullptr)) throw std::invalid_argument("device input is null");
+}
+
+}  // namespace
+
+void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
+                                   void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, nodes, edges);
+  if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const hipStream_t s = ChooseStream(w, stream);
+  const Launch l = w.MakeLaunch(w.real, batch, s);
+  ZeroEdgeScales(w, s);
+  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
+  const LinearBuffers buf = w.Linear(edges);
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, log_z);
+  CheckLaunches();
+}
+
+void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
+                                void *node_out, void *edge_out, void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, nodes, edges);
+  RequireMarginals(w);
+  if (log_z == nullptr || node_out == nullptr || (w.E() != 0 && edge_out == nullptr))
+    throw std::invalid_argument("marginal output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const hipStream_t s = ChooseStream(w, stream);
+  const Launch l = w.MakeLaunch(w.real, batch, s);
+  ZeroEdgeScales(w, s);
+  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
+  LinearBuffers buf = w.Linear(edges);
+  buf.edge_out = edge_out;
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, log_z);
+  LaunchSeedAdjoint(l, buf, nullptr);
+  ReverseLevels(l, w.levels, buf);
+  LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+  CheckLaunches();
+}
+
+void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
+                                  void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, log_nodes, log_edges);
+  if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const hipStream_t s = ChooseStream(w, stream);
+  const Launch l = w.MakeLaunch(w.real, batch, s);
+  LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
+  LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
+  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, log_z);
+  CheckLaunches();
+  w.last_batch = batch;
+  w.forward_ready = true;
+}
+
+void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *grad_edges, bool reduce_edges,
+                                   void *stream) {
+  Impl &w = *impl_;
+  if (!w.forward_ready) throw std::logic_error("thmm_logz_backward_device requires a preceding forward call");
+  RequireMarginals(w);
+  if (grad_nodes == nullptr || (w.E() != 0 && grad_edges == nullptr))
+    throw std::invalid_argument("gradient output is null");
+  if (reduce_edges && w.E() != 0 && w.edge_out.ptr == nullptr)
+    throw std::logic_error("batch-reduced edge gradients need workspace edge storage (no device-resident flag)");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const hipStream_t s = ChooseStream(w, stream);
+  const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
+  LinearBuffers buf = w.Linear(w.edges.ptr);
+  buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
+  LaunchSeedAdjoint(l, buf, grad);
+  ReverseLevels(l, w.levels, buf);
+  LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
+  if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
+  CheckLaunches();
+}
+
+void Workspace::Synchronize() {
+  Impl &w = *impl_;
+  if (w.stream == nullptr) return;
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  Hip(hipStreamSynchronize(w.stream), "tree-HMM accelerator execution");
+}
+
+}  // namespace thmm

@@ -1,0 +1,81 @@
+// Host executor behind the C-ABI: one workspace = one device, one stream,
+// device + pinned storage sized at Reserve, level-by-level kernel launches.
+// Mirrors the behaviour of the reference executor
+// (src/gpu_backend_impl.inc:1636-2403 and 2405-3130), redesigned around
+// renamed path slots (no tape copies) and runtime precision.
+#ifndef TREE_HMM_AMD_WORKSPACE_H_
+#define TREE_HMM_AMD_WORKSPACE_H_
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "launch.h"
+#include "plan.h"
+#include "tree_hmm_amd.h"
+
+namespace thmm {
+
+// Extra reserve flag (no reference counterpart): skip pinned host staging
+// and workspace-owned output buffers; only the device-resident entry points
+// may then be used. Keeps multi-GB edge-marginal outputs single-copy.
+constexpr unsigned kReserveDeviceResident = 8u;
+
+class Workspace {
+ public:
+  Workspace();
+  ~Workspace();
+  Workspace(const Workspace &) = delete;
+  Workspace &operator=(const Workspace &) = delete;
+
+  void Reserve(const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
+               Real real, unsigned flags, int device);
+  void ReserveCategorical(const Plan &plan, const void *identity, std::size_t states,
+                          std::size_t batch, std::size_t categories, const std::uint32_t *observation_nodes,
+                          std::size_t observation_count, Real real, unsigned flags, int device);
+
+  void Inputs(std::size_t batch, void **nodes, std::size_t *node_count, void **edges,
+              std::size_t *edge_count);
+  void CategoricalInputs(std::size_t batch, std::uint8_t **observations, std::size_t *observation_count,
+                         void **root, void **emissions, std::size_t *emission_count, void **edges,
+                         std::size_t *edge_count);
+  void Uniforms(std::size_t batch, void **uniforms, std::size_t *count);
+
+  void Partition(const thmm_dense_model &model, bool log_domain, const void **values, std::size_t *count,
+                 thmm_timings *timings);
+  void PartitionCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                            bool log_domain, const void **values, std::size_t *count, thmm_timings *timings);
+  void Maximum(const thmm_dense_model &model, const void **log_weights, const std::uint32_t **states,
+               thmm_timings *timings);
+  void MaximumCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                          const void **log_weights, const std::uint32_t **states, thmm_timings *timings);
+  void Sample(const thmm_dense_model &model, const void *uniforms, std::size_t uniform_count,
+              const std::uint32_t **states, thmm_timings *timings);
+  void SampleCategorical(const thmm_categorical_model &model, const void *uniforms, std::size_t uniform_count,
+                         thmm_categorical_update update, const std::uint32_t **states, thmm_timings *timings);
+  void Marginals(const thmm_dense_model &model, const void **log_partitions, const void **nodes,
+                 const void **edges, thmm_timings *timings);
+  void MarginalsCategorical(const thmm_categorical_model &model, thmm_categorical_update update,
+                            const void **log_partitions, const void **nodes, const void **edges,
+                            thmm_timings *timings);
+  void LogPartitionsF64(const double **values, std::size_t *count);
+
+  void LogPartitionDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z, void *stream);
+  void MarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z, void *node_out,
+                       void *edge_out, void *stream);
+  void LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
+                         void *stream);
+  void LogZBackwardDevice(const double *grad, void *grad_nodes, void *grad_edges, bool reduce_edges,
+                          void *stream);
+  void Synchronize();
+
+  struct Impl;
+
+ private:
+  std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace thmm
+
+#endif  // TREE_HMM_AMD_WORKSPACE_H_

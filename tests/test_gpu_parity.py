@@ -1,0 +1,196 @@
+"""Randomised GPU-vs-oracle parity over state counts, topologies, forests and
+batch shapes, for every kernel family:
+
+* log Z and marginals (the adjoint): FP32 form within the stated
+  tolerances (|dlogZ| <= 1e-5 * max(1, |logZ|), |dmarginal| <= 1e-5),
+  plain FP64 form within 1e-11;
+* MAP states and posterior samples: bit-exact against the oracle, except
+  where the oracle reports a near-tie / a variate within 1e-12 of a CDF
+  boundary (counted and required to be rare, never silently accepted);
+* device-resident entry points and the log-potential forward/backward pair.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import parallel_tree_hmm_amd as thmm
+from conftest import caterpillar, forest, path, random_potentials, random_recursive, star, yule
+
+pytestmark = pytest.mark.gpu
+
+LOGZ_TOL = {"f32": 1e-5, "f64": 1e-11}
+MARG_TOL = {"f32": 1e-5, "f64": 1e-11}
+
+
+def topology(name, n, rng):
+    if name == "recursive":
+        return random_recursive(n, rng)
+    if name == "path":
+        return path(n)
+    if name == "star":
+        return star(n)
+    if name == "caterpillar":
+        return caterpillar(n)
+    if name == "forest":
+        parts = [random_recursive(max(1, n // 4), rng), path(max(1, n // 4)), star(max(1, n // 4)), caterpillar(max(1, n - 3 * (n // 4)))]
+        return forest(parts)
+    raise KeyError(name)
+
+
+def as_precision(x, precision):
+    return np.asarray(x, np.float32).astype(np.float64) if precision == "f32" else np.asarray(x, np.float64)
+
+
+CASES = [
+    # (K, topology, nodes, batch)
+    (1, "path", 300, 3),
+    (2, "recursive", 200, 5),
+    (3, "star", 100, 4),
+    (4, "caterpillar", 301, 33),
+    (4, "forest", 400, 2),
+    (5, "recursive", 150, 7),
+    (8, "path", 1000, 4),
+    (8, "forest", 300, 3),
+    (9, "recursive", 120, 3),
+    (16, "caterpillar", 121, 2),
+    (16, "forest", 200, 1),
+    (17, "recursive", 60, 2),
+    (32, "path", 70, 2),
+    (33, "recursive", 40, 1),
+    (64, "caterpillar", 41, 1),
+    (64, "recursive", 40, 2),
+]
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k,topo,n,batch", CASES)
+def test_log_partition_and_marginals(require_gpu, precision, k, topo, n, batch):
+    rng = np.random.default_rng(k * 1000 + n)
+    plan = thmm.Plan(topology(topo, n, rng))
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=1.0)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, batch, precision=precision)
+    out = thmm.posterior_marginals_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), ws)
+    roots = plan.roots.size
+    log_z = ws.log_partitions_f64()[: batch * roots].reshape(batch, roots)
+    gnodes = out.nodes.reshape(batch, plan.num_nodes, k)
+    gedges = out.edges.reshape(batch, plan.num_edges, k, k)
+    olz, onm, oem = oracle.marginals_batched(plan, k, nodes, edges, threads=4)
+    assert np.all(np.abs(log_z - olz) <= LOGZ_TOL[precision] * np.maximum(1.0, np.abs(olz)))
+    assert np.max(np.abs(gnodes - onm)) <= MARG_TOL[precision]
+    if plan.num_edges:
+        assert np.max(np.abs(gedges - oem)) <= MARG_TOL[precision]
+    # log partition alone agrees with the marginal pass
+    lp = thmm.log_partition_function_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), ws)
+    assert np.allclose(ws.log_partitions_f64()[: batch * roots].reshape(batch, roots), log_z, rtol=0, atol=0)
+    assert lp.values.shape == (batch * roots,)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k,topo,n,batch", [c for c in CASES if c[0] <= 33])
+def test_maximum_matches_oracle(require_gpu, precision, k, topo, n, batch):
+    rng = np.random.default_rng(7 * k + n)
+    plan = thmm.Plan(topology(topo, n, rng))
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    ws = thmm.Workspace()
+    ws.reserve_maximum(plan, k, batch, precision=precision)
+    out = thmm.maximum_a_posteriori_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), ws)
+    states = out.states.reshape(batch, -1)
+    weights = out.log_weights.reshape(batch, -1)
+    for b in range(batch):
+        lw, expected = oracle.maximum(plan, k, nodes[b], edges)
+        assert np.all(np.abs(weights[b] - lw) <= (1e-6 if precision == "f32" else 1e-12) * np.maximum(1, np.abs(lw)))
+        assert np.array_equal(states[b], expected), f"MAP states differ for item {b}"
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k,topo,n,batch", [c for c in CASES if c[0] <= 33])
+def test_samples_match_oracle(require_gpu, precision, k, topo, n, batch):
+    rng = np.random.default_rng(11 * k + n)
+    plan = thmm.Plan(topology(topo, n, rng))
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    u = as_precision(rng.random((batch, plan.num_nodes)), precision)
+    ws = thmm.Workspace()
+    ws.reserve_sampling(plan, k, batch, precision=precision)
+    out = thmm.posterior_sample_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), u, ws)
+    states = out.states.reshape(batch, -1)
+    near_boundary = 0
+    for b in range(batch):
+        expected, dist = oracle.sample(plan, k, nodes[b], edges, u[b])
+        if np.min(dist) < 1e-12:
+            near_boundary += 1  # reported separately, not compared
+            continue
+        assert np.array_equal(states[b], expected), f"sample differs for item {b}"
+    assert near_boundary <= max(1, batch // 10)
+
+
+def test_device_entry_points_match_host(require_gpu):
+    import torch
+
+    rng = np.random.default_rng(5)
+    plan = thmm.Plan(forest([random_recursive(300, rng), path(200), star(100)]))
+    k, batch = 8, 3
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k)
+    nodes, edges = nodes.astype(np.float32), edges.astype(np.float32)
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, batch)
+    host = thmm.posterior_marginals_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), ws)
+    host_logz = ws.log_partitions_f64()[: batch * 3].copy()
+    host_nodes, host_edges = host.nodes.copy(), host.edges.copy()
+
+    dws = thmm.Workspace()
+    dws.reserve_marginals(plan, k, batch, device_resident=True)
+    d_nodes = torch.from_numpy(nodes).cuda()
+    d_edges = torch.from_numpy(edges).cuda()
+    d_logz = torch.empty(batch * 3, dtype=torch.float64, device="cuda")
+    d_nm = torch.empty(batch * plan.num_nodes * k, dtype=torch.float32, device="cuda")
+    d_em = torch.empty(batch * plan.num_edges * k * k, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    dws.marginals_device(
+        d_nodes.data_ptr(), d_edges.data_ptr(), batch, d_logz.data_ptr(), d_nm.data_ptr(), d_em.data_ptr(), stream
+    )
+    torch.cuda.synchronize()
+    assert np.array_equal(d_logz.cpu().numpy(), host_logz)
+    assert np.array_equal(d_nm.cpu().numpy(), host_nodes)
+    assert np.array_equal(d_em.cpu().numpy(), host_edges)
+    d_lz2 = torch.empty_like(d_logz)
+    dws.log_partition_device(d_nodes.data_ptr(), d_edges.data_ptr(), batch, d_lz2.data_ptr(), stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(d_lz2.cpu().numpy(), host_logz)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_log_potential_forward_backward(require_gpu, precision):
+    import torch
+
+    rng = np.random.default_rng(9)
+    plan = thmm.Plan(yule(40, rng))
+    k, batch = 6, 3
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    log_nodes = torch.tensor(rng.normal(size=(batch, plan.num_nodes, k)), dtype=dtype)
+    log_edges = torch.tensor(rng.normal(size=(plan.num_edges, k, k)), dtype=dtype)
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, batch, precision=precision)
+    d_ln, d_le = log_nodes.cuda(), log_edges.cuda()
+    d_logz = torch.empty(batch, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+    ws.logz_forward_device(d_ln.data_ptr(), d_le.data_ptr(), batch, d_logz.data_ptr(), stream)
+    grad = torch.tensor([1.0, -0.5, 2.0], dtype=torch.float64, device="cuda")
+    g_nodes = torch.empty_like(d_ln)
+    g_edges = torch.empty_like(d_le)
+    ws.logz_backward_device(grad.data_ptr(), g_nodes.data_ptr(), g_edges.data_ptr(), True, stream)
+    torch.cuda.synchronize()
+    nodes = np.exp(log_nodes.double().numpy())
+    edges = np.exp(log_edges.double().numpy())
+    olz, onm, oem = oracle.marginals_batched(plan, k, nodes, edges)
+    tol = 1e-5 if precision == "f32" else 1e-11
+    assert np.all(np.abs(d_logz.cpu().numpy() - olz[:, 0]) <= tol * np.maximum(1, np.abs(olz[:, 0])))
+    g = grad.cpu().numpy()
+    assert np.max(np.abs(g_nodes.cpu().double().numpy() - g[:, None, None] * onm)) <= tol * 2
+    assert np.max(np.abs(g_edges.cpu().double().numpy() - np.einsum("b,bekl->ekl", g, oem))) <= tol * 4
