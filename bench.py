@@ -1,0 +1,358 @@
+"""Throughput benchmark: nodes/s of log Z + adjoint (posterior marginals =
+d log Z / d log potentials) on one MI355X.
+
+Default workload (the one the headline target is quoted on): 4,096 trees
+with node counts log-uniform in [16, 65,536], topologies cycling random
+recursive / caterpillar / star, K = 16, FP32 form, contracted together as one
+forest (batch 1, one log partition per tree). Inputs are generated and kept
+resident in HBM before the timed region; one step is one full log Z +
+adjoint pass over the forest. The plain FP64 form is timed on the same
+inputs, and the FP64 CPU oracle is timed on a bounded sample of the trees.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 4]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "nodes/s logZ+adjoint: FP32-MFMA form vs plain FP64 at rel err <=1e-5; % roofline"
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# Workloads
+# ---------------------------------------------------------------------------
+
+
+def recursive_tree(n, rng):
+    p = np.empty(n, np.int64)
+    p[0] = -1
+    if n > 1:
+        p[1:] = (rng.random(n - 1) * np.arange(1, n)).astype(np.int64)
+    return p
+
+
+def caterpillar_tree(n):
+    i = np.arange(n, dtype=np.int64)
+    p = np.where(i % 2 == 0, i - 2, i - 1)
+    p[0] = -1
+    return p
+
+
+def star_tree(n):
+    p = np.zeros(n, np.int64)
+    p[0] = -1
+    return p
+
+
+def forest_workload(trees: int, low: int, high: int, seed: int, scale: float = 1.0):
+    """Per-tree parent arrays of the uneven mixed-topology forest."""
+    rng = np.random.default_rng(seed)
+    count = max(3, int(round(trees * scale)))
+    sizes = np.floor(np.exp(rng.uniform(math.log(low), math.log(high + 1), size=count))).astype(np.int64)
+    parts = []
+    for t, n in enumerate(sizes):
+        kind = t % 3
+        parts.append(recursive_tree(n, rng) if kind == 0 else caterpillar_tree(n) if kind == 1 else star_tree(n))
+    return parts
+
+
+def concatenate(parts):
+    offsets = np.zeros(len(parts) + 1, np.int64)
+    offsets[1:] = np.cumsum([p.size for p in parts])
+    out = np.empty(offsets[-1], np.int64)
+    for t, p in enumerate(parts):
+        out[offsets[t] : offsets[t + 1]] = np.where(p < 0, -1, p + offsets[t])
+    return out, offsets
+
+
+# ---------------------------------------------------------------------------
+# Algorithmic bytes per op (FP32 s = 4; per-item edge slots at batch 1)
+# ---------------------------------------------------------------------------
+
+
+def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
+    kk = k * k
+    if phase == 1:  # forward
+        return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s}[primitive]
+    if phase == 3:  # reverse
+        return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (6 * kk + 2 * k) * s}[primitive]
+    return 0
+
+
+KERNEL_NAMES = {
+    (1, 0): "RakeForward",
+    (1, 1): "CombineForward",
+    (1, 2): "AbsorbForward",
+    (1, 3): "CompressForward",
+    (3, 0): "RakeReverse",
+    (3, 1): "CombineReverse",
+    (3, 2): "AbsorbReverse",
+    (3, 3): "CompressReverse",
+}
+
+
+# ---------------------------------------------------------------------------
+# GPU leg
+# ---------------------------------------------------------------------------
+
+
+def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup):
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    nodes = nodes.to(dtype)
+    edges = edges.to(dtype)
+    n, e = plan.num_nodes, plan.num_edges
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, 1, precision=precision, device_resident=True)
+    log_z = torch.empty(plan.roots.size, dtype=torch.float64, device="cuda")
+    node_out = torch.empty(n * k, dtype=dtype, device="cuda")
+    edge_out = torch.empty(e * k * k, dtype=dtype, device="cuda")
+    stream = torch.cuda.current_stream()
+    args = (nodes.data_ptr(), edges.data_ptr(), 1, log_z.data_ptr(), node_out.data_ptr(), edge_out.data_ptr(), stream.cuda_stream)
+    for _ in range(warmup):
+        ws.marginals_device(*args)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        ws.marginals_device(*args)
+    stop.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / steps
+    timings = ws.profile_marginals_device(*args)
+    torch.cuda.synchronize()
+    return ws, ms, timings, log_z, node_out, edge_out
+
+
+def roofline_from(timings, k, s):
+    totals = {}
+    for phase, primitive, count, batch, ms in timings:
+        if phase not in (1, 3):
+            continue
+        key = (phase, primitive)
+        t = totals.setdefault(key, [0.0, 0, 0])
+        t[0] += ms
+        t[1] += op_bytes(phase, primitive, k, s) * count * batch
+        t[2] += 1
+    key = max(totals, key=lambda x: totals[x][0])
+    ms, nbytes, launches = totals[key]
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    breakdown = {KERNEL_NAMES[kk]: round(v[0], 4) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0])}
+    return {
+        "kernel": KERNEL_NAMES[key],
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "launches": launches,
+        "avg_launch_us": round(ms / launches * 1e3, 3),
+        "algorithmic_bytes_per_launch": int(nbytes / launches),
+        "kernel_ms_breakdown": breakdown,
+    }
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline + parity sample (oracle)
+# ---------------------------------------------------------------------------
+
+
+def tree_slices(offsets, t):
+    lo, hi = int(offsets[t]), int(offsets[t + 1])
+    return lo, hi, lo - t, hi - t - 1  # node range, edge range
+
+
+def tree_inputs(nodes, edges, offsets, t):
+    """FP64 host copies of one tree's inputs (sliced on the device)."""
+    lo, hi, elo, ehi = tree_slices(offsets, t)
+    return nodes[lo:hi].double().cpu().numpy(), edges[elo:ehi].double().cpu().numpy()
+
+
+def cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, budget_nodes, threads):
+    """Times the FP64 oracle on a bounded sample of the trees (one tree per
+    task, `threads` host threads)."""
+    order = np.random.default_rng(7).permutation(len(parts))
+    chosen, total = [], 0
+    for t in order:
+        if total >= budget_nodes:
+            break
+        if parts[t].size > budget_nodes // 2:
+            continue
+        chosen.append(int(t))
+        total += parts[t].size
+    plans = {t: thmm.Plan(parts[t]) for t in chosen}
+    jobs = [(t, *tree_inputs(nodes, edges, offsets, t)) for t in chosen]
+
+    def work(job):
+        t, nn, ee = job
+        return oracle.marginals(plans[t], k, nn, ee)
+
+    start = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(work, jobs))
+    seconds = time.perf_counter() - start
+    return total / seconds, total, len(chosen), seconds
+
+
+def parity_sample(thmm, oracle, parts, offsets, nodes, edges, k, log_z, node_out, edge_out, picks):
+    """Worst errors of the GPU outputs against the oracle on a few trees."""
+    worst = {"log_z_rel": 0.0, "node_abs": 0.0, "edge_abs": 0.0}
+    for t in picks:
+        plan = thmm.Plan(parts[t])
+        lo, hi, elo, ehi = tree_slices(offsets, t)
+        nn, ee = tree_inputs(nodes, edges, offsets, t)
+        lz, nm, em = oracle.marginals(plan, k, nn, ee)
+        worst["log_z_rel"] = max(worst["log_z_rel"], abs(log_z[t] - lz[0]) / max(1.0, abs(lz[0])))
+        gpu_nodes = node_out[lo:hi].double().cpu().numpy()
+        worst["node_abs"] = max(worst["node_abs"], float(np.max(np.abs(gpu_nodes - nm))))
+        if ehi > elo:
+            gpu_edges = edge_out[elo:ehi].double().cpu().numpy()
+            worst["edge_abs"] = max(worst["edge_abs"], float(np.max(np.abs(gpu_edges - em))))
+    return worst
+
+
+# ---------------------------------------------------------------------------
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=4, choices=[4])
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the trees (quick runs)")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-nodes", type=int, default=400_000)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU only (asked for {args.gpus}); batch items and trees are independent, "
+            "so N GPUs would each run their own shard with no exchange",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    import oracle
+    import parallel_tree_hmm_amd as thmm
+
+    if not thmm.available():
+        print("no HIP device visible", file=sys.stderr)
+        return 1
+    log(f"device: {thmm.device_description(0)}")
+
+    k = 16
+    seed = 0x54484D4D00000004
+    t0 = time.perf_counter()
+    parts = forest_workload(4096, 16, 65536, seed, args.scale)
+    parents, offsets = concatenate(parts)
+    plan = thmm.Plan(parents)
+    plan_s = time.perf_counter() - t0
+    n, e = plan.num_nodes, plan.num_edges
+    log(f"forest: {len(parts)} trees, {n} nodes, {plan.num_rounds} rounds, {plan.primitive_batches.size} levels, "
+        f"{plan.num_compressions} compressions; planned in {plan_s:.1f}s")
+
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed & 0x7FFFFFFFFFFFFFFF)
+    nodes = torch.randn((n, k), device="cuda", dtype=torch.float32, generator=gen).exp_()
+    edges = torch.empty((e, k, k), device="cuda", dtype=torch.float32).exponential_(1.0, generator=gen)
+    edges /= edges.sum(dim=2, keepdim=True)
+    torch.cuda.synchronize()
+
+    ws, ms, timings, log_z, node_out, edge_out = run_gpu(thmm, torch, plan, k, "f32", nodes, edges, args.steps, args.warmup)
+    value = n / (ms * 1e-3)
+    roofline = roofline_from(timings, k, 4)
+    log(f"FP32 form: {ms:.3f} ms/step, {value:.4g} nodes/s; dominant kernel {roofline['kernel']} "
+        f"{roofline['achieved']} GB/s ({roofline['frac']:.1%} of HBM peak)")
+
+    # Parity on a sample of trees (one of each size extreme, a few small).
+    log_z_h = log_z.cpu().numpy()
+    node_view = node_out.view(n, k)
+    edge_view = edge_out.view(e, k, k)
+    sizes = np.array([p.size for p in parts])
+    picks = [int(np.argmin(sizes))] + [int(t) for t in np.flatnonzero(sizes < 3000)[:6]] + [int(np.argmax(sizes))]
+    parity = parity_sample(thmm, oracle, parts, offsets, nodes, edges, k, log_z_h, node_view, edge_view, picks)
+    # Size-independent checks over the whole forest: every node marginal sums
+    # to one and every edge marginal's row sums equal its parent marginal.
+    parity["node_normalisation_abs"] = float((node_view.double().sum(1) - 1.0).abs().max())
+    eparents = torch.from_numpy(plan.edge_parents.astype(np.int64)).cuda()
+    parity["edge_row_sum_abs"] = float((edge_view.double().sum(2) - node_view.double()[eparents]).abs().max())
+    parity["trees_checked"] = len(picks)
+    del eparents
+    ws.close()
+    del node_out, edge_out, ws
+    torch.cuda.empty_cache()
+
+    fp64 = None
+    if not args.no_fp64:
+        ws64, ms64, _, lz64, _, _ = run_gpu(thmm, torch, plan, k, "f64", nodes, edges, max(2, args.steps // 2), 1)
+        fp64 = {"ms_per_step": round(ms64, 4), "value": n / (ms64 * 1e-3),
+                "fp32_speedup": round(ms64 / ms, 3),
+                "fp32_vs_fp64_log_z_rel": float(np.max(np.abs(lz64.cpu().numpy() - log_z_h) / np.maximum(1, np.abs(lz64.cpu().numpy()))))}
+        log(f"plain FP64 form: {ms64:.3f} ms/step; FP32 speedup {ms64 / ms:.2f}x")
+        ws64.close()
+        del ws64
+        torch.cuda.empty_cache()
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        rate, total, count, seconds = cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, args.cpu_nodes, threads)
+        cpu = {"value": rate, "unit": "nodes/s", "cores": threads, "kind": "port",
+               "sample": f"{count} whole trees, {total} nodes, FP64 oracle log Z + marginals, {seconds:.1f}s wall"}
+        log(f"CPU oracle: {rate:.4g} nodes/s on {threads} threads")
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (torch device RNG: node potentials exp(N(0,1)), edge rows Dirichlet(1))",
+        "config": {
+            "workload": "uneven forest: 4096 trees, N log-uniform [16, 65536], random recursive / caterpillar / star, "
+            "K=16, FP32 log Z + adjoint (node and edge marginals)",
+            "trees": len(parts),
+            "nodes": int(n),
+            "states": k,
+            "levels": int(plan.primitive_batches.size),
+            "scale": args.scale,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_plain": fp64,
+        "parity": parity,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

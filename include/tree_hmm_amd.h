@@ -329,6 +329,28 @@ thmm_status thmm_logz_backward_device(thmm_workspace *workspace,
 /* Blocks until the workspace stream has drained. */
 thmm_status thmm_synchronize(thmm_workspace *workspace);
 
+/* ------------------------------------------------------------------------ */
+/* Measurement                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* One launch of a profiled call. phase: 0 staging, 1 forward level,
+ * 2 finish + adjoint seed, 3 reverse level, 4 output transpose. primitive
+ * is a thmm_primitive for phases 1 and 3; count is the level's op count. */
+typedef struct thmm_launch_timing {
+  uint32_t phase, primitive, count, batch;
+  double ms;
+} thmm_launch_timing;
+
+/* thmm_marginals_device with a HIP event pair recorded on the launch stream
+ * around every kernel launch; blocks until done and reports up to
+ * `capacity` launch durations. No reference counterpart: it replaces the
+ * coarse AcceleratorTimings::kernel_ms with per-level figures. */
+thmm_status thmm_profile_marginals_device(
+    thmm_workspace *workspace, const void *d_node_potentials,
+    const void *d_edge_potentials, size_t batch, double *d_log_z,
+    void *d_node_marginals, void *d_edge_marginals, void *stream,
+    thmm_launch_timing *timings, size_t capacity, size_t *count);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -102,6 +102,16 @@ class Timings(ctypes.Structure):
     ]
 
 
+class LaunchTiming(ctypes.Structure):
+    _fields_ = [
+        ("phase", c_u32),
+        ("primitive", c_u32),
+        ("count", c_u32),
+        ("batch", c_u32),
+        ("ms", ctypes.c_double),
+    ]
+
+
 class DenseModel(ctypes.Structure):
     _fields_ = [
         ("identity", c_void_p),
@@ -190,6 +200,10 @@ PROTOTYPES = {
     "thmm_logz_forward_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
     "thmm_logz_backward_device": (c_status, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "thmm_synchronize": (c_status, [c_void_p]),
+    "thmm_profile_marginals_device": (
+        c_status,
+        [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, P(LaunchTiming), c_size, P(c_size)],
+    ),
 }
 
 _LIB: ctypes.CDLL | None = None

@@ -433,6 +433,20 @@ class Workspace:
             )
         )
 
+    def profile_marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream: int = 0):
+        """thmm_marginals_device with per-launch HIP event timing; returns a
+        list of (phase, primitive, count, batch, ms) tuples."""
+        capacity = 4 * (self._plan.primitive_batches.size + 8)
+        timings = (_abi.LaunchTiming * capacity)()
+        count = ctypes.c_size_t()
+        _check(
+            _lib().thmm_profile_marginals_device(
+                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream or None,
+                timings, capacity, ctypes.byref(count)
+            )
+        )
+        return [(t.phase, t.primitive, t.count, t.batch, t.ms) for t in timings[: min(count.value, capacity)]]
+
     def synchronize(self) -> None:
         _check(_lib().thmm_synchronize(self._handle))
 

@@ -95,9 +95,42 @@ def build_oracle() -> Path:
     return ORACLE_LIBRARY
 
 
+CONFORMANCE_SOURCE = ROOT / "tests" / "cpp" / "conformance.cc"
+CONFORMANCE_BINARIES = {
+    "f64": ROOT / "tests" / "cpp" / "conformance_f64",
+    "f32": ROOT / "tests" / "cpp" / "conformance_f32",
+}
+
+
+def build_conformance() -> None:
+    """The C++ drop-in conformance programs (tests/cpp), linked against the
+    library and the oracle."""
+    for precision, binary in CONFORMANCE_BINARIES.items():
+        _run(
+            [
+                "g++",
+                "-std=c++20",
+                "-O1",
+                "-Wall",
+                *(["-DTREE_HMM_USE_FLOAT"] if precision == "f32" else []),
+                f"-I{INCLUDE}",
+                str(CONFORMANCE_SOURCE),
+                "-o",
+                str(binary),
+                f"-L{PACKAGE}",
+                "-ltree_hmm_amd",
+                f"-L{ORACLE_DIR}",
+                "-loracle",
+                f"-Wl,-rpath,{PACKAGE}:{ORACLE_DIR}:/opt/rocm/lib",
+                "-Wl,-rpath,$ORIGIN/../../parallel_tree_hmm_amd:$ORIGIN/../../oracle",
+            ]
+        )
+
+
 def build_all() -> None:
     build_library()
     build_oracle()
+    build_conformance()
     shutil.rmtree(OBJ, ignore_errors=True)
 
 

@@ -274,6 +274,17 @@ thmm_status thmm_logz_backward_device(thmm_workspace *workspace, const double *d
   });
 }
 
+thmm_status thmm_profile_marginals_device(thmm_workspace *workspace, const void *d_node_potentials,
+                                          const void *d_edge_potentials, size_t batch, double *d_log_z,
+                                          void *d_node_marginals, void *d_edge_marginals, void *stream,
+                                          thmm_launch_timing *timings, size_t capacity, size_t *count) {
+  return Guard([&] {
+    Require(timings != nullptr && count != nullptr, "timing output is null");
+    W(workspace).MarginalsDevice(d_node_potentials, d_edge_potentials, batch, d_log_z, d_node_marginals,
+                                 d_edge_marginals, stream, timings, capacity, count);
+  });
+}
+
 thmm_status thmm_synchronize(thmm_workspace *workspace) {
   return Guard([&] { W(workspace).Synchronize(); });
 }

@@ -4,20 +4,30 @@
 // batch item. A group of G lanes (G = next power of two >= K, at most 64)
 // owns an item, lane s owning state s (the parent state of a rake, the
 // output row of a compression). Items are numbered op-major, batch-minor, so
-// the lanes of a wavefront share the op and read the batch-shared edge tile
+// the lanes of a wavefront share the op and read a batch-shared edge tile
 // once while their node/branch vectors are contiguous in [node][batch][state]
-// storage. Group reductions use cross-lane shuffles within the group.
+// storage.
 //
-// Algebra (scaled linear domain; each output divided by its maximum and the
-// log of that maximum added to an FP64 running log scale) follows
-// src/inference.cc:697-836 and src/gpu_device_algebra.h:41-52. The reverse
-// pass computes the log-domain adjoint of src/inference.cc:416-518 with
-// ratios of normalised values (term / output instead of exp(term - output)).
-// Max-product follows src/inference.cc:564-695 in FP64; sampling follows the
-// inverse-CDF rule of src/inference.cc:128-146 in FP64.
+// Number representation (element type T = float for the FP32 form, double
+// for the plain FP64 form and for sampling):
+//  * node and branch vectors are max-shifted logs (largest entry 0) plus an
+//    FP64 log scale, so products of messages of any dynamic range are sums;
+//  * composed path matrices are linear with every row divided by its
+//    maximum, plus one FP64 log scale per row ("per-row max shift"); edge
+//    slots hold the caller's linear potentials (row scale 0) or, for log
+//    inputs, row-max-shifted exponentials with their row scales;
+//  * adjoints are linear (posterior masses in [0, 1]).
+// A rake is then a linear mat-vec on exp(shifted log), a compression a
+// linear matrix product after folding the right factor's row scales into the
+// middle vector, and every normaliser is a logarithm added to an FP64 scale.
+// This is the scaled sum-product of src/inference.cc:697-836 with the
+// normalisation moved into the log domain; the reverse pass is the analytic
+// adjoint of src/inference.cc:416-518 written with ratios of normalised
+// terms (term / output instead of exp(term - output)), the ratios taken in
+// FP64. Max-product follows src/inference.cc:564-695 in FP64; sampling
+// follows the inverse-CDF rule of src/inference.cc:128-146 in FP64.
 #include <hip/hip_runtime.h>
 
-#include <cstdio>
 #include <string>
 
 #include "launch.h"
@@ -32,8 +42,14 @@ void Check(hipError_t status, const char *what) {
     g_launch_error = std::string(what) + ": " + hipGetErrorString(status);
 }
 
+constexpr double kNegInf = -__builtin_huge_val();
+
 __device__ inline float Max(float a, float b) { return fmaxf(a, b); }
 __device__ inline double Max(double a, double b) { return fmax(a, b); }
+__device__ inline float Exp(float x) { return expf(x); }
+__device__ inline double Exp(double x) { return exp(x); }
+
+template <typename T> __device__ inline T NegInf() { return static_cast<T>(kNegInf); }
 
 template <int G, typename V> __device__ inline V GroupMax(V v) {
 #pragma unroll
@@ -41,36 +57,27 @@ template <int G, typename V> __device__ inline V GroupMax(V v) {
   return v;
 }
 
-constexpr double kNegInf = -__builtin_huge_val();
-
-template <typename T> __device__ inline double UpdatedScale(double input_scale, T top) {
-  return top > T(0) ? input_scale + log(static_cast<double>(top)) : kNegInf;
-}
-
 template <typename T>
-__device__ inline const T *Slot(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
-                                std::uint32_t b) {
+__device__ inline const T *Slot(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
   return slot < d.E ? static_cast<const T *>(buf.edges) + slot * kk
-                    : static_cast<const T *>(buf.paths) +
-                          (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+                    : static_cast<const T *>(buf.paths) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
 }
 
-__device__ inline double SlotScale(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
-                                   std::uint32_t b) {
-  return slot < d.E ? buf.edge_scale[slot]
-                    : buf.path_scale[static_cast<std::uint64_t>(slot - d.E) * d.B + b];
+// FP64 log scale of row `row` of a slot.
+__device__ inline double RowScale(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                                  std::uint32_t row) {
+  if (slot < d.E) return buf.edge_scale ? buf.edge_scale[static_cast<std::uint64_t>(slot) * d.K + row] : 0.0;
+  return buf.path_scale[(static_cast<std::uint64_t>(slot - d.E) * d.B + b) * d.K + row];
 }
 
 // Adjoint storage of a slot: edge marginals are written batch-major
 // [B][E][K][K] straight into the output; composed slots are internal.
 template <typename T>
-__device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
-                                 std::uint32_t b) {
+__device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
   return slot < d.E ? static_cast<T *>(buf.edge_out) + (static_cast<std::uint64_t>(b) * d.E + slot) * kk
-                    : static_cast<T *>(buf.path_adj) +
-                          (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+                    : static_cast<T *>(buf.path_adj) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
 }
 
 struct Lane {
@@ -87,33 +94,42 @@ template <int G> __device__ inline Lane LaneOf() {
 // Scaled forward
 // ------------------------------------------------------------------------
 
-// branch = path . leaf (src/inference.cc:704-723)
+// branch[p] = log sum_c path[p][c] exp(leaf[c]) (src/inference.cc:704-723)
 template <typename T, int G>
-__global__ void RakeForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                            LinearBuffers buf) {
+__global__ void RakeForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
   const DeviceRake op = ops.rakes[offset + lane.item / d.B];
-  const T *v = static_cast<const T *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
-  const T *P = Slot<T>(buf, d, op.slot, b);
-  T m = T(0);
+  const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
+  const T *v = static_cast<const T *>(buf.nodes) + leaf_at * d.K;
+  double message = kNegInf;
   if (lane.s < d.K) {
-    const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
-    for (std::uint32_t c = 0; c < d.K; ++c) m += row[c] * v[c];
+    const T *row = Slot<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
+    T acc = T(0);
+    for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
+    if (acc > T(0)) message = RowScale(buf, d, op.slot, b, lane.s) + log(static_cast<double>(acc));
   }
-  const T top = GroupMax<G>(m);
+  const double top = GroupMax<G>(message);
   const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
-  if (lane.s < d.K) static_cast<T *>(buf.branches)[at * d.K + lane.s] = top > T(0) ? m * (T(1) / top) : m;
-  if (lane.s == 0)
-    buf.branch_scale[at] =
-        UpdatedScale(SlotScale(buf, d, op.slot, b) + buf.node_scale[static_cast<std::uint64_t>(op.leaf) * d.B + b], top);
+  const bool live = top > kNegInf;
+  if (lane.s < d.K) static_cast<T *>(buf.branches)[at * d.K + lane.s] = live ? static_cast<T>(message - top) : NegInf<T>();
+  if (lane.s == 0) buf.branch_scale[at] = live ? buf.node_scale[leaf_at] + top : kNegInf;
 }
 
-// branch[dst] *= branch[src] (src/inference.cc:725-735)
+// Sum of two shifted log vectors, re-shifted (src/inference.cc:725-747).
 template <typename T, int G>
-__global__ void CombineForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                               LinearBuffers buf) {
+__device__ inline void AddLogVectors(T *dst, double *dst_scale, const T *src, double src_scale, std::uint32_t s,
+                                     std::uint32_t K) {
+  const T x = s < K ? dst[s] + src[s] : NegInf<T>();
+  const T top = GroupMax<G>(x);
+  const bool live = top > NegInf<T>();
+  if (s < K) dst[s] = live ? x - top : NegInf<T>();
+  if (s == 0) *dst_scale = live ? *dst_scale + src_scale + static_cast<double>(top) : kNegInf;
+}
+
+template <typename T, int G>
+__global__ void CombineForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
@@ -121,75 +137,81 @@ __global__ void CombineForward(DeviceOps ops, Dims d, std::uint32_t offset, std:
   const std::uint64_t dst = static_cast<std::uint64_t>(op.destination) * d.B + b;
   const std::uint64_t src = static_cast<std::uint64_t>(op.source) * d.B + b;
   T *branches = static_cast<T *>(buf.branches);
-  T x = T(0);
-  if (lane.s < d.K) x = branches[dst * d.K + lane.s] * branches[src * d.K + lane.s];
-  const T top = GroupMax<G>(x);
-  if (lane.s < d.K) branches[dst * d.K + lane.s] = top > T(0) ? x * (T(1) / top) : x;
-  if (lane.s == 0) buf.branch_scale[dst] = UpdatedScale(buf.branch_scale[dst] + buf.branch_scale[src], top);
+  AddLogVectors<T, G>(branches + dst * d.K, buf.branch_scale + dst, branches + src * d.K, buf.branch_scale[src],
+                      lane.s, d.K);
 }
 
-// node[parent] *= branch (src/inference.cc:737-747)
 template <typename T, int G>
-__global__ void AbsorbForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                              LinearBuffers buf) {
+__global__ void AbsorbForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
   const DeviceAbsorb op = ops.absorbs[offset + lane.item / d.B];
   const std::uint64_t node = static_cast<std::uint64_t>(op.parent) * d.B + b;
   const std::uint64_t branch = static_cast<std::uint64_t>(op.branch) * d.B + b;
-  T *nodes = static_cast<T *>(buf.nodes);
-  T x = T(0);
-  if (lane.s < d.K) x = nodes[node * d.K + lane.s] * static_cast<const T *>(buf.branches)[branch * d.K + lane.s];
-  const T top = GroupMax<G>(x);
-  if (lane.s < d.K) nodes[node * d.K + lane.s] = top > T(0) ? x * (T(1) / top) : x;
-  if (lane.s == 0) buf.node_scale[node] = UpdatedScale(buf.node_scale[node] + buf.branch_scale[branch], top);
+  AddLogVectors<T, G>(static_cast<T *>(buf.nodes) + node * d.K, buf.node_scale + node,
+                      static_cast<const T *>(buf.branches) + branch * d.K, buf.branch_scale[branch], lane.s, d.K);
 }
 
-// out = L diag(mid) R into the fresh slot (src/inference.cc:749-776)
+// Middle weights of a compression: exp(mid[m] + r_m - W) with r_m the right
+// factor's row scales folded in and W their maximum (returned; -inf when no
+// middle state is supported).
 template <typename T, int G>
-__global__ void CompressForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                                LinearBuffers buf) {
+__device__ inline double MiddleWeights(const LinearBuffers &buf, const Dims &d, const DeviceCompress &op,
+                                       std::uint32_t b, const T *mid, T *e) {
+  double top = kNegInf;
+  for (std::uint32_t m = 0; m < d.K; ++m) top = fmax(top, static_cast<double>(mid[m]) + RowScale(buf, d, op.right, b, m));
+#pragma unroll
+  for (int m = 0; m < G; ++m) {
+    if (m < static_cast<int>(d.K)) {
+      const double w = static_cast<double>(mid[m]) + RowScale(buf, d, op.right, b, m);
+      e[m] = top > kNegInf ? static_cast<T>(exp(w - top)) : T(0);
+    }
+  }
+  return top;
+}
+
+// Row p of out = L diag(exp(mid)) R, each row max-normalised with its own
+// FP64 log scale (src/inference.cc:749-776).
+template <typename T, int G>
+__global__ void CompressForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
-  if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
+  if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
   const DeviceCompress op = ops.compresses[offset + lane.item / d.B];
-  const T *L = Slot<T>(buf, d, op.left, b);
+  const std::uint64_t p = lane.s;
+  const T *L = Slot<T>(buf, d, op.left, b) + p * d.K;
   const T *R = Slot<T>(buf, d, op.right, b);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-  const T *mid = static_cast<const T *>(buf.nodes) + mid_at * d.K;
+  T e[G];
+  const double w_top = MiddleWeights<T, G>(buf, d, op, b, static_cast<const T *>(buf.nodes) + mid_at * d.K, e);
   T acc[G];
 #pragma unroll
   for (int c = 0; c < G; ++c) acc[c] = T(0);
+  for (std::uint32_t m = 0; m < d.K; ++m) {
+    const T lm = L[m] * e[m];
+    const T *row = R + static_cast<std::uint64_t>(m) * d.K;
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < static_cast<int>(d.K)) acc[c] += lm * row[c];
+  }
   T top = T(0);
-  if (lane.s < d.K) {
-    for (std::uint32_t m = 0; m < d.K; ++m) {
-      const T lm = L[static_cast<std::uint64_t>(lane.s) * d.K + m] * mid[m];
-      const T *row = R + static_cast<std::uint64_t>(m) * d.K;
 #pragma unroll
-      for (int c = 0; c < G; ++c)
-        if (c < static_cast<int>(d.K)) acc[c] += lm * row[c];
-    }
-#pragma unroll
-    for (int c = 0; c < G; ++c)
-      if (c < static_cast<int>(d.K)) top = Max(top, acc[c]);
-  }
-  top = GroupMax<G>(top);
+  for (int c = 0; c < G; ++c)
+    if (c < static_cast<int>(d.K)) top = Max(top, acc[c]);
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
-  if (lane.s < d.K) {
-    T *out = static_cast<T *>(buf.paths) + out_at * d.K * d.K + static_cast<std::uint64_t>(lane.s) * d.K;
-    const T inverse = top > T(0) ? T(1) / top : T(1);
+  T *out = static_cast<T *>(buf.paths) + out_at * d.K * d.K + p * d.K;
+  const bool live = top > T(0) && w_top > kNegInf;
+  const T inverse = live ? T(1) / top : T(0);
 #pragma unroll
-    for (int c = 0; c < G; ++c)
-      if (c < static_cast<int>(d.K)) out[c] = acc[c] * inverse;
-  }
-  if (lane.s == 0)
-    buf.path_scale[out_at] = UpdatedScale(
-        SlotScale(buf, d, op.left, b) + buf.node_scale[mid_at] + SlotScale(buf, d, op.right, b), top);
+  for (int c = 0; c < G; ++c)
+    if (c < static_cast<int>(d.K)) out[c] = acc[c] * inverse;
+  buf.path_scale[out_at * d.K + p] =
+      live ? RowScale(buf, d, op.left, b, lane.s) + buf.node_scale[mid_at] + w_top + log(static_cast<double>(top))
+           : kNegInf;
 }
 
-// log Z = scale(root) + log sum(root), summed in FP64
-// (src/inference.cc:778-784).
+// log Z = scale(root) + log sum exp(root), in FP64 (src/inference.cc:778-784).
 template <typename T>
 __global__ void Finish(DeviceOps ops, Dims d, LinearBuffers buf, double *log_z) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -199,7 +221,7 @@ __global__ void Finish(DeviceOps ops, Dims d, LinearBuffers buf, double *log_z) 
   const std::uint64_t at = static_cast<std::uint64_t>(ops.roots[r]) * d.B + b;
   const T *x = static_cast<const T *>(buf.nodes) + at * d.K;
   double sum = 0.0;
-  for (std::uint32_t s = 0; s < d.K; ++s) sum += static_cast<double>(x[s]);
+  for (std::uint32_t s = 0; s < d.K; ++s) sum += exp(static_cast<double>(x[s]));
   log_z[static_cast<std::uint64_t>(b) * d.roots + r] = sum > 0.0 ? buf.node_scale[at] + log(sum) : kNegInf;
 }
 
@@ -207,7 +229,7 @@ __global__ void Finish(DeviceOps ops, Dims d, LinearBuffers buf, double *log_z) 
 // Reverse (adjoint)
 // ------------------------------------------------------------------------
 
-// root adjoint = root marginal (src/inference.cc:338-343), times grad.
+// Root adjoint = root marginal (src/inference.cc:338-343), times grad.
 template <typename T>
 __global__ void SeedAdjoint(DeviceOps ops, Dims d, LinearBuffers buf, const double *grad) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -218,17 +240,17 @@ __global__ void SeedAdjoint(DeviceOps ops, Dims d, LinearBuffers buf, const doub
   const T *x = static_cast<const T *>(buf.nodes) + at * d.K;
   T *adj = static_cast<T *>(buf.node_adj) + at * d.K;
   double sum = 0.0;
-  for (std::uint32_t s = 0; s < d.K; ++s) sum += static_cast<double>(x[s]);
+  for (std::uint32_t s = 0; s < d.K; ++s) sum += exp(static_cast<double>(x[s]));
   const double g = grad ? grad[static_cast<std::uint64_t>(b) * d.roots + r] : 1.0;
   const double factor = sum > 0.0 ? g / sum : 0.0;
-  for (std::uint32_t s = 0; s < d.K; ++s) adj[s] = static_cast<T>(static_cast<double>(x[s]) * factor);
+  for (std::uint32_t s = 0; s < d.K; ++s) adj[s] = static_cast<T>(exp(static_cast<double>(x[s])) * factor);
 }
 
-// Rake adjoint (src/inference.cc:488-518): with m = P v,
-// adj_P[p][c] = adj_m[p] P[p][c] v[c] / m[p], adj_v[c] = sum_p adj_P[p][c].
+// Rake adjoint (src/inference.cc:488-518): with t[p][c] = P[p][c] exp(v[c])
+// and m[p] = sum_c t[p][c], adj_P[p][c] = adj_m[p] t[p][c] / m[p] and
+// adj_v[c] = sum_p adj_P[p][c]; the ratio adj_m / m is taken in FP64.
 template <typename T, int G>
-__global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                            LinearBuffers buf) {
+__global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
@@ -236,28 +258,28 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
   const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
   const T *v = static_cast<const T *>(buf.nodes) + leaf_at * d.K;
   const T *P = Slot<T>(buf, d, op.slot, b);
-  T g = T(0);
+  double g = 0.0;
   if (lane.s < d.K) {
     const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
-    T m = T(0);
-    for (std::uint32_t c = 0; c < d.K; ++c) m += row[c] * v[c];
+    T acc = T(0);
+    for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
     const T a = static_cast<const T *>(buf.branch_adj)[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
-    g = (a != T(0) && m > T(0)) ? a / m : T(0);
+    g = (a != T(0) && acc > T(0)) ? static_cast<double>(a) / static_cast<double>(acc) : 0.0;
     T *out = SlotAdjoint<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
-    for (std::uint32_t c = 0; c < d.K; ++c) out[c] = g * row[c] * v[c];
+    for (std::uint32_t c = 0; c < d.K; ++c) out[c] = static_cast<T>(g * static_cast<double>(row[c] * Exp(v[c])));
   }
-  T acc = T(0);
+  double column = 0.0;
+  const T e_s = lane.s < d.K ? Exp(v[lane.s]) : T(0);
   for (std::uint32_t p = 0; p < d.K; ++p) {
-    const T gp = __shfl(g, static_cast<int>(p), G);
-    if (lane.s < d.K) acc += gp * P[static_cast<std::uint64_t>(p) * d.K + lane.s];
+    const double gp = __shfl(g, static_cast<int>(p), G);
+    if (lane.s < d.K) column += gp * static_cast<double>(P[static_cast<std::uint64_t>(p) * d.K + lane.s] * e_s);
   }
-  if (lane.s < d.K) static_cast<T *>(buf.node_adj)[leaf_at * d.K + lane.s] = acc * v[lane.s];
+  if (lane.s < d.K) static_cast<T *>(buf.node_adj)[leaf_at * d.K + lane.s] = static_cast<T>(column);
 }
 
 // adj(branch) = adj(parent) (src/inference.cc:466-475)
 template <typename T, int G>
-__global__ void AbsorbReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                              LinearBuffers buf) {
+__global__ void AbsorbReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
@@ -268,8 +290,7 @@ __global__ void AbsorbReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::
 
 // adj(source) = adj(destination) (src/inference.cc:477-486)
 template <typename T, int G>
-__global__ void CombineReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                               LinearBuffers buf) {
+__global__ void CombineReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
@@ -279,88 +300,87 @@ __global__ void CombineReverse(DeviceOps ops, Dims d, std::uint32_t offset, std:
       adj[(static_cast<std::uint64_t>(op.destination) * d.B + b) * d.K + lane.s];
 }
 
-// Compression adjoint (src/inference.cc:416-464). With o = L diag(mid) R and
-// Gr = adj_o / o (0 where adj_o or o is 0):
-//   adj_L = L diag(mid) .* (Gr R^T),  adj_R = diag(mid) R .* (L^T Gr),
+// Compression adjoint (src/inference.cc:416-464). With e the middle weights,
+// S = L diag(e) R (the forward product before row normalisation) and
+// Gr = adj_out / S (FP64; 0 where adj_out or S is 0):
+//   adj_L = L diag(e) .* (Gr R^T),  adj_R = diag(e) R .* (L^T Gr),
 //   adj_mid[m] = sum_c adj_R[m][c].
-// Lane p forms row p of o, Gr and adj_L; Gr goes through LDS so that lane m
+// Lane p forms row p of S, Gr and adj_L; Gr goes through LDS so that lane m
 // can form row m of L^T Gr.
 template <typename T, int G>
-__global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                                LinearBuffers buf) {
-  extern __shared__ unsigned char shared_bytes[];
-  T *gr = reinterpret_cast<T *>(shared_bytes) + (threadIdx.x / G) * G * G;
+__global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
+  extern __shared__ double shared_gr[];
+  double *gr = shared_gr + (threadIdx.x / G) * G * G;
   const Lane lane = LaneOf<G>();
-  const bool active = lane.item < static_cast<std::uint64_t>(count) * d.B && lane.s < d.K;
+  const bool in_range = lane.item < static_cast<std::uint64_t>(count) * d.B;
+  const bool active = in_range && lane.s < d.K;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
   DeviceCompress op{};
   const T *L = nullptr;
   const T *R = nullptr;
-  const T *mid = nullptr;
   std::uint64_t mid_at = 0;
-  if (lane.item < static_cast<std::uint64_t>(count) * d.B) {
+  T e[G];
+  if (in_range) {
     op = ops.compresses[offset + lane.item / d.B];
     L = Slot<T>(buf, d, op.left, b);
     R = Slot<T>(buf, d, op.right, b);
     mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-    mid = static_cast<const T *>(buf.nodes) + mid_at * d.K;
   }
   if (active) {
+    MiddleWeights<T, G>(buf, d, op, b, static_cast<const T *>(buf.nodes) + mid_at * d.K, e);
     const std::uint64_t p = lane.s;
     T acc[G];
 #pragma unroll
     for (int c = 0; c < G; ++c) acc[c] = T(0);
     for (std::uint32_t m = 0; m < d.K; ++m) {
-      const T lm = L[p * d.K + m] * mid[m];
+      const T lm = L[p * d.K + m] * e[m];
       const T *row = R + static_cast<std::uint64_t>(m) * d.K;
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < static_cast<int>(d.K)) acc[c] += lm * row[c];
     }
-    const T *adj_out = static_cast<const T *>(buf.path_adj) +
-                       (static_cast<std::uint64_t>(op.out - d.E) * d.B + b) * d.K * d.K + p * d.K;
+    const T *adj_out =
+        static_cast<const T *>(buf.path_adj) + (static_cast<std::uint64_t>(op.out - d.E) * d.B + b) * d.K * d.K + p * d.K;
 #pragma unroll
     for (int c = 0; c < G; ++c) {
       if (c < static_cast<int>(d.K)) {
         const T a = adj_out[c];
-        acc[c] = (a != T(0) && acc[c] > T(0)) ? a / acc[c] : T(0);
-        gr[p * G + c] = acc[c];
+        gr[p * G + c] = (a != T(0) && acc[c] > T(0)) ? static_cast<double>(a) / static_cast<double>(acc[c]) : 0.0;
       }
     }
     T *adj_left = SlotAdjoint<T>(buf, d, op.left, b) + p * d.K;
     for (std::uint32_t m = 0; m < d.K; ++m) {
       const T *row = R + static_cast<std::uint64_t>(m) * d.K;
-      T h = T(0);
-#pragma unroll
-      for (int c = 0; c < G; ++c)
-        if (c < static_cast<int>(d.K)) h += acc[c] * row[c];
-      adj_left[m] = L[p * d.K + m] * mid[m] * h;
+      double h = 0.0;
+      for (std::uint32_t c = 0; c < d.K; ++c) h += gr[p * G + c] * static_cast<double>(row[c]);
+      adj_left[m] = static_cast<T>(static_cast<double>(L[p * d.K + m] * e[m]) * h);
     }
   }
   __syncthreads();
   if (active) {
     const std::uint64_t m = lane.s;
-    T r[G];
+    double r[G];
 #pragma unroll
-    for (int c = 0; c < G; ++c) r[c] = T(0);
+    for (int c = 0; c < G; ++c) r[c] = 0.0;
     for (std::uint32_t p = 0; p < d.K; ++p) {
-      const T lp = L[static_cast<std::uint64_t>(p) * d.K + m];
+      const double lp = static_cast<double>(L[static_cast<std::uint64_t>(p) * d.K + m]);
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < static_cast<int>(d.K)) r[c] += lp * gr[p * G + c];
     }
     T *adj_right = SlotAdjoint<T>(buf, d, op.right, b) + m * d.K;
     const T *row = R + m * d.K;
-    T total = T(0);
+    const double em = static_cast<double>(e[m]);
+    double total = 0.0;
 #pragma unroll
     for (int c = 0; c < G; ++c) {
       if (c < static_cast<int>(d.K)) {
-        const T value = row[c] * mid[m] * r[c];
-        adj_right[c] = value;
+        const double value = static_cast<double>(row[c]) * em * r[c];
+        adj_right[c] = static_cast<T>(value);
         total += value;
       }
     }
-    static_cast<T *>(buf.node_adj)[mid_at * d.K + m] = total;
+    static_cast<T *>(buf.node_adj)[mid_at * d.K + m] = static_cast<T>(total);
   }
 }
 
@@ -368,13 +388,16 @@ __global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std
 // Staging
 // ------------------------------------------------------------------------
 
-// in [X][Y][K] -> out [Y][X][K] through a 64 x 64-element LDS tile.
-// The grid is one-dimensional: block i covers x-tile i % x_tiles and y-tile
+// in [X][Y][K] -> out [Y][X][K] through a 64 x 64-element LDS tile. The grid
+// is one-dimensional: block i covers x-tile i % x_tiles and y-tile
 // i / x_tiles. Requires K <= 64.
-template <typename In, typename Out>
-__global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
-                                 std::uint32_t K, double *scale_zero, std::uint32_t x_tiles) {
-  __shared__ Out tile[64][65];
+//   kMode 0: plain copy (adjoints);
+//   kMode 1: linear potentials -> max-shifted logs + FP64 scale;
+//   kMode 2: log potentials -> max-shifted logs + FP64 scale.
+template <typename In, typename Out, int kMode>
+__global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::uint32_t Y, std::uint32_t K,
+                                 double *scale, std::uint32_t x_tiles) {
+  __shared__ double tile[64][65];
   const std::uint32_t ty = 64 / K;  // y-vectors per tile row
   const std::uint32_t row = ty * K; // elements per tile row
   const std::uint64_t x0 = static_cast<std::uint64_t>(blockIdx.x % x_tiles) * 64;
@@ -382,85 +405,86 @@ __global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::u
   for (std::uint32_t i = threadIdx.x; i < 64 * row; i += blockDim.x) {
     const std::uint32_t x = i / row, j = i % row;
     const std::uint64_t gx = x0 + x;
-    if (gx < X && y0 * K + j < static_cast<std::uint64_t>(Y) * K)
-      tile[x][j] = static_cast<Out>(in[(gx * Y + y0) * K + j]);
+    if (gx < X && y0 * K + j < static_cast<std::uint64_t>(Y) * K) {
+      const double value = static_cast<double>(in[(gx * Y + y0) * K + j]);
+      tile[x][j] = kMode == 1 ? (value > 0.0 ? log(value) : kNegInf) : value;
+    }
   }
   __syncthreads();
-  const std::uint32_t xk = 64 * K;
-  for (std::uint32_t i = threadIdx.x; i < ty * xk; i += blockDim.x) {
-    const std::uint32_t y = i / xk, r = i % xk;
-    const std::uint32_t x = r / K, k = r % K;
-    const std::uint64_t gy = y0 + y, gx = x0 + x;
-    if (gy < Y && gx < X) out[(gy * X + gx) * K + k] = tile[x][y * K + k];
-  }
-  if (scale_zero != nullptr) {
+  if (kMode == 0) {
+    const std::uint32_t xk = 64 * K;
+    for (std::uint32_t i = threadIdx.x; i < ty * xk; i += blockDim.x) {
+      const std::uint32_t y = i / xk, r = i % xk;
+      const std::uint32_t x = r / K, k = r % K;
+      const std::uint64_t gy = y0 + y, gx = x0 + x;
+      if (gy < Y && gx < X) out[(gy * X + gx) * K + k] = static_cast<Out>(tile[x][y * K + k]);
+    }
+  } else {
+    // one thread per output vector
     for (std::uint32_t i = threadIdx.x; i < 64 * ty; i += blockDim.x) {
-      const std::uint64_t gy = y0 + i / 64, gx = x0 + i % 64;
-      if (gy < Y && gx < X) scale_zero[gy * X + gx] = 0.0;
+      const std::uint32_t y = i / 64, x = i % 64;
+      const std::uint64_t gy = y0 + y, gx = x0 + x;
+      if (gy >= Y || gx >= X) continue;
+      double top = kNegInf;
+      for (std::uint32_t k = 0; k < K; ++k) top = fmax(top, tile[x][y * K + k]);
+      const bool live = top > kNegInf;
+      Out *o = out + (gy * X + gx) * K;
+      for (std::uint32_t k = 0; k < K; ++k) o[k] = live ? static_cast<Out>(tile[x][y * K + k] - top) : NegInf<Out>();
+      scale[gy * X + gx] = top;
     }
   }
 }
 
-template <typename In, typename Out>
-__global__ void Convert(const In *in, Out *out, std::uint64_t count) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid < count) out[tid] = static_cast<Out>(in[tid]);
-}
-
-// log potentials [B][N][K] -> normalised linear [N][B][K] + FP64 scales.
-template <typename T>
-__global__ void InitNodesFromLogs(const T *in, T *nodes, double *scale, Dims d) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<std::uint64_t>(d.N) * d.B) return;
-  const std::uint32_t n = static_cast<std::uint32_t>(tid / d.B);
-  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
-  const T *x = in + (static_cast<std::uint64_t>(b) * d.N + n) * d.K;
-  T top = static_cast<T>(kNegInf);
-  for (std::uint32_t s = 0; s < d.K; ++s) top = Max(top, x[s]);
-  T *out = nodes + tid * d.K;
-  const bool finite = top > static_cast<T>(kNegInf);
-  for (std::uint32_t s = 0; s < d.K; ++s) out[s] = finite ? static_cast<T>(exp(x[s] - top)) : T(0);
-  scale[tid] = finite ? static_cast<double>(top) : kNegInf;
-}
-
+// Log edge potentials [E][K][K] -> row-max-shifted exponentials + FP64 row
+// scales [E][K].
 template <typename T>
 __global__ void InitEdgesFromLogs(const T *in, T *edges, double *scale, Dims d) {
-  const std::uint64_t e = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= d.E) return;
-  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  const T *x = in + e * kk;
-  T top = static_cast<T>(kNegInf);
-  for (std::uint64_t i = 0; i < kk; ++i) top = Max(top, x[i]);
-  const bool finite = top > static_cast<T>(kNegInf);
-  for (std::uint64_t i = 0; i < kk; ++i) edges[e * kk + i] = finite ? static_cast<T>(exp(x[i] - top)) : T(0);
-  scale[e] = finite ? static_cast<double>(top) : kNegInf;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.E) * d.K) return;
+  const T *x = in + tid * d.K;
+  T top = NegInf<T>();
+  for (std::uint32_t c = 0; c < d.K; ++c) top = Max(top, x[c]);
+  const bool live = top > NegInf<T>();
+  for (std::uint32_t c = 0; c < d.K; ++c) edges[tid * d.K + c] = live ? static_cast<T>(exp(static_cast<double>(x[c] - top))) : T(0);
+  scale[tid] = live ? static_cast<double>(top) : kNegInf;
 }
 
 // Categorical node factors (src/gpu_backend_impl.inc:326-382 semantics):
 // (root ? root[s] : 1) * (observed ? (cat < C ? emission[cat][s] : 0) : 1),
-// with the unobserved case not multiplied at all.
-template <typename T>
-__global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *obs_index,
-                                std::uint32_t obs_count, std::uint32_t categories,
-                                const std::uint8_t *is_root, const T *root, const T *emissions,
-                                T *nodes, double *scale, Dims d) {
+// the product formed in the Scalar type P so that it is bitwise equal to the
+// dense construction; stored raw (kLog false) or as max-shifted logs.
+template <typename P, typename Out, bool kLog>
+__global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *obs_index, std::uint32_t obs_count,
+                                std::uint32_t categories, const std::uint8_t *is_root, const P *root,
+                                const P *emissions, Out *nodes, double *scale, Dims d) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= static_cast<std::uint64_t>(d.N) * d.B) return;
   const std::uint32_t n = static_cast<std::uint32_t>(tid / d.B);
   const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
-  T *out = nodes + tid * d.K;
+  Out *out = nodes + tid * d.K;
   const bool rooted = is_root[n] != 0;
   const std::uint32_t o = obs_index[n];
-  if (o == 0xFFFFFFFFu) {
-    for (std::uint32_t s = 0; s < d.K; ++s) out[s] = rooted ? root[s] : T(1);
-  } else {
-    const std::uint32_t category = obs[static_cast<std::uint64_t>(b) * obs_count + o];
-    for (std::uint32_t s = 0; s < d.K; ++s) {
-      const T e = category < categories ? emissions[static_cast<std::uint64_t>(category) * d.K + s] : T(0);
-      out[s] = (rooted ? root[s] : T(1)) * e;
-    }
+  const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : obs[static_cast<std::uint64_t>(b) * obs_count + o];
+  auto value = [&](std::uint32_t s) -> P {
+    if (o == 0xFFFFFFFFu) return rooted ? root[s] : P(1);
+    const P e = category < categories ? emissions[static_cast<std::uint64_t>(category) * d.K + s] : P(0);
+    return (rooted ? root[s] : P(1)) * e;
+  };
+  if (!kLog) {
+    for (std::uint32_t s = 0; s < d.K; ++s) out[s] = static_cast<Out>(value(s));
+    return;
   }
-  scale[tid] = 0.0;
+  double top = kNegInf;
+  for (std::uint32_t s = 0; s < d.K; ++s) {
+    const double x = static_cast<double>(value(s));
+    top = fmax(top, x > 0.0 ? log(x) : kNegInf);
+  }
+  const bool live = top > kNegInf;
+  for (std::uint32_t s = 0; s < d.K; ++s) {
+    const double x = static_cast<double>(value(s));
+    out[s] = live && x > 0.0 ? static_cast<Out>(log(x) - top) : NegInf<Out>();
+  }
+  scale[tid] = top;
 }
 
 template <typename T>
@@ -474,27 +498,28 @@ __global__ void ReduceEdges(const T *edge_out, T *out, Dims d) {
   out[tid] = static_cast<T>(sum);
 }
 
-template <typename T>
-__global__ void CastF64(const double *in, T *out, std::uint64_t count) {
+template <typename In, typename Out>
+__global__ void Convert(const In *in, Out *out, std::uint64_t count) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid < count) out[tid] = static_cast<T>(in[tid]);
+  if (tid < count) out[tid] = static_cast<Out>(in[tid]);
 }
 
 // ------------------------------------------------------------------------
 // Max-product (FP64 log domain)
 // ------------------------------------------------------------------------
 
+// Logs of public [B][N][K] node potentials into internal [N][B][K], and of
+// the edges (src/inference.cc:95-98).
 template <typename In>
 __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
   const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
   if (tid < node_values) {
-    // internal [N][B][K] from public [B][N][K]
     const std::uint64_t s = tid % d.K, nb = tid / d.K;
     const std::uint64_t b = nb % d.B, n = nb / d.B;
     const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
-    nodes[tid] = x > 0.0 ? log(x) : kNegInf;  // src/inference.cc:95-98
+    nodes[tid] = x > 0.0 ? log(x) : kNegInf;
   } else if (tid < node_values + edge_values) {
     const std::uint64_t i = tid - node_values;
     const double x = static_cast<double>(edges_in[i]);
@@ -502,8 +527,7 @@ __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, d
   }
 }
 
-__device__ inline const double *MaxSlot(const MaxBuffers &buf, const Dims &d, std::uint32_t slot,
-                                        std::uint32_t b) {
+__device__ inline const double *MaxSlot(const MaxBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
   return slot < d.E ? buf.edges + slot * kk : buf.paths + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
 }
@@ -600,8 +624,8 @@ __global__ void MaxFinish(DeviceOps ops, Dims d, MaxBuffers buf, double *log_wei
 }
 
 // src/inference.cc:651-667
-__global__ void MaxExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                               MaxBuffers buf, std::uint32_t *assignment) {
+__global__ void MaxExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf,
+                               std::uint32_t *assignment) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
@@ -628,8 +652,7 @@ __global__ void MaxExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offse
 // Ascending inverse CDF over w / sum(w), skipping zero weights, falling back
 // to the last supported state (src/inference.cc:128-146). Returns K when no
 // state is supported.
-template <class W>
-__device__ inline std::uint32_t InverseCdf(std::uint32_t K, W weight, double u) {
+template <class W> __device__ inline std::uint32_t InverseCdf(std::uint32_t K, W weight, double u) {
   double total = 0.0;
   for (std::uint32_t s = 0; s < K; ++s) total += weight(s);
   double cumulative = 0.0;
@@ -645,8 +668,8 @@ __device__ inline std::uint32_t InverseCdf(std::uint32_t K, W weight, double u) 
 }
 
 template <typename U>
-__global__ void SampleSeed(DeviceOps ops, Dims d, LinearBuffers buf, const U *uniforms,
-                           std::uint32_t *assignment, std::uint32_t *failure) {
+__global__ void SampleSeed(DeviceOps ops, Dims d, LinearBuffers buf, const U *uniforms, std::uint32_t *assignment,
+                           std::uint32_t *failure) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= static_cast<std::uint64_t>(d.roots) * d.B) return;
   const std::uint32_t r = static_cast<std::uint32_t>(tid / d.B);
@@ -654,19 +677,15 @@ __global__ void SampleSeed(DeviceOps ops, Dims d, LinearBuffers buf, const U *un
   const std::uint32_t root = ops.roots[r];
   const double *x = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(root) * d.B + b) * d.K;
   const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + root;
-  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return x[i]; }, static_cast<double>(uniforms[at]));
-  if (s >= d.K) {
-    *failure = 1;
-    assignment[at] = 0;
-  } else {
-    assignment[at] = s;
-  }
+  const std::uint32_t s =
+      InverseCdf(d.K, [&](std::uint32_t i) { return exp(x[i]); }, static_cast<double>(uniforms[at]));
+  if (s >= d.K) *failure = 1;
+  assignment[at] = s >= d.K ? 0 : s;
 }
 
 template <typename U>
-__global__ void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                                  LinearBuffers buf, const U *uniforms, std::uint32_t *assignment,
-                                  std::uint32_t *failure) {
+__global__ void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf,
+                                  const U *uniforms, std::uint32_t *assignment, std::uint32_t *failure) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
@@ -674,14 +693,10 @@ __global__ void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, s
   std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
   const double *row = Slot<double>(buf, d, op.slot, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
   const double *v = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
-  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t c) { return row[c] * v[c]; },
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t c) { return row[c] * exp(v[c]); },
                                      static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.leaf]));
-  if (s >= d.K) {
-    *failure = 1;
-    a[op.leaf] = 0;
-  } else {
-    a[op.leaf] = s;
-  }
+  if (s >= d.K) *failure = 1;
+  a[op.leaf] = s >= d.K ? 0 : s;
 }
 
 template <typename U>
@@ -697,15 +712,17 @@ __global__ void SampleExpandCompressions(DeviceOps ops, Dims d, std::uint32_t of
   const double *L = Slot<double>(buf, d, op.left, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
   const double *R = Slot<double>(buf, d, op.right, b);
   const double *mid = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.middle) * d.B + b) * d.K;
+  double top = kNegInf;
+  for (std::uint32_t m = 0; m < d.K; ++m) top = fmax(top, mid[m] + RowScale(buf, d, op.right, b, m));
   const std::uint32_t s = InverseCdf(
-      d.K, [&](std::uint32_t m) { return L[m] * mid[m] * R[static_cast<std::uint64_t>(m) * d.K + cs]; },
+      d.K,
+      [&](std::uint32_t m) {
+        if (!(top > kNegInf)) return 0.0;
+        return L[m] * exp(mid[m] + RowScale(buf, d, op.right, b, m) - top) * R[static_cast<std::uint64_t>(m) * d.K + cs];
+      },
       static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.middle]));
-  if (s >= d.K) {
-    *failure = 1;
-    a[op.middle] = 0;
-  } else {
-    a[op.middle] = s;
-  }
+  if (s >= d.K) *failure = 1;
+  a[op.middle] = s >= d.K ? 0 : s;
 }
 
 // ------------------------------------------------------------------------
@@ -746,7 +763,7 @@ void ReverseLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
     case 1: CombineReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 2: AbsorbReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 3: {
-      const std::size_t shared = static_cast<std::size_t>(block / G) * G * G * sizeof(T);
+      const std::size_t shared = static_cast<std::size_t>(block / G) * G * G * sizeof(double);
       CompressReverse<T, G><<<grid, block, shared, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
       break;
     }
@@ -770,8 +787,8 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
 
 // Dispatches a runtime group width onto the template instantiations: runs
 // CALL with the constant G bound.
-#define THMM_WITH_GROUP(K, CALL)                  \
-  switch (GroupWidth(K)) {                        \
+#define THMM_WITH_GROUP(K, CALL)                    \
+  switch (GroupWidth(K)) {                          \
     case 1: { constexpr int G = 1; CALL; } break;   \
     case 2: { constexpr int G = 2; CALL; } break;   \
     case 4: { constexpr int G = 4; CALL; } break;   \
@@ -780,6 +797,16 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
     case 32: { constexpr int G = 32; CALL; } break; \
     default: { constexpr int G = 64; CALL; } break; \
   }
+
+template <typename In, typename Out, int kMode>
+void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
+  const std::uint32_t ty = 64 / l.dims.K;
+  const std::uint32_t x_tiles = (X + 63) / 64;
+  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty));
+  if (grid == 0) return;
+  TransposeVectors<In, Out, kMode><<<grid, kThreads, 0, StreamOf(l)>>>(
+      static_cast<const In *>(in), static_cast<Out *>(out), X, Y, l.dims.K, scale, x_tiles);
+}
 
 }  // namespace
 
@@ -798,65 +825,73 @@ const char *LastLaunchError() {
 
 void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes, double *node_scale) {
   const Dims &d = l.dims;
-  const std::uint32_t ty = 64 / d.K;
-  const std::uint32_t x_tiles = (d.B + 63) / 64;
-  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((d.N + ty - 1) / ty));
   if (l.real == Real::kF32) {
     if (in_real == Real::kF32)
-      TransposeVectors<float, float><<<grid, kThreads, 0, StreamOf(l)>>>(
-          static_cast<const float *>(input), static_cast<float *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+      LaunchTranspose<float, float, 1>(l, input, nodes, d.B, d.N, node_scale);
     else
-      TransposeVectors<double, float><<<grid, kThreads, 0, StreamOf(l)>>>(
-          static_cast<const double *>(input), static_cast<float *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+      LaunchTranspose<double, float, 1>(l, input, nodes, d.B, d.N, node_scale);
   } else {
     if (in_real == Real::kF32)
-      TransposeVectors<float, double><<<grid, kThreads, 0, StreamOf(l)>>>(
-          static_cast<const float *>(input), static_cast<double *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+      LaunchTranspose<float, double, 1>(l, input, nodes, d.B, d.N, node_scale);
     else
-      TransposeVectors<double, double><<<grid, kThreads, 0, StreamOf(l)>>>(
-          static_cast<const double *>(input), static_cast<double *>(nodes), d.B, d.N, d.K, node_scale, x_tiles);
+      LaunchTranspose<double, double, 1>(l, input, nodes, d.B, d.N, node_scale);
   }
   Check(hipGetLastError(), "node initialisation launch");
 }
 
 void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes, double *node_scale) {
-  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.N) * l.dims.B, kThreads);
+  const Dims &d = l.dims;
   if (l.real == Real::kF32)
-    InitNodesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const float *>(log_input), static_cast<float *>(nodes), node_scale, l.dims);
+    LaunchTranspose<float, float, 2>(l, log_input, nodes, d.B, d.N, node_scale);
   else
-    InitNodesFromLogs<double><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const double *>(log_input), static_cast<double *>(nodes), node_scale, l.dims);
+    LaunchTranspose<double, double, 2>(l, log_input, nodes, d.B, d.N, node_scale);
   Check(hipGetLastError(), "log node initialisation launch");
 }
 
 void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges, double *edge_scale) {
   if (l.dims.E == 0) return;
-  const unsigned grid = BlocksFor(l.dims.E, kThreads);
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.E) * l.dims.K, kThreads);
   if (l.real == Real::kF32)
-    InitEdgesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const float *>(log_input), static_cast<float *>(edges), edge_scale, l.dims);
+    InitEdgesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(log_input),
+                                                                static_cast<float *>(edges), edge_scale, l.dims);
   else
-    InitEdgesFromLogs<double><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const double *>(log_input), static_cast<double *>(edges), edge_scale, l.dims);
+    InitEdgesFromLogs<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(log_input),
+                                                                 static_cast<double *>(edges), edge_scale, l.dims);
   Check(hipGetLastError(), "log edge initialisation launch");
 }
 
-void LaunchInitCategorical(const Launch &l, const std::uint8_t *observations,
+void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const std::uint8_t *observations,
                            const std::uint32_t *observation_index_by_node, std::uint32_t observation_count,
                            std::uint32_t categories, const std::uint8_t *is_root, const void *root_potential,
                            const void *emissions, void *nodes, double *node_scale) {
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.N) * l.dims.B, kThreads);
-  if (l.real == Real::kF32)
-    InitCategorical<float><<<grid, kThreads, 0, StreamOf(l)>>>(
-        observations, observation_index_by_node, observation_count, categories, is_root,
-        static_cast<const float *>(root_potential), static_cast<const float *>(emissions),
-        static_cast<float *>(nodes), node_scale, l.dims);
-  else
-    InitCategorical<double><<<grid, kThreads, 0, StreamOf(l)>>>(
-        observations, observation_index_by_node, observation_count, categories, is_root,
-        static_cast<const double *>(root_potential), static_cast<const double *>(emissions),
-        static_cast<double *>(nodes), node_scale, l.dims);
+  const auto *obs = observations;
+  const auto *index = observation_index_by_node;
+  if (product_real == Real::kF32) {
+    const auto *root = static_cast<const float *>(root_potential);
+    const auto *em = static_cast<const float *>(emissions);
+    if (raw)
+      InitCategorical<float, float, false><<<grid, kThreads, 0, StreamOf(l)>>>(
+          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims);
+    else if (l.real == Real::kF32)
+      InitCategorical<float, float, true><<<grid, kThreads, 0, StreamOf(l)>>>(
+          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims);
+    else
+      InitCategorical<float, double, true><<<grid, kThreads, 0, StreamOf(l)>>>(
+          obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
+          l.dims);
+  } else {
+    const auto *root = static_cast<const double *>(root_potential);
+    const auto *em = static_cast<const double *>(emissions);
+    if (raw)
+      InitCategorical<double, double, false><<<grid, kThreads, 0, StreamOf(l)>>>(
+          obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
+          l.dims);
+    else
+      InitCategorical<double, double, true><<<grid, kThreads, 0, StreamOf(l)>>>(
+          obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
+          l.dims);
+  }
   Check(hipGetLastError(), "categorical initialisation launch");
 }
 
@@ -900,15 +935,10 @@ void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
 
 void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out) {
   const Dims &d = l.dims;
-  const std::uint32_t ty = 64 / d.K;
-  const std::uint32_t x_tiles = (d.N + 63) / 64;
-  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((d.B + ty - 1) / ty));
   if (l.real == Real::kF32)
-    TransposeVectors<float, float><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const float *>(node_adj), static_cast<float *>(out), d.N, d.B, d.K, nullptr, x_tiles);
+    LaunchTranspose<float, float, 0>(l, node_adj, out, d.N, d.B, nullptr);
   else
-    TransposeVectors<double, double><<<grid, kThreads, 0, StreamOf(l)>>>(
-        static_cast<const double *>(node_adj), static_cast<double *>(out), d.N, d.B, d.K, nullptr, x_tiles);
+    LaunchTranspose<double, double, 0>(l, node_adj, out, d.N, d.B, nullptr);
   Check(hipGetLastError(), "node marginal transpose launch");
 }
 
@@ -935,9 +965,9 @@ void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t cou
   if (count == 0) return;
   const unsigned grid = BlocksFor(count, kThreads);
   if (l.real == Real::kF32)
-    CastF64<float><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<float *>(out), count);
+    Convert<double, float><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<float *>(out), count);
   else
-    CastF64<double><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<double *>(out), count);
+    Convert<double, double><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<double *>(out), count);
   Check(hipGetLastError(), "cast launch");
 }
 
@@ -985,20 +1015,21 @@ void LaunchSampleSeed(const Launch &l, const LinearBuffers &buf, Real in_real, c
     SampleSeed<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, static_cast<const float *>(uniforms),
                                                          assignment, failure);
   else
-    SampleSeed<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf,
-                                                          static_cast<const double *>(uniforms), assignment, failure);
+    SampleSeed<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, buf, static_cast<const double *>(uniforms),
+                                                          assignment, failure);
   Check(hipGetLastError(), "sample seed launch");
 }
 
 void LaunchSampleExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
-                             const LinearBuffers &buf, Real in_real, const void *uniforms,
-                             std::uint32_t *assignment, std::uint32_t *failure) {
+                             const LinearBuffers &buf, Real in_real, const void *uniforms, std::uint32_t *assignment,
+                             std::uint32_t *failure) {
   if (count == 0 || (primitive != 0 && primitive != 3)) return;
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, kThreads);
   if (in_real == Real::kF32) {
     const float *u = static_cast<const float *>(uniforms);
     if (primitive == 0)
-      SampleExpandRakes<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment, failure);
+      SampleExpandRakes<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment,
+                                                                   failure);
     else
       SampleExpandCompressions<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u,
                                                                           assignment, failure);

@@ -42,12 +42,14 @@ struct Dims {
   std::uint32_t roots;  // number of roots
 };
 
-// Scaled linear-domain state (element type float or double, erased).
-//   nodes      [N][B][K]        node_scale   [N][B]   (FP64 log scales)
-//   edges      [E][K][K]        edge_scale   [E]      (shared slots)
-//   paths      [C][B][K][K]     path_scale   [C][B]   (composed slots)
-//   branches   [R][B][K]        branch_scale [R][B]
-//   node_adj   [N][B][K]        branch_adj   [R][B][K]
+// Scaled contraction state (element type float or double, erased).
+//   nodes      [N][B][K] max-shifted logs     node_scale   [N][B]   FP64
+//   branches   [R][B][K] max-shifted logs     branch_scale [R][B]   FP64
+//   edges      [E][K][K] linear               edge_scale   [E][K]   FP64 row
+//              scales, or null for unscaled caller potentials
+//   paths      [C][B][K][K] linear, rows max-normalised
+//                                             path_scale   [C][B][K] FP64
+//   node_adj   [N][B][K]        branch_adj   [R][B][K]   (linear adjoints)
 //   path_adj   [C][B][K][K]     edge_out     [B][E][K][K] (edge marginals)
 struct LinearBuffers {
   void *nodes;
@@ -86,19 +88,25 @@ struct Launch {
 };
 
 // ---- input staging ----
-// public [B][N][K] -> internal [N][B][K]; node_scale = 0. `in_real` is the
-// element type of `input`; `out` has element type launch.real.
+// Linear potentials, public [B][N][K] -> internal [N][B][K] max-shifted
+// logs + FP64 scales. `in_real` is the element type of `input`; `nodes`
+// has element type launch.real.
 void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes,
                      double *node_scale);
-// Log potentials [B][N][K] -> max-shifted linear values + FP64 scales.
+// Log potentials [B][N][K] -> max-shifted logs + FP64 scales.
 void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes,
                              double *node_scale);
-// Log edge potentials [E][K][K] -> max-shifted linear values + scales.
+// Log edge potentials [E][K][K] -> row-max-shifted exponentials + FP64 row
+// scales [E][K].
 void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges,
                              double *edge_scale);
-// Categorical observations -> internal [N][B][K] nodes, bitwise equal to the
-// dense construction root-or-one times emission.
-void LaunchInitCategorical(const Launch &l, const std::uint8_t *observations,
+// Categorical observations -> internal [N][B][K] node factors, the product
+// root-or-one times emission formed in `product_real` so that it is bitwise
+// equal to the dense construction. `raw` stores the products as they are
+// (element type product_real); otherwise max-shifted logs of element type
+// launch.real plus FP64 scales.
+void LaunchInitCategorical(const Launch &l, Real product_real, bool raw,
+                           const std::uint8_t *observations,
                            const std::uint32_t *observation_index_by_node,
                            std::uint32_t observation_count, std::uint32_t categories,
                            const std::uint8_t *is_root, const void *root_potential,

@@ -115,7 +115,7 @@ struct Workspace::Impl {
   // adjoint
   DeviceBuffer node_adj, branch_adj, path_adj, node_out, edge_out;
   // FP64 scaled state for sampling when Scalar is float
-  DeviceBuffer s_edges, s_edge_scale, s_nodes, s_node_scale, s_paths, s_path_scale, s_branches, s_branch_scale;
+  DeviceBuffer s_edges, s_nodes, s_node_scale, s_paths, s_path_scale, s_branches, s_branch_scale;
   DeviceBuffer assignment, failure;
   // max-product
   DeviceBuffer m_nodes, m_edges, m_paths, m_branches, rake_choice, compress_choice, log_weight;
@@ -133,7 +133,7 @@ struct Workspace::Impl {
                             &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
                             &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
-                            &s_edge_scale, &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
+                            &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight})
       b->Release();
@@ -181,17 +181,18 @@ struct Workspace::Impl {
     return l;
   }
 
-  LinearBuffers Linear(const void *edge_values) const {
-    return {nodes.ptr, node_scale.as<double>(), edge_values, edge_scale.as<double>(), paths.ptr,
+  // `edge_row_scales` is null for the caller's linear potentials and the
+  // [E][K] row scales after a log-potential initialisation.
+  LinearBuffers Linear(const void *edge_values, const double *edge_row_scales) const {
+    return {nodes.ptr, node_scale.as<double>(), edge_values, edge_row_scales, paths.ptr,
             path_scale.as<double>(), branches.ptr, branch_scale.as<double>(), node_adj.ptr, branch_adj.ptr,
             path_adj.ptr, edge_out.ptr};
   }
 
   LinearBuffers SamplingLinear() const {
-    if (real == Real::kF64) return Linear(edges.ptr);
-    return {s_nodes.ptr, s_node_scale.as<double>(), s_edges.ptr, s_edge_scale.as<double>(), s_paths.ptr,
-            s_path_scale.as<double>(), s_branches.ptr, s_branch_scale.as<double>(), nullptr, nullptr, nullptr,
-            nullptr};
+    if (real == Real::kF64) return Linear(edges.ptr, nullptr);
+    return {s_nodes.ptr, s_node_scale.as<double>(), s_edges.ptr, nullptr, s_paths.ptr, s_path_scale.as<double>(),
+            s_branches.ptr, s_branch_scale.as<double>(), nullptr, nullptr, nullptr, nullptr};
   }
 
   MaxBuffers Max() const {
@@ -285,11 +286,11 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   const std::size_t branch_values = Product({R, batch, K}, "branch workspace");
 
   w.edges.Allocate(edge_values * S, "edge storage");
-  w.edge_scale.Allocate(E * sizeof(double), "edge scales");
+  w.edge_scale.Allocate(E * K * sizeof(double), "edge row scales");
   w.nodes.Allocate(node_values * S, "node storage");
   w.node_scale.Allocate(N * batch * sizeof(double), "node scales");
   w.paths.Allocate(path_values * S, "path storage");
-  w.path_scale.Allocate(C * batch * sizeof(double), "path scales");
+  w.path_scale.Allocate(C * batch * K * sizeof(double), "path row scales");
   w.branches.Allocate(branch_values * S, "branch storage");
   w.branch_scale.Allocate(R * batch * sizeof(double), "branch scales");
   w.log_z.Allocate(batch * roots * sizeof(double), "log partitions");
@@ -312,11 +313,10 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   if (flags & THMM_RESERVE_SAMPLING) {
     if (real == Real::kF32) {
       w.s_edges.Allocate(edge_values * sizeof(double), "sampling edges");
-      w.s_edge_scale.Allocate(E * sizeof(double), "sampling edge scales");
       w.s_nodes.Allocate(node_values * sizeof(double), "sampling nodes");
       w.s_node_scale.Allocate(N * batch * sizeof(double), "sampling node scales");
       w.s_paths.Allocate(path_values * sizeof(double), "sampling paths");
-      w.s_path_scale.Allocate(C * batch * sizeof(double), "sampling path scales");
+      w.s_path_scale.Allocate(C * batch * K * sizeof(double), "sampling path row scales");
       w.s_branches.Allocate(branch_values * sizeof(double), "sampling branches");
       w.s_branch_scale.Allocate(R * batch * sizeof(double), "sampling branch scales");
     }
@@ -397,18 +397,67 @@ void UploadDense(Impl &w, const thmm_dense_model &model) {
         "GPU runtime MemcpyAsync edge upload");
 }
 
-void ZeroEdgeScales(Impl &w, hipStream_t stream) {
-  if (w.E() != 0)
-    Hip(hipMemsetAsync(w.edge_scale.ptr, 0, w.E() * sizeof(double), stream), "GPU runtime MemsetAsync edge scales");
+
+}  // namespace
+
+// Event pairs around individual launches (measurement only).
+class Profiler {
+ public:
+  Profiler(hipStream_t stream, std::uint32_t batch) : stream_(stream), batch_(batch) {}
+  ~Profiler() {
+    for (Entry &e : entries_) {
+      static_cast<void>(hipEventDestroy(e.start));
+      static_cast<void>(hipEventDestroy(e.stop));
+    }
+  }
+  void Begin(std::uint32_t phase, std::uint32_t primitive, std::uint32_t count) {
+    Entry e{};
+    e.info = {phase, primitive, count, batch_, 0.0};
+    Hip(hipEventCreate(&e.start), "GPU runtime EventCreate");
+    Hip(hipEventCreate(&e.stop), "GPU runtime EventCreate");
+    Hip(hipEventRecord(e.start, stream_), "GPU runtime EventRecord");
+    entries_.push_back(e);
+  }
+  void End() { Hip(hipEventRecord(entries_.back().stop, stream_), "GPU runtime EventRecord"); }
+  std::size_t Report(thmm_launch_timing *out, std::size_t capacity) {
+    Hip(hipStreamSynchronize(stream_), "profiled execution");
+    for (std::size_t i = 0; i < entries_.size() && i < capacity; ++i) {
+      float ms = 0.f;
+      Hip(hipEventElapsedTime(&ms, entries_[i].start, entries_[i].stop), "GPU runtime EventElapsedTime");
+      out[i] = entries_[i].info;
+      out[i].ms = ms;
+    }
+    return entries_.size();
+  }
+
+ private:
+  struct Entry {
+    hipEvent_t start, stop;
+    thmm_launch_timing info;
+  };
+  hipStream_t stream_;
+  std::uint32_t batch_;
+  std::vector<Entry> entries_;
+};
+
+namespace {
+
+void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf,
+                   Profiler *profiler = nullptr) {
+  for (const Level &level : levels) {
+    if (profiler) profiler->Begin(1, level.primitive, level.count);
+    LaunchForwardLevel(l, level.primitive, level.offset, level.count, buf);
+    if (profiler) profiler->End();
+  }
 }
 
-void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf) {
-  for (const Level &level : levels) LaunchForwardLevel(l, level.primitive, level.offset, level.count, buf);
-}
-
-void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf) {
-  for (std::size_t i = levels.size(); i-- > 0;)
+void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf,
+                   Profiler *profiler = nullptr) {
+  for (std::size_t i = levels.size(); i-- > 0;) {
+    if (profiler) profiler->Begin(3, levels[i].primitive, levels[i].count);
     LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
+    if (profiler) profiler->End();
+  }
 }
 
 // Categorical staging with residency policies (src/gpu_backend_impl.inc:2841-2947).
@@ -470,18 +519,23 @@ void FinishCategorical(Impl &w, const thmm_categorical_model &model, const Categ
   if (policy.update_factors) w.resident_factors = true;
 }
 
-// Writes the internal [N][B][K] Scalar node values for a call.
-void InitNodesDense(Impl &w, std::size_t batch) {
-  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  LaunchInitNodes(l, w.real, w.input_nodes.ptr, w.nodes.ptr, w.node_scale.as<double>());
-}
+// Where a call's node factors come from: the staged dense potentials or the
+// staged categorical observations.
+struct Source {
+  const thmm_categorical_model *categorical = nullptr;
+};
 
-void InitNodesCategorical(Impl &w, const thmm_categorical_model &model) {
-  const Launch l = w.MakeLaunch(w.real, model.batch, w.stream);
-  LaunchInitCategorical(l, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
-                        static_cast<std::uint32_t>(model.num_observation_nodes),
-                        static_cast<std::uint32_t>(model.categories), w.is_root.as<std::uint8_t>(),
-                        w.root_potential.ptr, w.emissions.ptr, w.nodes.ptr, w.node_scale.as<double>());
+// Writes the internal node state (max-shifted logs + scales, element type
+// l.real) of a call from its staged inputs.
+void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale) {
+  if (src.categorical == nullptr) {
+    LaunchInitNodes(l, w.real, w.input_nodes.ptr, nodes, scale);
+  } else {
+    LaunchInitCategorical(l, w.real, false, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+                          static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
+                          static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
+                          w.root_potential.ptr, w.emissions.ptr, nodes, scale);
+  }
 }
 
 void DownloadLogZ(Impl &w, std::size_t batch) {
@@ -512,21 +566,19 @@ void RequireFiniteLogZ(const Impl &w, std::size_t batch, const char *message) {
 }
 
 // Scaled forward + log Z for a staged call.
-template <class Init> void RunPartition(Impl &w, std::size_t batch, Init init) {
+void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  ZeroEdgeScales(w, w.stream);
-  init();
-  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
+  const LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   CheckLaunches();
 }
 
-template <class Init> void RunMarginals(Impl &w, std::size_t batch, Init init) {
+void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  ZeroEdgeScales(w, w.stream);
-  init();
-  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
+  const LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
@@ -535,17 +587,24 @@ template <class Init> void RunMarginals(Impl &w, std::size_t batch, Init init) {
   CheckLaunches();
 }
 
-template <class Init> void RunMaximum(Impl &w, std::size_t batch, Init init) {
+void RunMaximum(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  init();
-  // Logs of the staged factors, internal layout.
-  const Launch internal = w.MakeLaunch(w.real, batch, w.stream);
-  Launch as_public = internal;
-  // nodes are already internal [N][B][K]: present them to MaxInit as a
-  // [1][N*B][K] "public" array so that the transpose is the identity.
-  as_public.dims.B = 1;
-  as_public.dims.N = static_cast<std::uint32_t>(w.N() * batch);
-  LaunchMaxInit(as_public, w.real, w.nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
+  // Logs of the raw staged factors (bit-exact with the oracle's log of the
+  // same values), internal layout.
+  if (src.categorical == nullptr) {
+    LaunchMaxInit(l, w.real, w.input_nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
+  } else {
+    LaunchInitCategorical(l, w.real, true, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+                          static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
+                          static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
+                          w.root_potential.ptr, w.emissions.ptr, w.nodes.ptr, nullptr);
+    // The raw products are internal [N][B][K]: present them to MaxInit as a
+    // [1][N*B][K] "public" array so that its transpose is the identity.
+    Launch as_public = l;
+    as_public.dims.B = 1;
+    as_public.dims.N = static_cast<std::uint32_t>(w.N() * batch);
+    LaunchMaxInit(as_public, w.real, w.nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
+  }
   const MaxBuffers buf = w.Max();
   for (const Level &level : w.levels) LaunchMaxLevel(l, level.primitive, level.offset, level.count, buf);
   LaunchMaxFinish(l, buf, w.log_weight.as<double>(), w.assignment.as<std::uint32_t>());
@@ -555,19 +614,14 @@ template <class Init> void RunMaximum(Impl &w, std::size_t batch, Init init) {
   CheckLaunches();
 }
 
-template <class Init> void RunSample(Impl &w, std::size_t batch, Init init) {
+void RunSample(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(Real::kF64, batch, w.stream);
-  ZeroEdgeScales(w, w.stream);
-  init();
   if (w.real == Real::kF32) {
-    // Widen the staged float factors to the FP64 sampling state.
-    LaunchWidenF32(l, w.nodes.as<float>(), w.s_nodes.as<double>(), w.N() * batch * w.states);
+    // FP64 sampling state from the staged float factors.
+    InitState(w, src, l, w.s_nodes.ptr, w.s_node_scale.as<double>());
     LaunchWidenF32(l, w.edges.as<float>(), w.s_edges.as<double>(), w.E() * w.KK());
-    Hip(hipMemsetAsync(w.s_node_scale.ptr, 0, w.N() * batch * sizeof(double), w.stream),
-        "GPU runtime MemsetAsync sampling node scales");
-    if (w.E() != 0)
-      Hip(hipMemsetAsync(w.s_edge_scale.ptr, 0, w.E() * sizeof(double), w.stream),
-          "GPU runtime MemsetAsync sampling edge scales");
+  } else {
+    InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
   }
   Hip(hipMemsetAsync(w.failure.ptr, 0, sizeof(std::uint32_t), w.stream), "GPU runtime MemsetAsync sampling status");
   const LinearBuffers buf = w.SamplingLinear();
@@ -701,7 +755,7 @@ void Workspace::Partition(const thmm_dense_model &model, bool log_domain, const 
   UploadDense(w, model);
   Record(w, 1);
   Record(w, 2);
-  RunPartition(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  RunPartition(w, model.batch, Source{});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -726,7 +780,7 @@ void Workspace::PartitionCategorical(const thmm_categorical_model &model, thmm_c
   UploadCategorical(w, model, policy);
   Record(w, 1);
   Record(w, 2);
-  RunPartition(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  RunPartition(w, model.batch, Source{&model});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -826,7 +880,7 @@ void Workspace::Maximum(const thmm_dense_model &model, const void **log_weights,
   UploadDense(w, model);
   Record(w, 1);
   Record(w, 2);
-  RunMaximum(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  RunMaximum(w, model.batch, Source{});
   Record(w, 3);
   Record(w, 4);
   DownloadMaximum(w, model.batch, log_weights, states);
@@ -846,7 +900,7 @@ void Workspace::MaximumCategorical(const thmm_categorical_model &model, thmm_cat
   UploadCategorical(w, model, policy);
   Record(w, 1);
   Record(w, 2);
-  RunMaximum(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  RunMaximum(w, model.batch, Source{&model});
   Record(w, 3);
   Record(w, 4);
   DownloadMaximum(w, model.batch, log_weights, states);
@@ -868,7 +922,7 @@ void Workspace::Sample(const thmm_dense_model &model, const void *uniforms, std:
   UploadUniforms(w, uniforms, uniform_count);
   Record(w, 1);
   Record(w, 2);
-  RunSample(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  RunSample(w, model.batch, Source{});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -891,7 +945,7 @@ void Workspace::SampleCategorical(const thmm_categorical_model &model, const voi
   UploadUniforms(w, uniforms, uniform_count);
   Record(w, 1);
   Record(w, 2);
-  RunSample(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  RunSample(w, model.batch, Source{&model});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -913,7 +967,7 @@ void Workspace::Marginals(const thmm_dense_model &model, const void **log_partit
   UploadDense(w, model);
   Record(w, 1);
   Record(w, 2);
-  RunMarginals(w, model.batch, [&] { InitNodesDense(w, model.batch); });
+  RunMarginals(w, model.batch, Source{});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -935,7 +989,7 @@ void Workspace::MarginalsCategorical(const thmm_categorical_model &model, thmm_c
   UploadCategorical(w, model, policy);
   Record(w, 1);
   Record(w, 2);
-  RunMarginals(w, model.batch, [&] { InitNodesCategorical(w, model); });
+  RunMarginals(w, model.batch, Source{&model});
   Record(w, 3);
   Record(w, 4);
   DownloadLogZ(w, model.batch);
@@ -975,16 +1029,16 @@ void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::si
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, batch, s);
-  ZeroEdgeScales(w, s);
   LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-  const LinearBuffers buf = w.Linear(edges);
+  const LinearBuffers buf = w.Linear(edges, nullptr);
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, log_z);
   CheckLaunches();
 }
 
 void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
-                                void *node_out, void *edge_out, void *stream) {
+                                void *node_out, void *edge_out, void *stream, thmm_launch_timing *timings,
+                                std::size_t capacity, std::size_t *count) {
   Impl &w = *impl_;
   CheckDevice(w, batch, nodes, edges);
   RequireMarginals(w);
@@ -993,15 +1047,27 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, batch, s);
-  ZeroEdgeScales(w, s);
+  std::unique_ptr<Profiler> profiler;
+  if (timings != nullptr) profiler = std::make_unique<Profiler>(s, static_cast<std::uint32_t>(batch));
+  Profiler *p = profiler.get();
+  if (p) p->Begin(0, 0, 0);
   LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-  LinearBuffers buf = w.Linear(edges);
+  if (p) p->End();
+  LinearBuffers buf = w.Linear(edges, nullptr);
   buf.edge_out = edge_out;
-  ForwardLevels(l, w.levels, buf);
+  ForwardLevels(l, w.levels, buf, p);
+  if (p) p->Begin(2, 0, 0);
   LaunchFinish(l, buf, log_z);
   LaunchSeedAdjoint(l, buf, nullptr);
-  ReverseLevels(l, w.levels, buf);
+  if (p) p->End();
+  ReverseLevels(l, w.levels, buf, p);
+  if (p) p->Begin(4, 0, 0);
   LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+  if (p) {
+    p->End();
+    CheckLaunches();
+    *count = p->Report(timings, capacity);
+  }
   CheckLaunches();
 }
 
@@ -1015,7 +1081,7 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   const Launch l = w.MakeLaunch(w.real, batch, s);
   LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
   LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
-  const LinearBuffers buf = w.Linear(w.edges.ptr);
+  const LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, log_z);
   CheckLaunches();
@@ -1035,7 +1101,7 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
-  LinearBuffers buf = w.Linear(w.edges.ptr);
+  LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
   buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
   LaunchSeedAdjoint(l, buf, grad);
   ReverseLevels(l, w.levels, buf);

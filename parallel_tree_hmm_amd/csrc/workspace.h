@@ -62,8 +62,11 @@ class Workspace {
   void LogPartitionsF64(const double **values, std::size_t *count);
 
   void LogPartitionDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z, void *stream);
+  // With `timings` non-null, every launch is bracketed by an event pair and
+  // the call blocks to report the durations.
   void MarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z, void *node_out,
-                       void *edge_out, void *stream);
+                       void *edge_out, void *stream, thmm_launch_timing *timings = nullptr,
+                       std::size_t capacity = 0, std::size_t *count = nullptr);
   void LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
                          void *stream);
   void LogZBackwardDevice(const double *grad, void *grad_nodes, void *grad_edges, bool reduce_edges,

@@ -130,6 +130,28 @@ def test_samples_match_oracle(require_gpu, precision, k, topo, n, batch):
     assert near_boundary <= max(1, batch // 10)
 
 
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("topo,n", [("star", 30000), ("caterpillar", 20001), ("forest", 24000)])
+def test_dynamic_range_of_large_trees(require_gpu, precision, topo, n):
+    """Products of tens of thousands of messages span far more than FP32's
+    linear range (a 30,000-leaf star's root factor spans > e^200); the log
+    vectors with FP64 scales keep log Z and marginals within tolerance."""
+    rng = np.random.default_rng(n)
+    k = 16
+    plan = thmm.Plan(topology(topo, n, rng))
+    nodes, edges = random_potentials(rng, 1, plan.num_nodes, plan.num_edges, k)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, 1, precision=precision)
+    out = thmm.posterior_marginals_prepared(thmm.BatchedModelView(plan, k, 1, nodes, edges), ws)
+    log_z = ws.log_partitions_f64()[: plan.roots.size]
+    olz, onm, oem = oracle.marginals(plan, k, nodes[0], edges)
+    assert np.all(np.isfinite(out.nodes)) and np.all(np.isfinite(out.edges))
+    assert np.all(np.abs(log_z - olz) <= LOGZ_TOL[precision] * np.maximum(1.0, np.abs(olz)))
+    assert np.max(np.abs(out.nodes.reshape(onm.shape) - onm)) <= MARG_TOL[precision]
+    assert np.max(np.abs(out.edges.reshape(oem.shape) - oem)) <= MARG_TOL[precision]
+
+
 def test_device_entry_points_match_host(require_gpu):
     import torch
 
