@@ -287,11 +287,15 @@ thmm_status thmm_result_log_partitions_f64(thmm_workspace *workspace,
 /* Device-resident entry points                                             */
 /* ------------------------------------------------------------------------ */
 /* Inputs already in HBM, outputs written to caller device buffers, work
- * enqueued on `stream` (a hipStream_t; NULL = the workspace stream) without
- * a host synchronisation. Element type follows the workspace precision.
+ * enqueued on `stream` without a host synchronisation: a hipStream_t, NULL
+ * for the workspace's own stream, or THMM_HIP_NULL_STREAM for HIP's null
+ * (default) stream, whose handle is itself 0. Element type follows the
+ * workspace precision.
  * These have no reference counterpart; they serve callers that keep
  * potentials resident (the reference uploads every call,
  * src/gpu_backend_impl.inc:2197-2213). */
+
+#define THMM_HIP_NULL_STREAM ((void *)(~(uintptr_t)0))
 
 /* Scaled log partitions: d_log_z is double [batch * roots]. */
 thmm_status thmm_log_partition_device(thmm_workspace *workspace,

@@ -265,6 +265,16 @@ def device_description(device: int = 0) -> str:
 
 _PRECISIONS = {"f32": (_abi.THMM_F32, np.float32), "f64": (_abi.THMM_F64, np.float64)}
 
+_HIP_NULL_STREAM = ctypes.c_void_p(2**64 - 1)  # THMM_HIP_NULL_STREAM
+
+
+def _stream(stream):
+    """Maps a hipStream_t handle to the C-ABI convention (NULL means the
+    workspace's own stream, so HIP's null stream 0 gets its sentinel)."""
+    if stream is None:
+        return None
+    return _HIP_NULL_STREAM if int(stream) == 0 else ctypes.c_void_p(int(stream))
+
 
 class Workspace:
     """tree_hmm::rocm::Workspace: device + pinned storage for prepared calls."""
@@ -413,27 +423,31 @@ class Workspace:
         return _array(ptr, count.value, self.dtype).reshape(batch, self._plan.num_nodes)
 
     # -- device-resident entry points --------------------------------------
-    def log_partition_device(self, nodes_ptr: int, edges_ptr: int, batch: int, log_z_ptr: int, stream: int = 0):
-        _check(_lib().thmm_log_partition_device(self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, stream or None))
+    # `stream` is a hipStream_t handle (torch: stream.cuda_stream); 0 is
+    # HIP's null stream. None runs on the workspace's own stream.
+    def log_partition_device(self, nodes_ptr: int, edges_ptr: int, batch: int, log_z_ptr: int, stream=None):
+        _check(_lib().thmm_log_partition_device(self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, _stream(stream)))
 
-    def marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream: int = 0):
+    def marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream=None):
         _check(
             _lib().thmm_marginals_device(
-                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream or None
+                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, _stream(stream)
             )
         )
 
-    def logz_forward_device(self, log_nodes_ptr, log_edges_ptr, batch, log_z_ptr, stream: int = 0):
-        _check(_lib().thmm_logz_forward_device(self._handle, log_nodes_ptr, log_edges_ptr, batch, log_z_ptr, stream or None))
+    def logz_forward_device(self, log_nodes_ptr, log_edges_ptr, batch, log_z_ptr, stream=None):
+        _check(
+            _lib().thmm_logz_forward_device(self._handle, log_nodes_ptr, log_edges_ptr, batch, log_z_ptr, _stream(stream))
+        )
 
-    def logz_backward_device(self, grad_ptr, grad_nodes_ptr, grad_edges_ptr, reduce_edges: bool, stream: int = 0):
+    def logz_backward_device(self, grad_ptr, grad_nodes_ptr, grad_edges_ptr, reduce_edges: bool, stream=None):
         _check(
             _lib().thmm_logz_backward_device(
-                self._handle, grad_ptr or None, grad_nodes_ptr, grad_edges_ptr, int(reduce_edges), stream or None
+                self._handle, grad_ptr or None, grad_nodes_ptr, grad_edges_ptr, int(reduce_edges), _stream(stream)
             )
         )
 
-    def profile_marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream: int = 0):
+    def profile_marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream=None):
         """thmm_marginals_device with per-launch HIP event timing; returns a
         list of (phase, primitive, count, batch, ms) tuples."""
         capacity = 4 * (self._plan.primitive_batches.size + 8)
@@ -441,7 +455,7 @@ class Workspace:
         count = ctypes.c_size_t()
         _check(
             _lib().thmm_profile_marginals_device(
-                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream or None,
+                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, _stream(stream),
                 timings, capacity, ctypes.byref(count)
             )
         )

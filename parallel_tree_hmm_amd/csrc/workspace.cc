@@ -1008,6 +1008,7 @@ void Workspace::LogPartitionsF64(const double **values, std::size_t *count) {
 namespace {
 
 hipStream_t ChooseStream(const Impl &w, void *stream) {
+  if (stream == THMM_HIP_NULL_STREAM) return nullptr;  // HIP's null stream
   return stream != nullptr ? static_cast<hipStream_t>(stream) : w.stream;
 }
 
