@@ -1,0 +1,95 @@
+// Device helpers shared by the kernel translation units.
+#ifndef TREE_HMM_AMD_DEVICE_COMMON_H_
+#define TREE_HMM_AMD_DEVICE_COMMON_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "launch.h"
+
+namespace thmm {
+
+// Keeps the first failed launch of the calling host thread for
+// LastLaunchError().
+void RecordLaunchError(hipError_t status, const char *what);
+
+// Launches a level on the coalesced/MFMA kernels when they cover (K,
+// element type, primitive); returns false to leave it to the generic ones.
+bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std::uint32_t offset,
+                     std::uint32_t count, const LinearBuffers &buf);
+
+namespace dev {
+
+constexpr double kNegInf = -__builtin_huge_val();
+
+__device__ inline float Max(float a, float b) { return fmaxf(a, b); }
+__device__ inline double Max(double a, double b) { return fmax(a, b); }
+__device__ inline float Exp(float x) { return expf(x); }
+__device__ inline double Exp(double x) { return exp(x); }
+
+template <typename T> __device__ inline T NegInf() { return static_cast<T>(kNegInf); }
+
+template <int W, typename V> __device__ inline V GroupMax(V v) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v = Max(v, __shfl_xor(v, o, W));
+  return v;
+}
+
+// Storage of slot `slot` for batch item b: edges are batch-shared, composed
+// paths per item.
+template <typename T>
+__device__ inline const T *Slot(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  return slot < d.E ? static_cast<const T *>(buf.edges) + slot * kk
+                    : static_cast<const T *>(buf.paths) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+}
+
+// FP64 log scale of row `row` of a slot.
+__device__ inline double RowScale(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                                  std::uint32_t row) {
+  if (slot < d.E) return buf.edge_scale ? buf.edge_scale[static_cast<std::uint64_t>(slot) * d.K + row] : 0.0;
+  return buf.path_scale[(static_cast<std::uint64_t>(slot - d.E) * d.B + b) * d.K + row];
+}
+
+// Adjoint storage of a slot: edge marginals are written batch-major
+// [B][E][K][K] straight into the output; composed slots are internal.
+template <typename T>
+__device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  return slot < d.E ? static_cast<T *>(buf.edge_out) + (static_cast<std::uint64_t>(b) * d.E + slot) * kk
+                    : static_cast<T *>(buf.path_adj) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
+}
+
+// Four consecutive elements, moved with 16-byte (float) or 2 x 16-byte
+// (double) vector memory instructions.
+template <typename T> struct Quad {
+  T v[4];
+};
+
+template <typename T> __device__ inline Quad<T> LoadQuad(const T *p) {
+  Quad<T> q;
+  if constexpr (sizeof(T) == 4) {
+    const float4 x = *reinterpret_cast<const float4 *>(p);
+    q.v[0] = x.x, q.v[1] = x.y, q.v[2] = x.z, q.v[3] = x.w;
+  } else {
+    const double2 x = *reinterpret_cast<const double2 *>(p);
+    const double2 y = *reinterpret_cast<const double2 *>(p + 2);
+    q.v[0] = x.x, q.v[1] = x.y, q.v[2] = y.x, q.v[3] = y.y;
+  }
+  return q;
+}
+
+template <typename T> __device__ inline void StoreQuad(T *p, const Quad<T> &q) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(q.v[0], q.v[1], q.v[2], q.v[3]);
+  } else {
+    *reinterpret_cast<double2 *>(p) = make_double2(q.v[0], q.v[1]);
+    *reinterpret_cast<double2 *>(p + 2) = make_double2(q.v[2], q.v[3]);
+  }
+}
+
+}  // namespace dev
+}  // namespace thmm
+
+#endif  // TREE_HMM_AMD_DEVICE_COMMON_H_

@@ -30,55 +30,24 @@
 
 #include <string>
 
+#include "device_common.h"
 #include "launch.h"
 
 namespace thmm {
 namespace {
-
 thread_local std::string g_launch_error;
+}  // namespace
 
-void Check(hipError_t status, const char *what) {
+void RecordLaunchError(hipError_t status, const char *what) {
   if (status != hipSuccess && g_launch_error.empty())
     g_launch_error = std::string(what) + ": " + hipGetErrorString(status);
 }
 
-constexpr double kNegInf = -__builtin_huge_val();
+namespace {
 
-__device__ inline float Max(float a, float b) { return fmaxf(a, b); }
-__device__ inline double Max(double a, double b) { return fmax(a, b); }
-__device__ inline float Exp(float x) { return expf(x); }
-__device__ inline double Exp(double x) { return exp(x); }
+using namespace dev;
 
-template <typename T> __device__ inline T NegInf() { return static_cast<T>(kNegInf); }
-
-template <int G, typename V> __device__ inline V GroupMax(V v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v = Max(v, __shfl_xor(v, o, G));
-  return v;
-}
-
-template <typename T>
-__device__ inline const T *Slot(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
-  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  return slot < d.E ? static_cast<const T *>(buf.edges) + slot * kk
-                    : static_cast<const T *>(buf.paths) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
-}
-
-// FP64 log scale of row `row` of a slot.
-__device__ inline double RowScale(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
-                                  std::uint32_t row) {
-  if (slot < d.E) return buf.edge_scale ? buf.edge_scale[static_cast<std::uint64_t>(slot) * d.K + row] : 0.0;
-  return buf.path_scale[(static_cast<std::uint64_t>(slot - d.E) * d.B + b) * d.K + row];
-}
-
-// Adjoint storage of a slot: edge marginals are written batch-major
-// [B][E][K][K] straight into the output; composed slots are internal.
-template <typename T>
-__device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b) {
-  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  return slot < d.E ? static_cast<T *>(buf.edge_out) + (static_cast<std::uint64_t>(b) * d.E + slot) * kk
-                    : static_cast<T *>(buf.path_adj) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
-}
+void Check(hipError_t status, const char *what) { RecordLaunchError(status, what); }
 
 struct Lane {
   std::uint64_t item;
@@ -897,7 +866,7 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const s
 
 void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
-  if (count == 0) return;
+  if (count == 0 || LaunchFastLevel(l, false, primitive, offset, count, buf)) return;
   if (l.real == Real::kF32) {
     THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, offset, count, buf)));
   } else {
@@ -925,7 +894,7 @@ void LaunchSeedAdjoint(const Launch &l, const LinearBuffers &buf, const double *
 
 void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
-  if (count == 0) return;
+  if (count == 0 || LaunchFastLevel(l, true, primitive, offset, count, buf)) return;
   if (l.real == Real::kF32) {
     THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<float, G>(l, primitive, offset, count, buf)));
   } else {
