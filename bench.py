@@ -242,6 +242,8 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-nodes", type=int, default=400_000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="also run one plain device copy of MIB MiB (the reference point that scales HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -261,6 +263,19 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
     log(f"device: {thmm.device_description(0)}")
+
+    if args.copy_calibration:
+        # One plain device-to-device copy of known size: reading FETCH_SIZE and
+        # WRITE_SIZE for it gives the counters' bytes per algorithmic byte.
+        count = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(count, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {count * 4} bytes read, {count * 4} bytes written")
+        del src, dst
+        torch.cuda.empty_cache()
 
     k = 16
     seed = 0x54484D4D00000004

@@ -19,6 +19,8 @@
 // their maximum) kept in FP64.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "device_common.h"
 #include "launch.h"
 
@@ -34,6 +36,11 @@ template <int K> struct Team {
   static constexpr int kSteps = K / kRowsPerStep;
 };
 static_assert(Team<8>::kSteps == 1 && Team<16>::kSteps == 4 && Team<32>::kSteps == 4, "team tiling");
+// One logarithm per lane in RakeForwardTeam needs at most one step per lane
+// of a row.
+static_assert(Team<8>::kSteps <= Team<8>::kLanesPerRow && Team<16>::kSteps <= Team<16>::kLanesPerRow &&
+                  Team<32>::kSteps <= Team<32>::kLanesPerRow,
+              "row ownership");
 
 // branch[p] = log sum_c path[p][c] exp(leaf[c]) (src/inference.cc:704-723)
 template <typename T, int K>
@@ -54,28 +61,31 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
 #pragma unroll
   for (int i = 0; i < 4; ++i) e[i] = Exp(vq.v[i]);
   const T *P = Slot<T>(buf, d, op.slot, b);
-  double message[kJ];
+  // After the row reduction all kLpr lanes of a row hold its sum; the lane
+  // with l % kLpr == j % kLpr takes the logarithm, so each row's log is
+  // formed once and the team's rows are stored by one contiguous store.
+  T sums[kJ];
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {
     const Quad<T> p = LoadQuad(P + static_cast<std::uint32_t>(j * kT + l) * 4);
     T part = p.v[0] * e[0] + p.v[1] * e[1] + p.v[2] * e[2] + p.v[3] * e[3];
 #pragma unroll
     for (int o = kLpr / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kT);
-    const int row = j * kRps + l / kLpr;
-    message[j] = part > T(0) ? RowScale(buf, d, op.slot, b, row) + log(static_cast<double>(part)) : kNegInf;
+    sums[j] = part;
   }
-  double top = message[0];
+  int row = -1;
+  T part = T(0);
 #pragma unroll
-  for (int j = 1; j < kJ; ++j) top = fmax(top, message[j]);
+  for (int j = 0; j < kJ; ++j)
+    if (j % kLpr == l % kLpr) row = j * kRps + l / kLpr, part = sums[j];
+  const double message =
+      row >= 0 && part > T(0) ? RowScale(buf, d, op.slot, b, row) + log(static_cast<double>(part)) : kNegInf;
+  double top = message;
 #pragma unroll
-  for (int o = kLpr; o < kT; o <<= 1) top = fmax(top, __shfl_xor(top, o, kT));
+  for (int o = kT / 2; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kT));
   const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
   const bool live = top > kNegInf;
-  if (l % kLpr == 0) {
-    T *out = static_cast<T *>(buf.branches) + at * K;
-#pragma unroll
-    for (int j = 0; j < kJ; ++j) out[j * kRps + l / kLpr] = live ? static_cast<T>(message[j] - top) : NegInf<T>();
-  }
+  if (row >= 0) static_cast<T *>(buf.branches)[at * K + row] = live ? static_cast<T>(message - top) : NegInf<T>();
   if (l == 0) buf.branch_scale[at] = live ? buf.node_scale[leaf_at] + top : kNegInf;
 }
 
@@ -174,7 +184,7 @@ __device__ inline float StageCompression(const LinearBuffers &buf, const Dims &d
 #pragma unroll
   for (int o = 8; o > 0; o >>= 1) w_top = fmax(w_top, __shfl_xor(w_top, o, 16));
   *top = w_top;
-  return w_top > kNegInf ? static_cast<float>(exp(w - w_top)) : 0.0f;
+  return w_top > kNegInf ? expf(static_cast<float>(w - w_top)) : 0.0f;
 }
 
 // S = L diag(e) R on MFMA; lane holds S[4(lane/16) + i][lane % 16].
@@ -304,7 +314,54 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
   }
 }
 
+// ---------------------------------------------------------------------------
+// Vector transposition [X][Y][K] -> [Y][X][K], K in {4, 8, 16, 32, 64}
+// ---------------------------------------------------------------------------
+
+// A team of K/4 lanes moves one K-vector with 16-byte (float) or 32-byte
+// (double) accesses; consecutive teams take consecutive input vectors, so
+// reads are contiguous and, for X = 1 or Y = 1 (a forest, or one tree), so
+// are writes. kMode 0 copies; kMode 1 takes logs of linear potentials and
+// kMode 2 log potentials, both then shifted by the vector's maximum, which
+// goes to `scale` in FP64 (-inf for a vector without support).
+template <typename T> __device__ inline T LogOf(T v) { return v > T(0) ? log(v) : NegInf<T>(); }
+template <> __device__ inline float LogOf(float v) { return v > 0.0f ? logf(v) : NegInf<float>(); }
+
+template <typename In, typename Out, int kMode, int K>
+__global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
+                                                     double *scale) {
+  constexpr int kLanes = K / 4;
+  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const std::uint32_t v = tid / kLanes;
+  const int l = static_cast<int>(tid % kLanes);
+  if (v >= X * Y) return;
+  const std::uint32_t x = v / Y, y = v % Y;
+  const std::uint64_t to = static_cast<std::uint64_t>(y) * X + x;
+  const Quad<In> q = LoadQuad(in + static_cast<std::uint64_t>(v) * K + l * 4);
+  // Logs are taken in the wider of the two element types.
+  using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
+  Quad<Out> o;
+  if constexpr (kMode == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.v[i] = static_cast<Out>(q.v[i]);
+  } else {
+    W w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = kMode == 1 ? LogOf(static_cast<W>(q.v[i])) : static_cast<W>(q.v[i]);
+    W top = Max(Max(w[0], w[1]), Max(w[2], w[3]));
+#pragma unroll
+    for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
+    const bool live = top > NegInf<W>();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+    if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
+  }
+  StoreQuad(out + to * K + l * 4, o);
+}
+
 constexpr unsigned kBlock = 256;
+
+bool Aligned16(const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
 
 unsigned Blocks(std::uint64_t threads) { return static_cast<unsigned>((threads + kBlock - 1) / kBlock); }
 
@@ -325,6 +382,8 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
                      std::uint32_t count, const LinearBuffers &buf) {
   const std::uint32_t K = l.dims.K;
   // 32-bit item indexing: the grid must address every (op, item, lane).
+  // Caller-owned edge inputs and outputs must allow 16-byte accesses.
+  if (!Aligned16(buf.edges) || !Aligned16(buf.edge_out) || !Aligned16(buf.node_adj)) return false;
   if (static_cast<std::uint64_t>(count) * l.dims.B * 64 >= (1ull << 32)) return false;
   if (primitive == 0 && (K == 8 || K == 16 || K == 32)) {
     if (l.real == Real::kF32) {
@@ -347,6 +406,57 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
       CompressForwardMfma16<<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
     RecordLaunchError(hipGetLastError(), "MFMA compression launch");
     return true;
+  }
+  return false;
+}
+
+namespace {
+
+template <typename In, typename Out, int kMode>
+bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
+  const std::uint64_t threads = static_cast<std::uint64_t>(X) * Y * (l.dims.K / 4);
+  if (threads >= (1ull << 32)) return false;
+  const unsigned grid = Blocks(threads);
+  const auto *i = static_cast<const In *>(in);
+  auto *o = static_cast<Out *>(out);
+  switch (l.dims.K) {
+    case 4: TransposeTeam<In, Out, kMode, 4><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    case 8: TransposeTeam<In, Out, kMode, 8><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    case 16: TransposeTeam<In, Out, kMode, 16><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    case 32: TransposeTeam<In, Out, kMode, 32><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    case 64: TransposeTeam<In, Out, kMode, 64><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    default: return false;
+  }
+  RecordLaunchError(hipGetLastError(), "vector transpose launch");
+  return true;
+}
+
+}  // namespace
+
+bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
+                         std::uint32_t X, std::uint32_t Y, double *scale) {
+  const std::uint32_t K = l.dims.K;
+  if (K % 4 != 0 || K > 64 || (K & (K - 1)) != 0) return false;
+  // Below 8 states a vector is under 32 bytes: scattered writes only pay when
+  // one side of the transposition is trivial.
+  if (K < 8 && X != 1 && Y != 1) return false;
+  if (!Aligned16(in) || !Aligned16(out)) return false;
+  if (static_cast<std::uint64_t>(X) * Y == 0) return true;
+  const bool f_in = in_real == Real::kF32, f_out = out_real == Real::kF32;
+  switch (mode) {
+    case 0:
+      if (f_in != f_out) return false;
+      return f_in ? TransposeTeamFor<float, float, 0>(l, in, out, X, Y, scale)
+                  : TransposeTeamFor<double, double, 0>(l, in, out, X, Y, scale);
+    case 1:
+      if (f_in && f_out) return TransposeTeamFor<float, float, 1>(l, in, out, X, Y, scale);
+      if (f_in) return TransposeTeamFor<float, double, 1>(l, in, out, X, Y, scale);
+      if (f_out) return TransposeTeamFor<double, float, 1>(l, in, out, X, Y, scale);
+      return TransposeTeamFor<double, double, 1>(l, in, out, X, Y, scale);
+    case 2:
+      if (f_in != f_out) return false;
+      return f_in ? TransposeTeamFor<float, float, 2>(l, in, out, X, Y, scale)
+                  : TransposeTeamFor<double, double, 2>(l, in, out, X, Y, scale);
   }
   return false;
 }

@@ -769,6 +769,9 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
 
 template <typename In, typename Out, int kMode>
 void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
+  constexpr Real in_real = sizeof(In) == 4 ? Real::kF32 : Real::kF64;
+  constexpr Real out_real = sizeof(Out) == 4 ? Real::kF32 : Real::kF64;
+  if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale)) return;
   const std::uint32_t ty = 64 / l.dims.K;
   const std::uint32_t x_tiles = (X + 63) / 64;
   const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty));

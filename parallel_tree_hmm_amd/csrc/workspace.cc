@@ -565,6 +565,13 @@ void RequireFiniteLogZ(const Impl &w, std::size_t batch, const char *message) {
     if (!std::isfinite(src[i])) throw std::domain_error(message);
 }
 
+// With one batch item the internal [N][B][K] node layout is the public
+// [B][N][K] one, so the reverse pass writes node marginals straight into the
+// output instead of transposing them afterwards.
+void AdjointsInto(LinearBuffers &buf, std::size_t batch, void *node_out) {
+  if (batch == 1) buf.node_adj = node_out;
+}
+
 // Scaled forward + log Z for a staged call.
 void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
@@ -578,12 +585,13 @@ void RunPartition(Impl &w, std::size_t batch, const Source &src) {
 void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
   InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
-  const LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  AdjointsInto(buf, batch, w.node_out.ptr);
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
   ReverseLevels(l, w.levels, buf);
-  LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
+  if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
   CheckLaunches();
 }
 
@@ -1056,6 +1064,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   if (p) p->End();
   LinearBuffers buf = w.Linear(edges, nullptr);
   buf.edge_out = edge_out;
+  AdjointsInto(buf, batch, node_out);
   ForwardLevels(l, w.levels, buf, p);
   if (p) p->Begin(2, 0, 0);
   LaunchFinish(l, buf, log_z);
@@ -1063,7 +1072,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   if (p) p->End();
   ReverseLevels(l, w.levels, buf, p);
   if (p) p->Begin(4, 0, 0);
-  LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+  if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
   if (p) {
     p->End();
     CheckLaunches();
@@ -1104,9 +1113,10 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
   LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
   buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
+  AdjointsInto(buf, w.last_batch, grad_nodes);
   LaunchSeedAdjoint(l, buf, grad);
   ReverseLevels(l, w.levels, buf);
-  LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
+  if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
   if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
   CheckLaunches();
 }
