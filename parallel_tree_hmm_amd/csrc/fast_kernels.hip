@@ -149,168 +149,230 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
 // ---------------------------------------------------------------------------
 // FP32 MFMA compression, K = 16
 // ---------------------------------------------------------------------------
+//
+// v_mfma_f32_16x16x4_f32 accumulates D += A B over four k per instruction;
+// lane l supplies A[l%16][k] and B[k][l%16] for k = 4(l/16) + kb in step kb
+// (the k order within a product is free), and D[i][j] lands in register i%4
+// of lane j + 16(i/4). Two per-lane layouts of a 16 x 16 matrix X follow:
+//   column layout: lane l, register r holds X[4(l/16) + r][l%16]
+//                  (four 4-byte loads, each a 64-byte segment per 16 lanes);
+//   row layout:    lane l, register r holds X[l%16][4(l/16) + r]
+//                  (one 16-byte load per lane).
+// D comes out in column layout; a matrix in column layout is a B operand as
+// it stands and an A operand transposed, and the row layout of X is the
+// column layout of X^T. Hence S^T = R^T (L diag(e))^T takes R in column
+// layout and L in row layout and yields S in row layout, so rows of S are
+// reduced across 4 lanes and stored with 16-byte stores.
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kPad = 17;  // LDS row stride (floats)
-constexpr int kWaves = 4;  // items (wavefronts) per block
+constexpr int kItemsPerWave = 2;
+constexpr int kWaves = 4;  // wavefronts per block
 
-struct CompressTiles {
-  float L[16 * kPad];
-  float R[16 * kPad];
-  float G[16 * kPad];
-  double gscale[16];
-};
-
-// Stages L and R of an item in LDS and returns the lane's middle weight
-// e[m] for m = lane % 16, with W = max_m (mid[m] + r_m) through *top.
-__device__ inline float StageCompression(const LinearBuffers &buf, const Dims &d, const DeviceCompress &op,
-                                         std::uint32_t b, int lane, CompressTiles &tiles, double *top) {
-  const float *L = Slot<float>(buf, d, op.left, b);
-  const float *R = Slot<float>(buf, d, op.right, b);
-  const Quad<float> ql = LoadQuad(L + lane * 4);
-  const Quad<float> qr = LoadQuad(R + lane * 4);
-  const int row = lane / 4, col = (lane % 4) * 4;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    tiles.L[row * kPad + col + i] = ql.v[i];
-    tiles.R[row * kPad + col + i] = qr.v[i];
-  }
-  const int m = lane % 16;
-  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-  const double w = static_cast<double>(static_cast<const float *>(buf.nodes)[mid_at * 16 + m]) +
-                   RowScale(buf, d, op.right, b, m);
-  double w_top = w;
-#pragma unroll
-  for (int o = 8; o > 0; o >>= 1) w_top = fmax(w_top, __shfl_xor(w_top, o, 16));
-  *top = w_top;
-  return w_top > kNegInf ? expf(static_cast<float>(w - w_top)) : 0.0f;
-}
-
-// S = L diag(e) R on MFMA; lane holds S[4(lane/16) + i][lane % 16].
-__device__ inline floatx4 MiddleProduct(const CompressTiles &tiles, float e_mine, int lane) {
+__device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    const int k = 4 * kb + lane / 16;
-    const float ek = __shfl(e_mine, k, 64);
-    const float a = tiles.L[(lane % 16) * kPad + k] * ek;
-    const float bv = tiles.R[k * kPad + lane % 16];
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv, acc, 0, 0, 0);
-  }
+  for (int kb = 0; kb < 4; ++kb) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[kb], b[kb], acc, 0, 0, 0);
   return acc;
 }
 
+__device__ inline void LoadColumns(const float *X, int lane, float (&out)[4]) {
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = X[(4 * (lane / 16) + r) * 16 + lane % 16];
+}
+
+__device__ inline void LoadRows(const float *X, int lane, float (&out)[4]) {
+  const Quad<float> q = LoadQuad(X + (lane % 16) * 16 + 4 * (lane / 16));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[r] = q.v[r];
+}
+
+// Operands of one compression as seen by one lane.
+struct CompressOperands {
+  DeviceCompress op;
+  std::uint32_t b;
+  float left_rows[4];    // L, row layout
+  float right_cols[4];   // R, column layout
+  float mid[4];          // mid[4(l/16) + r]
+  double right_scale[4]; // row scales of R at m = 4(l/16) + r
+};
+
+__device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, const LinearBuffers &buf,
+                                       std::uint32_t offset, std::uint32_t item, int lane, CompressOperands &o) {
+  o.b = item % d.B;
+  o.op = ops.compresses[offset + item / d.B];
+  LoadRows(Slot<float>(buf, d, o.op.left, o.b), lane, o.left_rows);
+  LoadColumns(Slot<float>(buf, d, o.op.right, o.b), lane, o.right_cols);
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(o.op.middle) * d.B + o.b;
+  const Quad<float> m = LoadQuad(static_cast<const float *>(buf.nodes) + mid_at * 16 + 4 * (lane / 16));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    o.mid[r] = m.v[r];
+    o.right_scale[r] = RowScale(buf, d, o.op.right, o.b, 4 * (lane / 16) + r);
+  }
+}
+
+// Middle weights e[m] = exp(mid[m] + r_m - W) for m = 4(l/16) + r, with
+// W = max_m (mid[m] + r_m) returned (-inf when no middle state is live).
+__device__ inline double MiddleWeights16(const CompressOperands &o, float (&e)[4]) {
+  double w[4];
+  double top = kNegInf;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    w[r] = static_cast<double>(o.mid[r]) + o.right_scale[r];
+    top = fmax(top, w[r]);
+  }
+  top = fmax(top, __shfl_xor(top, 16, 64));
+  top = fmax(top, __shfl_xor(top, 32, 64));
+#pragma unroll
+  for (int r = 0; r < 4; ++r) e[r] = top > kNegInf ? expf(static_cast<float>(w[r] - top)) : 0.0f;
+  return top;
+}
+
+// S = L diag(e) R in row layout.
+__device__ inline floatx4 MiddleProduct16(const CompressOperands &o, const float (&e)[4]) {
+  float scaled[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) scaled[r] = o.left_rows[r] * e[r];
+  return Mfma(o.right_cols, scaled);
+}
+
+// Row p of out = L diag(exp(mid)) R, max-normalised with its own FP64 log
+// scale (src/inference.cc:749-776).
 __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
                                                              std::uint32_t count, LinearBuffers buf) {
-  __shared__ CompressTiles shared[kWaves];
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const std::uint32_t item = blockIdx.x * kWaves + wave;
-  if (item >= count * d.B) return;  // whole wavefront; no block barrier below
-  CompressTiles &tiles = shared[wave];
-  const std::uint32_t b = item % d.B;
-  const DeviceCompress op = ops.compresses[offset + item / d.B];
-  double w_top = 0.0;
-  const float e_mine = StageCompression(buf, d, op, b, lane, tiles, &w_top);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const floatx4 s = MiddleProduct(tiles, e_mine, lane);
-  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
-  float *out = static_cast<float *>(buf.paths) + out_at * 256;
-  const int c = lane % 16;
+  const int lane = threadIdx.x % 64;
+  const std::uint32_t total = count * d.B;
+  const std::uint32_t first = (blockIdx.x * kWaves + threadIdx.x / 64) * kItemsPerWave;
+  if (first >= total) return;
+  CompressOperands o[kItemsPerWave];
+  double left_scale[kItemsPerWave], mid_scale[kItemsPerWave];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int p = 4 * (lane / 16) + i;
-    float rmax = s[i];
+  for (int t = 0; t < kItemsPerWave; ++t) {
+    const std::uint32_t item = first + t < total ? first + t : first;
+    LoadCompression(ops, d, buf, offset, item, lane, o[t]);
+    left_scale[t] = RowScale(buf, d, o[t].op.left, o[t].b, lane % 16);
+    mid_scale[t] = buf.node_scale[static_cast<std::uint64_t>(o[t].op.middle) * d.B + o[t].b];
+  }
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) rmax = fmaxf(rmax, __shfl_xor(rmax, o, 16));
-    const bool live = rmax > 0.0f && w_top > kNegInf;
-    out[p * 16 + c] = live ? s[i] / rmax : 0.0f;
-    if (c == 0)
-      buf.path_scale[out_at * 16 + p] =
-          live ? RowScale(buf, d, op.left, b, p) + buf.node_scale[mid_at] + w_top + log(static_cast<double>(rmax))
-               : kNegInf;
+  for (int t = 0; t < kItemsPerWave; ++t) {
+    float e[4];
+    const double w_top = MiddleWeights16(o[t], e);
+    const floatx4 s = MiddleProduct16(o[t], e);
+    float top = fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3]));
+    top = fmaxf(top, __shfl_xor(top, 16, 64));
+    top = fmaxf(top, __shfl_xor(top, 32, 64));
+    if (first + t >= total) continue;
+    const bool live = top > 0.0f && w_top > kNegInf;
+    const float inverse = live ? 1.0f / top : 0.0f;
+    const std::uint64_t out_at = static_cast<std::uint64_t>(o[t].op.out - d.E) * d.B + o[t].b;
+    Quad<float> q;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q.v[r] = s[r] * inverse;
+    StoreQuad(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
+    if (lane < 16)
+      buf.path_scale[out_at * 16 + lane] =
+          live ? left_scale[t] + mid_scale[t] + w_top + log(static_cast<double>(top)) : kNegInf;
   }
 }
 
 // Compression adjoint on MFMA (src/inference.cc:416-464). With S = L' R,
-// L' = L diag(e), Gr = adj_out / S row-normalised as Gr[p][c] = g_p Gh[p][c]
-// and Gamma = max_p g_p:
+// L' = L diag(e), Gr = adj_out / S (FP64 ratios; 0 where adj_out or S is 0),
+// g_p = max_c Gr[p][c], Gh = Gr / g_p, Gamma = max_p g_p, Gt = Gr / Gamma:
 //   adj_L[p][m] = L'[p][m] g_p (Gh R^T)[p][m]
-//   adj_R[m][c] = R[m][c] e[m] Gamma (A^T Gh)[m][c],  A[p][m] = L[p][m] g_p / Gamma
+//   adj_R[m][c] = R[m][c] e[m] Gamma (L^T Gt)[m][c]
 //   adj_mid[m]  = sum_c adj_R[m][c]
+// H^T = R Gh^T takes R and Gh in row layout; Q^T = Gt^T L takes Gt and L in
+// column layout, Gt reaching it through one LDS transposition.
+constexpr int kPad = 17;  // LDS row stride (floats)
+
 __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
                                                              std::uint32_t count, LinearBuffers buf) {
-  __shared__ CompressTiles shared[kWaves];
+  __shared__ float tiles[kWaves][kItemsPerWave][16 * kPad];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const std::uint32_t item = blockIdx.x * kWaves + wave;
-  if (item >= count * d.B) return;
-  CompressTiles &tiles = shared[wave];
-  const std::uint32_t b = item % d.B;
-  const DeviceCompress op = ops.compresses[offset + item / d.B];
-  double w_top = 0.0;
-  const float e_mine = StageCompression(buf, d, op, b, lane, tiles, &w_top);
+  const std::uint32_t total = count * d.B;
+  const std::uint32_t first = (blockIdx.x * kWaves + wave) * kItemsPerWave;
+  if (first >= total) return;  // whole wavefront; only wavefront barriers below
+  CompressOperands o[kItemsPerWave];
+  float right_rows[kItemsPerWave][4], left_cols[kItemsPerWave][4], adj[kItemsPerWave][4];
+  float mid_own[kItemsPerWave];
+  double right_scale_own[kItemsPerWave];
+#pragma unroll
+  for (int t = 0; t < kItemsPerWave; ++t) {
+    const std::uint32_t item = first + t < total ? first + t : first;
+    LoadCompression(ops, d, buf, offset, item, lane, o[t]);
+    LoadRows(Slot<float>(buf, d, o[t].op.right, o[t].b), lane, right_rows[t]);
+    LoadColumns(Slot<float>(buf, d, o[t].op.left, o[t].b), lane, left_cols[t]);
+    const std::uint64_t out_at = static_cast<std::uint64_t>(o[t].op.out - d.E) * d.B + o[t].b;
+    LoadRows(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj[t]);
+    const std::uint64_t mid_at = static_cast<std::uint64_t>(o[t].op.middle) * d.B + o[t].b;
+    mid_own[t] = static_cast<const float *>(buf.nodes)[mid_at * 16 + lane % 16];
+    right_scale_own[t] = RowScale(buf, d, o[t].op.right, o[t].b, lane % 16);
+  }
+  float e[kItemsPerWave][4], e_own[kItemsPerWave];
+  double g[kItemsPerWave], gamma[kItemsPerWave];
+  float gh[kItemsPerWave][4];
+#pragma unroll
+  for (int t = 0; t < kItemsPerWave; ++t) {
+    const double w_top = MiddleWeights16(o[t], e[t]);
+    e_own[t] = w_top > kNegInf ? expf(static_cast<float>(static_cast<double>(mid_own[t]) + right_scale_own[t] - w_top))
+                               : 0.0f;
+    const floatx4 s = MiddleProduct16(o[t], e[t]);
+    double gr[4];
+    double top = 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      gr[r] = (adj[t][r] != 0.0f && s[r] > 0.0f) ? static_cast<double>(adj[t][r]) / static_cast<double>(s[r]) : 0.0;
+      top = fmax(top, gr[r]);
+    }
+    top = fmax(top, __shfl_xor(top, 16, 64));
+    top = fmax(top, __shfl_xor(top, 32, 64));
+    g[t] = top;  // g_p for p = lane % 16
+    double big = top;
+#pragma unroll
+    for (int x = 1; x < 16; x <<= 1) big = fmax(big, __shfl_xor(big, x, 64));
+    gamma[t] = big;
+    const double inverse_gamma = big > 0.0 ? 1.0 / big : 0.0;
+    float *tile = tiles[wave][t];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      gh[t][r] = top > 0.0 ? static_cast<float>(gr[r] / top) : 0.0f;
+      tile[(lane % 16) * kPad + 4 * (lane / 16) + r] = static_cast<float>(gr[r] * inverse_gamma);
+    }
+  }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const floatx4 s = MiddleProduct(tiles, e_mine, lane);
-  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
-  const float *adj_out = static_cast<const float *>(buf.path_adj) + out_at * 256;
-  const int c = lane % 16;
-  // Row-normalised ratios Gh and their FP64 row scales g_p.
-  double gamma = 0.0;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int p = 4 * (lane / 16) + i;
-    const float a = adj_out[p * 16 + c];
-    const double gr = (a != 0.0f && s[i] > 0.0f) ? static_cast<double>(a) / static_cast<double>(s[i]) : 0.0;
-    double g = gr;
+  for (int t = 0; t < kItemsPerWave; ++t) {
+    float gt_cols[4];
+    const float *tile = tiles[wave][t];
 #pragma unroll
-    for (int o = 8; o > 0; o >>= 1) g = fmax(g, __shfl_xor(g, o, 16));
-    tiles.G[p * kPad + c] = g > 0.0 ? static_cast<float>(gr / g) : 0.0f;
-    if (c == 0) tiles.gscale[p] = g;
-    gamma = fmax(gamma, g);
-  }
+    for (int r = 0; r < 4; ++r) gt_cols[r] = tile[(4 * (lane / 16) + r) * kPad + lane % 16];
+    const floatx4 h = Mfma(right_rows[t], gh[t]);   // H in row layout: [p = l%16][m = 4(l/16) + r]
+    const floatx4 q = Mfma(gt_cols, left_cols[t]);  // Q in row layout: [m = l%16][c = 4(l/16) + r]
+    if (first + t >= total) continue;
+    const DeviceCompress &op = o[t].op;
+    const std::uint32_t b = o[t].b;
+    const int col = 4 * (lane / 16);
+    Quad<float> adj_left, adj_right;
+    double row_total = 0.0;
 #pragma unroll
-  for (int o = 16; o < 64; o <<= 1) gamma = fmax(gamma, __shfl_xor(gamma, o, 64));
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-
-  // H = Gh R^T; lane holds H[p = 4(lane/16) + i][m = lane % 16].
-  floatx4 h = {0.f, 0.f, 0.f, 0.f};
-  // Q = A^T Gh; lane holds Q[m = 4(lane/16) + i][c = lane % 16].
-  floatx4 q = {0.f, 0.f, 0.f, 0.f};
-  const double inverse_gamma = gamma > 0.0 ? 1.0 / gamma : 0.0;
-#pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    const int k = 4 * kb + lane / 16;
-    const int r = lane % 16;
-    h = __builtin_amdgcn_mfma_f32_16x16x4f32(tiles.G[r * kPad + k], tiles.R[r * kPad + k], h, 0, 0, 0);
-    const float a = tiles.L[k * kPad + r] * static_cast<float>(tiles.gscale[k] * inverse_gamma);
-    q = __builtin_amdgcn_mfma_f32_16x16x4f32(a, tiles.G[k * kPad + r], q, 0, 0, 0);
-  }
-  float *adj_left = SlotAdjoint<float>(buf, d, op.left, b);
-  float *adj_right = SlotAdjoint<float>(buf, d, op.right, b);
-  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int row = 4 * (lane / 16) + i;
-    // adj_L[p = row][m = c]
-    const double left = static_cast<double>(tiles.L[row * kPad + c] * e_mine) * tiles.gscale[row] *
-                        static_cast<double>(h[i]);
-    adj_left[row * 16 + c] = static_cast<float>(left);
-    // adj_R[m = row][c]
-    const float e_row = __shfl(e_mine, row, 64);
-    const double right = static_cast<double>(tiles.R[row * kPad + c]) * static_cast<double>(e_row) * gamma *
-                         static_cast<double>(q[i]);
-    adj_right[row * 16 + c] = static_cast<float>(right);
-    double total = right;
-#pragma unroll
-    for (int o = 8; o > 0; o >>= 1) total += __shfl_xor(total, o, 16);
-    if (c == 0) static_cast<float *>(buf.node_adj)[mid_at * 16 + row] = static_cast<float>(total);
+    for (int r = 0; r < 4; ++r) {
+      adj_left.v[r] = static_cast<float>(static_cast<double>(o[t].left_rows[r]) * static_cast<double>(e[t][r]) * g[t] *
+                                         static_cast<double>(h[r]));
+      const double right = static_cast<double>(right_rows[t][r]) * static_cast<double>(e_own[t]) * gamma[t] *
+                           static_cast<double>(q[r]);
+      adj_right.v[r] = static_cast<float>(right);
+      row_total += right;
+    }
+    StoreQuad(SlotAdjoint<float>(buf, d, op.left, b) + (lane % 16) * 16 + col, adj_left);
+    StoreQuad(SlotAdjoint<float>(buf, d, op.right, b) + (lane % 16) * 16 + col, adj_right);
+    row_total += __shfl_xor(row_total, 16, 64);
+    row_total += __shfl_xor(row_total, 32, 64);
+    if (lane < 16)
+      static_cast<float *>(buf.node_adj)[(static_cast<std::uint64_t>(op.middle) * d.B + b) * 16 + lane] =
+          static_cast<float>(row_total);
   }
 }
 
@@ -399,7 +461,8 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
     return true;
   }
   if (primitive == 3 && K == 16 && l.real == Real::kF32) {
-    const unsigned grid = static_cast<unsigned>((static_cast<std::uint64_t>(count) * l.dims.B + kWaves - 1) / kWaves);
+    const std::uint64_t per_block = static_cast<std::uint64_t>(kWaves) * kItemsPerWave;
+    const unsigned grid = static_cast<unsigned>((static_cast<std::uint64_t>(count) * l.dims.B + per_block - 1) / per_block);
     if (reverse)
       CompressReverseMfma16<<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
     else
