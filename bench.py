@@ -84,7 +84,15 @@ def concatenate(parts):
 
 
 # ---------------------------------------------------------------------------
-# Algorithmic bytes per op (FP32 s = 4; per-item edge slots at batch 1)
+# Algorithmic bytes per op (FP32 s = 4; per-item edge slots at batch 1).
+# Values only; the FP64 scale words (one per vector, K per matrix) are left
+# out. Forward: rake reads the leaf vector and the K x K slot and writes the
+# branch; combination/absorption read two vectors and write one; compression
+# reads L, R and the middle vector and writes the composed path. Reverse:
+# rake reads the slot, the leaf vector and the branch adjoint and writes the
+# slot adjoint (an edge marginal) and the leaf adjoint; combination and
+# absorption copy one adjoint vector; compression reads L, R, the middle
+# vector and the path adjoint and writes the adjoints of L, R and the middle.
 # ---------------------------------------------------------------------------
 
 
@@ -93,7 +101,7 @@ def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
     if phase == 1:  # forward
         return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s}[primitive]
     if phase == 3:  # reverse
-        return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (6 * kk + 2 * k) * s}[primitive]
+        return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s}[primitive]
     return 0
 
 
@@ -141,6 +149,18 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup):
     return ws, ms, timings, log_z, node_out, edge_out
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
+    None when no summary covers it."""
+    for path in sorted((ROOT / "profiles").glob("traffic_*.md")):
+        for line in path.read_text().splitlines():
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def roofline_from(timings, k, s):
     totals = {}
     for phase, primitive, count, batch, ms in timings:
@@ -162,7 +182,7 @@ def roofline_from(timings, k, s):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(KERNEL_NAMES[key]),
         "launches": launches,
         "avg_launch_us": round(ms / launches * 1e3, 3),
         "algorithmic_bytes_per_launch": int(nbytes / launches),
