@@ -284,6 +284,32 @@ thmm_status thmm_result_log_partitions_f64(thmm_workspace *workspace,
                                            size_t *count);
 
 /* ------------------------------------------------------------------------ */
+/* Branch-length gradient pair (categorical models)                         */
+/* ------------------------------------------------------------------------ */
+/* No reference counterpart. For edge factors P_e = expm(Q t_e) staged by the
+ * caller (e.g. through thmm_categorical_inputs with THMM_UPDATE_FACTORS
+ * reuse of the observations), the forward half returns the FP64 log
+ * partitions [batch * roots] and keeps the contraction state; the backward
+ * half runs the adjoint pass seeded with grad_log_z (host, FP64
+ * [batch * roots]; NULL = all ones) and returns
+ *   grad[e] = d (sum_b grad_log_z[b] log Z_b) / d t_e
+ *           = sum_{p,c} M_e[p][c] (Q P_e)[p][c] / P_e[p][c]
+ * (host, FP64 [num_edges]), M_e the batch-summed edge marginals, Q the
+ * row-major K x K rate matrix (host, FP64). Entries with P_e[p][c] = 0 add
+ * nothing, so the chain needs P_e > 0, which expm(Q t) has for t > 0 and an
+ * irreducible Q. Needs a ReserveCategoricalMarginals workspace; any other
+ * call in between invalidates the forward state (logic_error). Result spans
+ * live in workspace pinned memory until the next call. */
+thmm_status thmm_categorical_logz_forward(thmm_workspace *workspace,
+                                          const thmm_categorical_model *model,
+                                          thmm_categorical_update update,
+                                          const double **log_z, size_t *count);
+thmm_status thmm_branch_length_backward(thmm_workspace *workspace,
+                                        const double *grad_log_z,
+                                        const double *rate_matrix,
+                                        const double **grad, size_t *count);
+
+/* ------------------------------------------------------------------------ */
 /* Device-resident entry points                                             */
 /* ------------------------------------------------------------------------ */
 /* Inputs already in HBM, outputs written to caller device buffers, work

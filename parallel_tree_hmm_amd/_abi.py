@@ -192,6 +192,11 @@ PROTOTYPES = {
         [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_void_p), P(c_void_p), P(Timings)],
     ),
     "thmm_result_log_partitions_f64": (c_status, [c_void_p, P(c_void_p), P(c_size)]),
+    "thmm_categorical_logz_forward": (
+        c_status,
+        [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_size)],
+    ),
+    "thmm_branch_length_backward": (c_status, [c_void_p, c_void_p, c_void_p, P(c_void_p), P(c_size)]),
     "thmm_log_partition_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
     "thmm_marginals_device": (
         c_status,

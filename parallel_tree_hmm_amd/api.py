@@ -464,6 +464,37 @@ class Workspace:
     def synchronize(self) -> None:
         _check(_lib().thmm_synchronize(self._handle))
 
+    # -- branch-length gradient pair (categorical) --------------------------
+    def categorical_logz_forward(self, model: "BatchedCategoricalModelView", update=None) -> np.ndarray:
+        """Forward half: FP64 log partitions [batch * roots] of a categorical
+        model (edge factors P_e = expm(Q t_e)); keeps the contraction state
+        for branch_length_backward."""
+        m, keep = _categorical(model, self)
+        update = CategoricalInputUpdate.kAll if update is None else update
+        ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib().thmm_categorical_logz_forward(self._handle, ctypes.byref(m), int(update), ctypes.byref(ptr), ctypes.byref(count)))
+        del keep
+        return _array(ptr, count.value, np.float64).copy()
+
+    def branch_length_backward(self, rate_matrix, grad_log_z=None) -> np.ndarray:
+        """Backward half: d (sum_b g_b log Z_b) / d t_e for every edge, FP64
+        [num_edges], given the K x K rate matrix Q and the seeds g (None =
+        all ones)."""
+        k = self.states
+        q = np.ascontiguousarray(np.asarray(rate_matrix, dtype=np.float64))
+        if q.shape != (k, k):
+            raise InvalidArgument(f"rate matrix must be {k} x {k}")
+        g = None
+        if grad_log_z is not None:
+            g = np.ascontiguousarray(np.asarray(grad_log_z, dtype=np.float64)).ravel()
+        ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(
+            _lib().thmm_branch_length_backward(
+                self._handle, None if g is None else g.ctypes.data, q.ctypes.data, ctypes.byref(ptr), ctypes.byref(count)
+            )
+        )
+        return _array(ptr, count.value, np.float64).copy()
+
     def log_partitions_f64(self) -> np.ndarray:
         ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
         _check(_lib().thmm_result_log_partitions_f64(self._handle, ctypes.byref(ptr), ctypes.byref(count)))

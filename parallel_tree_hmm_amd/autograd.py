@@ -1,17 +1,38 @@
-"""torch.autograd.Function over the forward/backward C-ABI pair
-(thmm_logz_forward_device / thmm_logz_backward_device).
+"""torch.autograd.Functions over the forward/backward C-ABI pairs.
 
-log Z(log node potentials, log edge potentials) for a batch of items sharing
-one plan; the backward pass is the hand-written adjoint kernel, whose
-outputs are the posterior marginals scaled by the incoming gradient.
-Everything stays on the device; torch provides memory and the stream.
+LogPartition (thmm_logz_forward_device / thmm_logz_backward_device): log Z
+of log node and edge potentials for a batch of items sharing one plan; the
+backward pass is the hand-written adjoint kernel, whose outputs are the
+posterior marginals scaled by the incoming gradient. Everything stays on the
+device; torch provides memory and the stream.
+
+BranchLengthLogPartition (thmm_categorical_logz_forward /
+thmm_branch_length_backward): log Z per site of a categorical model whose
+edge factors are P_e = expm(Q t_e), differentiated with respect to the
+branch lengths t.
+
+A workspace holds the state of its latest forward call only: a backward
+whose forward was followed by another forward on the same workspace raises
+instead of returning the gradient of the later call.
 """
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
-from .api import Plan, Workspace
+from .api import BatchedCategoricalModelView, CategoricalInputUpdate, LogicError, Plan, Workspace
+
+
+def _claim(workspace: Workspace) -> int:
+    generation = getattr(workspace, "_forward_generation", 0) + 1
+    workspace._forward_generation = generation
+    return generation
+
+
+def _require_latest(workspace: Workspace, generation: int) -> None:
+    if getattr(workspace, "_forward_generation", 0) != generation:
+        raise LogicError("the workspace ran another forward call after this one; its contraction state is gone")
 
 
 class LogPartition(torch.autograd.Function):
@@ -28,6 +49,7 @@ class LogPartition(torch.autograd.Function):
         log_z = torch.empty((batch, roots), dtype=torch.float64, device=log_nodes.device)
         stream = torch.cuda.current_stream().cuda_stream
         workspace.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), batch, log_z.data_ptr(), stream)
+        ctx.generation = _claim(workspace)
         ctx.workspace = workspace
         ctx.reduce_edges = reduce_edges
         ctx.shapes = (log_nodes.shape, log_edges.shape, log_nodes.dtype)
@@ -35,6 +57,7 @@ class LogPartition(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_log_z: torch.Tensor):
+        _require_latest(ctx.workspace, ctx.generation)
         node_shape, edge_shape, dtype = ctx.shapes
         device = grad_log_z.device
         grad = grad_log_z.to(torch.float64).contiguous()
@@ -65,3 +88,43 @@ def reserve_for(plan: Plan, states: int, batch: int, dtype: torch.dtype) -> Work
     ws = Workspace()
     ws.reserve_marginals(plan, states, batch, precision="f32" if dtype == torch.float32 else "f64")
     return ws
+
+
+class BranchLengthLogPartition(torch.autograd.Function):
+    """log_z[b * roots + r] of a categorical model with edge factors
+    P_e = expm(Q t_e), formed in FP64 on the host."""
+
+    @staticmethod
+    def forward(ctx, t: torch.Tensor, rate_matrix: torch.Tensor, model: BatchedCategoricalModelView,
+                workspace: Workspace, update: CategoricalInputUpdate):
+        q = rate_matrix.detach().to(torch.float64).cpu()
+        p = torch.linalg.matrix_exp(q.unsqueeze(0) * t.detach().to(torch.float64).cpu().reshape(-1, 1, 1))
+        staged = BatchedCategoricalModelView(
+            model.plan, model.states, model.batch, model.categories, model.observation_nodes, model.observations,
+            model.root_potential, model.emission_potentials, p.numpy(),
+        )
+        log_z = workspace.categorical_logz_forward(staged, update)
+        ctx.generation = _claim(workspace)
+        ctx.workspace = workspace
+        ctx.rate = q.numpy()
+        ctx.t_meta = (t.dtype, t.device)
+        return torch.from_numpy(log_z).to(t.device)
+
+    @staticmethod
+    def backward(ctx, grad_log_z: torch.Tensor):
+        _require_latest(ctx.workspace, ctx.generation)
+        seeds = grad_log_z.detach().to(torch.float64).cpu().numpy()
+        grad = ctx.workspace.branch_length_backward(ctx.rate, seeds)
+        dtype, device = ctx.t_meta
+        return torch.from_numpy(grad).to(device=device, dtype=dtype), None, None, None, None
+
+
+def branch_length_log_partition(t: torch.Tensor, rate_matrix, model: BatchedCategoricalModelView,
+                                workspace: Workspace, update=CategoricalInputUpdate.kAll) -> torch.Tensor:
+    """Differentiable per-site log Z ([batch * roots], FP64) as a function of
+    the branch lengths t ([num_edges]). `model` supplies the observations,
+    root and emission factors (its edge potentials are replaced by
+    expm(Q t_e)); the workspace must come from reserve_categorical_marginals.
+    Use update=kFactors to reuse observations staged by an earlier call."""
+    q = rate_matrix if isinstance(rate_matrix, torch.Tensor) else torch.as_tensor(np.asarray(rate_matrix))
+    return BranchLengthLogPartition.apply(t, q, model, workspace, update)

@@ -245,6 +245,22 @@ thmm_status thmm_result_log_partitions_f64(thmm_workspace *workspace, const doub
   });
 }
 
+thmm_status thmm_categorical_logz_forward(thmm_workspace *workspace, const thmm_categorical_model *model,
+                                          thmm_categorical_update update, const double **log_z, size_t *count) {
+  return Guard([&] {
+    Require(model && log_z && count, "argument is null");
+    W(workspace).CategoricalLogZForward(*model, update, log_z, count);
+  });
+}
+
+thmm_status thmm_branch_length_backward(thmm_workspace *workspace, const double *grad_log_z,
+                                        const double *rate_matrix, const double **grad, size_t *count) {
+  return Guard([&] {
+    Require(grad && count, "argument is null");
+    W(workspace).BranchLengthBackward(grad_log_z, rate_matrix, grad, count);
+  });
+}
+
 thmm_status thmm_log_partition_device(thmm_workspace *workspace, const void *d_node_potentials,
                                       const void *d_edge_potentials, size_t batch, double *d_log_z, void *stream) {
   return Guard([&] { W(workspace).LogPartitionDevice(d_node_potentials, d_edge_potentials, batch, d_log_z, stream); });

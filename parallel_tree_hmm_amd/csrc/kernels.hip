@@ -914,6 +914,55 @@ void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out) {
   Check(hipGetLastError(), "node marginal transpose launch");
 }
 
+namespace {
+
+// One thread per (e, p, c): the batch-summed adjoint times the rate of change
+// of log P_e[p][c] with respect to t_e.
+template <typename T>
+__global__ void BranchChainTerms(const T *edge_out, const T *edges, const double *rate, double *terms, Dims d) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= static_cast<std::uint64_t>(d.E) * kk) return;
+  const std::uint64_t e = tid / kk;
+  const std::uint32_t p = static_cast<std::uint32_t>((tid % kk) / d.K), c = static_cast<std::uint32_t>(tid % d.K);
+  double mass = 0.0;
+  for (std::uint32_t b = 0; b < d.B; ++b) mass += static_cast<double>(edge_out[static_cast<std::uint64_t>(b) * d.E * kk + tid]);
+  const T *P = edges + e * kk;
+  const double value = static_cast<double>(P[p * d.K + c]);
+  double term = 0.0;
+  if (value > 0.0 && mass != 0.0) {
+    double slope = 0.0;
+    for (std::uint32_t j = 0; j < d.K; ++j) slope += rate[p * d.K + j] * static_cast<double>(P[j * d.K + c]);
+    term = mass * slope / value;
+  }
+  terms[tid] = term;
+}
+
+__global__ void BranchChainSums(const double *terms, double *grad, Dims d) {
+  const std::uint64_t e = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= d.E) return;
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  double total = 0.0;
+  for (std::uint64_t i = 0; i < kk; ++i) total += terms[e * kk + i];
+  grad[e] = total;
+}
+
+}  // namespace
+
+void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *edges, const double *rate,
+                             double *terms, double *grad) {
+  const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  if (entries == 0) return;
+  if (l.real == Real::kF32)
+    BranchChainTerms<float><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+        static_cast<const float *>(edge_out), static_cast<const float *>(edges), rate, terms, l.dims);
+  else
+    BranchChainTerms<double><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+        static_cast<const double *>(edge_out), static_cast<const double *>(edges), rate, terms, l.dims);
+  BranchChainSums<<<BlocksFor(l.dims.E, kThreads), kThreads, 0, StreamOf(l)>>>(terms, grad, l.dims);
+  Check(hipGetLastError(), "branch-length chain launch");
+}
+
 void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out) {
   const std::uint64_t total = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
   if (total == 0) return;

@@ -127,6 +127,12 @@ void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
 void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out);
 // out[e][.] = sum_b edge_out[b][e][.]
 void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out);
+// Branch-length chain for edge factors P_e = expm(Q t_e), dP_e/dt_e = Q P_e:
+//   grad[e] = sum_{p,c} M_e[p][c] (Q P_e)[p][c] / P_e[p][c],
+// M_e = sum_b edge_out[b][e] (the adjoint of P_e, batch-summed), FP64.
+// `terms` is [E][K][K] FP64 scratch; entries with P_e[p][c] = 0 contribute 0.
+void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *edges, const double *rate,
+                             double *terms, double *grad);
 // Widens `count` FP32 values to FP64.
 void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t count);
 // Casts `count` FP64 values to the launch element type.

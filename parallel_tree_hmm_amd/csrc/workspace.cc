@@ -98,8 +98,11 @@ struct Workspace::Impl {
   std::vector<std::uint32_t> observation_nodes;
   std::size_t resident_observation_batch = 0;
   bool resident_factors = false;
+  // Which forward pass the contraction state holds for a following backward
+  // call; any other call overwrites that state and resets it.
+  enum class Forward { kNone, kLogPotentials, kCategorical };
+  Forward forward = Forward::kNone;
   std::size_t last_batch = 0;
-  bool forward_ready = false;  // logz_forward state present for backward
 
   hipStream_t stream = nullptr;
   hipEvent_t events[6] = {};
@@ -121,6 +124,9 @@ struct Workspace::Impl {
   DeviceBuffer m_nodes, m_edges, m_paths, m_branches, rake_choice, compress_choice, log_weight;
   // results
   PinnedBuffer host_values, host_log_z, host_assignment, host_node_out, host_edge_out, host_failure;
+  // branch-length chain (categorical marginal workspaces)
+  DeviceBuffer chain_seed, chain_rate, chain_terms, chain_grad;
+  PinnedBuffer host_chain_seed, host_chain_rate, host_chain_grad;
 
   ~Impl() { Clear(); }
 
@@ -135,11 +141,12 @@ struct Workspace::Impl {
                             &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
                             &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
-                            &rake_choice, &compress_choice, &log_weight})
+                            &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
+                            &chain_grad})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
-                            &host_edge_out, &host_failure})
+                            &host_edge_out, &host_failure, &host_chain_seed, &host_chain_rate, &host_chain_grad})
       b->Release();
     for (hipEvent_t &e : events) {
       if (e != nullptr) static_cast<void>(hipEventDestroy(e));
@@ -157,7 +164,7 @@ struct Workspace::Impl {
     resident_observation_batch = 0;
     resident_factors = false;
     last_batch = 0;
-    forward_ready = false;
+    forward = Forward::kNone;
     levels.clear();
   }
 
@@ -701,6 +708,16 @@ void Workspace::ReserveCategorical(const Plan &plan, const void *identity, std::
     w.observations.Allocate(observation_values, "observations");
     w.root_potential.Allocate(states * w.S(), "root potential");
     w.emissions.Allocate(emission_values * w.S(), "emissions");
+    if (flags & THMM_RESERVE_MARGINALS) {
+      const std::size_t seeds = batch * plan.roots.size(), E = plan.num_edges;
+      w.chain_seed.Allocate(seeds * sizeof(double), "branch-length gradient seeds");
+      w.chain_rate.Allocate(states * states * sizeof(double), "rate matrix");
+      w.chain_terms.Allocate(E * states * states * sizeof(double), "branch-length chain terms");
+      w.chain_grad.Allocate(E * sizeof(double), "branch-length gradients");
+      w.host_chain_seed.Allocate(seeds * sizeof(double), "pinned gradient seeds");
+      w.host_chain_rate.Allocate(states * states * sizeof(double), "pinned rate matrix");
+      w.host_chain_grad.Allocate(E * sizeof(double), "pinned branch-length gradients");
+    }
     std::vector<std::uint32_t> index(plan.num_nodes, 0xFFFFFFFFu);
     for (std::size_t i = 0; i < observation_count; ++i) index[observation_nodes[i]] = static_cast<std::uint32_t>(i);
     Upload(w.observation_index, index, w.stream, "categorical topology upload");
@@ -758,6 +775,7 @@ void Workspace::Partition(const thmm_dense_model &model, bool log_domain, const 
   const Timer timer;
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "Workspace::Reserve");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   StageDense(w, model);
   Record(w, 0);
   UploadDense(w, model);
@@ -783,6 +801,7 @@ void Workspace::PartitionCategorical(const thmm_categorical_model &model, thmm_c
   const Timer timer;
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "Workspace::Reserve");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const CategoricalPolicy policy = StageCategorical(w, model, update);
   Record(w, 0);
   UploadCategorical(w, model, policy);
@@ -883,6 +902,7 @@ void Workspace::Maximum(const thmm_dense_model &model, const void **log_weights,
   RequireMaximum(w);
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveMaximum");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   StageDense(w, model);
   Record(w, 0);
   UploadDense(w, model);
@@ -903,6 +923,7 @@ void Workspace::MaximumCategorical(const thmm_categorical_model &model, thmm_cat
   RequireMaximum(w);
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalMaximum");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const CategoricalPolicy policy = StageCategorical(w, model, update);
   Record(w, 0);
   UploadCategorical(w, model, policy);
@@ -924,6 +945,7 @@ void Workspace::Sample(const thmm_dense_model &model, const void *uniforms, std:
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveSampling");
   ValidateUniforms(w, uniforms, uniform_count, model.batch);
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   StageDense(w, model);
   Record(w, 0);
   UploadDense(w, model);
@@ -947,6 +969,7 @@ void Workspace::SampleCategorical(const thmm_categorical_model &model, const voi
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalSampling");
   ValidateUniforms(w, uniforms, uniform_count, model.batch);
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const CategoricalPolicy policy = StageCategorical(w, model, update);
   Record(w, 0);
   UploadCategorical(w, model, policy);
@@ -970,6 +993,7 @@ void Workspace::Marginals(const thmm_dense_model &model, const void **log_partit
   RequireMarginals(w);
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveMarginals");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   StageDense(w, model);
   Record(w, 0);
   UploadDense(w, model);
@@ -992,6 +1016,7 @@ void Workspace::MarginalsCategorical(const thmm_categorical_model &model, thmm_c
   RequireMarginals(w);
   CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "ReserveCategoricalMarginals");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const CategoricalPolicy policy = StageCategorical(w, model, update);
   Record(w, 0);
   UploadCategorical(w, model, policy);
@@ -1004,6 +1029,64 @@ void Workspace::MarginalsCategorical(const thmm_categorical_model &model, thmm_c
   DownloadMarginals(w, model.batch, log_partitions, nodes, edges);
   FinishCategorical(w, model, policy);
   FillTimings(w, timer, timings);
+}
+
+void Workspace::CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
+                                       const double **log_z, std::size_t *count) {
+  Impl &w = *impl_;
+  RequireHost(w);
+  RequireMarginals(w);
+  if (!w.categorical) throw std::logic_error("the branch-length pair needs a ReserveCategoricalMarginals workspace");
+  CheckCall(w, model.identity, model.states, model.batch, model.edge_count, "Workspace::ReserveCategoricalMarginals");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
+  const CategoricalPolicy policy = StageCategorical(w, model, update);
+  UploadCategorical(w, model, policy);
+  RunPartition(w, model.batch, Source{&model});
+  DownloadLogZ(w, model.batch);
+  Record(w, 5);
+  WaitDownload(w);
+  FinishCategorical(w, model, policy);
+  RequireFiniteLogZ(w, model.batch, "the tree HMM has a nonpositive partition function");
+  w.last_batch = model.batch;
+  w.forward = Impl::Forward::kCategorical;
+  *log_z = w.host_log_z.as<double>();
+  *count = model.batch * w.Roots();
+}
+
+void Workspace::BranchLengthBackward(const double *grad_log_z, const double *rate_matrix, const double **grad,
+                                     std::size_t *count) {
+  Impl &w = *impl_;
+  if (w.forward != Impl::Forward::kCategorical)
+    throw std::logic_error("thmm_branch_length_backward requires a preceding thmm_categorical_logz_forward call");
+  if (rate_matrix == nullptr) throw std::invalid_argument("rate matrix is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  const std::size_t seeds = w.last_batch * w.Roots();
+  if (grad_log_z != nullptr) {
+    std::memcpy(w.host_chain_seed.ptr, grad_log_z, seeds * sizeof(double));
+    Hip(hipMemcpyAsync(w.chain_seed.ptr, w.host_chain_seed.ptr, seeds * sizeof(double), hipMemcpyHostToDevice,
+                       w.stream),
+        "GPU runtime MemcpyAsync gradient-seed upload");
+  }
+  std::memcpy(w.host_chain_rate.ptr, rate_matrix, w.KK() * sizeof(double));
+  Hip(hipMemcpyAsync(w.chain_rate.ptr, w.host_chain_rate.ptr, w.KK() * sizeof(double), hipMemcpyHostToDevice,
+                     w.stream),
+      "GPU runtime MemcpyAsync rate-matrix upload");
+  const Launch l = w.MakeLaunch(w.real, w.last_batch, w.stream);
+  LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  AdjointsInto(buf, w.last_batch, w.node_out.ptr);
+  LaunchSeedAdjoint(l, buf, grad_log_z != nullptr ? w.chain_seed.as<double>() : nullptr);
+  ReverseLevels(l, w.levels, buf);
+  LaunchBranchLengthChain(l, w.edge_out.ptr, w.edges.ptr, w.chain_rate.as<double>(), w.chain_terms.as<double>(),
+                          w.chain_grad.as<double>());
+  CheckLaunches();
+  Hip(hipMemcpyAsync(w.host_chain_grad.ptr, w.chain_grad.ptr, w.E() * sizeof(double), hipMemcpyDeviceToHost,
+                     w.stream),
+      "GPU runtime MemcpyAsync branch-length gradient download");
+  Record(w, 5);
+  WaitDownload(w);
+  *grad = w.host_chain_grad.as<double>();
+  *count = w.E();
 }
 
 void Workspace::LogPartitionsF64(const double **values, std::size_t *count) {
@@ -1036,6 +1119,7 @@ void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::si
   CheckDevice(w, batch, nodes, edges);
   if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, batch, s);
   LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
@@ -1054,6 +1138,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   if (log_z == nullptr || node_out == nullptr || (w.E() != 0 && edge_out == nullptr))
     throw std::invalid_argument("marginal output is null");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, batch, s);
   std::unique_ptr<Profiler> profiler;
@@ -1087,6 +1172,7 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   CheckDevice(w, batch, log_nodes, log_edges);
   if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);
   const Launch l = w.MakeLaunch(w.real, batch, s);
   LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
@@ -1096,13 +1182,14 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   LaunchFinish(l, buf, log_z);
   CheckLaunches();
   w.last_batch = batch;
-  w.forward_ready = true;
+  w.forward = Impl::Forward::kLogPotentials;
 }
 
 void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *grad_edges, bool reduce_edges,
                                    void *stream) {
   Impl &w = *impl_;
-  if (!w.forward_ready) throw std::logic_error("thmm_logz_backward_device requires a preceding forward call");
+  if (w.forward != Impl::Forward::kLogPotentials)
+    throw std::logic_error("thmm_logz_backward_device requires a preceding thmm_logz_forward_device call");
   RequireMarginals(w);
   if (grad_nodes == nullptr || (w.E() != 0 && grad_edges == nullptr))
     throw std::invalid_argument("gradient output is null");

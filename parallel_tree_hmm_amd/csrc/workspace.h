@@ -60,6 +60,10 @@ class Workspace {
                             const void **log_partitions, const void **nodes, const void **edges,
                             thmm_timings *timings);
   void LogPartitionsF64(const double **values, std::size_t *count);
+  void CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
+                              const double **log_z, std::size_t *count);
+  void BranchLengthBackward(const double *grad_log_z, const double *rate_matrix, const double **grad,
+                            std::size_t *count);
 
   void LogPartitionDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z, void *stream);
   // With `timings` non-null, every launch is bracketed by an event pair and

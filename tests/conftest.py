@@ -124,3 +124,55 @@ TOPOLOGIES = {
 }
 
 os.environ.setdefault("PYTHONHASHSEED", "0")
+
+
+# ---------------------------------------------------------------------------
+# Categorical phylogenetic-style model (config 3 shape)
+# ---------------------------------------------------------------------------
+
+
+def reversible_rate_matrix(rng, k: int) -> np.ndarray:
+    """Q = S Pi with S symmetric Exp(1) exchangeabilities and pi ~ Dirichlet(1),
+    rows summing to 0, scaled to unit mean rate (-sum_i pi_i Q_ii = 1)."""
+    s = rng.exponential(1.0, size=(k, k))
+    s = (s + s.T) / 2.0
+    np.fill_diagonal(s, 0.0)
+    pi = rng.dirichlet(np.ones(k))
+    q = s * pi[None, :]
+    np.fill_diagonal(q, -q.sum(axis=1))
+    return q / -np.sum(pi * np.diag(q))
+
+
+def transition_matrices(q: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """P_e = expm(Q t_e) in FP64 ([edge, parent, child])."""
+    from scipy.linalg import expm
+
+    return np.stack([expm(q * te) for te in t])
+
+
+def phylo_model(rng, depth: int, sites: int, k: int = 4, missing: float = 0.05):
+    """Balanced binary tree with 2**depth leaves (parent(i) = (i-1)//2), the
+    leaves observed per site with values in {0..k-1} and a fraction coded k
+    (missing: an all-ones emission row); root ~ Dirichlet(1); emission
+    [I_k; 1^T]; a reversible rate matrix; branch lengths t_e ~ Exp(mean 0.1),
+    t_e the length of the edge into node e + 1."""
+    n = 2 ** (depth + 1) - 1
+    parents = balanced_binary(n)
+    leaves = np.arange(n // 2, n, dtype=np.uint32)
+    obs = rng.integers(0, k, size=(sites, leaves.size)).astype(np.uint8)
+    obs[rng.random(obs.shape) < missing] = k
+    root = rng.dirichlet(np.ones(k))
+    emission = np.vstack([np.eye(k), np.ones((1, k))])
+    q = reversible_rate_matrix(rng, k)
+    t = rng.exponential(0.1, size=n - 1)
+    return parents, leaves, obs, root, emission, q, t
+
+
+def dense_categorical_nodes(n, leaves, obs, root, emission) -> np.ndarray:
+    """The dense node factors of a categorical model: (root at node 0, else 1)
+    times the emission row of the observed category at observed nodes."""
+    sites, k = obs.shape[0], root.size
+    nodes = np.ones((sites, n, k))
+    nodes[:, 0, :] = root
+    nodes[:, leaves, :] *= emission[obs]
+    return nodes
