@@ -68,6 +68,17 @@ __device__ inline T *SlotAdjoint(const LinearBuffers &buf, const Dims &d, std::u
                     : static_cast<T *>(buf.path_adj) + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * kk;
 }
 
+// Adjoint of a rake's branch message: the branch's own adjoint, or, when the
+// executor keeps no branch adjoints (every branch is absorbed into the rake's
+// parent and combination/absorption adjoints are copies), the parent's node
+// adjoint.
+template <typename T>
+__device__ inline const T *RakeAdjoint(const LinearBuffers &buf, const Dims &d, const DeviceRake &op, std::uint32_t b) {
+  if (buf.branch_adj != nullptr)
+    return static_cast<const T *>(buf.branch_adj) + (static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K;
+  return static_cast<const T *>(buf.node_adj) + (static_cast<std::uint64_t>(op.parent) * d.B + b) * d.K;
+}
+
 // Four consecutive elements, moved with 16-byte (float) or 2 x 16-byte
 // (double) vector memory instructions.
 template <typename T> struct Quad {

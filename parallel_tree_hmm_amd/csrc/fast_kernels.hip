@@ -111,7 +111,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   for (int i = 0; i < 4; ++i) e[i] = Exp(vq.v[i]);
   const T *P = Slot<T>(buf, d, op.slot, b);
   T *adj_p = SlotAdjoint<T>(buf, d, op.slot, b);
-  const T *adj_m = static_cast<const T *>(buf.branch_adj) + (static_cast<std::uint64_t>(op.branch) * d.B + b) * K;
+  const T *adj_m = RakeAdjoint<T>(buf, d, op, b);
   double column[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {

@@ -232,7 +232,7 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
     const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
     T acc = T(0);
     for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
-    const T a = static_cast<const T *>(buf.branch_adj)[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
+    const T a = RakeAdjoint<T>(buf, d, op, b)[lane.s];
     g = (a != T(0) && acc > T(0)) ? static_cast<double>(a) / static_cast<double>(acc) : 0.0;
     T *out = SlotAdjoint<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
     for (std::uint32_t c = 0; c < d.K; ++c) out[c] = static_cast<T>(g * static_cast<double>(row[c] * Exp(v[c])));

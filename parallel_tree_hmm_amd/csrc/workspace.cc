@@ -107,6 +107,11 @@ struct Workspace::Impl {
   hipStream_t stream = nullptr;
   hipEvent_t events[6] = {};
   std::vector<Level> levels;
+  // Every rake's branch ends up absorbed into the rake's parent (through
+  // combinations that only merge branches), so a branch adjoint equals the
+  // parent's node adjoint: the reverse pass skips the combination and
+  // absorption levels and rakes read node_adj[parent].
+  bool rake_adjoint_from_parent = false;
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
@@ -166,6 +171,7 @@ struct Workspace::Impl {
     last_batch = 0;
     forward = Forward::kNone;
     levels.clear();
+    rake_adjoint_from_parent = false;
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
@@ -282,6 +288,19 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   for (std::uint32_t r : plan.roots) is_root[r] = 1;
   Upload(w.is_root, is_root, w.stream, "topology upload");
   for (const thmm_primitive_batch &b : plan.batches) w.levels.push_back({b.primitive, b.offset, b.count});
+  {
+    std::vector<std::uint32_t> branch_parent(plan.num_branches, 0xFFFFFFFFu);
+    for (const thmm_branch_absorption &a : plan.absorptions) branch_parent[a.branch] = a.parent;
+    for (std::size_t i = plan.batches.size(); i-- > 0;) {
+      const thmm_primitive_batch &b = plan.batches[i];
+      if (b.primitive != THMM_BRANCH_COMBINATION) continue;
+      for (std::uint32_t j = b.offset + b.count; j-- > b.offset;)
+        branch_parent[plan.combinations[j].source] = branch_parent[plan.combinations[j].destination];
+    }
+    w.rake_adjoint_from_parent = true;
+    for (const thmm_rake &r : plan.rakes)
+      if (branch_parent[r.branch] != r.parent) w.rake_adjoint_from_parent = false;
+  }
 
   const std::size_t K = states, KK = K * K, N = plan.num_nodes, R = plan.num_branches, C = plan.num_compressions();
   const std::size_t S = RealSize(real);
@@ -338,7 +357,7 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   }
   if (flags & THMM_RESERVE_MARGINALS) {
     w.node_adj.Allocate(node_values * S, "node adjoints");
-    w.branch_adj.Allocate(branch_values * S, "branch adjoints");
+    if (!w.rake_adjoint_from_parent) w.branch_adj.Allocate(branch_values * S, "branch adjoints");
     w.path_adj.Allocate(path_values * S, "path adjoints");
     if (!resident) {
       w.node_out.Allocate(node_values * S, "node marginals");
@@ -458,9 +477,14 @@ void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const Line
   }
 }
 
+// Without branch adjoint storage (buf.branch_adj null) rakes take their
+// adjoint from the parent node and the pure-copy levels are skipped.
 void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf,
                    Profiler *profiler = nullptr) {
   for (std::size_t i = levels.size(); i-- > 0;) {
+    if (buf.branch_adj == nullptr &&
+        (levels[i].primitive == THMM_BRANCH_COMBINATION || levels[i].primitive == THMM_BRANCH_ABSORPTION))
+      continue;
     if (profiler) profiler->Begin(3, levels[i].primitive, levels[i].count);
     LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
     if (profiler) profiler->End();
