@@ -87,8 +87,11 @@ def concatenate(parts):
 # Algorithmic bytes per op (FP32 s = 4; per-item edge slots at batch 1).
 # Values only; the FP64 scale words (one per vector, K per matrix) are left
 # out. Forward: rake reads the leaf vector and the K x K slot and writes the
-# branch; combination/absorption read two vectors and write one; compression
-# reads L, R and the middle vector and writes the composed path. Reverse:
+# branch; combination/absorption read two vectors and write one; a fused
+# rake level (the rakes, combinations and absorptions of a round in one
+# launch) counts as its rakes, its partial-sum levels as the FP64 partials
+# they read; compression reads L, R and the middle vector and writes the
+# composed path. Reverse:
 # rake reads the slot, the leaf vector and the branch adjoint and writes the
 # slot adjoint (an edge marginal) and the leaf adjoint; combination and
 # absorption copy one adjoint vector; compression reads L, R, the middle
@@ -98,8 +101,9 @@ def concatenate(parts):
 
 def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
     kk = k * k
-    if phase == 1:  # forward
-        return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s}[primitive]
+    if phase == 1:  # forward (4: fused rake level, per rake; 5: partial sums, per FP64 partial read)
+        return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s, 4: (2 * k + kk) * s,
+                5: 8 * k}[primitive]
     if phase == 3:  # reverse
         return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s}[primitive]
     return 0
@@ -110,6 +114,8 @@ KERNEL_NAMES = {
     (1, 1): "CombineForward",
     (1, 2): "AbsorbForward",
     (1, 3): "CompressForward",
+    (1, 4): "RakeGather",
+    (1, 5): "GatherSegments",
     (3, 0): "RakeReverse",
     (3, 1): "CombineReverse",
     (3, 2): "AbsorbReverse",

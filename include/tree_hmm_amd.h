@@ -365,7 +365,10 @@ thmm_status thmm_synchronize(thmm_workspace *workspace);
 
 /* One launch of a profiled call. phase: 0 staging, 1 forward level,
  * 2 finish + adjoint seed, 3 reverse level, 4 output transpose. primitive
- * is a thmm_primitive for phases 1 and 3; count is the level's op count. */
+ * is a thmm_primitive for phases 1 and 3, or, in phase 1, 4 for a fused
+ * rake level (rakes, combinations and absorptions of a round in one launch;
+ * count = rakes) and 5 for its partial-absorption level (count = partials);
+ * otherwise count is the level's op count. */
 typedef struct thmm_launch_timing {
   uint32_t phase, primitive, count, batch;
   double ms;

@@ -9,14 +9,16 @@
 // step). Row sums reduce over the LPR lanes of a row, column sums over the
 // lanes that share columns.
 //
-// MFMA compression (FP32, K = 16): one wavefront per item; L and R are
-// staged through LDS with coalesced 16-byte loads and the 16 x 16 x 16
-// products run on v_mfma_f32_16x16x4_f32 (four k-steps). Operand layouts:
-// A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i%4
-// of lane j + 16(i/4). Every MFMA operand lies in [0, 1]: L and R are
-// row-normalised, the middle weights are exp(w - max w), and in the reverse
-// pass the ratio matrix adj/S is row-normalised with its row scales (and
-// their maximum) kept in FP64.
+// Fused rakes (K in {8, 16, 32}): the same team walks a chunk of rakes that
+// share a parent and folds their summed messages into the parent, replacing
+// the combination and absorption levels of the sum-product forward pass.
+//
+// MFMA compression (FP32, K = 16): two items per wavefront, operands loaded
+// straight from HBM in the layouts v_mfma_f32_16x16x4_f32 consumes (see the
+// section below); the reverse pass transposes one 16 x 16 tile through LDS.
+// Every MFMA operand lies in [0, 1]: L and R are row-normalised, the middle
+// weights are exp(w - max w), and in the reverse pass the ratio matrix adj/S
+// is row-normalised with its row scales (and their maximum) kept in FP64.
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -144,6 +146,101 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
     for (int i = 0; i < 4; ++i) q.v[i] = static_cast<T>(column[i]);
     StoreQuad(static_cast<T *>(buf.node_adj) + leaf_at * K + c0, q);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Fused rake + combination + absorption
+// ---------------------------------------------------------------------------
+//
+// Combining branches multiplies messages and absorbing multiplies the product
+// into the parent (src/inference.cc:173-189, 725-747): in logs, the parent's
+// new vector is its old one plus the sum of its raked children's messages. A
+// team walks one chunk of rakes that share a parent, accumulates their
+// absolute log messages in FP64 (lane-owned rows, as in RakeForwardTeam) and
+// either folds the sum into the parent or leaves it as a partial.
+
+// Adds the FP64 absolute log vector `acc` (rows owned by lanes with row >= 0)
+// into node vector `at` and re-shifts it by its maximum.
+template <typename T, int K, int kT>
+__device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int row, double acc, int lane) {
+  T *x = static_cast<T *>(buf.nodes) + at * K;
+  const double scale = buf.node_scale[at];
+  const double y = row >= 0 ? static_cast<double>(x[row]) + scale + acc : kNegInf;
+  double top = y;
+#pragma unroll
+  for (int o = kT / 2; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kT));
+  const bool live = top > kNegInf;
+  if (row >= 0) x[row] = live ? static_cast<T>(y - top) : NegInf<T>();
+  if (lane == 0) buf.node_scale[at] = live ? top : kNegInf;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                      LinearBuffers buf) {
+  using TM = Team<K>;
+  constexpr int kT = TM::kLanes, kLpr = TM::kLanesPerRow, kRps = TM::kRowsPerStep, kJ = TM::kSteps;
+  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const std::uint32_t item = tid / kT;
+  const int l = static_cast<int>(tid % kT);
+  if (item >= count * d.B) return;
+  const std::uint32_t b = item % d.B;
+  const DeviceGatherChunk chunk = g.chunks[offset + item / d.B];
+  int owned = -1;
+#pragma unroll
+  for (int j = 0; j < kJ; ++j)
+    if (j % kLpr == l % kLpr) owned = j;
+  const int row = owned >= 0 ? owned * kRps + l / kLpr : -1;
+  const int c0 = (l % kLpr) * 4;
+  double acc = 0.0;
+  DeviceRake op = g.rakes[chunk.first];
+  for (std::uint32_t i = 0; i < chunk.count; ++i) {
+    const DeviceRake next = g.rakes[chunk.first + (i + 1 < chunk.count ? i + 1 : i)];
+    const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
+    const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
+    const T *P = Slot<T>(buf, d, op.slot, b);
+    Quad<T> pq[kJ];
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) pq[j] = LoadQuad(P + static_cast<std::uint32_t>(j * kT + l) * 4);
+    const double leaf_scale = buf.node_scale[leaf_at];
+    const double row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
+    T e[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) e[k] = Exp(vq.v[k]);
+    T mine = T(0);
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+      T part = pq[j].v[0] * e[0] + pq[j].v[1] * e[1] + pq[j].v[2] * e[2] + pq[j].v[3] * e[3];
+#pragma unroll
+      for (int o = kLpr / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kT);
+      if (j == owned) mine = part;
+    }
+    acc += (row >= 0 && mine > T(0)) ? leaf_scale + row_scale + log(static_cast<double>(mine)) : kNegInf;
+    op = next;
+  }
+  if (chunk.partial == kDirectChunk)
+    AbsorbSum<T, K, kT>(buf, static_cast<std::uint64_t>(chunk.parent) * d.B + b, row, acc, l);
+  else if (row >= 0)
+    g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + row] = acc;
+}
+
+// Sums a run of partials (K lanes, lane = row) into a partial of the next
+// reduction level or, at the last level, into the parent.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, std::uint32_t offset,
+                                                          std::uint32_t count, LinearBuffers buf) {
+  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const std::uint32_t item = tid / K;
+  const int row = static_cast<int>(tid % K);
+  if (item >= count * d.B) return;
+  const std::uint32_t b = item % d.B;
+  const DeviceGatherSegment seg = g.segments[offset + item / d.B];
+  double acc = 0.0;
+  for (std::uint32_t i = 0; i < seg.count; ++i)
+    acc += g.partials[(static_cast<std::uint64_t>(seg.first_partial + i) * d.B + b) * K + row];
+  if (seg.out == kDirectChunk)
+    AbsorbSum<T, K, K>(buf, static_cast<std::uint64_t>(seg.parent) * d.B + b, row, acc, row);
+  else
+    g.partials[(static_cast<std::uint64_t>(seg.out) * d.B + b) * K + row] = acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,6 +536,48 @@ template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, 
 }
 
 }  // namespace
+
+bool FusedForwardSupported(std::uint32_t states) { return states == 8 || states == 16 || states == 32; }
+
+void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
+                        const LinearBuffers &buf) {
+  if (count == 0) return;
+  const std::uint32_t K = l.dims.K;
+  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * (K == 32 ? 64 : 16));
+  const hipStream_t s = StreamOf(l);
+#define THMM_GATHER(T, KK) RakeGatherTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf)
+  if (l.real == Real::kF32) {
+    if (K == 8) THMM_GATHER(float, 8);
+    if (K == 16) THMM_GATHER(float, 16);
+    if (K == 32) THMM_GATHER(float, 32);
+  } else {
+    if (K == 8) THMM_GATHER(double, 8);
+    if (K == 16) THMM_GATHER(double, 16);
+    if (K == 32) THMM_GATHER(double, 32);
+  }
+#undef THMM_GATHER
+  RecordLaunchError(hipGetLastError(), "fused rake level launch");
+}
+
+void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
+                          const LinearBuffers &buf) {
+  if (count == 0) return;
+  const std::uint32_t K = l.dims.K;
+  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * K);
+  const hipStream_t s = StreamOf(l);
+#define THMM_SEGMENTS(T, KK) GatherSegmentsTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf)
+  if (l.real == Real::kF32) {
+    if (K == 8) THMM_SEGMENTS(float, 8);
+    if (K == 16) THMM_SEGMENTS(float, 16);
+    if (K == 32) THMM_SEGMENTS(float, 32);
+  } else {
+    if (K == 8) THMM_SEGMENTS(double, 8);
+    if (K == 16) THMM_SEGMENTS(double, 16);
+    if (K == 32) THMM_SEGMENTS(double, 32);
+  }
+#undef THMM_SEGMENTS
+  RecordLaunchError(hipGetLastError(), "partial absorption launch");
+}
 
 bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std::uint32_t offset,
                      std::uint32_t count, const LinearBuffers &buf) {

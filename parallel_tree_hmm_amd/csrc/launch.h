@@ -112,6 +112,33 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw,
                            const std::uint8_t *is_root, const void *root_potential,
                            const void *emissions, void *nodes, double *node_scale);
 
+// ---- fused rake + combination + absorption (sum-product forward) ----
+// The rakes of one round, grouped by the absorption their branch reaches
+// (the parent), in chunks of at most a few rakes: a chunk's messages are
+// summed in FP64 and either absorbed into the parent at once (partial ==
+// kDirectChunk) or written to partials[partial][b][K] for a segment pass.
+constexpr std::uint32_t kDirectChunk = 0xFFFFFFFFu;
+struct DeviceGatherChunk {
+  std::uint32_t first, count, parent, partial;
+};
+// Sums `count` consecutive partials into partial `out`, or, for out ==
+// kDirectChunk, absorbs the sum into `parent`.
+struct DeviceGatherSegment {
+  std::uint32_t parent, first_partial, count, out;
+};
+struct GatherOps {
+  const DeviceRake *rakes;  // sorted by target absorption within a round
+  const DeviceGatherChunk *chunks;
+  const DeviceGatherSegment *segments;
+  double *partials;
+};
+// True when the fused kernels cover this state count and element type.
+bool FusedForwardSupported(std::uint32_t states);
+void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
+                        const LinearBuffers &buf);
+void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
+                          const LinearBuffers &buf);
+
 // ---- scaled forward (log Z) ----
 void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
                         std::uint32_t count, const LinearBuffers &buf);

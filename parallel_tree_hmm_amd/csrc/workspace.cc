@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -112,6 +113,14 @@ struct Workspace::Impl {
   // parent's node adjoint: the reverse pass skips the combination and
   // absorption levels and rakes read node_adj[parent].
   bool rake_adjoint_from_parent = false;
+  // Sum-product forward program with fused rake/combination/absorption
+  // levels (empty when the plan or state count does not allow it).
+  struct FusedStep {
+    std::uint32_t kind, offset, count;  // kind: 0 chunks, 1 segments, 3 compressions
+    std::uint32_t work;                 // rakes (kind 0), partials (kind 1), compressions (kind 3)
+  };
+  std::vector<FusedStep> fused;
+  DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
@@ -147,7 +156,7 @@ struct Workspace::Impl {
                             &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
-                            &chain_grad})
+                            &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
@@ -172,6 +181,7 @@ struct Workspace::Impl {
     forward = Forward::kNone;
     levels.clear();
     rake_adjoint_from_parent = false;
+    fused.clear();
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
@@ -182,6 +192,11 @@ struct Workspace::Impl {
   std::size_t Roots() const { return plan->roots.size(); }
   std::size_t KK() const { return states * states; }
   std::size_t S() const { return RealSize(real); }
+
+  GatherOps Gather() const {
+    return {gather_rakes.as<DeviceRake>(), gather_chunks.as<DeviceGatherChunk>(),
+            gather_segments.as<DeviceGatherSegment>(), gather_partials.as<double>()};
+  }
 
   Launch MakeLaunch(Real r, std::size_t batch, hipStream_t s) const {
     Launch l{};
@@ -221,6 +236,114 @@ using Impl = Workspace::Impl;
 template <typename V> void Upload(DeviceBuffer &dst, const std::vector<V> &src, hipStream_t stream, const char *what) {
   dst.Allocate(src.size() * sizeof(V), what);
   if (!src.empty()) Hip(hipMemcpyAsync(dst.ptr, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, stream), what);
+}
+
+// Rakes per chunk of the fused forward (a team walks the chunk serially) and
+// partials per team of a partial-reduction level.
+constexpr std::uint32_t kGatherChunk = 16;
+constexpr std::uint32_t kGatherFanIn = 32;
+
+// Groups the plan's batches into rounds of non-decreasing primitive kind
+// (rakes, combinations, absorptions, compressions) and, when every rake's
+// branch is absorbed in its own round, lays out the fused forward program:
+// per round one chunk level over the rakes sorted by absorption, then for
+// parents with more than kGatherChunk raked children levels of partial sums
+// of fan-in kGatherFanIn until one sum per parent is absorbed, then the
+// round's compressions. Leaves w.fused empty otherwise.
+void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> &device_rakes, std::size_t batch) {
+  std::vector<std::uint32_t> branch_absorb(plan.num_branches, 0xFFFFFFFFu);
+  for (std::size_t a = 0; a < plan.absorptions.size(); ++a)
+    branch_absorb[plan.absorptions[a].branch] = static_cast<std::uint32_t>(a);
+  for (std::size_t i = plan.batches.size(); i-- > 0;) {
+    const thmm_primitive_batch &b = plan.batches[i];
+    if (b.primitive != THMM_BRANCH_COMBINATION) continue;
+    for (std::uint32_t j = b.offset + b.count; j-- > b.offset;)
+      branch_absorb[plan.combinations[j].source] = branch_absorb[plan.combinations[j].destination];
+  }
+  std::vector<std::uint32_t> absorb_round(plan.absorptions.size(), 0xFFFFFFFFu);
+  std::vector<std::pair<std::size_t, std::size_t>> rounds;  // batch ranges
+  for (std::size_t i = 0; i < plan.batches.size(); ++i) {
+    if (i == 0 || plan.batches[i].primitive < plan.batches[i - 1].primitive) rounds.push_back({i, i});
+    rounds.back().second = i + 1;
+  }
+  for (std::size_t r = 0; r < rounds.size(); ++r)
+    for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i)
+      if (plan.batches[i].primitive == THMM_BRANCH_ABSORPTION)
+        for (std::uint32_t j = plan.batches[i].offset; j < plan.batches[i].offset + plan.batches[i].count; ++j)
+          absorb_round[j] = static_cast<std::uint32_t>(r);
+
+  std::vector<Impl::FusedStep> steps;
+  std::vector<DeviceRake> sorted;
+  std::vector<DeviceGatherChunk> chunks;
+  std::vector<DeviceGatherSegment> segments;
+  std::uint32_t partials = 0;
+  for (std::size_t r = 0; r < rounds.size(); ++r) {
+    std::vector<std::pair<std::uint32_t, std::uint32_t>> keyed;  // (absorption, rake)
+    for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i) {
+      const thmm_primitive_batch &b = plan.batches[i];
+      if (b.primitive != THMM_RAKE) continue;
+      for (std::uint32_t j = b.offset; j < b.offset + b.count; ++j) {
+        const std::uint32_t a = branch_absorb[plan.rakes[j].branch];
+        if (a == 0xFFFFFFFFu || absorb_round[a] != r || plan.absorptions[a].parent != plan.rakes[j].parent) return;
+        keyed.push_back({a, j});
+      }
+    }
+    std::stable_sort(keyed.begin(), keyed.end(),
+                     [](const auto &x, const auto &y) { return x.first < y.first; });
+    const std::uint32_t chunk_offset = static_cast<std::uint32_t>(chunks.size());
+    // Parents whose chunks leave partials: (parent, first partial, count).
+    std::vector<std::array<std::uint32_t, 3>> open;
+    for (std::size_t i = 0; i < keyed.size();) {
+      std::size_t end = i;
+      while (end < keyed.size() && keyed[end].first == keyed[i].first) ++end;
+      const std::uint32_t parent = plan.absorptions[keyed[i].first].parent;
+      const bool direct = end - i <= kGatherChunk;
+      const std::uint32_t first_partial = partials;
+      for (std::size_t c = i; c < end; c += kGatherChunk) {
+        const std::uint32_t count = static_cast<std::uint32_t>(std::min<std::size_t>(kGatherChunk, end - c));
+        chunks.push_back({static_cast<std::uint32_t>(sorted.size()), count, parent, direct ? kDirectChunk : partials++});
+        for (std::size_t k = c; k < c + count; ++k) sorted.push_back(device_rakes[keyed[k].second]);
+      }
+      if (!direct) open.push_back({parent, first_partial, partials - first_partial});
+      i = end;
+    }
+    if (chunks.size() > chunk_offset)
+      steps.push_back({0, chunk_offset, static_cast<std::uint32_t>(chunks.size() - chunk_offset),
+                       static_cast<std::uint32_t>(keyed.size())});
+    // Reduction levels: runs of up to kGatherFanIn partials per team; a
+    // parent whose partials fit one run is absorbed at that level.
+    while (!open.empty()) {
+      const std::uint32_t segment_offset = static_cast<std::uint32_t>(segments.size());
+      std::uint32_t parts = 0;
+      std::vector<std::array<std::uint32_t, 3>> next;
+      for (const auto &[parent, first, count] : open) {
+        parts += count;
+        if (count <= kGatherFanIn) {
+          segments.push_back({parent, first, count, kDirectChunk});
+          continue;
+        }
+        const std::uint32_t next_first = partials;
+        for (std::uint32_t c = 0; c < count; c += kGatherFanIn)
+          segments.push_back({parent, first + c, std::min(kGatherFanIn, count - c), partials++});
+        next.push_back({parent, next_first, partials - next_first});
+      }
+      steps.push_back({1, segment_offset, static_cast<std::uint32_t>(segments.size() - segment_offset), parts});
+      open = std::move(next);
+    }
+    for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i)
+      if (plan.batches[i].primitive == THMM_COMPRESSION)
+        steps.push_back({3, plan.batches[i].offset, plan.batches[i].count, plan.batches[i].count});
+  }
+  if (sorted.size() != plan.rakes.size()) return;
+  // The fused kernels index (item, lane) in 32 bits.
+  for (const Impl::FusedStep &step : steps)
+    if (static_cast<std::uint64_t>(step.count) * batch * 64 >= (1ull << 32)) return;
+  Upload(w.gather_rakes, sorted, w.stream, "fused topology upload");
+  Upload(w.gather_chunks, chunks, w.stream, "fused topology upload");
+  Upload(w.gather_segments, segments, w.stream, "fused topology upload");
+  w.gather_partials.Allocate(Product({partials, batch, w.states, sizeof(double)}, "partial messages"),
+                             "partial messages");
+  w.fused = std::move(steps);
 }
 
 void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
@@ -301,6 +424,8 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
     for (const thmm_rake &r : plan.rakes)
       if (branch_parent[r.branch] != r.parent) w.rake_adjoint_from_parent = false;
   }
+  if (w.rake_adjoint_from_parent && FusedForwardSupported(static_cast<std::uint32_t>(states)))
+    BuildFusedForward(w, plan, rakes, batch);
 
   const std::size_t K = states, KK = K * K, N = plan.num_nodes, R = plan.num_branches, C = plan.num_compressions();
   const std::size_t S = RealSize(real);
@@ -477,6 +602,28 @@ void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const Line
   }
 }
 
+// The sum-product forward pass: the fused program when the workspace has
+// one, else the plan's levels.
+void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
+  if (w.fused.empty()) {
+    ForwardLevels(l, w.levels, buf, profiler);
+    return;
+  }
+  const GatherOps g = w.Gather();
+  for (const Impl::FusedStep &step : w.fused) {
+    // Profiled as primitive 4 (fused rake chunks, count = rakes), 5 (partial
+    // absorptions, count = partials) or a compression level.
+    if (profiler) profiler->Begin(1, step.kind == 3 ? THMM_COMPRESSION : 4 + step.kind, step.work);
+    if (step.kind == 0)
+      LaunchGatherChunks(l, g, step.offset, step.count, buf);
+    else if (step.kind == 1)
+      LaunchGatherSegments(l, g, step.offset, step.count, buf);
+    else
+      LaunchForwardLevel(l, THMM_COMPRESSION, step.offset, step.count, buf);
+    if (profiler) profiler->End();
+  }
+}
+
 // Without branch adjoint storage (buf.branch_adj null) rakes take their
 // adjoint from the parent node and the pure-copy levels are skipped.
 void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf,
@@ -608,7 +755,7 @@ void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
   InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
   const LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
-  ForwardLevels(l, w.levels, buf);
+  SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   CheckLaunches();
 }
@@ -618,7 +765,7 @@ void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
   LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
   AdjointsInto(buf, batch, w.node_out.ptr);
-  ForwardLevels(l, w.levels, buf);
+  SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
   ReverseLevels(l, w.levels, buf);
@@ -1148,7 +1295,7 @@ void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::si
   const Launch l = w.MakeLaunch(w.real, batch, s);
   LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
   const LinearBuffers buf = w.Linear(edges, nullptr);
-  ForwardLevels(l, w.levels, buf);
+  SumProductForward(w, l, buf);
   LaunchFinish(l, buf, log_z);
   CheckLaunches();
 }
@@ -1174,7 +1321,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   LinearBuffers buf = w.Linear(edges, nullptr);
   buf.edge_out = edge_out;
   AdjointsInto(buf, batch, node_out);
-  ForwardLevels(l, w.levels, buf, p);
+  SumProductForward(w, l, buf, p);
   if (p) p->Begin(2, 0, 0);
   LaunchFinish(l, buf, log_z);
   LaunchSeedAdjoint(l, buf, nullptr);
@@ -1202,7 +1349,7 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
   LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
   const LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-  ForwardLevels(l, w.levels, buf);
+  SumProductForward(w, l, buf);
   LaunchFinish(l, buf, log_z);
   CheckLaunches();
   w.last_batch = batch;
