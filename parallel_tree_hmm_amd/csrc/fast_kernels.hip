@@ -1,7 +1,7 @@
 // Coalesced and MFMA kernels for the hot levels (see kernels.hip for the
 // number representation and the generic forms these replace).
 //
-// Team layout (rakes, K in {8, 16, 32}): a team of T lanes owns one item.
+// Team layout (rakes, K in {8, 16, 32, 64}): a team of T lanes owns one item.
 // The item's K x K matrix is read and written as row-major 16-byte quads:
 // in step j lane l moves quad j*T + l, i.e. row j*RPS + l/LPR, columns
 // (l%LPR)*4 .. +3, so every memory instruction of a team touches one
@@ -9,7 +9,7 @@
 // step). Row sums reduce over the LPR lanes of a row, column sums over the
 // lanes that share columns.
 //
-// Fused rakes (K in {8, 16, 32}): the same team walks a chunk of rakes that
+// Fused rakes (K in {8, 16, 32, 64}): the same team walks a chunk of rakes that
 // share a parent and folds their summed messages into the parent, replacing
 // the combination and absorption levels of the sum-product forward pass.
 //
@@ -37,11 +37,12 @@ template <int K> struct Team {
   static constexpr int kRowsPerStep = 4 * kLanes / K;
   static constexpr int kSteps = K / kRowsPerStep;
 };
-static_assert(Team<8>::kSteps == 1 && Team<16>::kSteps == 4 && Team<32>::kSteps == 4, "team tiling");
+static_assert(Team<8>::kSteps == 1 && Team<16>::kSteps == 4 && Team<32>::kSteps == 4 && Team<64>::kSteps == 16,
+              "team tiling");
 // One logarithm per lane in RakeForwardTeam needs at most one step per lane
 // of a row.
 static_assert(Team<8>::kSteps <= Team<8>::kLanesPerRow && Team<16>::kSteps <= Team<16>::kLanesPerRow &&
-                  Team<32>::kSteps <= Team<32>::kLanesPerRow,
+                  Team<32>::kSteps <= Team<32>::kLanesPerRow && Team<64>::kSteps <= Team<64>::kLanesPerRow,
               "row ownership");
 
 // branch[p] = log sum_c path[p][c] exp(leaf[c]) (src/inference.cc:704-723)
@@ -537,23 +538,27 @@ template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, 
 
 }  // namespace
 
-bool FusedForwardSupported(std::uint32_t states) { return states == 8 || states == 16 || states == 32; }
+bool FusedForwardSupported(std::uint32_t states) {
+  return states == 8 || states == 16 || states == 32 || states == 64;
+}
 
 void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
   if (count == 0) return;
   const std::uint32_t K = l.dims.K;
-  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * (K == 32 ? 64 : 16));
+  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * (K >= 32 ? 64 : 16));
   const hipStream_t s = StreamOf(l);
 #define THMM_GATHER(T, KK) RakeGatherTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf)
   if (l.real == Real::kF32) {
     if (K == 8) THMM_GATHER(float, 8);
     if (K == 16) THMM_GATHER(float, 16);
     if (K == 32) THMM_GATHER(float, 32);
+    if (K == 64) THMM_GATHER(float, 64);
   } else {
     if (K == 8) THMM_GATHER(double, 8);
     if (K == 16) THMM_GATHER(double, 16);
     if (K == 32) THMM_GATHER(double, 32);
+    if (K == 64) THMM_GATHER(double, 64);
   }
 #undef THMM_GATHER
   RecordLaunchError(hipGetLastError(), "fused rake level launch");
@@ -570,10 +575,12 @@ void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t off
     if (K == 8) THMM_SEGMENTS(float, 8);
     if (K == 16) THMM_SEGMENTS(float, 16);
     if (K == 32) THMM_SEGMENTS(float, 32);
+    if (K == 64) THMM_SEGMENTS(float, 64);
   } else {
     if (K == 8) THMM_SEGMENTS(double, 8);
     if (K == 16) THMM_SEGMENTS(double, 16);
     if (K == 32) THMM_SEGMENTS(double, 32);
+    if (K == 64) THMM_SEGMENTS(double, 64);
   }
 #undef THMM_SEGMENTS
   RecordLaunchError(hipGetLastError(), "partial absorption launch");
@@ -586,15 +593,17 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
   // Caller-owned edge inputs and outputs must allow 16-byte accesses.
   if (!Aligned16(buf.edges) || !Aligned16(buf.edge_out) || !Aligned16(buf.node_adj)) return false;
   if (static_cast<std::uint64_t>(count) * l.dims.B * 64 >= (1ull << 32)) return false;
-  if (primitive == 0 && (K == 8 || K == 16 || K == 32)) {
+  if (primitive == 0 && (K == 8 || K == 16 || K == 32 || K == 64)) {
     if (l.real == Real::kF32) {
       if (K == 8) LaunchRakeTeam<float, 8>(l, reverse, offset, count, buf);
       if (K == 16) LaunchRakeTeam<float, 16>(l, reverse, offset, count, buf);
       if (K == 32) LaunchRakeTeam<float, 32>(l, reverse, offset, count, buf);
+      if (K == 64) LaunchRakeTeam<float, 64>(l, reverse, offset, count, buf);
     } else {
       if (K == 8) LaunchRakeTeam<double, 8>(l, reverse, offset, count, buf);
       if (K == 16) LaunchRakeTeam<double, 16>(l, reverse, offset, count, buf);
       if (K == 32) LaunchRakeTeam<double, 32>(l, reverse, offset, count, buf);
+      if (K == 64) LaunchRakeTeam<double, 64>(l, reverse, offset, count, buf);
     }
     RecordLaunchError(hipGetLastError(), "rake level launch");
     return true;
