@@ -13,9 +13,10 @@
 // share a parent and folds their summed messages into the parent, replacing
 // the combination and absorption levels of the sum-product forward pass.
 //
-// MFMA compression (FP32, K = 16): two items per wavefront, operands loaded
-// straight from HBM in the layouts v_mfma_f32_16x16x4_f32 consumes (see the
-// section below); the reverse pass transposes one 16 x 16 tile through LDS.
+// MFMA compression (FP32, K = 16, 32, 64): operands loaded straight from HBM
+// in the layouts v_mfma_f32_16x16x4_f32 consumes (see the sections below;
+// two items per wavefront at K = 16, one at K = 32 and 64); the reverse pass
+// transposes one K x K tile through LDS.
 // Every MFMA operand lies in [0, 1]: L and R are row-normalised, the middle
 // weights are exp(w - max w), and in the reverse pass the ratio matrix adj/S
 // is row-normalised with its row scales (and their maximum) kept in FP64.
@@ -475,6 +476,267 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
 }
 
 // ---------------------------------------------------------------------------
+// FP32 MFMA compression, K = 32 and 64 (one wavefront per item)
+// ---------------------------------------------------------------------------
+//
+// With NB = K/16 blocks, lane l (i = l%16, g = l/16) holds a K x K matrix X
+//   in row layout:    X[16I + i][16J + 4g + r]   (NB*NB 16-byte loads),
+//   in column layout: X[16J + 4g + r][16I + i]   (the row layout of X^T).
+// Taking step s = (J, r) of the k loop as k = 16J + 4g + r, the MFMA tile
+// (Ia, Ib) over two row-layout operands Y, Z accumulates (Y Z^T) with
+// D[4g + r][i] = (Y Z^T)[16Ia + 4g + r][16Ib + i]: output row 16Ib + i in
+// lane i, columns 16Ia + 4g + r, i.e. the row layout of the product's
+// transpose. Hence
+//   S^T = R^T (L diag e)^T  from R in column and L in row layout,
+//   H^T = R Gh^T            from R and Gh in row layout,
+//   Q^T = Gt^T L            from Gt and L in column layout,
+// and S, H, Q come out in row layout: rows reduce over 4 lanes, stores are
+// 16-byte quads. Gh/Gt pass through one LDS tile per wavefront.
+
+// Middle weights e[m] for m = 16J + 4g + r; returns W (-inf: no live state).
+template <int K>
+__device__ inline double MiddleWeightsWide(const LinearBuffers &buf, const Dims &d, const DeviceCompress &op,
+                                           std::uint32_t b, int g, float (&e)[K / 16][4]) {
+  constexpr int NB = K / 16;
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+  double w[NB][4];
+  double top = kNegInf;
+#pragma unroll
+  for (int J = 0; J < NB; ++J) {
+    const Quad<float> m = LoadQuad(static_cast<const float *>(buf.nodes) + mid_at * K + 16 * J + 4 * g);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      w[J][r] = static_cast<double>(m.v[r]) + RowScale(buf, d, op.right, b, 16 * J + 4 * g + r);
+      top = fmax(top, w[J][r]);
+    }
+  }
+  top = fmax(top, __shfl_xor(top, 16, 64));
+  top = fmax(top, __shfl_xor(top, 32, 64));
+#pragma unroll
+  for (int J = 0; J < NB; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) e[J][r] = top > kNegInf ? expf(static_cast<float>(w[J][r] - top)) : 0.0f;
+  return top;
+}
+
+template <int K>
+__device__ inline void LoadRowBand(const float *X, int I, int i, int g, float (&out)[K / 16][4]) {
+#pragma unroll
+  for (int J = 0; J < K / 16; ++J) {
+    const Quad<float> q = LoadQuad(X + (16 * I + i) * K + 16 * J + 4 * g);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[J][r] = q.v[r];
+  }
+}
+
+template <int K>
+__device__ inline void LoadColumnBand(const float *X, int I, int i, int g, float (&out)[K / 16][4]) {
+#pragma unroll
+  for (int J = 0; J < K / 16; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) out[J][r] = X[(16 * J + 4 * g + r) * K + 16 * I + i];
+}
+
+template <int K>
+__device__ inline floatx4 TileProduct(const float (&y)[K / 16][4], const float (&z)[K / 16][4]) {
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int J = 0; J < K / 16; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(y[J][r], z[J][r], acc, 0, 0, 0);
+  return acc;
+}
+
+constexpr int kWideWaves = 2;  // wavefronts (items) per block
+
+template <int K>
+__global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                                           std::uint32_t count, LinearBuffers buf) {
+  constexpr int NB = K / 16;
+  const int lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
+  const std::uint32_t item = blockIdx.x * kWideWaves + threadIdx.x / 64;
+  if (item >= count * d.B) return;
+  const std::uint32_t b = item % d.B;
+  const DeviceCompress op = ops.compresses[offset + item / d.B];
+  const float *L = Slot<float>(buf, d, op.left, b);
+  const float *R = Slot<float>(buf, d, op.right, b);
+  float e[NB][4];
+  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, g, e);
+  float rc[NB][NB][4];  // R column layout: rc[Ic] is the band Ic of R^T
+#pragma unroll
+  for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K>(R, Ic, i, g, rc[Ic]);
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+  const double mid_scale = buf.node_scale[mid_at];
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  float *out = static_cast<float *>(buf.paths) + out_at * K * K;
+#pragma unroll 1
+  for (int Ip = 0; Ip < NB; ++Ip) {
+    float lb[NB][4];
+    LoadRowBand<K>(L, Ip, i, g, lb);
+    const double left_scale = RowScale(buf, d, op.left, b, 16 * Ip + i);
+#pragma unroll
+    for (int J = 0; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) lb[J][r] *= e[J][r];
+    floatx4 s[NB];
+    float top = 0.0f;
+#pragma unroll
+    for (int Ic = 0; Ic < NB; ++Ic) {
+      s[Ic] = TileProduct<K>(rc[Ic], lb);  // S[16Ip + i][16Ic + 4g + r]
+      top = fmaxf(top, fmaxf(fmaxf(s[Ic][0], s[Ic][1]), fmaxf(s[Ic][2], s[Ic][3])));
+    }
+    top = fmaxf(top, __shfl_xor(top, 16, 64));
+    top = fmaxf(top, __shfl_xor(top, 32, 64));
+    const bool live = top > 0.0f && w_top > kNegInf;
+    const float inverse = live ? 1.0f / top : 0.0f;
+#pragma unroll
+    for (int Ic = 0; Ic < NB; ++Ic) {
+      Quad<float> q;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q.v[r] = s[Ic][r] * inverse;
+      StoreQuad(out + (16 * Ip + i) * K + 16 * Ic + 4 * g, q);
+    }
+    if (g == 0)
+      buf.path_scale[out_at * K + 16 * Ip + i] =
+          live ? left_scale + mid_scale + w_top + log(static_cast<double>(top)) : kNegInf;
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                                           std::uint32_t count, LinearBuffers buf) {
+  constexpr int NB = K / 16;
+  constexpr int kStride = K + 1;  // LDS row stride (floats)
+  __shared__ float tiles[kWideWaves][K * kStride];
+  __shared__ double row_ratio[kWideWaves][K];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
+  const std::uint32_t item = blockIdx.x * kWideWaves + wave;
+  if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
+  float *G = tiles[wave];
+  double *gs = row_ratio[wave];
+  const std::uint32_t b = item % d.B;
+  const DeviceCompress op = ops.compresses[offset + item / d.B];
+  const float *L = Slot<float>(buf, d, op.left, b);
+  const float *R = Slot<float>(buf, d, op.right, b);
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  const float *adj_out = static_cast<const float *>(buf.path_adj) + out_at * K * K;
+  float e[NB][4];
+  MiddleWeightsWide<K>(buf, d, op, b, g, e);
+
+  // S (recomputed), Gr = adj/S, g_p, Gh = Gr/g_p -> LDS; Gamma = max_p g_p.
+  double gamma = 0.0;
+  {
+    float rc[NB][NB][4];
+#pragma unroll
+    for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K>(R, Ic, i, g, rc[Ic]);
+#pragma unroll 1
+    for (int Ip = 0; Ip < NB; ++Ip) {
+      float lb[NB][4], ab[NB][4];
+      LoadRowBand<K>(L, Ip, i, g, lb);
+      LoadRowBand<K>(adj_out, Ip, i, g, ab);
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lb[J][r] *= e[J][r];
+      double gr[NB][4];
+      double top = 0.0;
+#pragma unroll
+      for (int Ic = 0; Ic < NB; ++Ic) {
+        const floatx4 s = TileProduct<K>(rc[Ic], lb);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          gr[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f) ? static_cast<double>(ab[Ic][r]) / static_cast<double>(s[r]) : 0.0;
+          top = fmax(top, gr[Ic][r]);
+        }
+      }
+      top = fmax(top, __shfl_xor(top, 16, 64));
+      top = fmax(top, __shfl_xor(top, 32, 64));
+      gamma = fmax(gamma, top);
+#pragma unroll
+      for (int Ic = 0; Ic < NB; ++Ic)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] = top > 0.0 ? static_cast<float>(gr[Ic][r] / top) : 0.0f;
+      if (g == 0) gs[16 * Ip + i] = top;
+    }
+  }
+#pragma unroll
+  for (int x = 1; x < 16; x <<= 1) gamma = fmax(gamma, __shfl_xor(gamma, x, 64));
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+  float rr[NB][NB][4];  // R row layout: rr[Im] is the band Im of R
+#pragma unroll
+  for (int Im = 0; Im < NB; ++Im) LoadRowBand<K>(R, Im, i, g, rr[Im]);
+
+  // adj_L[p][m] = L'[p][m] g_p H[p][m], H^T = R Gh^T.
+  float *adj_left = SlotAdjoint<float>(buf, d, op.left, b);
+#pragma unroll 1
+  for (int Ip = 0; Ip < NB; ++Ip) {
+    float gb[NB][4], lb[NB][4];
+#pragma unroll
+    for (int J = 0; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) gb[J][r] = G[(16 * Ip + i) * kStride + 16 * J + 4 * g + r];
+    LoadRowBand<K>(L, Ip, i, g, lb);
+    const double g_row = gs[16 * Ip + i];
+#pragma unroll
+    for (int Im = 0; Im < NB; ++Im) {
+      const floatx4 h = TileProduct<K>(rr[Im], gb);  // H[16Ip + i][16Im + 4g + r]
+      Quad<float> q;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        q.v[r] = static_cast<float>(static_cast<double>(lb[Im][r]) * static_cast<double>(e[Im][r]) * g_row *
+                                    static_cast<double>(h[r]));
+      StoreQuad(adj_left + (16 * Ip + i) * K + 16 * Im + 4 * g, q);
+    }
+  }
+
+  // adj_R[m][c] = R[m][c] e[m] Gamma Q[m][c], Q^T = Gt^T L, Gt = Gr / Gamma;
+  // adj_mid[m] = sum_c adj_R[m][c].
+  const double inverse_gamma = gamma > 0.0 ? 1.0 / gamma : 0.0;
+  float scale[NB][4];  // g_p / Gamma for p = 16J + 4g + r
+#pragma unroll
+  for (int J = 0; J < NB; ++J)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) scale[J][r] = static_cast<float>(gs[16 * J + 4 * g + r] * inverse_gamma);
+  float *adj_right = SlotAdjoint<float>(buf, d, op.right, b);
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+#pragma unroll 1
+  for (int Im = 0; Im < NB; ++Im) {
+    float lc[NB][4];
+    LoadColumnBand<K>(L, Im, i, g, lc);
+    // e[m] for m = 16Im + i (the row this lane holds of Q and adj_R) lives in
+    // register (Im, i % 4) of the lanes with g = i / 4; the source lane
+    // 16(i/4) + i has the same i, so it evaluates the same register index.
+    const float e_row = __shfl(e[Im][i % 4], 16 * (i / 4) + i, 64);
+    double row_total = 0.0;
+#pragma unroll
+    for (int Ic = 0; Ic < NB; ++Ic) {
+      float gt[NB][4];
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) gt[J][r] = G[(16 * J + 4 * g + r) * kStride + 16 * Ic + i] * scale[J][r];
+      const floatx4 q = TileProduct<K>(gt, lc);  // Q[16Im + i][16Ic + 4g + r]
+      Quad<float> out;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double v = static_cast<double>(rr[Im][Ic][r]) * static_cast<double>(e_row) * gamma *
+                         static_cast<double>(q[r]);
+        out.v[r] = static_cast<float>(v);
+        row_total += v;
+      }
+      StoreQuad(adj_right + (16 * Im + i) * K + 16 * Ic + 4 * g, out);
+    }
+    row_total += __shfl_xor(row_total, 16, 64);
+    row_total += __shfl_xor(row_total, 32, 64);
+    if (g == 0) static_cast<float *>(buf.node_adj)[mid_at * K + 16 * Im + i] = static_cast<float>(row_total);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Vector transposition [X][Y][K] -> [Y][X][K], K in {4, 8, 16, 32, 64}
 // ---------------------------------------------------------------------------
 
@@ -606,6 +868,24 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
       if (K == 64) LaunchRakeTeam<double, 64>(l, reverse, offset, count, buf);
     }
     RecordLaunchError(hipGetLastError(), "rake level launch");
+    return true;
+  }
+  if (primitive == 3 && (K == 32 || K == 64) && l.real == Real::kF32) {
+    const unsigned grid =
+        static_cast<unsigned>((static_cast<std::uint64_t>(count) * l.dims.B + kWideWaves - 1) / kWideWaves);
+    const hipStream_t s = StreamOf(l);
+    if (K == 32) {
+      if (reverse)
+        CompressReverseMfmaWide<32><<<grid, 64 * kWideWaves, 0, s>>>(l.ops, l.dims, offset, count, buf);
+      else
+        CompressForwardMfmaWide<32><<<grid, 64 * kWideWaves, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    } else {
+      if (reverse)
+        CompressReverseMfmaWide<64><<<grid, 64 * kWideWaves, 0, s>>>(l.ops, l.dims, offset, count, buf);
+      else
+        CompressForwardMfmaWide<64><<<grid, 64 * kWideWaves, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    }
+    RecordLaunchError(hipGetLastError(), "MFMA compression launch");
     return true;
   }
   if (primitive == 3 && K == 16 && l.real == Real::kF32) {
