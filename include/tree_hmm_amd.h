@@ -131,7 +131,13 @@ enum {
   /* No reference counterpart: allocate no pinned staging and no
    * workspace-owned outputs; only the *_device entry points may be used.
    * Keeps multi-GB edge-marginal outputs single-copy in HBM. */
-  THMM_RESERVE_DEVICE_RESIDENT = 8u
+  THMM_RESERVE_DEVICE_RESIDENT = 8u,
+  /* No reference counterpart: the device-resident entry points replay a
+   * hipGraph of their whole launch sequence, captured at the first call and
+   * again whenever the pointers, batch or reduction flag change; the replay
+   * is enqueued on the call's stream. thmm_profile_marginals_device stays
+   * eager. */
+  THMM_RESERVE_GRAPH = 16u
 };
 
 /* tree_hmm::CategoricalInputUpdate (include/tree_hmm/accelerator.h:92-96). */

@@ -35,6 +35,7 @@ THMM_RESERVE_MAXIMUM = 1
 THMM_RESERVE_SAMPLING = 2
 THMM_RESERVE_MARGINALS = 4
 THMM_RESERVE_DEVICE_RESIDENT = 8
+THMM_RESERVE_GRAPH = 16
 
 THMM_RAKE = 0
 THMM_BRANCH_COMBINATION = 1

@@ -343,8 +343,13 @@ class Workspace:
     def reserve_sampling(self, plan, states, batch_capacity, device=0, precision="f32"):
         self._reserve(plan, states, batch_capacity, _abi.THMM_RESERVE_SAMPLING, device, precision)
 
-    def reserve_marginals(self, plan, states, batch_capacity, device=0, precision="f32", device_resident=False):
+    def reserve_marginals(self, plan, states, batch_capacity, device=0, precision="f32", device_resident=False,
+                          graphs=False):
+        """ReserveMarginals. device_resident: no pinned host mirrors (device
+        entry points only); graphs: device entry points replay a captured
+        hipGraph of their launch sequence."""
         flags = _abi.THMM_RESERVE_MARGINALS | (_abi.THMM_RESERVE_DEVICE_RESIDENT if device_resident else 0)
+        flags |= _abi.THMM_RESERVE_GRAPH if graphs else 0
         self._reserve(plan, states, batch_capacity, flags, device, precision)
 
     def reserve_categorical(self, plan, states, batch_capacity, categories, observation_nodes, device=0, precision="f32"):

@@ -121,6 +121,25 @@ struct Workspace::Impl {
   };
   std::vector<FusedStep> fused;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
+  // Instantiated hipGraphs of the device-resident entry points (reserve flag
+  // THMM_RESERVE_GRAPH), one per entry point, keyed by its arguments.
+  struct GraphKey {
+    const void *args[6] = {};
+    std::size_t batch = 0;
+    int flag = 0;
+    bool operator==(const GraphKey &o) const {
+      return std::equal(args, args + 6, o.args) && batch == o.batch && flag == o.flag;
+    }
+  };
+  struct GraphCache {
+    GraphKey key;
+    hipGraphExec_t exec = nullptr;
+    void Release() noexcept {
+      if (exec != nullptr) static_cast<void>(hipGraphExecDestroy(exec));
+      exec = nullptr;
+    }
+  };
+  GraphCache graphs[4];
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
@@ -148,7 +167,9 @@ struct Workspace::Impl {
     if (stream != nullptr) {
       static_cast<void>(hipSetDevice(device));
       static_cast<void>(hipStreamSynchronize(stream));
+      static_cast<void>(hipDeviceSynchronize());  // graph replays may run on other streams
     }
+    for (GraphCache &g : graphs) g.Release();
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
                             &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
@@ -1274,6 +1295,35 @@ hipStream_t ChooseStream(const Impl &w, void *stream) {
   return stream != nullptr ? static_cast<hipStream_t>(stream) : w.stream;
 }
 
+// Runs `enqueue` on stream s, or, with THMM_RESERVE_GRAPH, replays a hipGraph
+// of it on s, capturing it (on the workspace stream) the first time and
+// whenever the key changes. The captured launches contain no memory set or
+// copy nodes.
+template <typename F>
+void RunOrReplay(Impl &w, int kind, const Impl::GraphKey &key, hipStream_t s, const F &enqueue) {
+  if (!(w.flags & kReserveGraph)) {
+    enqueue(s);
+    CheckLaunches();
+    return;
+  }
+  Impl::GraphCache &cache = w.graphs[kind];
+  if (cache.exec == nullptr || !(cache.key == key)) {
+    cache.Release();
+    Hip(hipStreamBeginCapture(w.stream, hipStreamCaptureModeThreadLocal), "GPU runtime StreamBeginCapture");
+    enqueue(w.stream);
+    hipGraph_t graph = nullptr;
+    const hipError_t ended = hipStreamEndCapture(w.stream, &graph);
+    CheckLaunches();
+    Hip(ended, "GPU runtime StreamEndCapture");
+    const hipError_t made = hipGraphInstantiate(&cache.exec, graph, nullptr, nullptr, 0);
+    static_cast<void>(hipGraphDestroy(graph));
+    if (made != hipSuccess) cache.exec = nullptr;
+    Hip(made, "GPU runtime GraphInstantiate");
+    cache.key = key;
+  }
+  Hip(hipGraphLaunch(cache.exec, s), "GPU runtime GraphLaunch");
+}
+
 void CheckDevice(const Impl &w, std::size_t batch, const void *nodes, const void *edges) {
   if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
   if (w.categorical) throw std::invalid_argument("device-resident calls take dense potentials");
@@ -1291,13 +1341,15 @@ void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::si
   if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   w.forward = Impl::Forward::kNone;
-  const hipStream_t s = ChooseStream(w, stream);
-  const Launch l = w.MakeLaunch(w.real, batch, s);
-  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-  const LinearBuffers buf = w.Linear(edges, nullptr);
-  SumProductForward(w, l, buf);
-  LaunchFinish(l, buf, log_z);
-  CheckLaunches();
+  Impl::GraphKey key;
+  key.args[0] = nodes, key.args[1] = edges, key.args[2] = log_z, key.batch = batch;
+  RunOrReplay(w, 0, key, ChooseStream(w, stream), [&](hipStream_t s) {
+    const Launch l = w.MakeLaunch(w.real, batch, s);
+    LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
+    const LinearBuffers buf = w.Linear(edges, nullptr);
+    SumProductForward(w, l, buf);
+    LaunchFinish(l, buf, log_z);
+  });
 }
 
 void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
@@ -1311,30 +1363,38 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);
-  const Launch l = w.MakeLaunch(w.real, batch, s);
+  // With `timings` every launch is bracketed by an event pair (eager only).
   std::unique_ptr<Profiler> profiler;
   if (timings != nullptr) profiler = std::make_unique<Profiler>(s, static_cast<std::uint32_t>(batch));
   Profiler *p = profiler.get();
-  if (p) p->Begin(0, 0, 0);
-  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-  if (p) p->End();
-  LinearBuffers buf = w.Linear(edges, nullptr);
-  buf.edge_out = edge_out;
-  AdjointsInto(buf, batch, node_out);
-  SumProductForward(w, l, buf, p);
-  if (p) p->Begin(2, 0, 0);
-  LaunchFinish(l, buf, log_z);
-  LaunchSeedAdjoint(l, buf, nullptr);
-  if (p) p->End();
-  ReverseLevels(l, w.levels, buf, p);
-  if (p) p->Begin(4, 0, 0);
-  if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+  const auto enqueue = [&](hipStream_t on) {
+    const Launch l = w.MakeLaunch(w.real, batch, on);
+    if (p) p->Begin(0, 0, 0);
+    LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
+    if (p) p->End();
+    LinearBuffers buf = w.Linear(edges, nullptr);
+    buf.edge_out = edge_out;
+    AdjointsInto(buf, batch, node_out);
+    SumProductForward(w, l, buf, p);
+    if (p) p->Begin(2, 0, 0);
+    LaunchFinish(l, buf, log_z);
+    LaunchSeedAdjoint(l, buf, nullptr);
+    if (p) p->End();
+    ReverseLevels(l, w.levels, buf, p);
+    if (p) p->Begin(4, 0, 0);
+    if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+    if (p) p->End();
+  };
   if (p) {
-    p->End();
+    enqueue(s);
     CheckLaunches();
     *count = p->Report(timings, capacity);
+    return;
   }
-  CheckLaunches();
+  Impl::GraphKey key;
+  key.args[0] = nodes, key.args[1] = edges, key.args[2] = log_z, key.args[3] = node_out, key.args[4] = edge_out;
+  key.batch = batch;
+  RunOrReplay(w, 1, key, s, enqueue);
 }
 
 void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
@@ -1344,14 +1404,16 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   if (log_z == nullptr) throw std::invalid_argument("log-partition output is null");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   w.forward = Impl::Forward::kNone;
-  const hipStream_t s = ChooseStream(w, stream);
-  const Launch l = w.MakeLaunch(w.real, batch, s);
-  LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
-  LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
-  const LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-  SumProductForward(w, l, buf);
-  LaunchFinish(l, buf, log_z);
-  CheckLaunches();
+  Impl::GraphKey key;
+  key.args[0] = log_nodes, key.args[1] = log_edges, key.args[2] = log_z, key.batch = batch;
+  RunOrReplay(w, 2, key, ChooseStream(w, stream), [&](hipStream_t s) {
+    const Launch l = w.MakeLaunch(w.real, batch, s);
+    LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
+    LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
+    const LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+    SumProductForward(w, l, buf);
+    LaunchFinish(l, buf, log_z);
+  });
   w.last_batch = batch;
   w.forward = Impl::Forward::kLogPotentials;
 }
@@ -1367,16 +1429,19 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   if (reduce_edges && w.E() != 0 && w.edge_out.ptr == nullptr)
     throw std::logic_error("batch-reduced edge gradients need workspace edge storage (no device-resident flag)");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
-  const hipStream_t s = ChooseStream(w, stream);
-  const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
-  LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-  buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
-  AdjointsInto(buf, w.last_batch, grad_nodes);
-  LaunchSeedAdjoint(l, buf, grad);
-  ReverseLevels(l, w.levels, buf);
-  if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
-  if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
-  CheckLaunches();
+  Impl::GraphKey key;
+  key.args[0] = grad, key.args[1] = grad_nodes, key.args[2] = grad_edges, key.batch = w.last_batch;
+  key.flag = reduce_edges ? 1 : 0;
+  RunOrReplay(w, 3, key, ChooseStream(w, stream), [&](hipStream_t s) {
+    const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
+    LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+    buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
+    AdjointsInto(buf, w.last_batch, grad_nodes);
+    LaunchSeedAdjoint(l, buf, grad);
+    ReverseLevels(l, w.levels, buf);
+    if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
+    if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
+  });
 }
 
 void Workspace::Synchronize() {

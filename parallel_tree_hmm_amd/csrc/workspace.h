@@ -21,6 +21,8 @@ namespace thmm {
 // and workspace-owned output buffers; only the device-resident entry points
 // may then be used. Keeps multi-GB edge-marginal outputs single-copy.
 constexpr unsigned kReserveDeviceResident = 8u;
+// Extra reserve flag: device-resident entry points replay captured hipGraphs.
+constexpr unsigned kReserveGraph = 16u;
 
 class Workspace {
  public:
