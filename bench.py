@@ -128,7 +128,7 @@ KERNEL_NAMES = {
 # ---------------------------------------------------------------------------
 
 
-def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup):
+def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup, tune=True):
     dtype = torch.float32 if precision == "f32" else torch.float64
     nodes = nodes.to(dtype)
     edges = edges.to(dtype)
@@ -143,6 +143,10 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup):
     for _ in range(warmup):
         ws.marginals_device(*args)
     torch.cuda.synchronize()
+    # Launch variants picked by the library's autotuner on these inputs
+    # (outside the timed region).
+    tuning = ws.autotune_marginals_device(*args) if tune else dict(ws.tuning(), ms=None)
+    torch.cuda.synchronize()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(steps):
@@ -152,7 +156,7 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup):
     ms = start.elapsed_time(stop) / steps
     timings = ws.profile_marginals_device(*args)
     torch.cuda.synchronize()
-    return ws, ms, timings, log_z, node_out, edge_out
+    return ws, ms, timings, log_z, node_out, edge_out, tuning
 
 
 def measured_traffic(kernel: str):
@@ -268,6 +272,7 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-nodes", type=int, default=400_000)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (the reference point that scales HBM counters)")
     args = ap.parse_args()
@@ -321,9 +326,12 @@ def main() -> int:
     edges /= edges.sum(dim=2, keepdim=True)
     torch.cuda.synchronize()
 
-    ws, ms, timings, log_z, node_out, edge_out = run_gpu(thmm, torch, plan, k, "f32", nodes, edges, args.steps, args.warmup)
+    ws, ms, timings, log_z, node_out, edge_out, tuning = run_gpu(
+        thmm, torch, plan, k, "f32", nodes, edges, args.steps, args.warmup, not args.no_tune
+    )
     value = n / (ms * 1e-3)
     roofline = roofline_from(timings, k, 4)
+    log(f"autotuned launch variants: {tuning}")
     log(f"FP32 form: {ms:.3f} ms/step, {value:.4g} nodes/s; dominant kernel {roofline['kernel']} "
         f"{roofline['achieved']} GB/s ({roofline['frac']:.1%} of HBM peak)")
 
@@ -362,9 +370,12 @@ def main() -> int:
         del nodes, edges
         torch.cuda.empty_cache()
         try:
-            ws64, ms64, _, lz64, _, _ = run_gpu(thmm, torch, plan, k, "f64", nodes64, edges64, max(2, args.steps // 2), 1)
+            ws64, ms64, _, lz64, _, _, tuning64 = run_gpu(
+                thmm, torch, plan, k, "f64", nodes64, edges64, max(2, args.steps // 2), 1, not args.no_tune
+            )
             lz64_h = lz64.cpu().numpy()
             fp64 = {"ms_per_step": round(ms64, 4), "value": n / (ms64 * 1e-3), "fp32_speedup": round(ms64 / ms, 3),
+                    "tuning": tuning64,
                     "fp32_vs_fp64_log_z_rel": float(np.max(np.abs(lz64_h - log_z_h) / np.maximum(1, np.abs(lz64_h))))}
             log(f"plain FP64 form: {ms64:.3f} ms/step; FP32 speedup {ms64 / ms:.2f}x")
             ws64.close()
@@ -396,6 +407,7 @@ def main() -> int:
             "scale": args.scale,
         },
         "roofline": roofline,
+        "tuning": tuning,
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
         "parity": parity,

@@ -366,6 +366,36 @@ thmm_status thmm_logz_backward_device(thmm_workspace *workspace,
 thmm_status thmm_synchronize(thmm_workspace *workspace);
 
 /* ------------------------------------------------------------------------ */
+/* Launch tuning                                                            */
+/* ------------------------------------------------------------------------ */
+/* No reference counterpart. Launch variants of a reserved workspace:
+ * compress_items_forward / _reverse: K = 16 FP32 MFMA compressions per
+ * wavefront (1 or 2); rake_block: threads per block of the team rake kernels
+ * (64, 128 or 256); gather_chunk: rakes a team folds into one parent per
+ * step of the fused forward pass (1..64). Values outside these sets are
+ * invalid_argument. Setting them drops captured graphs; a new gather_chunk
+ * rebuilds the fused forward program. ms is filled by the autotuner. */
+typedef struct thmm_tuning {
+  uint32_t compress_items_forward, compress_items_reverse;
+  uint32_t rake_block, gather_chunk;
+  double ms;
+} thmm_tuning;
+
+thmm_status thmm_get_tuning(const thmm_workspace *workspace, thmm_tuning *out);
+thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning);
+
+/* Autotuner: runs thmm_marginals_device on the given inputs and outputs once
+ * per candidate launch variant (coordinate descent over the four knobs,
+ * each candidate the best of three event-timed runs), keeps the fastest
+ * setting in the workspace and reports it, with its milliseconds per call,
+ * in *chosen. Blocks until done; the outputs hold valid marginals. */
+thmm_status thmm_autotune_marginals_device(
+    thmm_workspace *workspace, const void *d_node_potentials,
+    const void *d_edge_potentials, size_t batch, double *d_log_z,
+    void *d_node_marginals, void *d_edge_marginals, void *stream,
+    thmm_tuning *chosen);
+
+/* ------------------------------------------------------------------------ */
 /* Measurement                                                              */
 /* ------------------------------------------------------------------------ */
 

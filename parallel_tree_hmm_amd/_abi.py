@@ -113,6 +113,16 @@ class LaunchTiming(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("compress_items_forward", c_u32),
+        ("compress_items_reverse", c_u32),
+        ("rake_block", c_u32),
+        ("gather_chunk", c_u32),
+        ("ms", ctypes.c_double),
+    ]
+
+
 class DenseModel(ctypes.Structure):
     _fields_ = [
         ("identity", c_void_p),
@@ -198,6 +208,12 @@ PROTOTYPES = {
         [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_size)],
     ),
     "thmm_branch_length_backward": (c_status, [c_void_p, c_void_p, c_void_p, P(c_void_p), P(c_size)]),
+    "thmm_get_tuning": (c_status, [c_void_p, P(Tuning)]),
+    "thmm_set_tuning": (c_status, [c_void_p, P(Tuning)]),
+    "thmm_autotune_marginals_device": (
+        c_status,
+        [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, P(Tuning)],
+    ),
     "thmm_log_partition_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
     "thmm_marginals_device": (
         c_status,

@@ -466,6 +466,38 @@ class Workspace:
         )
         return [(t.phase, t.primitive, t.count, t.batch, t.ms) for t in timings[: min(count.value, capacity)]]
 
+    # -- launch tuning -----------------------------------------------------
+    _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk")
+
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        _check(_lib().thmm_get_tuning(self._handle, ctypes.byref(t)))
+        return {f: getattr(t, f) for f in self._TUNING_FIELDS}
+
+    def set_tuning(self, **values) -> None:
+        current = self.tuning()
+        unknown = set(values) - set(current)
+        if unknown:
+            raise InvalidArgument(f"unknown tuning fields {sorted(unknown)}")
+        current.update(values)
+        t = _abi.Tuning(*(int(current[f]) for f in self._TUNING_FIELDS), 0.0)
+        _check(_lib().thmm_set_tuning(self._handle, ctypes.byref(t)))
+
+    def autotune_marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr,
+                                  stream=None) -> dict:
+        """Times the launch variants on these inputs, keeps the fastest and
+        returns it with its milliseconds per marginal pass ("ms")."""
+        t = _abi.Tuning()
+        _check(
+            _lib().thmm_autotune_marginals_device(
+                self._handle, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, _stream(stream),
+                ctypes.byref(t)
+            )
+        )
+        out = {f: getattr(t, f) for f in self._TUNING_FIELDS}
+        out["ms"] = t.ms
+        return out
+
     def synchronize(self) -> None:
         _check(_lib().thmm_synchronize(self._handle))
 

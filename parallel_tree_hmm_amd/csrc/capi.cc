@@ -65,6 +65,10 @@ thmm::Workspace &W(thmm_workspace *workspace) {
   return workspace->workspace;
 }
 
+thmm_tuning ToC(const thmm::Tuning &t, double ms) {
+  return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, ms};
+}
+
 }  // namespace
 
 extern "C" {
@@ -258,6 +262,38 @@ thmm_status thmm_branch_length_backward(thmm_workspace *workspace, const double 
   return Guard([&] {
     Require(grad && count, "argument is null");
     W(workspace).BranchLengthBackward(grad_log_z, rate_matrix, grad, count);
+  });
+}
+
+thmm_status thmm_get_tuning(const thmm_workspace *workspace, thmm_tuning *out) {
+  return Guard([&] {
+    Require(workspace && out, "argument is null");
+    *out = ToC(workspace->workspace.GetTuning(), 0.0);
+  });
+}
+
+thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning) {
+  return Guard([&] {
+    Require(tuning, "argument is null");
+    thmm::Tuning t;
+    t.compress_items_forward = tuning->compress_items_forward;
+    t.compress_items_reverse = tuning->compress_items_reverse;
+    t.rake_block = tuning->rake_block;
+    t.gather_chunk = tuning->gather_chunk;
+    W(workspace).SetTuning(t);
+  });
+}
+
+thmm_status thmm_autotune_marginals_device(thmm_workspace *workspace, const void *d_node_potentials,
+                                           const void *d_edge_potentials, size_t batch, double *d_log_z,
+                                           void *d_node_marginals, void *d_edge_marginals, void *stream,
+                                           thmm_tuning *chosen) {
+  return Guard([&] {
+    Require(chosen, "argument is null");
+    thmm::Workspace &w = W(workspace);
+    const double ms = w.AutotuneMarginalsDevice(d_node_potentials, d_edge_potentials, batch, d_log_z,
+                                                d_node_marginals, d_edge_marginals, stream);
+    *chosen = ToC(w.GetTuning(), ms);
   });
 }
 

@@ -265,7 +265,6 @@ __global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, s
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int kItemsPerWave = 2;
 constexpr int kWaves = 4;  // wavefronts per block
 
 __device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
@@ -338,6 +337,7 @@ __device__ inline floatx4 MiddleProduct16(const CompressOperands &o, const float
 
 // Row p of out = L diag(exp(mid)) R, max-normalised with its own FP64 log
 // scale (src/inference.cc:749-776).
+template <int kItemsPerWave>
 __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
                                                              std::uint32_t count, LinearBuffers buf) {
   const int lane = threadIdx.x % 64;
@@ -385,6 +385,7 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
 // column layout, Gt reaching it through one LDS transposition.
 constexpr int kPad = 17;  // LDS row stride (floats)
 
+template <int kItemsPerWave>
 __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
                                                              std::uint32_t count, LinearBuffers buf) {
   __shared__ float tiles[kWaves][kItemsPerWave][16 * kPad];
@@ -789,13 +790,16 @@ unsigned Blocks(std::uint64_t threads) { return static_cast<unsigned>((threads +
 
 hipStream_t StreamOf(const Launch &l) { return static_cast<hipStream_t>(l.stream); }
 
+unsigned BlocksOf(std::uint64_t threads, unsigned block) { return static_cast<unsigned>((threads + block - 1) / block); }
+
 template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, std::uint32_t offset, std::uint32_t count,
                                                  const LinearBuffers &buf) {
-  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * Team<K>::kLanes);
+  const unsigned block = l.tuning.rake_block;
+  const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * Team<K>::kLanes, block);
   if (reverse)
-    RakeReverseTeam<T, K><<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+    RakeReverseTeam<T, K><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
   else
-    RakeForwardTeam<T, K><<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+    RakeForwardTeam<T, K><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
 }
 
 }  // namespace
@@ -808,9 +812,10 @@ void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offse
                         const LinearBuffers &buf) {
   if (count == 0) return;
   const std::uint32_t K = l.dims.K;
-  const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * (K >= 32 ? 64 : 16));
+  const unsigned block = l.tuning.rake_block;
+  const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * (K >= 32 ? 64 : 16), block);
   const hipStream_t s = StreamOf(l);
-#define THMM_GATHER(T, KK) RakeGatherTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf)
+#define THMM_GATHER(T, KK) RakeGatherTeam<T, KK><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf)
   if (l.real == Real::kF32) {
     if (K == 8) THMM_GATHER(float, 8);
     if (K == 16) THMM_GATHER(float, 16);
@@ -889,12 +894,14 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
     return true;
   }
   if (primitive == 3 && K == 16 && l.real == Real::kF32) {
-    const std::uint64_t per_block = static_cast<std::uint64_t>(kWaves) * kItemsPerWave;
+    const int items = reverse ? l.tuning.compress_items_reverse : l.tuning.compress_items_forward;
+    const std::uint64_t per_block = static_cast<std::uint64_t>(kWaves) * items;
     const unsigned grid = static_cast<unsigned>((static_cast<std::uint64_t>(count) * l.dims.B + per_block - 1) / per_block);
-    if (reverse)
-      CompressReverseMfma16<<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
-    else
-      CompressForwardMfma16<<<grid, kBlock, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+    const hipStream_t s = StreamOf(l);
+    if (reverse && items == 1) CompressReverseMfma16<1><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    if (reverse && items != 1) CompressReverseMfma16<2><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    if (!reverse && items == 1) CompressForwardMfma16<1><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    if (!reverse && items != 1) CompressForwardMfma16<2><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
     RecordLaunchError(hipGetLastError(), "MFMA compression launch");
     return true;
   }

@@ -80,11 +80,21 @@ struct MaxBuffers {
 
 enum class Real { kF32 = 0, kF64 = 1 };
 
+// Launch variants chosen per workspace (defaults, thmm_set_tuning or the
+// autotuner).
+struct Tuning {
+  std::uint32_t compress_items_forward = 2;  // K = 16 MFMA compressions per wavefront: 1 or 2
+  std::uint32_t compress_items_reverse = 2;
+  std::uint32_t rake_block = 256;            // threads per block of the team rake kernels: 64, 128, 256
+  std::uint32_t gather_chunk = 16;           // rakes per team in the fused forward: 1 .. 64
+};
+
 struct Launch {
   Real real;
   Dims dims;
   DeviceOps ops;
   void *stream;  // hipStream_t
+  Tuning tuning;
 };
 
 // ---- input staging ----

@@ -140,6 +140,7 @@ struct Workspace::Impl {
     }
   };
   GraphCache graphs[4];
+  Tuning tuning;
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
@@ -170,6 +171,7 @@ struct Workspace::Impl {
       static_cast<void>(hipDeviceSynchronize());  // graph replays may run on other streams
     }
     for (GraphCache &g : graphs) g.Release();
+    tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
                             &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
@@ -227,6 +229,7 @@ struct Workspace::Impl {
     l.ops = {rakes.as<DeviceRake>(), combines.as<DeviceCombine>(), absorbs.as<DeviceAbsorb>(),
              compresses.as<DeviceCompress>(), roots.as<std::uint32_t>()};
     l.stream = s;
+    l.tuning = tuning;
     return l;
   }
 
@@ -259,16 +262,15 @@ template <typename V> void Upload(DeviceBuffer &dst, const std::vector<V> &src, 
   if (!src.empty()) Hip(hipMemcpyAsync(dst.ptr, src.data(), src.size() * sizeof(V), hipMemcpyHostToDevice, stream), what);
 }
 
-// Rakes per chunk of the fused forward (a team walks the chunk serially) and
-// partials per team of a partial-reduction level.
-constexpr std::uint32_t kGatherChunk = 16;
+// Partials per team of a partial-reduction level (the rakes per chunk of the
+// fused forward, which a team walks serially, are w.tuning.gather_chunk).
 constexpr std::uint32_t kGatherFanIn = 32;
 
 // Groups the plan's batches into rounds of non-decreasing primitive kind
 // (rakes, combinations, absorptions, compressions) and, when every rake's
 // branch is absorbed in its own round, lays out the fused forward program:
 // per round one chunk level over the rakes sorted by absorption, then for
-// parents with more than kGatherChunk raked children levels of partial sums
+// parents with more than one chunk of raked children levels of partial sums
 // of fan-in kGatherFanIn until one sum per parent is absorbed, then the
 // round's compressions. Leaves w.fused empty otherwise.
 void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> &device_rakes, std::size_t batch) {
@@ -293,6 +295,7 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
         for (std::uint32_t j = plan.batches[i].offset; j < plan.batches[i].offset + plan.batches[i].count; ++j)
           absorb_round[j] = static_cast<std::uint32_t>(r);
 
+  const std::uint32_t chunk_size = w.tuning.gather_chunk;
   std::vector<Impl::FusedStep> steps;
   std::vector<DeviceRake> sorted;
   std::vector<DeviceGatherChunk> chunks;
@@ -318,10 +321,10 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
       std::size_t end = i;
       while (end < keyed.size() && keyed[end].first == keyed[i].first) ++end;
       const std::uint32_t parent = plan.absorptions[keyed[i].first].parent;
-      const bool direct = end - i <= kGatherChunk;
+      const bool direct = end - i <= chunk_size;
       const std::uint32_t first_partial = partials;
-      for (std::size_t c = i; c < end; c += kGatherChunk) {
-        const std::uint32_t count = static_cast<std::uint32_t>(std::min<std::size_t>(kGatherChunk, end - c));
+      for (std::size_t c = i; c < end; c += chunk_size) {
+        const std::uint32_t count = static_cast<std::uint32_t>(std::min<std::size_t>(chunk_size, end - c));
         chunks.push_back({static_cast<std::uint32_t>(sorted.size()), count, parent, direct ? kDirectChunk : partials++});
         for (std::size_t k = c; k < c + count; ++k) sorted.push_back(device_rakes[keyed[k].second]);
       }
@@ -364,6 +367,8 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
   Upload(w.gather_segments, segments, w.stream, "fused topology upload");
   w.gather_partials.Allocate(Product({partials, batch, w.states, sizeof(double)}, "partial messages"),
                              "partial messages");
+  // The uploads read the local vectors above.
+  Hip(hipStreamSynchronize(w.stream), "fused topology upload");
   w.fused = std::move(steps);
 }
 
@@ -1324,6 +1329,53 @@ void RunOrReplay(Impl &w, int kind, const Impl::GraphKey &key, hipStream_t s, co
   Hip(hipGraphLaunch(cache.exec, s), "GPU runtime GraphLaunch");
 }
 
+// The launch sequence of thmm_marginals_device on stream `on`.
+void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *edges, std::size_t batch,
+                      double *log_z, void *node_out, void *edge_out, Profiler *p) {
+  const Launch l = w.MakeLaunch(w.real, batch, on);
+  if (p) p->Begin(0, 0, 0);
+  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
+  if (p) p->End();
+  LinearBuffers buf = w.Linear(edges, nullptr);
+  buf.edge_out = edge_out;
+  AdjointsInto(buf, batch, node_out);
+  SumProductForward(w, l, buf, p);
+  if (p) p->Begin(2, 0, 0);
+  LaunchFinish(l, buf, log_z);
+  LaunchSeedAdjoint(l, buf, nullptr);
+  if (p) p->End();
+  ReverseLevels(l, w.levels, buf, p);
+  if (p) p->Begin(4, 0, 0);
+  if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
+  if (p) p->End();
+}
+
+void ValidateTuning(const Tuning &t) {
+  if (t.compress_items_forward < 1 || t.compress_items_forward > 2 || t.compress_items_reverse < 1 ||
+      t.compress_items_reverse > 2)
+    throw std::invalid_argument("compressions per wavefront must be 1 or 2");
+  if (t.rake_block != 64 && t.rake_block != 128 && t.rake_block != 256)
+    throw std::invalid_argument("rake block size must be 64, 128 or 256");
+  if (t.gather_chunk < 1 || t.gather_chunk > 64) throw std::invalid_argument("gather chunk must be in [1, 64]");
+}
+
+// Installs `t`; a new chunk size rebuilds the fused forward program from the
+// device's renamed rakes. Captured graphs are dropped (they bake in kernels).
+void ApplyTuning(Impl &w, const Tuning &t) {
+  ValidateTuning(t);
+  const bool rebuild = !w.fused.empty() && t.gather_chunk != w.tuning.gather_chunk;
+  w.tuning = t;
+  for (Impl::GraphCache &g : w.graphs) g.Release();
+  if (!rebuild) return;
+  std::vector<DeviceRake> rakes(w.plan->rakes.size());
+  if (!rakes.empty())
+    Hip(hipMemcpy(rakes.data(), w.rakes.ptr, rakes.size() * sizeof(DeviceRake), hipMemcpyDeviceToHost),
+        "GPU runtime Memcpy rake download");
+  w.fused.clear();
+  BuildFusedForward(w, *w.plan, rakes, w.capacity);
+  if (w.fused.empty()) throw std::logic_error("the fused forward program could not be rebuilt");
+}
+
 void CheckDevice(const Impl &w, std::size_t batch, const void *nodes, const void *edges) {
   if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
   if (w.categorical) throw std::invalid_argument("device-resident calls take dense potentials");
@@ -1368,22 +1420,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   if (timings != nullptr) profiler = std::make_unique<Profiler>(s, static_cast<std::uint32_t>(batch));
   Profiler *p = profiler.get();
   const auto enqueue = [&](hipStream_t on) {
-    const Launch l = w.MakeLaunch(w.real, batch, on);
-    if (p) p->Begin(0, 0, 0);
-    LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-    if (p) p->End();
-    LinearBuffers buf = w.Linear(edges, nullptr);
-    buf.edge_out = edge_out;
-    AdjointsInto(buf, batch, node_out);
-    SumProductForward(w, l, buf, p);
-    if (p) p->Begin(2, 0, 0);
-    LaunchFinish(l, buf, log_z);
-    LaunchSeedAdjoint(l, buf, nullptr);
-    if (p) p->End();
-    ReverseLevels(l, w.levels, buf, p);
-    if (p) p->Begin(4, 0, 0);
-    if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
-    if (p) p->End();
+    EnqueueMarginals(w, on, nodes, edges, batch, log_z, node_out, edge_out, p);
   };
   if (p) {
     enqueue(s);
@@ -1442,6 +1479,80 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
     if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
     if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
   });
+}
+
+void Workspace::SetTuning(const Tuning &t) {
+  Impl &w = *impl_;
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  Hip(hipStreamSynchronize(w.stream), "tree-HMM accelerator execution");
+  ApplyTuning(w, t);
+}
+
+Tuning Workspace::GetTuning() const { return impl_->tuning; }
+
+// Coordinate descent over the launch variants: each candidate runs the whole
+// marginal pass on the caller's inputs (eagerly, on `stream`), timed with an
+// event pair as the best of three runs; a candidate replaces the current
+// value when it is at least 1 % faster. Outputs hold the last run's (valid)
+// results.
+double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
+                                          void *node_out, void *edge_out, void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, nodes, edges);
+  RequireMarginals(w);
+  if (log_z == nullptr || node_out == nullptr || (w.E() != 0 && edge_out == nullptr))
+    throw std::invalid_argument("marginal output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
+  const hipStream_t s = ChooseStream(w, stream);
+  hipEvent_t start = nullptr, stop = nullptr;
+  Hip(hipEventCreate(&start), "GPU runtime EventCreate");
+  Hip(hipEventCreate(&stop), "GPU runtime EventCreate");
+  const auto time_current = [&]() {
+    float best = std::numeric_limits<float>::infinity();
+    for (int rep = 0; rep < 3; ++rep) {
+      Hip(hipEventRecord(start, s), "GPU runtime EventRecord");
+      EnqueueMarginals(w, s, nodes, edges, batch, log_z, node_out, edge_out, nullptr);
+      Hip(hipEventRecord(stop, s), "GPU runtime EventRecord");
+      Hip(hipEventSynchronize(stop), "tree-HMM accelerator execution");
+      CheckLaunches();
+      float ms = 0.0f;
+      Hip(hipEventElapsedTime(&ms, start, stop), "GPU runtime EventElapsedTime");
+      best = std::min(best, ms);
+    }
+    return static_cast<double>(best);
+  };
+  double best_ms = 0.0;
+  try {
+    ApplyTuning(w, w.tuning);
+    best_ms = time_current();
+    const auto search = [&](std::uint32_t Tuning::*knob, std::initializer_list<std::uint32_t> values) {
+      const Tuning kept = w.tuning;
+      Tuning best = kept;
+      for (std::uint32_t v : values) {
+        if (v == kept.*knob) continue;
+        Tuning t = kept;
+        t.*knob = v;
+        ApplyTuning(w, t);
+        const double ms = time_current();
+        if (ms < best_ms * 0.99) best_ms = ms, best = t;
+      }
+      ApplyTuning(w, best);
+    };
+    const bool mfma16 = w.states == 16 && w.real == Real::kF32;
+    if (!w.fused.empty()) search(&Tuning::gather_chunk, {4, 8, 16, 32});
+    if (mfma16) search(&Tuning::compress_items_forward, {1, 2});
+    if (mfma16) search(&Tuning::compress_items_reverse, {1, 2});
+    search(&Tuning::rake_block, {64, 128, 256});
+  } catch (...) {
+    static_cast<void>(hipEventDestroy(start));
+    static_cast<void>(hipEventDestroy(stop));
+    throw;
+  }
+  static_cast<void>(hipEventDestroy(start));
+  static_cast<void>(hipEventDestroy(stop));
+  return best_ms;
 }
 
 void Workspace::Synchronize() {

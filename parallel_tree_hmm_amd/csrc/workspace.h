@@ -79,6 +79,14 @@ class Workspace {
                           void *stream);
   void Synchronize();
 
+  // Launch variants (see Tuning in launch.h).
+  void SetTuning(const Tuning &t);
+  Tuning GetTuning() const;
+  // Times the variants on the caller's inputs and keeps the fastest; returns
+  // the best measured milliseconds per marginal pass.
+  double AutotuneMarginalsDevice(const void *nodes, const void *edges, std::size_t batch, double *log_z,
+                                 void *node_out, void *edge_out, void *stream);
+
   struct Impl;
 
  private:
