@@ -285,45 +285,49 @@ __device__ inline void LoadRows(const float *X, int lane, float (&out)[4]) {
   for (int r = 0; r < 4; ++r) out[r] = q.v[r];
 }
 
-// Operands of one compression as seen by one lane.
+// Operands of one compression as seen by one lane. An item is handled by a
+// whole wavefront, so the op and slot bases are made wave-uniform
+// (readfirstlane): their arithmetic runs on the scalar unit and the vector
+// loads add only the lane's offset.
 struct CompressOperands {
   DeviceCompress op;
   std::uint32_t b;
-  float left_rows[4];    // L, row layout
-  float right_cols[4];   // R, column layout
-  float mid[4];          // mid[4(l/16) + r]
-  double right_scale[4]; // row scales of R at m = 4(l/16) + r
+  const float *left, *right;
+  float left_rows[4];       // L, row layout
+  float right_cols[4];      // R, column layout
+  float mid_own;            // mid[l%16]
+  double right_scale_own;   // row scale of R at m = l%16
 };
+
+__device__ inline std::uint32_t Uniform(std::uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, const LinearBuffers &buf,
                                        std::uint32_t offset, std::uint32_t item, int lane, CompressOperands &o) {
+  item = Uniform(item);
   o.b = item % d.B;
-  o.op = ops.compresses[offset + item / d.B];
-  LoadRows(Slot<float>(buf, d, o.op.left, o.b), lane, o.left_rows);
-  LoadColumns(Slot<float>(buf, d, o.op.right, o.b), lane, o.right_cols);
+  const DeviceCompress op = ops.compresses[offset + item / d.B];
+  o.op.left = Uniform(op.left), o.op.middle = Uniform(op.middle), o.op.right = Uniform(op.right);
+  o.op.out = Uniform(op.out);
+  o.left = Slot<float>(buf, d, o.op.left, o.b);
+  o.right = Slot<float>(buf, d, o.op.right, o.b);
+  LoadRows(o.left, lane, o.left_rows);
+  LoadColumns(o.right, lane, o.right_cols);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(o.op.middle) * d.B + o.b;
-  const Quad<float> m = LoadQuad(static_cast<const float *>(buf.nodes) + mid_at * 16 + 4 * (lane / 16));
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    o.mid[r] = m.v[r];
-    o.right_scale[r] = RowScale(buf, d, o.op.right, o.b, 4 * (lane / 16) + r);
-  }
+  o.mid_own = static_cast<const float *>(buf.nodes)[mid_at * 16 + lane % 16];
+  o.right_scale_own = RowScale(buf, d, o.op.right, o.b, lane % 16);
 }
 
-// Middle weights e[m] = exp(mid[m] + r_m - W) for m = 4(l/16) + r, with
-// W = max_m (mid[m] + r_m) returned (-inf when no middle state is live).
-__device__ inline double MiddleWeights16(const CompressOperands &o, float (&e)[4]) {
-  double w[4];
-  double top = kNegInf;
+// Middle weights e[m] = exp(mid[m] + r_m - W): each lane forms the weight of
+// m = l%16 (e_own) and gathers those of m = 4(l/16) + r into e[r]; returns
+// W = max_m (mid[m] + r_m) (-inf when no middle state is live).
+__device__ inline double MiddleWeights16(const CompressOperands &o, int lane, float (&e)[4], float *e_own) {
+  const double w = static_cast<double>(o.mid_own) + o.right_scale_own;
+  double top = w;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    w[r] = static_cast<double>(o.mid[r]) + o.right_scale[r];
-    top = fmax(top, w[r]);
-  }
-  top = fmax(top, __shfl_xor(top, 16, 64));
-  top = fmax(top, __shfl_xor(top, 32, 64));
+  for (int x = 1; x < 16; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
+  *e_own = top > kNegInf ? expf(static_cast<float>(w - top)) : 0.0f;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) e[r] = top > kNegInf ? expf(static_cast<float>(w[r] - top)) : 0.0f;
+  for (int r = 0; r < 4; ++r) e[r] = __shfl(*e_own, 4 * (lane / 16) + r, 64);
   return top;
 }
 
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
                                                              std::uint32_t count, LinearBuffers buf) {
   const int lane = threadIdx.x % 64;
   const std::uint32_t total = count * d.B;
-  const std::uint32_t first = (blockIdx.x * kWaves + threadIdx.x / 64) * kItemsPerWave;
+  const std::uint32_t first = Uniform((blockIdx.x * kWaves + threadIdx.x / 64) * kItemsPerWave);
   if (first >= total) return;
   CompressOperands o[kItemsPerWave];
   double left_scale[kItemsPerWave], mid_scale[kItemsPerWave];
@@ -355,8 +359,8 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
   }
 #pragma unroll
   for (int t = 0; t < kItemsPerWave; ++t) {
-    float e[4];
-    const double w_top = MiddleWeights16(o[t], e);
+    float e[4], e_own;
+    const double w_top = MiddleWeights16(o[t], lane, e, &e_own);
     const floatx4 s = MiddleProduct16(o[t], e);
     float top = fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3]));
     top = fmaxf(top, __shfl_xor(top, 16, 64));
@@ -369,9 +373,11 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
 #pragma unroll
     for (int r = 0; r < 4; ++r) q.v[r] = s[r] * inverse;
     StoreQuad(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
+    // top lies in (0, 16]: its FP32 logarithm is exact to an ulp of a number
+    // of order one; the sum of scales stays FP64.
     if (lane < 16)
       buf.path_scale[out_at * 16 + lane] =
-          live ? left_scale[t] + mid_scale[t] + w_top + log(static_cast<double>(top)) : kNegInf;
+          live ? left_scale[t] + mid_scale[t] + w_top + static_cast<double>(logf(top)) : kNegInf;
   }
 }
 
@@ -391,32 +397,25 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
   __shared__ float tiles[kWaves][kItemsPerWave][16 * kPad];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const std::uint32_t total = count * d.B;
-  const std::uint32_t first = (blockIdx.x * kWaves + wave) * kItemsPerWave;
+  const std::uint32_t first = Uniform((blockIdx.x * kWaves + wave) * kItemsPerWave);
   if (first >= total) return;  // whole wavefront; only wavefront barriers below
   CompressOperands o[kItemsPerWave];
   float right_rows[kItemsPerWave][4], left_cols[kItemsPerWave][4], adj[kItemsPerWave][4];
-  float mid_own[kItemsPerWave];
-  double right_scale_own[kItemsPerWave];
 #pragma unroll
   for (int t = 0; t < kItemsPerWave; ++t) {
     const std::uint32_t item = first + t < total ? first + t : first;
     LoadCompression(ops, d, buf, offset, item, lane, o[t]);
-    LoadRows(Slot<float>(buf, d, o[t].op.right, o[t].b), lane, right_rows[t]);
-    LoadColumns(Slot<float>(buf, d, o[t].op.left, o[t].b), lane, left_cols[t]);
+    LoadRows(o[t].right, lane, right_rows[t]);
+    LoadColumns(o[t].left, lane, left_cols[t]);
     const std::uint64_t out_at = static_cast<std::uint64_t>(o[t].op.out - d.E) * d.B + o[t].b;
     LoadRows(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj[t]);
-    const std::uint64_t mid_at = static_cast<std::uint64_t>(o[t].op.middle) * d.B + o[t].b;
-    mid_own[t] = static_cast<const float *>(buf.nodes)[mid_at * 16 + lane % 16];
-    right_scale_own[t] = RowScale(buf, d, o[t].op.right, o[t].b, lane % 16);
   }
   float e[kItemsPerWave][4], e_own[kItemsPerWave];
   double g[kItemsPerWave], gamma[kItemsPerWave];
   float gh[kItemsPerWave][4];
 #pragma unroll
   for (int t = 0; t < kItemsPerWave; ++t) {
-    const double w_top = MiddleWeights16(o[t], e[t]);
-    e_own[t] = w_top > kNegInf ? expf(static_cast<float>(static_cast<double>(mid_own[t]) + right_scale_own[t] - w_top))
-                               : 0.0f;
+    MiddleWeights16(o[t], lane, e[t], &e_own[t]);
     const floatx4 s = MiddleProduct16(o[t], e[t]);
     double gr[4];
     double top = 0.0;
@@ -433,10 +432,11 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
     for (int x = 1; x < 16; x <<= 1) big = fmax(big, __shfl_xor(big, x, 64));
     gamma[t] = big;
     const double inverse_gamma = big > 0.0 ? 1.0 / big : 0.0;
+    const double inverse_top = top > 0.0 ? 1.0 / top : 0.0;
     float *tile = tiles[wave][t];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      gh[t][r] = top > 0.0 ? static_cast<float>(gr[r] / top) : 0.0f;
+      gh[t][r] = static_cast<float>(gr[r] * inverse_top);
       tile[(lane % 16) * kPad + 4 * (lane / 16) + r] = static_cast<float>(gr[r] * inverse_gamma);
     }
   }
