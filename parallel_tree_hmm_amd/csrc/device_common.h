@@ -107,6 +107,36 @@ template <typename T> __device__ inline void StoreQuad(T *p, const Quad<T> &q) {
   }
 }
 
+// Streaming forms for data read or written exactly once by a level (edge
+// slots, edge marginals): non-temporal, so they do not displace reusable
+// lines in the caches.
+typedef float StreamF4 __attribute__((ext_vector_type(4)));
+typedef double StreamD2 __attribute__((ext_vector_type(2)));
+
+template <typename T> __device__ inline Quad<T> LoadQuadStream(const T *p) {
+  Quad<T> q;
+  if constexpr (sizeof(T) == 4) {
+    const StreamF4 x = __builtin_nontemporal_load(reinterpret_cast<const StreamF4 *>(p));
+    q.v[0] = x[0], q.v[1] = x[1], q.v[2] = x[2], q.v[3] = x[3];
+  } else {
+    const StreamD2 x = __builtin_nontemporal_load(reinterpret_cast<const StreamD2 *>(p));
+    const StreamD2 y = __builtin_nontemporal_load(reinterpret_cast<const StreamD2 *>(p + 2));
+    q.v[0] = x[0], q.v[1] = x[1], q.v[2] = y[0], q.v[3] = y[1];
+  }
+  return q;
+}
+
+template <typename T> __device__ inline void StoreQuadStream(T *p, const Quad<T> &q) {
+  if constexpr (sizeof(T) == 4) {
+    const StreamF4 x = {q.v[0], q.v[1], q.v[2], q.v[3]};
+    __builtin_nontemporal_store(x, reinterpret_cast<StreamF4 *>(p));
+  } else {
+    const StreamD2 x = {q.v[0], q.v[1]}, y = {q.v[2], q.v[3]};
+    __builtin_nontemporal_store(x, reinterpret_cast<StreamD2 *>(p));
+    __builtin_nontemporal_store(y, reinterpret_cast<StreamD2 *>(p + 2));
+  }
+}
+
 }  // namespace dev
 }  // namespace thmm
 

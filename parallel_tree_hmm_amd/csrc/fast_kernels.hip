@@ -46,6 +46,26 @@ static_assert(Team<8>::kSteps <= Team<8>::kLanesPerRow && Team<16>::kSteps <= Te
                   Team<32>::kSteps <= Team<32>::kLanesPerRow && Team<64>::kSteps <= Team<64>::kLanesPerRow,
               "row ownership");
 
+// exp of the team's quad of a K-vector: the lanes sharing a quad (same
+// l % kLpr) each exponentiate one element and exchange the results, so every
+// element is exponentiated once per team.
+template <typename T, int kT, int kLpr>
+__device__ inline void TeamExp(const Quad<T> &v, int l, T (&e)[4]) {
+  static_assert(kT / kLpr >= 4, "a quad needs four lanes");
+  const T mine = Exp(v.v[(l / kLpr) % 4]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) e[i] = __shfl(mine, l % kLpr + kLpr * i, kT);
+}
+
+// log of a positive value: FP32 mantissa logarithm plus the exact exponent
+// term in FP64 (absolute error about 6e-8 whatever the magnitude).
+__device__ inline double LogPositive(float v) {
+  int exponent = 0;
+  const float mantissa = frexpf(v, &exponent);
+  return static_cast<double>(logf(mantissa)) + 0.6931471805599453 * exponent;
+}
+__device__ inline double LogPositive(double v) { return log(v); }
+
 // branch[p] = log sum_c path[p][c] exp(leaf[c]) (src/inference.cc:704-723)
 template <typename T, int K>
 __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, std::uint32_t offset,
@@ -62,8 +82,7 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
   const int c0 = (l % kLpr) * 4;
   const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
   T e[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) e[i] = Exp(vq.v[i]);
+  TeamExp<T, kT, kLpr>(vq, l, e);
   const T *P = Slot<T>(buf, d, op.slot, b);
   // After the row reduction all kLpr lanes of a row hold its sum; the lane
   // with l % kLpr == j % kLpr takes the logarithm, so each row's log is
@@ -82,8 +101,7 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
 #pragma unroll
   for (int j = 0; j < kJ; ++j)
     if (j % kLpr == l % kLpr) row = j * kRps + l / kLpr, part = sums[j];
-  const double message =
-      row >= 0 && part > T(0) ? RowScale(buf, d, op.slot, b, row) + log(static_cast<double>(part)) : kNegInf;
+  const double message = row >= 0 && part > T(0) ? RowScale(buf, d, op.slot, b, row) + LogPositive(part) : kNegInf;
   double top = message;
 #pragma unroll
   for (int o = kT / 2; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kT));
@@ -111,8 +129,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   const int c0 = (l % kLpr) * 4;
   const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
   T e[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) e[i] = Exp(vq.v[i]);
+  TeamExp<T, kT, kLpr>(vq, l, e);
   const T *P = Slot<T>(buf, d, op.slot, b);
   T *adj_p = SlotAdjoint<T>(buf, d, op.slot, b);
   const T *adj_m = RakeAdjoint<T>(buf, d, op, b);
@@ -120,7 +137,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {
     const std::uint32_t quad = static_cast<std::uint32_t>(j * kT + l) * 4;
-    const Quad<T> p = LoadQuad(P + quad);
+    const Quad<T> p = LoadQuadStream(P + quad);
     T t[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) t[i] = p.v[i] * e[i];
@@ -136,7 +153,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
       out.v[i] = static_cast<T>(c);
       column[i] += c;
     }
-    StoreQuad(adj_p + quad, out);
+    StoreQuadStream(adj_p + quad, out);
   }
 #pragma unroll
   for (int o = kLpr; o < kT; o <<= 1)
@@ -202,12 +219,11 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
     const T *P = Slot<T>(buf, d, op.slot, b);
     Quad<T> pq[kJ];
 #pragma unroll
-    for (int j = 0; j < kJ; ++j) pq[j] = LoadQuad(P + static_cast<std::uint32_t>(j * kT + l) * 4);
+    for (int j = 0; j < kJ; ++j) pq[j] = LoadQuadStream(P + static_cast<std::uint32_t>(j * kT + l) * 4);
     const double leaf_scale = buf.node_scale[leaf_at];
     const double row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
     T e[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) e[k] = Exp(vq.v[k]);
+    TeamExp<T, kT, kLpr>(vq, l, e);
     T mine = T(0);
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
@@ -216,7 +232,7 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
       for (int o = kLpr / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kT);
       if (j == owned) mine = part;
     }
-    acc += (row >= 0 && mine > T(0)) ? leaf_scale + row_scale + log(static_cast<double>(mine)) : kNegInf;
+    acc += (row >= 0 && mine > T(0)) ? leaf_scale + row_scale + LogPositive(mine) : kNegInf;
     op = next;
   }
   if (chunk.partial == kDirectChunk)
@@ -274,13 +290,18 @@ __device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
   return acc;
 }
 
-__device__ inline void LoadColumns(const float *X, int lane, float (&out)[4]) {
+// kStream: the kernel's last read of these values (non-temporal loads).
+template <bool kStream = false> __device__ inline void LoadColumns(const float *X, int lane, float (&out)[4]) {
 #pragma unroll
-  for (int r = 0; r < 4; ++r) out[r] = X[(4 * (lane / 16) + r) * 16 + lane % 16];
+  for (int r = 0; r < 4; ++r) {
+    const float *at = X + (4 * (lane / 16) + r) * 16 + lane % 16;
+    out[r] = kStream ? __builtin_nontemporal_load(at) : *at;
+  }
 }
 
-__device__ inline void LoadRows(const float *X, int lane, float (&out)[4]) {
-  const Quad<float> q = LoadQuad(X + (lane % 16) * 16 + 4 * (lane / 16));
+template <bool kStream = false> __device__ inline void LoadRows(const float *X, int lane, float (&out)[4]) {
+  const float *at = X + (lane % 16) * 16 + 4 * (lane / 16);
+  const Quad<float> q = kStream ? LoadQuadStream(at) : LoadQuad(at);
 #pragma unroll
   for (int r = 0; r < 4; ++r) out[r] = q.v[r];
 }
@@ -301,6 +322,7 @@ struct CompressOperands {
 
 __device__ inline std::uint32_t Uniform(std::uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+template <bool kStream>
 __device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, const LinearBuffers &buf,
                                        std::uint32_t offset, std::uint32_t item, int lane, CompressOperands &o) {
   item = Uniform(item);
@@ -310,8 +332,8 @@ __device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, cons
   o.op.out = Uniform(op.out);
   o.left = Slot<float>(buf, d, o.op.left, o.b);
   o.right = Slot<float>(buf, d, o.op.right, o.b);
-  LoadRows(o.left, lane, o.left_rows);
-  LoadColumns(o.right, lane, o.right_cols);
+  LoadRows<kStream>(o.left, lane, o.left_rows);
+  LoadColumns<kStream>(o.right, lane, o.right_cols);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(o.op.middle) * d.B + o.b;
   o.mid_own = static_cast<const float *>(buf.nodes)[mid_at * 16 + lane % 16];
   o.right_scale_own = RowScale(buf, d, o.op.right, o.b, lane % 16);
@@ -353,7 +375,7 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
 #pragma unroll
   for (int t = 0; t < kItemsPerWave; ++t) {
     const std::uint32_t item = first + t < total ? first + t : first;
-    LoadCompression(ops, d, buf, offset, item, lane, o[t]);
+    LoadCompression<true>(ops, d, buf, offset, item, lane, o[t]);
     left_scale[t] = RowScale(buf, d, o[t].op.left, o[t].b, lane % 16);
     mid_scale[t] = buf.node_scale[static_cast<std::uint64_t>(o[t].op.middle) * d.B + o[t].b];
   }
@@ -372,7 +394,7 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
     Quad<float> q;
 #pragma unroll
     for (int r = 0; r < 4; ++r) q.v[r] = s[r] * inverse;
-    StoreQuad(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
+    StoreQuadStream(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
     // top lies in (0, 16]: its FP32 logarithm is exact to an ulp of a number
     // of order one; the sum of scales stays FP64.
     if (lane < 16)
@@ -404,11 +426,11 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
 #pragma unroll
   for (int t = 0; t < kItemsPerWave; ++t) {
     const std::uint32_t item = first + t < total ? first + t : first;
-    LoadCompression(ops, d, buf, offset, item, lane, o[t]);
-    LoadRows(o[t].right, lane, right_rows[t]);
-    LoadColumns(o[t].left, lane, left_cols[t]);
+    LoadCompression<false>(ops, d, buf, offset, item, lane, o[t]);
+    LoadRows<true>(o[t].right, lane, right_rows[t]);
+    LoadColumns<true>(o[t].left, lane, left_cols[t]);
     const std::uint64_t out_at = static_cast<std::uint64_t>(o[t].op.out - d.E) * d.B + o[t].b;
-    LoadRows(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj[t]);
+    LoadRows<true>(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj[t]);
   }
   float e[kItemsPerWave][4], e_own[kItemsPerWave];
   double g[kItemsPerWave], gamma[kItemsPerWave];
@@ -466,8 +488,8 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
       adj_right.v[r] = static_cast<float>(right);
       row_total += right;
     }
-    StoreQuad(SlotAdjoint<float>(buf, d, op.left, b) + (lane % 16) * 16 + col, adj_left);
-    StoreQuad(SlotAdjoint<float>(buf, d, op.right, b) + (lane % 16) * 16 + col, adj_right);
+    StoreQuadStream(SlotAdjoint<float>(buf, d, op.left, b) + (lane % 16) * 16 + col, adj_left);
+    StoreQuadStream(SlotAdjoint<float>(buf, d, op.right, b) + (lane % 16) * 16 + col, adj_right);
     row_total += __shfl_xor(row_total, 16, 64);
     row_total += __shfl_xor(row_total, 32, 64);
     if (lane < 16)
