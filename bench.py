@@ -74,6 +74,23 @@ def forest_workload(trees: int, low: int, high: int, seed: int, scale: float = 1
     return parts
 
 
+def yule_tree(leaves: int, rng):
+    """Random full binary tree: split a uniform random leaf until `leaves`."""
+    parents = np.empty(2 * leaves - 1, np.int64)
+    parents[0] = -1
+    current = [0]
+    size = 1
+    for draw in rng.integers(0, 1 << 62, size=leaves - 1):
+        i = int(draw % len(current))
+        leaf = current[i]
+        parents[size] = leaf
+        parents[size + 1] = leaf
+        current[i] = size
+        current.append(size + 1)
+        size += 2
+    return parents
+
+
 def concatenate(parts):
     offsets = np.zeros(len(parts) + 1, np.int64)
     offsets[1:] = np.cumsum([p.size for p in parts])
@@ -262,12 +279,103 @@ def parity_sample(thmm, oracle, parts, offsets, nodes, edges, k, log_z, node_out
 # ---------------------------------------------------------------------------
 
 
+def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
+    """Random full binary (Yule) trees of 16,383 nodes, K = 64, log node and
+    edge potentials N(0, 1), FP32: log Z plus the gradients of every node and
+    edge log potential through the forward/backward pair. A step is one
+    forward + backward over the forest."""
+    k = 64
+    seed = 0x54484D4D00000005
+    rng = np.random.default_rng(seed)
+    trees = max(1, int(round(256 * args.scale)))
+    t0 = time.perf_counter()
+    parts = [yule_tree(8192, rng) for _ in range(trees)]
+    parents, offsets = concatenate(parts)
+    plan = thmm.Plan(parents)
+    n, e, c = plan.num_nodes, plan.num_edges, plan.num_compressions
+    log(f"forest: {trees} Yule trees, {n} nodes, {c} compressions; built and planned in {time.perf_counter() - t0:.1f}s")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed & 0x7FFFFFFFFFFFFFFF)
+    log_nodes = torch.randn((1, n, k), device="cuda", generator=gen)
+    log_edges = torch.randn((e, k, k), device="cuda", generator=gen)
+    grad_nodes = torch.empty((1, n, k), device="cuda")
+    grad_edges = torch.empty((e, k, k), device="cuda")
+    log_z = torch.empty(plan.roots.size, dtype=torch.float64, device="cuda")
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, 1, precision="f32", device_resident=True)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        ws.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), 1, log_z.data_ptr(), stream)
+        ws.logz_backward_device(0, grad_nodes.data_ptr(), grad_edges.data_ptr(), False, stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(args.steps):
+        step()
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / args.steps
+    value = n / (ms * 1e-3)
+    # Algorithmic bytes of one step (values only, s = 4): log-edge staging
+    # (read + write), node staging, then per rake / compression the forward
+    # and reverse figures of op_bytes (kind 0 rakes, 3 compressions).
+    s4, rakes = 4, n - c - plan.roots.size
+    step_bytes = (2 * e * k * k + 2 * n * k) * s4 + rakes * (op_bytes(1, 0, k, s4) + op_bytes(3, 0, k, s4)) + c * (
+        op_bytes(1, 3, k, s4) + op_bytes(3, 3, k, s4)
+    )
+    achieved = step_bytes / (ms * 1e-3) / 1e9
+    log(f"K=64 FP32 adjoint pair: {ms:.2f} ms/step, {value:.4g} nodes/s, {achieved:.0f} GB/s over the whole step")
+    # Parity on two whole trees against the FP64 oracle, and the
+    # size-independent normalisation over the forest.
+    worst = {"log_z_rel": 0.0, "node_grad_abs": 0.0, "edge_grad_abs": 0.0}
+    lz_host = log_z.cpu().numpy()
+    for t in (0, trees - 1):
+        lo, hi, elo, ehi = tree_slices(offsets, t)
+        tree_plan = thmm.Plan(parts[t])
+        nn = torch.exp(log_nodes[0, lo:hi].double()).cpu().numpy()
+        ee = torch.exp(log_edges[elo:ehi].double()).cpu().numpy()
+        olz, onm, oem = oracle.marginals(tree_plan, k, nn, ee)
+        worst["log_z_rel"] = max(worst["log_z_rel"], abs(lz_host[t] - olz[0]) / max(1.0, abs(olz[0])))
+        worst["node_grad_abs"] = max(worst["node_grad_abs"], float(np.max(np.abs(grad_nodes[0, lo:hi].double().cpu().numpy() - onm))))
+        worst["edge_grad_abs"] = max(worst["edge_grad_abs"], float(np.max(np.abs(grad_edges[elo:ehi].double().cpu().numpy() - oem))))
+    worst["node_normalisation_abs"] = float((grad_nodes[0].double().sum(1) - 1.0).abs().max())
+    worst["trees_checked"] = 2
+    log(f"parity: {worst}")
+    result = {
+        "metric": "nodes/s log Z + adjoint (dense-state forest, log potentials)",
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (torch device RNG: log node and edge potentials N(0,1))",
+        "config": {"workload": f"{trees} random full binary (Yule) trees of 16383 nodes, K=64, FP32 log Z + gradients of "
+                   "all node and edge log potentials", "nodes": int(n), "compressions": int(c), "scale": args.scale},
+        "roofline": {"kernel": "whole step", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_step": int(step_bytes)},
+        "parity": worst,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=4, choices=[4])
+    ap.add_argument("--config", type=int, default=4, choices=[4, 5],
+                    help="4: the uneven K=16 forest (default); 5: the K=64 Yule forest through the adjoint pair")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the trees (quick runs)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -294,6 +402,8 @@ def main() -> int:
         print("no HIP device visible", file=sys.stderr)
         return 1
     log(f"device: {thmm.device_description(0)}")
+    if args.config == 5:
+        return run_dense_state_adjoint(args, thmm, oracle, torch)
 
     if args.copy_calibration:
         # One plain device-to-device copy of known size: reading FETCH_SIZE and

@@ -211,9 +211,10 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
   const int row = owned >= 0 ? owned * kRps + l / kLpr : -1;
   const int c0 = (l % kLpr) * 4;
   double acc = 0.0;
-  DeviceRake op = g.rakes[chunk.first];
+  DeviceRake op{chunk.leaf0, chunk.slot0, 0, 0};
   for (std::uint32_t i = 0; i < chunk.count; ++i) {
-    const DeviceRake next = g.rakes[chunk.first + (i + 1 < chunk.count ? i + 1 : i)];
+    DeviceRake next = op;
+    if (i + 1 < chunk.count) next = g.rakes[chunk.first + i + 1];
     const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
     const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
     const T *P = Slot<T>(buf, d, op.slot, b);
@@ -516,48 +517,57 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
 // and S, H, Q come out in row layout: rows reduce over 4 lanes, stores are
 // 16-byte quads. Gh/Gt pass through one LDS tile per wavefront.
 
-// Middle weights e[m] for m = 16J + 4g + r; returns W (-inf: no live state).
+// Middle weights: lane l forms e[m] for m = l % K (*e_own) and gathers those
+// of m = 16J + 4g + r into e[J][r]; returns W (-inf: no live state).
 template <int K>
 __device__ inline double MiddleWeightsWide(const LinearBuffers &buf, const Dims &d, const DeviceCompress &op,
-                                           std::uint32_t b, int g, float (&e)[K / 16][4]) {
+                                           std::uint32_t b, int lane, float (&e)[K / 16][4], float *e_own) {
   constexpr int NB = K / 16;
+  const int g = lane / 16;
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-  double w[NB][4];
-  double top = kNegInf;
+  const int m = lane % K;
+  const double w = static_cast<double>(static_cast<const float *>(buf.nodes)[mid_at * K + m]) +
+                   RowScale(buf, d, op.right, b, m);
+  double top = w;
 #pragma unroll
-  for (int J = 0; J < NB; ++J) {
-    const Quad<float> m = LoadQuad(static_cast<const float *>(buf.nodes) + mid_at * K + 16 * J + 4 * g);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      w[J][r] = static_cast<double>(m.v[r]) + RowScale(buf, d, op.right, b, 16 * J + 4 * g + r);
-      top = fmax(top, w[J][r]);
-    }
-  }
-  top = fmax(top, __shfl_xor(top, 16, 64));
-  top = fmax(top, __shfl_xor(top, 32, 64));
+  for (int x = 1; x < K; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
+  *e_own = top > kNegInf ? expf(static_cast<float>(w - top)) : 0.0f;
 #pragma unroll
   for (int J = 0; J < NB; ++J)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) e[J][r] = top > kNegInf ? expf(static_cast<float>(w[J][r] - top)) : 0.0f;
+    for (int r = 0; r < 4; ++r) e[J][r] = __shfl(*e_own, 16 * J + 4 * g + r, 64);
   return top;
 }
 
-template <int K>
+// kStream: the kernel's last read of these values (non-temporal loads).
+template <int K, bool kStream = false>
 __device__ inline void LoadRowBand(const float *X, int I, int i, int g, float (&out)[K / 16][4]) {
 #pragma unroll
   for (int J = 0; J < K / 16; ++J) {
-    const Quad<float> q = LoadQuad(X + (16 * I + i) * K + 16 * J + 4 * g);
+    const float *at = X + (16 * I + i) * K + 16 * J + 4 * g;
+    const Quad<float> q = kStream ? LoadQuadStream(at) : LoadQuad(at);
 #pragma unroll
     for (int r = 0; r < 4; ++r) out[J][r] = q.v[r];
   }
 }
 
-template <int K>
+template <int K, bool kStream = false>
 __device__ inline void LoadColumnBand(const float *X, int I, int i, int g, float (&out)[K / 16][4]) {
 #pragma unroll
   for (int J = 0; J < K / 16; ++J)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) out[J][r] = X[(16 * J + 4 * g + r) * K + 16 * I + i];
+    for (int r = 0; r < 4; ++r) {
+      const float *at = X + (16 * J + 4 * g + r) * K + 16 * I + i;
+      out[J][r] = kStream ? __builtin_nontemporal_load(at) : *at;
+    }
+}
+
+// The op of a wavefront's item with its fields in scalar registers.
+__device__ inline DeviceCompress UniformOp(const DeviceOps &ops, std::uint32_t index) {
+  const DeviceCompress op = ops.compresses[index];
+  DeviceCompress u{};
+  u.left = Uniform(op.left), u.middle = Uniform(op.middle), u.right = Uniform(op.right), u.out = Uniform(op.out);
+  return u;
 }
 
 template <int K>
@@ -577,17 +587,17 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
                                                                            std::uint32_t count, LinearBuffers buf) {
   constexpr int NB = K / 16;
   const int lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
-  const std::uint32_t item = blockIdx.x * kWideWaves + threadIdx.x / 64;
+  const std::uint32_t item = Uniform(blockIdx.x * kWideWaves + threadIdx.x / 64);
   if (item >= count * d.B) return;
   const std::uint32_t b = item % d.B;
-  const DeviceCompress op = ops.compresses[offset + item / d.B];
+  const DeviceCompress op = UniformOp(ops, offset + item / d.B);
   const float *L = Slot<float>(buf, d, op.left, b);
   const float *R = Slot<float>(buf, d, op.right, b);
-  float e[NB][4];
-  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, g, e);
+  float e[NB][4], e_own;
+  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
   float rc[NB][NB][4];  // R column layout: rc[Ic] is the band Ic of R^T
 #pragma unroll
-  for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K>(R, Ic, i, g, rc[Ic]);
+  for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K, true>(R, Ic, i, g, rc[Ic]);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
   const double mid_scale = buf.node_scale[mid_at];
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
@@ -595,7 +605,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
 #pragma unroll 1
   for (int Ip = 0; Ip < NB; ++Ip) {
     float lb[NB][4];
-    LoadRowBand<K>(L, Ip, i, g, lb);
+    LoadRowBand<K, true>(L, Ip, i, g, lb);
     const double left_scale = RowScale(buf, d, op.left, b, 16 * Ip + i);
 #pragma unroll
     for (int J = 0; J < NB; ++J)
@@ -617,11 +627,11 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
       Quad<float> q;
 #pragma unroll
       for (int r = 0; r < 4; ++r) q.v[r] = s[Ic][r] * inverse;
-      StoreQuad(out + (16 * Ip + i) * K + 16 * Ic + 4 * g, q);
+      StoreQuadStream(out + (16 * Ip + i) * K + 16 * Ic + 4 * g, q);
     }
     if (g == 0)
       buf.path_scale[out_at * K + 16 * Ip + i] =
-          live ? left_scale + mid_scale + w_top + log(static_cast<double>(top)) : kNegInf;
+          live ? left_scale + mid_scale + w_top + static_cast<double>(logf(top)) : kNegInf;
   }
 }
 
@@ -633,18 +643,18 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   __shared__ float tiles[kWideWaves][K * kStride];
   __shared__ double row_ratio[kWideWaves][K];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
-  const std::uint32_t item = blockIdx.x * kWideWaves + wave;
+  const std::uint32_t item = Uniform(blockIdx.x * kWideWaves + wave);
   if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
   float *G = tiles[wave];
   double *gs = row_ratio[wave];
   const std::uint32_t b = item % d.B;
-  const DeviceCompress op = ops.compresses[offset + item / d.B];
+  const DeviceCompress op = UniformOp(ops, offset + item / d.B);
   const float *L = Slot<float>(buf, d, op.left, b);
   const float *R = Slot<float>(buf, d, op.right, b);
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
   const float *adj_out = static_cast<const float *>(buf.path_adj) + out_at * K * K;
-  float e[NB][4];
-  MiddleWeightsWide<K>(buf, d, op, b, g, e);
+  float e[NB][4], e_own;
+  MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
 
   // S (recomputed), Gr = adj/S, g_p, Gh = Gr/g_p -> LDS; Gamma = max_p g_p.
   double gamma = 0.0;
@@ -656,7 +666,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
     for (int Ip = 0; Ip < NB; ++Ip) {
       float lb[NB][4], ab[NB][4];
       LoadRowBand<K>(L, Ip, i, g, lb);
-      LoadRowBand<K>(adj_out, Ip, i, g, ab);
+      LoadRowBand<K, true>(adj_out, Ip, i, g, ab);
 #pragma unroll
       for (int J = 0; J < NB; ++J)
 #pragma unroll
@@ -691,7 +701,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 
   float rr[NB][NB][4];  // R row layout: rr[Im] is the band Im of R
 #pragma unroll
-  for (int Im = 0; Im < NB; ++Im) LoadRowBand<K>(R, Im, i, g, rr[Im]);
+  for (int Im = 0; Im < NB; ++Im) LoadRowBand<K, true>(R, Im, i, g, rr[Im]);
 
   // adj_L[p][m] = L'[p][m] g_p H[p][m], H^T = R Gh^T.
   float *adj_left = SlotAdjoint<float>(buf, d, op.left, b);
@@ -712,7 +722,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
       for (int r = 0; r < 4; ++r)
         q.v[r] = static_cast<float>(static_cast<double>(lb[Im][r]) * static_cast<double>(e[Im][r]) * g_row *
                                     static_cast<double>(h[r]));
-      StoreQuad(adj_left + (16 * Ip + i) * K + 16 * Im + 4 * g, q);
+      StoreQuadStream(adj_left + (16 * Ip + i) * K + 16 * Im + 4 * g, q);
     }
   }
 
@@ -729,11 +739,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 #pragma unroll 1
   for (int Im = 0; Im < NB; ++Im) {
     float lc[NB][4];
-    LoadColumnBand<K>(L, Im, i, g, lc);
-    // e[m] for m = 16Im + i (the row this lane holds of Q and adj_R) lives in
-    // register (Im, i % 4) of the lanes with g = i / 4; the source lane
-    // 16(i/4) + i has the same i, so it evaluates the same register index.
-    const float e_row = __shfl(e[Im][i % 4], 16 * (i / 4) + i, 64);
+    LoadColumnBand<K, true>(L, Im, i, g, lc);
+    // e[m] for m = 16Im + i (the row this lane holds of Q and adj_R).
+    const float e_row = __shfl(e_own, 16 * Im + i, 64);
     double row_total = 0.0;
 #pragma unroll
     for (int Ic = 0; Ic < NB; ++Ic) {
@@ -751,7 +759,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
         out.v[r] = static_cast<float>(v);
         row_total += v;
       }
-      StoreQuad(adj_right + (16 * Im + i) * K + 16 * Ic + 4 * g, out);
+      StoreQuadStream(adj_right + (16 * Im + i) * K + 16 * Ic + 4 * g, out);
     }
     row_total += __shfl_xor(row_total, 16, 64);
     row_total += __shfl_xor(row_total, 32, 64);

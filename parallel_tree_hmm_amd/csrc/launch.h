@@ -128,8 +128,11 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw,
 // summed in FP64 and either absorbed into the parent at once (partial ==
 // kDirectChunk) or written to partials[partial][b][K] for a segment pass.
 constexpr std::uint32_t kDirectChunk = 0xFFFFFFFFu;
+// leaf0/slot0 repeat the chunk's first rake, so that a one-rake chunk needs
+// no second dependent load before its data.
 struct DeviceGatherChunk {
   std::uint32_t first, count, parent, partial;
+  std::uint32_t leaf0, slot0, pad0, pad1;
 };
 // Sums `count` consecutive partials into partial `out`, or, for out ==
 // kDirectChunk, absorbs the sum into `parent`.

@@ -325,7 +325,9 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
       const std::uint32_t first_partial = partials;
       for (std::size_t c = i; c < end; c += chunk_size) {
         const std::uint32_t count = static_cast<std::uint32_t>(std::min<std::size_t>(chunk_size, end - c));
-        chunks.push_back({static_cast<std::uint32_t>(sorted.size()), count, parent, direct ? kDirectChunk : partials++});
+        const DeviceRake &head = device_rakes[keyed[c].second];
+        chunks.push_back({static_cast<std::uint32_t>(sorted.size()), count, parent, direct ? kDirectChunk : partials++,
+                          head.leaf, head.slot, 0, 0});
         for (std::size_t k = c; k < c + count; ++k) sorted.push_back(device_rakes[keyed[k].second]);
       }
       if (!direct) open.push_back({parent, first_partial, partials - first_partial});
@@ -631,7 +633,8 @@ void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const Line
 // The sum-product forward pass: the fused program when the workspace has
 // one, else the plan's levels.
 void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
-  if (w.fused.empty()) {
+  // The fused kernels read edge slots with 16-byte accesses.
+  if (w.fused.empty() || (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) != 0) {
     ForwardLevels(l, w.levels, buf, profiler);
     return;
   }

@@ -216,3 +216,34 @@ def test_log_potential_forward_backward(require_gpu, precision):
     g = grad.cpu().numpy()
     assert np.max(np.abs(g_nodes.cpu().double().numpy() - g[:, None, None] * onm)) <= tol * 2
     assert np.max(np.abs(g_edges.cpu().double().numpy() - np.einsum("b,bekl->ekl", g, oem))) <= tol * 4
+
+
+def test_unaligned_device_buffers_fall_back(require_gpu):
+    """Edge potentials and outputs that are only 4-byte aligned take the
+    generic kernels (the coalesced ones use 16-byte accesses) and agree with
+    the aligned call."""
+    import torch
+
+    rng = np.random.default_rng(17)
+    plan = thmm.Plan(forest([random_recursive(300, rng), star(200), caterpillar(201)]))
+    k = 16
+    nodes, edges = random_potentials(rng, 1, plan.num_nodes, plan.num_edges, k)
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, 1, precision="f32", device_resident=True)
+    d_nodes = torch.tensor(nodes, dtype=torch.float32, device="cuda")
+    storage = torch.zeros(plan.num_edges * k * k + 1, dtype=torch.float32, device="cuda")
+    shifted = storage[1:]  # 4 bytes past a 256-byte boundary
+    shifted.copy_(torch.tensor(edges.ravel(), dtype=torch.float32))
+    aligned = shifted.clone()
+    results = []
+    for e in (aligned, shifted):
+        assert (e.data_ptr() % 16 == 0) == (e is aligned)
+        lz = torch.empty(plan.roots.size, dtype=torch.float64, device="cuda")
+        nm = torch.empty(plan.num_nodes * k + 1, dtype=torch.float32, device="cuda")[1:]
+        em = torch.empty(plan.num_edges * k * k + 1, dtype=torch.float32, device="cuda")[1:]
+        ws.marginals_device(d_nodes.data_ptr(), e.data_ptr(), 1, lz.data_ptr(), nm.data_ptr(), em.data_ptr(), 0)
+        torch.cuda.synchronize()
+        results.append((lz.cpu().numpy(), nm.cpu().numpy(), em.cpu().numpy()))
+    (lz0, n0, e0), (lz1, n1, e1) = results
+    assert np.max(np.abs(lz0 - lz1) / np.maximum(1, np.abs(lz0))) <= 1e-6
+    assert np.max(np.abs(n0 - n1)) <= 1e-6 and np.max(np.abs(e0 - e1)) <= 1e-6
