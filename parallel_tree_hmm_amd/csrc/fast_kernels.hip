@@ -776,7 +776,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 // reads are contiguous and, for X = 1 or Y = 1 (a forest, or one tree), so
 // are writes. kMode 0 copies; kMode 1 takes logs of linear potentials and
 // kMode 2 log potentials, both then shifted by the vector's maximum, which
-// goes to `scale` in FP64 (-inf for a vector without support).
+// goes to `scale` in FP64 (-inf for a vector without support); kMode 3 turns
+// log potentials into exp(v - max) with the same FP64 scale (edge rows).
 template <typename T> __device__ inline T LogOf(T v) { return v > T(0) ? log(v) : NegInf<T>(); }
 template <> __device__ inline float LogOf(float v) { return v > 0.0f ? logf(v) : NegInf<float>(); }
 
@@ -784,13 +785,13 @@ template <typename In, typename Out, int kMode, int K>
 __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
                                                      double *scale) {
   constexpr int kLanes = K / 4;
-  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const std::uint32_t v = tid / kLanes;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t v = tid / kLanes;
   const int l = static_cast<int>(tid % kLanes);
-  if (v >= X * Y) return;
-  const std::uint32_t x = v / Y, y = v % Y;
-  const std::uint64_t to = static_cast<std::uint64_t>(y) * X + x;
-  const Quad<In> q = LoadQuad(in + static_cast<std::uint64_t>(v) * K + l * 4);
+  if (v >= static_cast<std::uint64_t>(X) * Y) return;
+  const std::uint64_t x = v / Y, y = v % Y;
+  const std::uint64_t to = y * X + x;
+  const Quad<In> q = LoadQuad(in + v * K + l * 4);
   // Logs are taken in the wider of the two element types.
   using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
   Quad<Out> o;
@@ -806,7 +807,12 @@ __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std
     for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
     const bool live = top > NegInf<W>();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (kMode == 3)
+        o.v[i] = live ? Exp(static_cast<Out>(w[i] - top)) : Out(0);
+      else
+        o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+    }
     if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
   }
   StoreQuad(out + to * K + l * 4, o);
@@ -943,7 +949,7 @@ namespace {
 template <typename In, typename Out, int kMode>
 bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
   const std::uint64_t threads = static_cast<std::uint64_t>(X) * Y * (l.dims.K / 4);
-  if (threads >= (1ull << 32)) return false;
+  if (threads / kBlock >= (1ull << 31)) return false;  // grid size limit
   const unsigned grid = Blocks(threads);
   const auto *i = static_cast<const In *>(in);
   auto *o = static_cast<Out *>(out);
@@ -985,6 +991,10 @@ bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real,
       if (f_in != f_out) return false;
       return f_in ? TransposeTeamFor<float, float, 2>(l, in, out, X, Y, scale)
                   : TransposeTeamFor<double, double, 2>(l, in, out, X, Y, scale);
+    case 3:
+      if (f_in != f_out) return false;
+      return f_in ? TransposeTeamFor<float, float, 3>(l, in, out, X, Y, scale)
+                  : TransposeTeamFor<double, double, 3>(l, in, out, X, Y, scale);
   }
   return false;
 }

@@ -822,6 +822,12 @@ void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes
 
 void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges, double *edge_scale) {
   if (l.dims.E == 0) return;
+  // Rows are K-vectors: the coalesced team kernel over E*K of them.
+  const Real real = l.real;
+  if (LaunchFastTranspose(l, 3, real, real, log_input, edges, 1, l.dims.E * l.dims.K, edge_scale)) {
+    Check(hipGetLastError(), "log edge initialisation launch");
+    return;
+  }
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.E) * l.dims.K, kThreads);
   if (l.real == Real::kF32)
     InitEdgesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(log_input),
