@@ -415,6 +415,22 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
         log(f"copy calibration: {count * 4} bytes read, {count * 4} bytes written")
+        # Streaming references on the same buffers (event-timed, best of 5):
+        # what plain library kernels reach for read+write and read-only.
+        refs = {}
+        for name, fn, moved in (("copy", lambda: dst.copy_(src), 8 * count),
+                                ("add_in_place", lambda: dst.add_(1.0), 8 * count),
+                                ("sum", lambda: src.sum(), 4 * count)):
+            best = float("inf")
+            for _ in range(5):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                fn()
+                b.record()
+                torch.cuda.synchronize()
+                best = min(best, a.elapsed_time(b))
+            refs[name] = round(moved / (best * 1e-3) / 1e9, 1)
+        log(f"streaming references (GB/s): {refs}")
         del src, dst
         torch.cuda.empty_cache()
 
