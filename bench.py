@@ -113,6 +113,11 @@ def concatenate(parts):
 # slot adjoint (an edge marginal) and the leaf adjoint; combination and
 # absorption copy one adjoint vector; compression reads L, R, the middle
 # vector and the path adjoint and writes the adjoints of L, R and the middle.
+# A grouped compression of n slots (primitive 6) reads the n slots and the
+# n - 1 middle vectors and writes one path; its reverse reads the slots, the
+# middles and the path adjoint and writes n slot adjoints and n - 1 middle
+# adjoints. Its levels report these as K-vectors moved per item (reverse
+# levels split into groups of at most 4 slots, primitive 7, and the rest).
 # ---------------------------------------------------------------------------
 
 
@@ -120,9 +125,10 @@ def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
     kk = k * k
     if phase == 1:  # forward (4: fused rake level, per rake; 5: partial sums, per FP64 partial read)
         return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s, 4: (2 * k + kk) * s,
-                5: 8 * k}[primitive]
+                5: 8 * k, 6: k * s}[primitive]
     if phase == 3:  # reverse
-        return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s}[primitive]
+        return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s, 6: k * s,
+                7: k * s}[primitive]
     return 0
 
 
@@ -133,10 +139,13 @@ KERNEL_NAMES = {
     (1, 3): "CompressForward",
     (1, 4): "RakeGather",
     (1, 5): "GatherSegments",
+    (1, 6): "ChainForward",
     (3, 0): "RakeReverse",
     (3, 1): "CombineReverse",
     (3, 2): "AbsorbReverse",
     (3, 3): "CompressReverse",
+    (3, 6): "ChainReverse",
+    (3, 7): "ChainReverseSmall",
 }
 
 
@@ -202,6 +211,8 @@ def roofline_from(timings, k, s):
     ms, nbytes, launches = totals[key]
     achieved = nbytes / (ms * 1e-3) / 1e9
     breakdown = {KERNEL_NAMES[kk]: round(v[0], 4) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0])}
+    gbs = {KERNEL_NAMES[kk]: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0])
+           if v[0] > 0 and v[1] > 0}
     return {
         "kernel": KERNEL_NAMES[key],
         "bound": "hbm",
@@ -214,6 +225,7 @@ def roofline_from(timings, k, s):
         "avg_launch_us": round(ms / launches * 1e3, 3),
         "algorithmic_bytes_per_launch": int(nbytes / launches),
         "kernel_ms_breakdown": breakdown,
+        "kernel_algorithmic_gbs": gbs,
     }
 
 

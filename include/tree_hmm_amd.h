@@ -372,12 +372,17 @@ thmm_status thmm_synchronize(thmm_workspace *workspace);
  * compress_items_forward / _reverse: K = 16 FP32 MFMA compressions per
  * wavefront (1 or 2); rake_block: threads per block of the team rake kernels
  * (64, 128 or 256); gather_chunk: rakes a team folds into one parent per
- * step of the fused forward pass (1..64). Values outside these sets are
- * invalid_argument. Setting them drops captured graphs; a new gather_chunk
- * rebuilds the fused forward program. ms is filled by the autotuner. */
+ * step of the fused forward pass (1..64); compress_depth: compression-tree
+ * levels evaluated as one grouped product of up to 2^depth slots on the
+ * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2).
+ * Values outside these sets are invalid_argument. Setting them drops
+ * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
+ * forward program, and a new compress_depth also drops a held
+ * thmm_logz_forward_device pass. ms is filled by the autotuner. */
 typedef struct thmm_tuning {
   uint32_t compress_items_forward, compress_items_reverse;
   uint32_t rake_block, gather_chunk;
+  uint32_t compress_depth;
   double ms;
 } thmm_tuning;
 
@@ -404,7 +409,11 @@ thmm_status thmm_autotune_marginals_device(
  * is a thmm_primitive for phases 1 and 3, or, in phase 1, 4 for a fused
  * rake level (rakes, combinations and absorptions of a round in one launch;
  * count = rakes) and 5 for its partial-absorption level (count = partials);
- * otherwise count is the level's op count. */
+ * in phases 1 and 3, 6 for a level of grouped compressions (count = K-element
+ * vectors read and written per batch item: (n + 1) K + n - 1 forward and
+ * (2n + 1) K + 2 (n - 1) reverse for a group of n slots), and in phase 3, 7
+ * for the part of such a level whose groups have at most 4 slots; otherwise
+ * count is the level's op count. */
 typedef struct thmm_launch_timing {
   uint32_t phase, primitive, count, batch;
   double ms;

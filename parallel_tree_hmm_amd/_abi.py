@@ -119,6 +119,7 @@ class Tuning(ctypes.Structure):
         ("compress_items_reverse", c_u32),
         ("rake_block", c_u32),
         ("gather_chunk", c_u32),
+        ("compress_depth", c_u32),
         ("ms", ctypes.c_double),
     ]
 

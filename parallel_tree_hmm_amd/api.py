@@ -467,7 +467,8 @@ class Workspace:
         return [(t.phase, t.primitive, t.count, t.batch, t.ms) for t in timings[: min(count.value, capacity)]]
 
     # -- launch tuning -----------------------------------------------------
-    _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk")
+    _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk",
+                      "compress_depth")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()

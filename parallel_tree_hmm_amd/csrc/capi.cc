@@ -66,7 +66,7 @@ thmm::Workspace &W(thmm_workspace *workspace) {
 }
 
 thmm_tuning ToC(const thmm::Tuning &t, double ms) {
-  return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, ms};
+  return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, t.compress_depth, ms};
 }
 
 }  // namespace
@@ -280,6 +280,7 @@ thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning
     t.compress_items_reverse = tuning->compress_items_reverse;
     t.rake_block = tuning->rake_block;
     t.gather_chunk = tuning->gather_chunk;
+    t.compress_depth = tuning->compress_depth;
     W(workspace).SetTuning(t);
   });
 }

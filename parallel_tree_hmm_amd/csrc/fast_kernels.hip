@@ -343,8 +343,9 @@ __device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, cons
 // Middle weights e[m] = exp(mid[m] + r_m - W): each lane forms the weight of
 // m = l%16 (e_own) and gathers those of m = 4(l/16) + r into e[r]; returns
 // W = max_m (mid[m] + r_m) (-inf when no middle state is live).
-__device__ inline double MiddleWeights16(const CompressOperands &o, int lane, float (&e)[4], float *e_own) {
-  const double w = static_cast<double>(o.mid_own) + o.right_scale_own;
+__device__ inline double StepWeights16(float mid_own, double right_scale_own, int lane, float (&e)[4],
+                                      float *e_own) {
+  const double w = static_cast<double>(mid_own) + right_scale_own;
   double top = w;
 #pragma unroll
   for (int x = 1; x < 16; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
@@ -352,6 +353,10 @@ __device__ inline double MiddleWeights16(const CompressOperands &o, int lane, fl
 #pragma unroll
   for (int r = 0; r < 4; ++r) e[r] = __shfl(*e_own, 4 * (lane / 16) + r, 64);
   return top;
+}
+
+__device__ inline double MiddleWeights16(const CompressOperands &o, int lane, float (&e)[4], float *e_own) {
+  return StepWeights16(o.mid_own, o.right_scale_own, lane, e, e_own);
 }
 
 // S = L diag(e) R in row layout.
@@ -406,7 +411,9 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
 
 // Compression adjoint on MFMA (src/inference.cc:416-464). With S = L' R,
 // L' = L diag(e), Gr = adj_out / S (FP64 ratios; 0 where adj_out or S is 0),
-// g_p = max_c Gr[p][c], Gh = Gr / g_p, Gamma = max_p g_p, Gt = Gr / Gamma:
+// g_p = max_c |Gr[p][c]|, Gh = Gr / g_p, Gamma = max_p g_p, Gt = Gr / Gamma
+// (magnitudes: an adjoint seeded by a negative upstream gradient keeps its
+// sign through Gh and Gt):
 //   adj_L[p][m] = L'[p][m] g_p (Gh R^T)[p][m]
 //   adj_R[m][c] = R[m][c] e[m] Gamma (L^T Gt)[m][c]
 //   adj_mid[m]  = sum_c adj_R[m][c]
@@ -445,7 +452,7 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       gr[r] = (adj[t][r] != 0.0f && s[r] > 0.0f) ? static_cast<double>(adj[t][r]) / static_cast<double>(s[r]) : 0.0;
-      top = fmax(top, gr[r]);
+      top = fmax(top, fabs(gr[r]));
     }
     top = fmax(top, __shfl_xor(top, 16, 64));
     top = fmax(top, __shfl_xor(top, 32, 64));
@@ -497,6 +504,333 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
       static_cast<float *>(buf.node_adj)[(static_cast<std::uint64_t>(op.middle) * d.B + b) * 16 + lane] =
           static_cast<float>(row_total);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Grouped compressions, K = 16 (one wavefront per (group, item))
+// ---------------------------------------------------------------------------
+//
+// A group's product X_0 D_1 X_1 ... D_{n-1} X_{n-1} is accumulated left to
+// right, T_0 = X_0 and T_j = rownorm(T_{j-1} diag(e_j) X_j), each step being
+// the binary compression above with T_{j-1} as its left operand: T stays in
+// registers in row layout (the layout of the left operand and of the MFMA
+// result), X_j arrives in column layout, and the row scales add up in FP64.
+// The reverse pass recomputes T_0 .. T_{n-2} and runs the binary adjoint
+// steps from j = n-1 down to 1; the adjoint of T_{j-1} stays in registers as
+// the output adjoint of the next step. Adjoints are in marginal form (value
+// times gradient), which does not depend on how T's rows are scaled.
+
+struct ChainView {  // wave-uniform
+  std::uint32_t n, out, b;
+  std::uint32_t slots[kMaxChain], middles[kMaxChain - 1];
+};
+
+__device__ inline void LoadChain(const DeviceOps &ops, const Dims &d, std::uint32_t offset, std::uint32_t item,
+                                 ChainView &c) {
+  item = Uniform(item);
+  c.b = item % d.B;
+  const DeviceChain &src = ops.chains[offset + item / d.B];
+  c.n = Uniform(src.count);
+  c.out = Uniform(src.out);
+#pragma unroll
+  for (std::uint32_t i = 0; i < kMaxChain; ++i) c.slots[i] = Uniform(src.slots[i]);
+#pragma unroll
+  for (std::uint32_t i = 0; i + 1 < kMaxChain; ++i) c.middles[i] = Uniform(src.middles[i]);
+}
+
+__device__ inline float RowMax16(const floatx4 &s) {
+  float top = fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3]));
+  top = fmaxf(top, __shfl_xor(top, 16, 64));
+  return fmaxf(top, __shfl_xor(top, 32, 64));
+}
+
+// Middle node j's value for m = l%16 and the row scale of X_j at m.
+__device__ inline void ChainMiddle(const LinearBuffers &buf, const Dims &d, const ChainView &c, int j, int lane,
+                                   float *mid_own, double *right_scale_own) {
+  const std::uint64_t mid_at = static_cast<std::uint64_t>(c.middles[j - 1]) * d.B + c.b;
+  *mid_own = static_cast<const float *>(buf.nodes)[mid_at * 16 + lane % 16];
+  *right_scale_own = RowScale(buf, d, c.slots[j], c.b, lane % 16);
+}
+
+template <int kMax>
+__global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                          std::uint32_t count, LinearBuffers buf) {
+  const int lane = threadIdx.x % 64;
+  const std::uint32_t item = Uniform(blockIdx.x * kWaves + threadIdx.x / 64);
+  if (item >= count * d.B) return;
+  ChainView c;
+  LoadChain(ops, d, offset, item, c);
+  float acc[4], cols[kMax][4], mid_own[kMax];
+  double right_scale_own[kMax];
+  LoadRows<true>(Slot<float>(buf, d, c.slots[0], c.b), lane, acc);
+  double scale = RowScale(buf, d, c.slots[0], c.b, lane % 16);
+#pragma unroll
+  for (int j = 1; j < kMax; ++j) {
+    if (j >= static_cast<int>(c.n)) break;
+    LoadColumns<true>(Slot<float>(buf, d, c.slots[j], c.b), lane, cols[j]);
+    ChainMiddle(buf, d, c, j, lane, &mid_own[j], &right_scale_own[j]);
+  }
+#pragma unroll
+  for (int j = 1; j < kMax; ++j) {
+    if (j >= static_cast<int>(c.n)) break;
+    float e[4], e_own;
+    const double w_top = StepWeights16(mid_own[j], right_scale_own[j], lane, e, &e_own);
+    float scaled[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) scaled[r] = acc[r] * e[r];
+    const floatx4 s = Mfma(cols[j], scaled);
+    const float top = RowMax16(s);
+    const bool live = top > 0.0f && w_top > kNegInf;
+    const float inverse = live ? 1.0f / top : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = s[r] * inverse;
+    const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(c.middles[j - 1]) * d.B + c.b];
+    scale = live ? scale + mid_scale + w_top + static_cast<double>(logf(top)) : kNegInf;
+  }
+  const std::uint64_t out_at = static_cast<std::uint64_t>(c.out - d.E) * d.B + c.b;
+  Quad<float> q;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) q.v[r] = acc[r];
+  StoreQuadStream(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
+  if (lane < 16) buf.path_scale[out_at * 16 + lane] = scale;
+}
+
+// Row-layout quad of an adjoint; edge marginals may be caller memory
+// without 16-byte alignment (quad_edges == 0), then stored per element.
+__device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                                       int lane, const Quad<float> &q, bool quad_edges) {
+  float *at = SlotAdjoint<float>(buf, d, slot, b) + (lane % 16) * 16 + 4 * (lane / 16);
+  if (slot >= d.E || quad_edges) {
+    StoreQuadStream(at, q);
+  } else {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) __builtin_nontemporal_store(q.v[r], at + r);
+  }
+}
+
+// Reverse pass of one group over a balanced bracketing of its product
+// (matrix products associate): span [lo, hi) splits at mid = (lo + hi) / 2,
+// whose middle node is middles[mid - 1]. Build() forms every span's
+// row-normalised product bottom-up in row and column layout (the operand
+// layouts of the adjoint step), so the spans of one depth are independent
+// and the critical path is log2(n) adjoint steps. Reverse() runs the binary
+// compression adjoint at each split with the span's normalised product P and
+// its row normaliser in place of the recomputed S (S = P / normaliser row by
+// row), top-down, each split with its own LDS tile for the Gt transposition.
+struct Part {
+  float row[4], col[4];  // normalised product: row layout, column layout
+  double scale;          // FP64 log scale of row l%16 (Build only)
+  float inverse;         // normaliser of row l%16 (1 for a stored slot)
+  float e_own;           // split's middle weight e[m], m = l%16
+};
+
+struct GroupState {
+  const ChainView &c;
+  const LinearBuffers &buf;
+  const Dims &d;
+  int lane;
+  float (*tiles)[16 * kPad];  // one per split (index mid - 1)
+  bool quad_edges;
+};
+
+template <int kLo, int kHi, bool kLeaf = (kHi - kLo == 1)> struct Span;
+
+template <int kLo, int kHi> struct Span<kLo, kHi, true> {
+  Part part;
+  __device__ inline void Build(const GroupState &g) {
+    const float *x = Slot<float>(g.buf, g.d, g.c.slots[kLo], g.c.b);
+    LoadRows<true>(x, g.lane, part.row);
+    LoadColumns<false>(x, g.lane, part.col);
+    part.scale = RowScale(g.buf, g.d, g.c.slots[kLo], g.c.b, g.lane % 16);
+    part.inverse = 1.0f;
+  }
+  __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
+    Quad<float> q;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) q.v[r] = adj[r];
+    StoreAdjointRow(g.buf, g.d, g.c.slots[kLo], g.c.b, g.lane, q, g.quad_edges);
+  }
+};
+
+template <int kLo, int kHi> struct Span<kLo, kHi, false> {
+  static constexpr int kMid = (kLo + kHi) / 2;
+  Span<kLo, kMid> left;
+  Span<kMid, kHi> right;
+  Part part;
+
+  __device__ inline void Build(const GroupState &g) {
+    left.Build(g);
+    right.Build(g);
+    const int lane = g.lane;
+    const std::uint64_t mid_at = static_cast<std::uint64_t>(g.c.middles[kMid - 1]) * g.d.B + g.c.b;
+    const float mid_own = static_cast<const float *>(g.buf.nodes)[mid_at * 16 + lane % 16];
+    float e[4];
+    const double w_top = StepWeights16(mid_own, right.part.scale, lane, e, &part.e_own);
+    float scaled[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) scaled[r] = left.part.row[r] * e[r];
+    const floatx4 s = Mfma(right.part.col, scaled);  // row layout
+    const floatx4 u = Mfma(scaled, right.part.col);  // column layout
+    const float top = RowMax16(s);
+    const bool live = top > 0.0f && w_top > kNegInf;
+    part.inverse = live ? 1.0f / top : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      part.row[r] = s[r] * part.inverse;
+      part.col[r] = u[r] * __shfl(part.inverse, 4 * (lane / 16) + r, 64);
+    }
+    part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + static_cast<double>(logf(top)) : kNegInf;
+  }
+
+  // adj: marginal-form adjoint of this span's product, row layout. The
+  // ratios Gr = adj / S (S = P / normaliser, row by row) are formed in FP32
+  // when every one is finite and below 1e30 in magnitude, else (wave-uniform)
+  // in FP64 as in the binary compression adjoint; the row and global scales
+  // g_p and Gamma are applied in FP64 when they exceed 1e30.
+  __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
+    const int lane = g.lane;
+    float e[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) e[r] = __shfl(part.e_own, 4 * (lane / 16) + r, 64);
+    float gr[4];
+    bool small = true;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      gr[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? adj[r] * part.inverse * __builtin_amdgcn_rcpf(part.row[r]) : 0.0f;
+      small = small && fabsf(gr[r]) < 1e30f;
+    }
+    float gh[4], gt[4];
+    double top, gamma;
+    if (__all(small)) {
+      float t = fmaxf(fmaxf(fabsf(gr[0]), fabsf(gr[1])), fmaxf(fabsf(gr[2]), fabsf(gr[3])));
+      t = fmaxf(t, __shfl_xor(t, 16, 64));
+      t = fmaxf(t, __shfl_xor(t, 32, 64));
+      float u = t;
+#pragma unroll
+      for (int x = 1; x < 16; x <<= 1) u = fmaxf(u, __shfl_xor(u, x, 64));
+      const float inverse_top = t > 0.0f ? 1.0f / t : 0.0f;
+      const float inverse_gamma = u > 0.0f ? 1.0f / u : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) gh[r] = gr[r] * inverse_top, gt[r] = gr[r] * inverse_gamma;
+      top = t, gamma = u;
+    } else {
+      double wide[4];
+      top = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        wide[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? static_cast<double>(adj[r]) *
+                                                               static_cast<double>(part.inverse) /
+                                                               static_cast<double>(part.row[r])
+                                                         : 0.0;
+        top = fmax(top, fabs(wide[r]));
+      }
+      top = fmax(top, __shfl_xor(top, 16, 64));
+      top = fmax(top, __shfl_xor(top, 32, 64));
+      gamma = top;
+#pragma unroll
+      for (int x = 1; x < 16; x <<= 1) gamma = fmax(gamma, __shfl_xor(gamma, x, 64));
+      const double inverse_top = top > 0.0 ? 1.0 / top : 0.0;
+      const double inverse_gamma = gamma > 0.0 ? 1.0 / gamma : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        gh[r] = static_cast<float>(wide[r] * inverse_top), gt[r] = static_cast<float>(wide[r] * inverse_gamma);
+    }
+    float *tile = g.tiles[kMid - 1];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tile[(lane % 16) * kPad + 4 * (lane / 16) + r] = gt[r];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float gt_cols[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) gt_cols[r] = tile[(4 * (lane / 16) + r) * kPad + lane % 16];
+    const floatx4 h = Mfma(right.part.row, gh);      // [p = l%16][m = 4(l/16) + r]
+    const floatx4 q = Mfma(gt_cols, left.part.col);  // [m = l%16][c = 4(l/16) + r]
+    float adj_left[4], adj_right[4], row_total;
+    if (__all(top < 1e30 && gamma < 1e30)) {
+      const float top_f = static_cast<float>(top), right_weight = part.e_own * static_cast<float>(gamma);
+      row_total = 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        adj_left[r] = (left.part.row[r] * e[r]) * (top_f * h[r]);
+        adj_right[r] = right.part.row[r] * right_weight * q[r];
+        row_total += adj_right[r];
+      }
+      row_total += __shfl_xor(row_total, 16, 64);
+      row_total += __shfl_xor(row_total, 32, 64);
+    } else {
+      double total = 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        adj_left[r] = static_cast<float>(static_cast<double>(left.part.row[r]) * static_cast<double>(e[r]) * top *
+                                         static_cast<double>(h[r]));
+        const double value = static_cast<double>(right.part.row[r]) * static_cast<double>(part.e_own) * gamma *
+                             static_cast<double>(q[r]);
+        adj_right[r] = static_cast<float>(value);
+        total += value;
+      }
+      total += __shfl_xor(total, 16, 64);
+      total += __shfl_xor(total, 32, 64);
+      row_total = static_cast<float>(total);
+    }
+    if (lane < 16)
+      static_cast<float *>(g.buf.node_adj)[(static_cast<std::uint64_t>(g.c.middles[kMid - 1]) * g.d.B + g.c.b) * 16 +
+                                           lane] = row_total;
+    left.Reverse(g, adj_left);
+    right.Reverse(g, adj_right);
+  }
+};
+
+template <int kN> __device__ inline void ReverseGroup(const GroupState &g, const float (&adj)[4]) {
+  Span<0, kN> root;
+  root.Build(g);
+  root.Reverse(g, adj);
+}
+
+// kMax: the largest group a launch may hold (4 or 8); it bounds the
+// registers and LDS tiles of the kernel.
+template <int kMax>
+__device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
+                                        std::uint32_t count, const LinearBuffers &buf, int quad_edges,
+                                        float (*tiles)[kMax - 1][16 * kPad]) {
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const std::uint32_t item = Uniform(blockIdx.x * kWaves + wave);
+  if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
+  ChainView c;
+  LoadChain(ops, d, offset, item, c);
+  const GroupState g{c, buf, d, lane, tiles[wave], quad_edges != 0};
+  float adj[4];
+  const std::uint64_t out_at = static_cast<std::uint64_t>(c.out - d.E) * d.B + c.b;
+  LoadRows<true>(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj);
+  switch (c.n) {
+    case 2: ReverseGroup<2>(g, adj); break;
+    case 3: ReverseGroup<3>(g, adj); break;
+    case 4: ReverseGroup<4>(g, adj); break;
+    default:
+      if constexpr (kMax > 4) {
+        switch (c.n) {
+          case 5: ReverseGroup<5>(g, adj); break;
+          case 6: ReverseGroup<6>(g, adj); break;
+          case 7: ReverseGroup<7>(g, adj); break;
+          case 8: ReverseGroup<8>(g, adj); break;
+          default: break;
+        }
+      }
+      break;
+  }
+}
+
+__global__ __launch_bounds__(256) void ChainReverseSmall(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                         std::uint32_t count, LinearBuffers buf, int quad_edges) {
+  __shared__ float tiles[kWaves][3][16 * kPad];
+  ChainReverseBody<4>(ops, d, offset, count, buf, quad_edges, tiles);
+}
+
+// Three wavefronts per SIMD (at most 168 registers per lane).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ChainReverseLarge(
+    DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf, int quad_edges) {
+  __shared__ float tiles[kWaves][kMaxChain - 1][16 * kPad];
+  ChainReverseBody<kMaxChain>(ops, d, offset, count, buf, quad_edges, tiles);
 }
 
 // ---------------------------------------------------------------------------
@@ -656,7 +990,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   float e[NB][4], e_own;
   MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
 
-  // S (recomputed), Gr = adj/S, g_p, Gh = Gr/g_p -> LDS; Gamma = max_p g_p.
+  // S (recomputed), Gr = adj/S, g_p = max |Gr| over row p, Gh = Gr/g_p -> LDS;
+  // Gamma = max_p g_p.
   double gamma = 0.0;
   {
     float rc[NB][NB][4];
@@ -679,7 +1014,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           gr[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f) ? static_cast<double>(ab[Ic][r]) / static_cast<double>(s[r]) : 0.0;
-          top = fmax(top, gr[Ic][r]);
+          top = fmax(top, fabs(gr[Ic][r]));
         }
       }
       top = fmax(top, __shfl_xor(top, 16, 64));
@@ -892,6 +1227,24 @@ void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t off
 bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std::uint32_t offset,
                      std::uint32_t count, const LinearBuffers &buf) {
   const std::uint32_t K = l.dims.K;
+  if (primitive == kChainPrimitive || primitive == kSmallChainPrimitive) {
+    // Built only for K = 16 FP32 workspaces; there is no generic fallback
+    // (the inner compressions' slots are not materialised).
+    if (K != 16 || l.real != Real::kF32 || l.ops.chains == nullptr) {
+      RecordLaunchError(hipErrorInvalidValue, "grouped compression launch (K = 16 FP32 only)");
+      return true;
+    }
+    const unsigned grid = static_cast<unsigned>((static_cast<std::uint64_t>(count) * l.dims.B + kWaves - 1) / kWaves);
+    const hipStream_t s = StreamOf(l);
+    if (reverse && primitive == kChainPrimitive)
+      ChainReverseLarge<<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf, Aligned16(buf.edge_out));
+    else if (reverse)
+      ChainReverseSmall<<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf, Aligned16(buf.edge_out));
+    else
+      ChainForwardMfma16<kMaxChain><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
+    RecordLaunchError(hipGetLastError(), "grouped compression launch");
+    return true;
+  }
   // 32-bit item indexing: the grid must address every (op, item, lane).
   // Caller-owned edge inputs and outputs must allow 16-byte accesses.
   if (!Aligned16(buf.edges) || !Aligned16(buf.edge_out) || !Aligned16(buf.node_adj)) return false;

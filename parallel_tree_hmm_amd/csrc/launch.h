@@ -26,12 +26,32 @@ struct DeviceCompress {
   std::uint32_t parent, child, pad0, pad1;
 };
 
+// Grouped compressions: a connected piece of one compression tree (a root
+// compression and the compressions below it down to a fixed height) taken
+// as one product S = X_0 D_1 X_1 ... D_{n-1} X_{n-1} over its n input slots
+// in tree order; middles[i] is the node between slots[i] and slots[i+1].
+// Only `out`, the root compression's slot, is written, so the slots of the
+// inner compressions are never stored or read back.
+constexpr std::uint32_t kMaxChain = 8;
+struct DeviceChain {
+  std::uint32_t count, out;
+  std::uint32_t slots[kMaxChain];
+  std::uint32_t middles[kMaxChain - 1];
+  std::uint32_t pad;
+};
+// Level primitives of grouped compressions (after the plan's 0..3 and the
+// fused forward's 4 and 5): 6 for groups of any size, 7 for a range of
+// groups of at most 4 slots (a reverse launch with fewer registers).
+constexpr std::uint32_t kChainPrimitive = 6;
+constexpr std::uint32_t kSmallChainPrimitive = 7;
+
 struct DeviceOps {
   const DeviceRake *rakes;
   const DeviceCombine *combines;
   const DeviceAbsorb *absorbs;
   const DeviceCompress *compresses;
   const std::uint32_t *roots;
+  const DeviceChain *chains;
 };
 
 struct Dims {
@@ -87,6 +107,7 @@ struct Tuning {
   std::uint32_t compress_items_reverse = 2;
   std::uint32_t rake_block = 256;            // threads per block of the team rake kernels: 64, 128, 256
   std::uint32_t gather_chunk = 16;           // rakes per team in the fused forward: 1 .. 64
+  std::uint32_t compress_depth = 2;          // compression-tree levels per grouped compression: 1 (none) .. 3
 };
 
 struct Launch {

@@ -84,6 +84,9 @@ struct PinnedBuffer {
 
 struct Level {
   std::uint32_t primitive, offset, count;
+  // Grouped compression levels: K-vectors moved per batch item in the
+  // forward and reverse pass (profiling only).
+  std::uint32_t forward_work = 0, reverse_work = 0;
 };
 
 }  // namespace
@@ -116,11 +119,18 @@ struct Workspace::Impl {
   // Sum-product forward program with fused rake/combination/absorption
   // levels (empty when the plan or state count does not allow it).
   struct FusedStep {
-    std::uint32_t kind, offset, count;  // kind: 0 chunks, 1 segments, 3 compressions
-    std::uint32_t work;                 // rakes (kind 0), partials (kind 1), compressions (kind 3)
+    std::uint32_t kind, offset, count;  // kind: 0 chunks, 1 segments, 3 compressions, 6 grouped compressions
+    std::uint32_t work;                 // rakes (kind 0), partials (kind 1), compressions (kind 3), K-vectors (6)
   };
   std::vector<FusedStep> fused;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
+  // Grouped compressions (K = 16 FP32 with the fused forward): the level of
+  // each plan batch that holds compressions (count 0: none rooted there),
+  // and the reverse program, the plan's levels with those in place of the
+  // compression levels. Empty when compressions are not grouped.
+  DeviceBuffer chains;
+  std::vector<Level> chain_levels;
+  std::vector<Level> grouped_levels;
   // Instantiated hipGraphs of the device-resident entry points (reserve flag
   // THMM_RESERVE_GRAPH), one per entry point, keyed by its arguments.
   struct GraphKey {
@@ -179,7 +189,8 @@ struct Workspace::Impl {
                             &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
-                            &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials})
+                            &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials,
+                            &chains})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
@@ -205,6 +216,8 @@ struct Workspace::Impl {
     levels.clear();
     rake_adjoint_from_parent = false;
     fused.clear();
+    chain_levels.clear();
+    grouped_levels.clear();
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
@@ -227,7 +240,7 @@ struct Workspace::Impl {
     l.dims = {static_cast<std::uint32_t>(states), static_cast<std::uint32_t>(batch), static_cast<std::uint32_t>(E()),
               static_cast<std::uint32_t>(N()), static_cast<std::uint32_t>(Roots())};
     l.ops = {rakes.as<DeviceRake>(), combines.as<DeviceCombine>(), absorbs.as<DeviceAbsorb>(),
-             compresses.as<DeviceCompress>(), roots.as<std::uint32_t>()};
+             compresses.as<DeviceCompress>(), roots.as<std::uint32_t>(), chains.as<DeviceChain>()};
     l.stream = s;
     l.tuning = tuning;
     return l;
@@ -356,9 +369,15 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
       steps.push_back({1, segment_offset, static_cast<std::uint32_t>(segments.size() - segment_offset), parts});
       open = std::move(next);
     }
-    for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i)
-      if (plan.batches[i].primitive == THMM_COMPRESSION)
+    for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i) {
+      if (plan.batches[i].primitive != THMM_COMPRESSION) continue;
+      if (w.chain_levels.empty()) {
         steps.push_back({3, plan.batches[i].offset, plan.batches[i].count, plan.batches[i].count});
+      } else if (w.chain_levels[i].count != 0) {
+        const Level &level = w.chain_levels[i];
+        steps.push_back({kChainPrimitive, level.offset, level.count, level.forward_work});
+      }
+    }
   }
   if (sorted.size() != plan.rakes.size()) return;
   // The fused kernels index (item, lane) in 32 bits.
@@ -372,6 +391,133 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
   // The uploads read the local vectors above.
   Hip(hipStreamSynchronize(w.stream), "fused topology upload");
   w.fused = std::move(steps);
+}
+
+// Groups the renamed compressions for the K = 16 FP32 MFMA path. A
+// compression's height is 1 plus the largest height of the compressions
+// producing its inputs. Groups are formed top-down (consumers come after
+// their inputs in plan order): a compression whose slot feeds a rake or a
+// root starts a group; a group takes in an input compression, in place and
+// left first, when that compression's height is not a multiple of
+// w.tuning.compress_depth and the group stays within 2^depth slots (<=
+// kMaxChain); every other input compression starts a group of its own.
+// Balanced compression trees thus split into depth-level groups. A group
+// runs at its root's level: every input is an edge or an earlier group's
+// slot, and every middle node's value is final by then.
+void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &compresses, std::size_t batch) {
+  w.chain_levels.clear();
+  w.grouped_levels.clear();
+  w.chains.Release();
+  const std::uint32_t depth = w.tuning.compress_depth;
+  if (depth < 2 || w.states != 16 || w.real != Real::kF32 || compresses.empty()) return;
+  const std::size_t limit = std::min<std::size_t>(std::size_t{1} << depth, kMaxChain);
+  const std::uint32_t E = static_cast<std::uint32_t>(plan.num_edges);
+  std::vector<std::uint32_t> height(compresses.size(), 0);
+  std::vector<std::uint8_t> root(compresses.size(), 1);  // starts a group
+  for (const thmm_primitive_batch &b : plan.batches) {
+    if (b.primitive != THMM_COMPRESSION) continue;
+    for (std::uint32_t c = b.offset; c < b.offset + b.count; ++c) {
+      std::uint32_t h = 1;
+      for (std::uint32_t slot : {compresses[c].left, compresses[c].right})
+        if (slot >= E) h = std::max(h, height[slot - E] + 1), root[slot - E] = 0;
+      height[c] = h;
+    }
+  }
+  // In-order inputs and middles of the group rooted at c. With `decide`,
+  // input compressions that the rule above leaves out become group roots;
+  // without, every input compression that is not a root is taken in.
+  std::vector<std::uint32_t> slots, middles;
+  const auto expand = [&](std::uint32_t c, bool decide) {
+    slots = {compresses[c].left, compresses[c].right};
+    middles = {compresses[c].middle};
+    for (std::size_t k = 0; k < slots.size();) {
+      const std::uint32_t slot = slots[k];
+      if (slot >= E && !root[slot - E]) {
+        if (!decide || (height[slot - E] % depth != 0 && slots.size() < limit)) {
+          const DeviceCompress &op = compresses[slot - E];
+          slots[k] = op.left;
+          slots.insert(slots.begin() + static_cast<std::ptrdiff_t>(k) + 1, op.right);
+          middles.insert(middles.begin() + static_cast<std::ptrdiff_t>(k), op.middle);
+          continue;  // look at the new left input
+        }
+        root[slot - E] = 1;
+      }
+      ++k;
+    }
+  };
+  for (std::size_t i = plan.batches.size(); i-- > 0;) {
+    const thmm_primitive_batch &b = plan.batches[i];
+    if (b.primitive != THMM_COMPRESSION) continue;
+    for (std::uint32_t c = b.offset + b.count; c-- > b.offset;)
+      if (root[c]) expand(c, true);
+  }
+  std::vector<DeviceChain> chains;
+  std::vector<Level> levels(plan.batches.size(), Level{kChainPrimitive, 0, 0});
+  // Per compression batch, groups of at most 4 slots first: the reverse
+  // pass launches them separately (kSmallChainPrimitive, fewer registers).
+  std::vector<Level> small(plan.batches.size(), Level{kSmallChainPrimitive, 0, 0});
+  const auto forward_work = [](std::uint32_t n) { return (n + 1) * 16 + (n - 1); };
+  const auto reverse_work = [](std::uint32_t n) { return (2 * n + 1) * 16 + 2 * (n - 1); };
+  for (std::size_t i = 0; i < plan.batches.size(); ++i) {
+    const thmm_primitive_batch &b = plan.batches[i];
+    if (b.primitive != THMM_COMPRESSION) continue;
+    Level &level = levels[i];
+    level.offset = static_cast<std::uint32_t>(chains.size());
+    for (std::uint32_t c = b.offset; c < b.offset + b.count; ++c) {
+      if (!root[c]) continue;
+      expand(c, false);
+      if (slots.size() < 2 || slots.size() > limit) throw std::logic_error("compression group exceeds its input limit");
+      DeviceChain group{};
+      group.count = static_cast<std::uint32_t>(slots.size());
+      group.out = compresses[c].out;
+      std::copy(slots.begin(), slots.end(), group.slots);
+      std::copy(middles.begin(), middles.end(), group.middles);
+      chains.push_back(group);
+      level.forward_work += forward_work(group.count);
+      level.reverse_work += reverse_work(group.count);
+    }
+    level.count = static_cast<std::uint32_t>(chains.size()) - level.offset;
+    const auto first = chains.begin() + level.offset;
+    const auto split =
+        std::stable_partition(first, chains.end(), [](const DeviceChain &x) { return x.count <= 4; });
+    small[i].offset = level.offset;
+    small[i].count = static_cast<std::uint32_t>(split - first);
+    for (auto it = first; it != split; ++it) {
+      small[i].forward_work += forward_work(it->count);
+      small[i].reverse_work += reverse_work(it->count);
+    }
+    // The kernels index (group, item) in 32 bits.
+    if (static_cast<std::uint64_t>(level.count) * batch * 64 >= (1ull << 32)) return;
+  }
+  Upload(w.chains, chains, w.stream, "grouped compression upload");
+  Hip(hipStreamSynchronize(w.stream), "grouped compression upload");
+  for (std::size_t i = 0; i < plan.batches.size(); ++i) {
+    if (plan.batches[i].primitive != THMM_COMPRESSION) {
+      w.grouped_levels.push_back(w.levels[i]);
+      continue;
+    }
+    const Level &all = levels[i];
+    if (small[i].count != 0) w.grouped_levels.push_back(small[i]);
+    if (all.count > small[i].count)
+      w.grouped_levels.push_back({kChainPrimitive, all.offset + small[i].count, all.count - small[i].count,
+                                  all.forward_work - small[i].forward_work, all.reverse_work - small[i].reverse_work});
+  }
+  w.chain_levels = std::move(levels);
+}
+
+// The fused forward program and, for K = 16 FP32, the grouped compressions
+// it runs; grouping is dropped when the fused program cannot be built (the
+// plan-level forward stores every compression).
+void BuildPrograms(Impl &w, const Plan &plan, const std::vector<DeviceRake> &rakes,
+                   const std::vector<DeviceCompress> &compresses, std::size_t batch) {
+  w.fused.clear();
+  BuildChains(w, plan, compresses, batch);
+  BuildFusedForward(w, plan, rakes, batch);
+  if (w.fused.empty()) {
+    w.chain_levels.clear();
+    w.grouped_levels.clear();
+    w.chains.Release();
+  }
 }
 
 void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
@@ -453,7 +599,7 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
       if (branch_parent[r.branch] != r.parent) w.rake_adjoint_from_parent = false;
   }
   if (w.rake_adjoint_from_parent && FusedForwardSupported(static_cast<std::uint32_t>(states)))
-    BuildFusedForward(w, plan, rakes, batch);
+    BuildPrograms(w, plan, rakes, compresses, batch);
 
   const std::size_t K = states, KK = K * K, N = plan.num_nodes, R = plan.num_branches, C = plan.num_compressions();
   const std::size_t S = RealSize(real);
@@ -630,27 +776,40 @@ void ForwardLevels(const Launch &l, const std::vector<Level> &levels, const Line
   }
 }
 
+// Whether SumProductForward runs the fused program for these buffers (its
+// kernels read edge slots with 16-byte accesses).
+bool UsesFusedForward(const Impl &w, const LinearBuffers &buf) {
+  return !w.fused.empty() && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0;
+}
+
 // The sum-product forward pass: the fused program when the workspace has
 // one, else the plan's levels.
 void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
-  // The fused kernels read edge slots with 16-byte accesses.
-  if (w.fused.empty() || (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) != 0) {
+  if (!UsesFusedForward(w, buf)) {
     ForwardLevels(l, w.levels, buf, profiler);
     return;
   }
   const GatherOps g = w.Gather();
   for (const Impl::FusedStep &step : w.fused) {
     // Profiled as primitive 4 (fused rake chunks, count = rakes), 5 (partial
-    // absorptions, count = partials) or a compression level.
-    if (profiler) profiler->Begin(1, step.kind == 3 ? THMM_COMPRESSION : 4 + step.kind, step.work);
+    // absorptions, count = partials), a compression level, or a grouped
+    // compression level (kChainPrimitive, count = K-vectors moved per item).
+    if (profiler)
+      profiler->Begin(1, step.kind == 3 || step.kind == kChainPrimitive ? step.kind : 4 + step.kind, step.work);
     if (step.kind == 0)
       LaunchGatherChunks(l, g, step.offset, step.count, buf);
     else if (step.kind == 1)
       LaunchGatherSegments(l, g, step.offset, step.count, buf);
     else
-      LaunchForwardLevel(l, THMM_COMPRESSION, step.offset, step.count, buf);
+      LaunchForwardLevel(l, step.kind, step.offset, step.count, buf);
     if (profiler) profiler->End();
   }
+}
+
+// The levels whose reverse matches the forward pass SumProductForward ran
+// for these buffers: with grouped compressions only the groups' slots exist.
+const std::vector<Level> &ReverseProgram(const Impl &w, const LinearBuffers &buf) {
+  return UsesFusedForward(w, buf) && !w.grouped_levels.empty() ? w.grouped_levels : w.levels;
 }
 
 // Without branch adjoint storage (buf.branch_adj null) rakes take their
@@ -661,7 +820,8 @@ void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const Line
     if (buf.branch_adj == nullptr &&
         (levels[i].primitive == THMM_BRANCH_COMBINATION || levels[i].primitive == THMM_BRANCH_ABSORPTION))
       continue;
-    if (profiler) profiler->Begin(3, levels[i].primitive, levels[i].count);
+    const bool grouped = levels[i].primitive == kChainPrimitive || levels[i].primitive == kSmallChainPrimitive;
+    if (profiler) profiler->Begin(3, levels[i].primitive, grouped ? levels[i].reverse_work : levels[i].count);
     LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
     if (profiler) profiler->End();
   }
@@ -797,7 +957,7 @@ void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
-  ReverseLevels(l, w.levels, buf);
+  ReverseLevels(l, ReverseProgram(w, buf), buf);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
   CheckLaunches();
 }
@@ -1276,7 +1436,7 @@ void Workspace::BranchLengthBackward(const double *grad_log_z, const double *rat
   LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
   AdjointsInto(buf, w.last_batch, w.node_out.ptr);
   LaunchSeedAdjoint(l, buf, grad_log_z != nullptr ? w.chain_seed.as<double>() : nullptr);
-  ReverseLevels(l, w.levels, buf);
+  ReverseLevels(l, ReverseProgram(w, buf), buf);
   LaunchBranchLengthChain(l, w.edge_out.ptr, w.edges.ptr, w.chain_rate.as<double>(), w.chain_terms.as<double>(),
                           w.chain_grad.as<double>());
   CheckLaunches();
@@ -1347,7 +1507,7 @@ void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *ed
   LaunchFinish(l, buf, log_z);
   LaunchSeedAdjoint(l, buf, nullptr);
   if (p) p->End();
-  ReverseLevels(l, w.levels, buf, p);
+  ReverseLevels(l, ReverseProgram(w, buf), buf, p);
   if (p) p->Begin(4, 0, 0);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
   if (p) p->End();
@@ -1360,13 +1520,18 @@ void ValidateTuning(const Tuning &t) {
   if (t.rake_block != 64 && t.rake_block != 128 && t.rake_block != 256)
     throw std::invalid_argument("rake block size must be 64, 128 or 256");
   if (t.gather_chunk < 1 || t.gather_chunk > 64) throw std::invalid_argument("gather chunk must be in [1, 64]");
+  if (t.compress_depth < 1 || t.compress_depth > 3) throw std::invalid_argument("compression depth must be 1, 2 or 3");
 }
 
-// Installs `t`; a new chunk size rebuilds the fused forward program from the
-// device's renamed rakes. Captured graphs are dropped (they bake in kernels).
+// Installs `t`; a new chunk size or compression depth rebuilds the fused
+// forward program (and the grouped compressions) from the device's renamed
+// ops. Captured graphs are dropped (they bake in kernels). A new depth also
+// drops a held forward pass: its reverse needs the groups it ran.
 void ApplyTuning(Impl &w, const Tuning &t) {
   ValidateTuning(t);
-  const bool rebuild = !w.fused.empty() && t.gather_chunk != w.tuning.gather_chunk;
+  const bool rebuild =
+      !w.fused.empty() && (t.gather_chunk != w.tuning.gather_chunk || t.compress_depth != w.tuning.compress_depth);
+  if (t.compress_depth != w.tuning.compress_depth) w.forward = Impl::Forward::kNone;
   w.tuning = t;
   for (Impl::GraphCache &g : w.graphs) g.Release();
   if (!rebuild) return;
@@ -1374,8 +1539,12 @@ void ApplyTuning(Impl &w, const Tuning &t) {
   if (!rakes.empty())
     Hip(hipMemcpy(rakes.data(), w.rakes.ptr, rakes.size() * sizeof(DeviceRake), hipMemcpyDeviceToHost),
         "GPU runtime Memcpy rake download");
-  w.fused.clear();
-  BuildFusedForward(w, *w.plan, rakes, w.capacity);
+  std::vector<DeviceCompress> compresses(w.plan->compressions.size());
+  if (!compresses.empty())
+    Hip(hipMemcpy(compresses.data(), w.compresses.ptr, compresses.size() * sizeof(DeviceCompress),
+                  hipMemcpyDeviceToHost),
+        "GPU runtime Memcpy compression download");
+  BuildPrograms(w, *w.plan, rakes, compresses, w.capacity);
   if (w.fused.empty()) throw std::logic_error("the fused forward program could not be rebuilt");
 }
 
@@ -1478,7 +1647,7 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
     buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
     AdjointsInto(buf, w.last_batch, grad_nodes);
     LaunchSeedAdjoint(l, buf, grad);
-    ReverseLevels(l, w.levels, buf);
+    ReverseLevels(l, ReverseProgram(w, buf), buf);
     if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
     if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
   });
@@ -1547,6 +1716,7 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
     if (!w.fused.empty()) search(&Tuning::gather_chunk, {4, 8, 16, 32});
     if (mfma16) search(&Tuning::compress_items_forward, {1, 2});
     if (mfma16) search(&Tuning::compress_items_reverse, {1, 2});
+    if (mfma16 && !w.fused.empty() && !w.plan->compressions.empty()) search(&Tuning::compress_depth, {1, 2, 3});
     search(&Tuning::rake_block, {64, 128, 256});
   } catch (...) {
     static_cast<void>(hipEventDestroy(start));

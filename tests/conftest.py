@@ -80,6 +80,21 @@ def forest(trees: list[np.ndarray]) -> np.ndarray:
     return np.concatenate(out)
 
 
+def root_index_of_nodes(plan) -> np.ndarray:
+    """For every node, the position of its tree's root in plan.roots (the
+    order of per-root log partitions and gradient seeds)."""
+    parents = np.full(plan.num_nodes, -1, np.int64)
+    parents[plan.edge_children] = plan.edge_parents
+    position = {int(r): i for i, r in enumerate(plan.roots)}
+    out = np.empty(plan.num_nodes, np.int64)
+    for v in range(plan.num_nodes):
+        u = v
+        while parents[u] >= 0:
+            u = parents[u]
+        out[v] = position[u]
+    return out
+
+
 def random_potentials(rng, batch, n_nodes, n_edges, k, zero_fraction=0.0, sigma=1.0):
     nodes = np.exp(rng.normal(0.0, sigma, size=(batch, n_nodes, k)))
     if zero_fraction:

@@ -17,7 +17,7 @@ import pytest
 
 import oracle
 import parallel_tree_hmm_amd as thmm
-from conftest import caterpillar, forest, path, random_potentials, random_recursive, star, yule
+from conftest import caterpillar, forest, path, random_potentials, random_recursive, root_index_of_nodes, star, yule
 
 pytestmark = pytest.mark.gpu
 
@@ -216,6 +216,40 @@ def test_log_potential_forward_backward(require_gpu, precision):
     g = grad.cpu().numpy()
     assert np.max(np.abs(g_nodes.cpu().double().numpy() - g[:, None, None] * onm)) <= tol * 2
     assert np.max(np.abs(g_edges.cpu().double().numpy() - np.einsum("b,bekl->ekl", g, oem))) <= tol * 4
+
+
+@pytest.mark.parametrize("k", [16, 32, 64])
+def test_gradient_seeds_of_either_sign(require_gpu, k):
+    """The adjoint pair with per-(item, tree) seeds of both signs: the MFMA
+    compression adjoints normalise their ratios by magnitude, so a negative
+    seed scales every node and edge gradient of its tree, compressions
+    included (ungrouped compressions: compress_depth 1)."""
+    import torch
+
+    rng = np.random.default_rng(k)
+    plan = thmm.Plan(forest([caterpillar(121), path(90), random_recursive(60, rng)]))
+    batch = 2
+    log_nodes = torch.tensor(rng.normal(size=(batch, plan.num_nodes, k)), dtype=torch.float32, device="cuda")
+    log_edges = torch.tensor(rng.normal(size=(plan.num_edges, k, k)), dtype=torch.float32, device="cuda")
+    seeds = np.array([[-1.5, 0.7, -0.2], [2.0, -0.6, 1.1]])
+    grad = torch.tensor(seeds.ravel(), dtype=torch.float64, device="cuda")
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, batch, precision="f32", device_resident=True)
+    if k == 16:
+        ws.set_tuning(compress_depth=1)
+    lz = torch.empty(batch * 3, dtype=torch.float64, device="cuda")
+    ws.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), batch, lz.data_ptr(), 0)
+    gn = torch.empty_like(log_nodes)
+    ge = torch.empty((batch, plan.num_edges, k, k), device="cuda")
+    ws.logz_backward_device(grad.data_ptr(), gn.data_ptr(), ge.data_ptr(), False, 0)
+    torch.cuda.synchronize()
+    olz, onm, oem = oracle.marginals_batched(plan, k, np.exp(log_nodes.double().cpu().numpy()),
+                                             np.exp(log_edges.double().cpu().numpy()), threads=4)
+    per_node = seeds[:, root_index_of_nodes(plan)]
+    assert np.all(np.abs(lz.cpu().numpy().reshape(batch, 3) - olz) <= 1e-5 * np.maximum(1, np.abs(olz)))
+    assert np.max(np.abs(gn.cpu().double().numpy() - per_node[:, :, None] * onm)) <= 4e-5
+    per_edge = per_node[:, plan.edge_children]
+    assert np.max(np.abs(ge.cpu().double().numpy() - per_edge[:, :, None, None] * oem)) <= 4e-5
 
 
 def test_unaligned_device_buffers_fall_back(require_gpu):

@@ -1,7 +1,8 @@
 """Launch variants (thmm_set_tuning) and the autotuner: every variant gives
 the same marginals (bit-exact where only the launch shape changes; within
-1e-6 where the fused forward's chunking reorders FP64 sums), invalid values
-are rejected, and the autotuner leaves a valid setting and valid outputs."""
+1e-6 where the fused forward's chunking reorders FP64 sums or compression
+grouping reorders FP32 products), invalid values are rejected, and the
+autotuner leaves a valid setting and valid outputs."""
 
 from __future__ import annotations
 
@@ -39,21 +40,29 @@ def test_variants_agree(require_gpu):
 
     plan, nodes, edges, ws = setup(torch)
     assert ws.tuning() == {"compress_items_forward": 2, "compress_items_reverse": 2, "rake_block": 256,
-                           "gather_chunk": 16}
+                           "gather_chunk": 16, "compress_depth": 2}
     base = run(torch, ws, plan, nodes, edges)
-    for fields in ({"compress_items_forward": 1}, {"compress_items_reverse": 1}, {"rake_block": 64},
-                   {"rake_block": 128}):
-        ws.set_tuning(**fields)
-        for a, b in zip(base, run(torch, ws, plan, nodes, edges)):
-            assert torch.equal(a, b), fields
-        ws.set_tuning(compress_items_forward=2, compress_items_reverse=2, rake_block=256)
-    for chunk in (1, 4, 32, 64):
-        ws.set_tuning(gather_chunk=chunk)
+    # Launch shapes only: bit-exact. The binary compression kernels'
+    # variants are seen with grouping off (depth 1).
+    for depth in (2, 1):
+        ws.set_tuning(compress_depth=depth)
+        same = run(torch, ws, plan, nodes, edges)
+        for fields in ({"compress_items_forward": 1}, {"compress_items_reverse": 1}, {"rake_block": 64},
+                       {"rake_block": 128}):
+            ws.set_tuning(**fields)
+            for a, b in zip(same, run(torch, ws, plan, nodes, edges)):
+                assert torch.equal(a, b), (depth, fields)
+            ws.set_tuning(compress_items_forward=2, compress_items_reverse=2, rake_block=256)
+    # Reordered FP64 sums (chunk size) or FP32 products (grouping depth).
+    for fields in ({"gather_chunk": 1}, {"gather_chunk": 4}, {"gather_chunk": 32}, {"gather_chunk": 64},
+                   {"compress_depth": 1}, {"compress_depth": 3}):
+        ws.set_tuning(**{"gather_chunk": 16, "compress_depth": 2, **fields})
         lz, nm, em = run(torch, ws, plan, nodes, edges)
-        assert float(((lz - base[0]).abs() / base[0].abs().clamp(min=1)).max()) <= 1e-9
-        assert float((nm - base[1]).abs().max()) <= 1e-6
-        assert float((em - base[2]).abs().max()) <= 1e-6
-    for bad in ({"compress_items_forward": 3}, {"rake_block": 96}, {"gather_chunk": 0}, {"gather_chunk": 65}):
+        assert float(((lz - base[0]).abs() / base[0].abs().clamp(min=1)).max()) <= 1e-6, fields
+        assert float((nm - base[1]).abs().max()) <= 1e-6, fields
+        assert float((em - base[2]).abs().max()) <= 1e-6, fields
+    for bad in ({"compress_items_forward": 3}, {"rake_block": 96}, {"gather_chunk": 0}, {"gather_chunk": 65},
+                {"compress_depth": 0}, {"compress_depth": 4}):
         with pytest.raises(thmm.InvalidArgument):
             ws.set_tuning(**bad)
 
