@@ -213,6 +213,7 @@ def roofline_from(timings, k, s):
     breakdown = {KERNEL_NAMES[kk]: round(v[0], 4) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0])}
     gbs = {KERNEL_NAMES[kk]: round(v[1] / (v[0] * 1e-3) / 1e9, 1) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0])
            if v[0] > 0 and v[1] > 0}
+    per_launch = {KERNEL_NAMES[kk]: int(v[1] / v[2]) for kk, v in sorted(totals.items(), key=lambda x: -x[1][0]) if v[1] > 0}
     return {
         "kernel": KERNEL_NAMES[key],
         "bound": "hbm",
@@ -226,6 +227,7 @@ def roofline_from(timings, k, s):
         "algorithmic_bytes_per_launch": int(nbytes / launches),
         "kernel_ms_breakdown": breakdown,
         "kernel_algorithmic_gbs": gbs,
+        "kernel_algorithmic_bytes_per_launch": per_launch,
     }
 
 
