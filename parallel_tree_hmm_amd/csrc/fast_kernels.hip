@@ -284,6 +284,44 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;  // wavefronts per block
 
+// Cross-lane maxima and exchanges without an LDS round trip: DPP moves
+// inside a 16-lane row (quad permutations, then row rotations by 4 and 8
+// cover all 16 lanes), and ds_swizzle's bitmask mode for lane ^ 16 inside a
+// 32-lane half. No source lane is ever out of its row, so every lane reads
+// a valid value.
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowRotate4 = 0x124, kRowRotate8 = 0x128;
+constexpr int kSwizzleXor16 = 0x401F;  // and_mask 0x1F, or_mask 0, xor_mask 0x10
+
+template <int kCtrl> __device__ inline float DppMove(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl> __device__ inline double DppMove(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Maximum over the 16 lanes of each row (lane l and l ^ 1, ..., l ^ 8).
+__device__ inline float RowLanesMax(float v) {
+  v = fmaxf(v, DppMove<kQuadXor1>(v));
+  v = fmaxf(v, DppMove<kQuadXor2>(v));
+  v = fmaxf(v, DppMove<kRowRotate4>(v));
+  return fmaxf(v, DppMove<kRowRotate8>(v));
+}
+__device__ inline double RowLanesMax(double v) {
+  v = fmax(v, DppMove<kQuadXor1>(v));
+  v = fmax(v, DppMove<kQuadXor2>(v));
+  v = fmax(v, DppMove<kRowRotate4>(v));
+  return fmax(v, DppMove<kRowRotate8>(v));
+}
+
+// The value of lane l ^ 16.
+__device__ inline float Xor16(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), kSwizzleXor16));
+}
+
+
 __device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -346,9 +384,7 @@ __device__ inline void LoadCompression(const DeviceOps &ops, const Dims &d, cons
 __device__ inline double StepWeights16(float mid_own, double right_scale_own, int lane, float (&e)[4],
                                       float *e_own) {
   const double w = static_cast<double>(mid_own) + right_scale_own;
-  double top = w;
-#pragma unroll
-  for (int x = 1; x < 16; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
+  const double top = RowLanesMax(w);
   *e_own = top > kNegInf ? expf(static_cast<float>(w - top)) : 0.0f;
 #pragma unroll
   for (int r = 0; r < 4; ++r) e[r] = __shfl(*e_own, 4 * (lane / 16) + r, 64);
@@ -540,7 +576,7 @@ __device__ inline void LoadChain(const DeviceOps &ops, const Dims &d, std::uint3
 
 __device__ inline float RowMax16(const floatx4 &s) {
   float top = fmaxf(fmaxf(s[0], s[1]), fmaxf(s[2], s[3]));
-  top = fmaxf(top, __shfl_xor(top, 16, 64));
+  top = fmaxf(top, Xor16(top));
   return fmaxf(top, __shfl_xor(top, 32, 64));
 }
 
@@ -622,6 +658,7 @@ struct Part {
   double scale;          // FP64 log scale of row l%16 (Build only)
   float inverse;         // normaliser of row l%16 (1 for a stored slot)
   float e_own;           // split's middle weight e[m], m = l%16
+  float e[4];            // split's middle weights e[m], m = 4(l/16) + r (kept when kKeepWeights)
 };
 
 struct GroupState {
@@ -633,9 +670,11 @@ struct GroupState {
   bool quad_edges;
 };
 
-template <int kLo, int kHi, bool kLeaf = (kHi - kLo == 1)> struct Span;
+// kKeepWeights: splits keep their gathered middle weights from Build for
+// Reverse (four registers per split) instead of gathering them again.
+template <int kLo, int kHi, bool kKeepWeights, bool kLeaf = (kHi - kLo == 1)> struct Span;
 
-template <int kLo, int kHi> struct Span<kLo, kHi, true> {
+template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeights, true> {
   Part part;
   __device__ inline void Build(const GroupState &g) {
     const float *x = Slot<float>(g.buf, g.d, g.c.slots[kLo], g.c.b);
@@ -652,10 +691,10 @@ template <int kLo, int kHi> struct Span<kLo, kHi, true> {
   }
 };
 
-template <int kLo, int kHi> struct Span<kLo, kHi, false> {
+template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeights, false> {
   static constexpr int kMid = (kLo + kHi) / 2;
-  Span<kLo, kMid> left;
-  Span<kMid, kHi> right;
+  Span<kLo, kMid, kKeepWeights> left;
+  Span<kMid, kHi, kKeepWeights> right;
   Part part;
 
   __device__ inline void Build(const GroupState &g) {
@@ -664,11 +703,10 @@ template <int kLo, int kHi> struct Span<kLo, kHi, false> {
     const int lane = g.lane;
     const std::uint64_t mid_at = static_cast<std::uint64_t>(g.c.middles[kMid - 1]) * g.d.B + g.c.b;
     const float mid_own = static_cast<const float *>(g.buf.nodes)[mid_at * 16 + lane % 16];
-    float e[4];
-    const double w_top = StepWeights16(mid_own, right.part.scale, lane, e, &part.e_own);
+    const double w_top = StepWeights16(mid_own, right.part.scale, lane, part.e, &part.e_own);
     float scaled[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) scaled[r] = left.part.row[r] * e[r];
+    for (int r = 0; r < 4; ++r) scaled[r] = left.part.row[r] * part.e[r];
     const floatx4 s = Mfma(right.part.col, scaled);  // row layout
     const floatx4 u = Mfma(scaled, right.part.col);  // column layout
     const float top = RowMax16(s);
@@ -691,7 +729,7 @@ template <int kLo, int kHi> struct Span<kLo, kHi, false> {
     const int lane = g.lane;
     float e[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) e[r] = __shfl(part.e_own, 4 * (lane / 16) + r, 64);
+    for (int r = 0; r < 4; ++r) e[r] = kKeepWeights ? part.e[r] : __shfl(part.e_own, 4 * (lane / 16) + r, 64);
     float gr[4];
     bool small = true;
 #pragma unroll
@@ -703,11 +741,9 @@ template <int kLo, int kHi> struct Span<kLo, kHi, false> {
     double top, gamma;
     if (__all(small)) {
       float t = fmaxf(fmaxf(fabsf(gr[0]), fabsf(gr[1])), fmaxf(fabsf(gr[2]), fabsf(gr[3])));
-      t = fmaxf(t, __shfl_xor(t, 16, 64));
+      t = fmaxf(t, Xor16(t));
       t = fmaxf(t, __shfl_xor(t, 32, 64));
-      float u = t;
-#pragma unroll
-      for (int x = 1; x < 16; x <<= 1) u = fmaxf(u, __shfl_xor(u, x, 64));
+      const float u = RowLanesMax(t);
       const float inverse_top = t > 0.0f ? 1.0f / t : 0.0f;
       const float inverse_gamma = u > 0.0f ? 1.0f / u : 0.0f;
 #pragma unroll
@@ -756,7 +792,7 @@ template <int kLo, int kHi> struct Span<kLo, kHi, false> {
         adj_right[r] = right.part.row[r] * right_weight * q[r];
         row_total += adj_right[r];
       }
-      row_total += __shfl_xor(row_total, 16, 64);
+      row_total += Xor16(row_total);
       row_total += __shfl_xor(row_total, 32, 64);
     } else {
       double total = 0.0;
@@ -781,8 +817,8 @@ template <int kLo, int kHi> struct Span<kLo, kHi, false> {
   }
 };
 
-template <int kN> __device__ inline void ReverseGroup(const GroupState &g, const float (&adj)[4]) {
-  Span<0, kN> root;
+template <int kN, bool kKeepWeights> __device__ inline void ReverseGroup(const GroupState &g, const float (&adj)[4]) {
+  Span<0, kN, kKeepWeights> root;
   root.Build(g);
   root.Reverse(g, adj);
 }
@@ -803,16 +839,16 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
   const std::uint64_t out_at = static_cast<std::uint64_t>(c.out - d.E) * d.B + c.b;
   LoadRows<true>(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj);
   switch (c.n) {
-    case 2: ReverseGroup<2>(g, adj); break;
-    case 3: ReverseGroup<3>(g, adj); break;
-    case 4: ReverseGroup<4>(g, adj); break;
+    case 2: ReverseGroup<2, (kMax <= 4)>(g, adj); break;
+    case 3: ReverseGroup<3, (kMax <= 4)>(g, adj); break;
+    case 4: ReverseGroup<4, (kMax <= 4)>(g, adj); break;
     default:
       if constexpr (kMax > 4) {
         switch (c.n) {
-          case 5: ReverseGroup<5>(g, adj); break;
-          case 6: ReverseGroup<6>(g, adj); break;
-          case 7: ReverseGroup<7>(g, adj); break;
-          case 8: ReverseGroup<8>(g, adj); break;
+          case 5: ReverseGroup<5, (kMax <= 4)>(g, adj); break;
+          case 6: ReverseGroup<6, (kMax <= 4)>(g, adj); break;
+          case 7: ReverseGroup<7, (kMax <= 4)>(g, adj); break;
+          case 8: ReverseGroup<8, (kMax <= 4)>(g, adj); break;
           default: break;
         }
       }
