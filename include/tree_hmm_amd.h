@@ -97,7 +97,11 @@ typedef struct thmm_plan thmm_plan;
 /* Builds a rake-compress plan from a parent array (parents[i] == -1 marks a
  * root; several roots make a forest whose trees are contracted together).
  * Replaces btrc::MakePlan(std::vector<int64_t>) as called at
- * tests/inference_test.cc:159-160 and tests/accelerator_test.h:19-20. */
+ * tests/inference_test.cc:159-160 and tests/accelerator_test.h:19-20.
+ * A forest of at least 65,536 nodes whose trees each occupy the node range
+ * from their root to the next root is planned tree by tree on host threads
+ * (environment variable THMM_PLAN_THREADS, default the hardware threads up
+ * to 16; 1 plans sequentially); the plan is the same either way. */
 thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes,
                              thmm_plan **out);
 

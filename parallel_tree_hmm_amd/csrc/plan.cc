@@ -2,8 +2,13 @@
 #include "plan.h"
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <exception>
 #include <limits>
+#include <mutex>
 #include <numeric>
+#include <thread>
 #include <utility>
 
 namespace thmm {
@@ -11,8 +16,12 @@ namespace {
 
 constexpr Index kNone = std::numeric_limits<Index>::max();
 
-void Require(bool condition, const std::string &message) {
+// Checks run once per node or op: the message is built only on failure.
+void Require(bool condition, const char *message) {
   if (!condition) throw std::invalid_argument(message);
+}
+template <typename Message> void RequireWith(bool condition, Message &&message) {
+  if (!condition) throw std::invalid_argument(message());
 }
 
 // Returns false when the parent links contain a cycle. parent[i] == kNone
@@ -62,26 +71,15 @@ thmm_plan_arrays Plan::Arrays() const {
   return a;
 }
 
-Plan MakePlan(const std::int64_t *parents, std::size_t num_nodes) {
-  if (num_nodes == 0) throw std::invalid_argument("a plan needs at least one node");
-  if (num_nodes >= kNone) throw std::length_error("plan node count exceeds uint32_t");
-  const std::size_t n = num_nodes;
-  std::vector<Index> parent(n, kNone);
-  Plan plan;
-  plan.num_nodes = n;
-  for (std::size_t i = 0; i < n; ++i) {
-    const std::int64_t p = parents[i];
-    if (p == -1) {
-      plan.roots.push_back(static_cast<Index>(i));
-      continue;
-    }
-    Require(p >= 0 && static_cast<std::size_t>(p) < n && static_cast<std::size_t>(p) != i,
-            "parent of node " + std::to_string(i) + " is out of range");
-    parent[i] = static_cast<Index>(p);
-  }
-  Require(!plan.roots.empty(), "a plan needs at least one root (parent -1)");
-  Require(Acyclic(parent), "parent links contain a cycle");
+namespace {
 
+// Contracts the forest `parent` (kNone marks a root) round by round into
+// `plan`: edge arrays, ops, batches and round offsets. Ops are listed in
+// ascending node order within each batch, so the plan of a forest whose
+// trees occupy consecutive node ranges is, round by round and batch by
+// batch, the concatenation of its trees' plans.
+void Contract(const std::vector<Index> &parent, Plan &plan) {
+  const std::size_t n = parent.size();
   // Canonical edge order: one edge per non-root node, in ascending node
   // order; edge e joins edge_parents[e] -> edge_children[e].
   std::vector<Index> slot(n, kNone);  // slot joining cur_parent[i] -> i
@@ -216,6 +214,229 @@ Plan MakePlan(const std::int64_t *parents, std::size_t num_nodes) {
     ++round;
   }
   plan.round_batch_offsets.push_back(static_cast<Index>(plan.batches.size()));
+}
+
+// Threads for planning a forest: THMM_PLAN_THREADS when set (1 plans
+// sequentially), else the host's hardware threads, at most 16.
+unsigned PlannerThreads() {
+  if (const char *v = std::getenv("THMM_PLAN_THREADS")) {
+    const long t = std::strtol(v, nullptr, 10);
+    return t >= 1 ? static_cast<unsigned>(std::min<long>(t, 64)) : 1u;
+  }
+  return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+// True when every tree occupies the node range from its root to the next
+// root (roots ascending, node 0 a root): the trees are then planned apart.
+bool ConsecutiveTrees(const std::vector<Index> &parent, const std::vector<Index> &roots) {
+  if (roots.empty() || roots[0] != 0) return false;
+  for (std::size_t t = 0; t < roots.size(); ++t) {
+    const Index first = roots[t];
+    const Index end = t + 1 < roots.size() ? roots[t + 1] : static_cast<Index>(parent.size());
+    for (Index i = first + 1; i < end; ++i)
+      if (parent[i] == kNone || parent[i] < first || parent[i] >= end) return false;
+  }
+  return true;
+}
+
+// Local op ranges of one tree in one round.
+struct TreeRound {
+  std::uint32_t rake_first = 0, rakes = 0, absorb_first = 0, absorbs = 0, compress_first = 0, compresses = 0;
+  std::vector<std::pair<std::uint32_t, std::uint32_t>> stages;  // (first, count) per combination stage
+};
+
+std::vector<TreeRound> RoundsOf(const Plan &p) {
+  std::vector<TreeRound> rounds(p.num_rounds());
+  for (std::size_t r = 0; r < rounds.size(); ++r) {
+    TreeRound &tr = rounds[r];
+    bool seen_rake = false, seen_absorb = false, seen_compress = false;
+    for (Index b = p.round_batch_offsets[r]; b < p.round_batch_offsets[r + 1]; ++b) {
+      const thmm_primitive_batch &batch = p.batches[b];
+      switch (batch.primitive) {
+        case THMM_RAKE: tr.rake_first = batch.offset, tr.rakes = batch.count, seen_rake = true; break;
+        case THMM_BRANCH_COMBINATION: tr.stages.push_back({batch.offset, batch.count}); break;
+        case THMM_BRANCH_ABSORPTION:
+          tr.absorb_first = batch.offset, tr.absorbs = batch.count, seen_absorb = true;
+          break;
+        default: tr.compress_first = batch.offset, tr.compresses = batch.count, seen_compress = true; break;
+      }
+    }
+    // Empty batches are not listed; their ranges start where the ops of
+    // that kind continue.
+    if (!seen_rake) tr.rake_first = r == 0 ? 0 : rounds[r - 1].rake_first + rounds[r - 1].rakes;
+    if (!seen_absorb) tr.absorb_first = r == 0 ? 0 : rounds[r - 1].absorb_first + rounds[r - 1].absorbs;
+    if (!seen_compress) tr.compress_first = r == 0 ? 0 : rounds[r - 1].compress_first + rounds[r - 1].compresses;
+  }
+  return rounds;
+}
+
+// Plans each tree of a consecutive forest on its own thread pool slot and
+// merges the plans round by round. Branch and compression ids are numbered
+// round-major and tree-minor, as the single contraction numbers them, so
+// the result equals Contract() over the whole forest.
+void ContractTrees(const std::vector<Index> &parent, Plan &plan, unsigned threads) {
+  const std::vector<Index> &roots = plan.roots;
+  const std::size_t trees = roots.size(), n = parent.size();
+  const auto first = [&](std::size_t t) { return roots[t]; };
+  const auto end = [&](std::size_t t) { return t + 1 < trees ? roots[t + 1] : static_cast<Index>(n); };
+  std::vector<Plan> local(trees);
+  std::vector<std::vector<TreeRound>> rounds(trees);
+  std::vector<std::size_t> order(trees);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](std::size_t a, std::size_t b) {
+    return end(a) - first(a) > end(b) - first(b);
+  });
+  const auto parallel = [&](auto &&job) {
+    std::atomic<std::size_t> next{0};
+    std::exception_ptr failure;
+    std::mutex failure_lock;
+    const auto work = [&] {
+      for (std::size_t i; (i = next.fetch_add(1)) < trees;) {
+        try {
+          job(order[i]);
+        } catch (...) {
+          std::lock_guard<std::mutex> hold(failure_lock);
+          if (!failure) failure = std::current_exception();
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (unsigned k = 1; k < std::min<std::size_t>(threads, trees); ++k) pool.emplace_back(work);
+    work();
+    for (std::thread &th : pool) th.join();
+    if (failure) std::rethrow_exception(failure);
+  };
+  parallel([&](std::size_t t) {
+    std::vector<Index> sub(parent.begin() + first(t), parent.begin() + end(t));
+    for (Index &p : sub) p = p == kNone ? kNone : p - first(t);
+    Contract(sub, local[t]);
+    rounds[t] = RoundsOf(local[t]);
+  });
+
+  std::size_t round_count = 0, max_stages = 0;
+  for (std::size_t t = 0; t < trees; ++t) {
+    round_count = std::max(round_count, rounds[t].size());
+    for (const TreeRound &tr : rounds[t]) max_stages = std::max(max_stages, tr.stages.size());
+  }
+  // Global offsets per (round, tree) of each op kind, round-major.
+  const auto at = [&](std::size_t r, std::size_t t) { return r * trees + t; };
+  const std::size_t cells = round_count * trees;
+  std::vector<std::size_t> rake_at(cells), absorb_at(cells), compress_at(cells), stage_at(cells * max_stages);
+  std::vector<thmm_primitive_batch> batches;
+  std::vector<Index> round_offsets;
+  std::size_t rakes = 0, combinations = 0, absorptions = 0, compressions = 0;
+  const auto round_of = [&](std::size_t t, std::size_t r) -> const TreeRound * {
+    return r < rounds[t].size() ? &rounds[t][r] : nullptr;
+  };
+  for (std::size_t r = 0; r < round_count; ++r) {
+    round_offsets.push_back(static_cast<Index>(batches.size()));
+    const auto push = [&](thmm_primitive primitive, std::size_t offset, std::size_t count) {
+      if (count != 0)
+        batches.push_back({static_cast<std::uint32_t>(primitive), static_cast<std::uint32_t>(offset),
+                           static_cast<std::uint32_t>(count)});
+    };
+    std::size_t begin = rakes;
+    for (std::size_t t = 0; t < trees; ++t)
+      if (const TreeRound *tr = round_of(t, r)) rake_at[at(r, t)] = rakes, rakes += tr->rakes;
+    push(THMM_RAKE, begin, rakes - begin);
+    for (std::size_t s = 0; s < max_stages; ++s) {
+      begin = combinations;
+      for (std::size_t t = 0; t < trees; ++t) {
+        const TreeRound *tr = round_of(t, r);
+        if (tr != nullptr && s < tr->stages.size())
+          stage_at[at(r, t) * max_stages + s] = combinations, combinations += tr->stages[s].second;
+      }
+      push(THMM_BRANCH_COMBINATION, begin, combinations - begin);
+    }
+    begin = absorptions;
+    for (std::size_t t = 0; t < trees; ++t)
+      if (const TreeRound *tr = round_of(t, r)) absorb_at[at(r, t)] = absorptions, absorptions += tr->absorbs;
+    push(THMM_BRANCH_ABSORPTION, begin, absorptions - begin);
+    begin = compressions;
+    for (std::size_t t = 0; t < trees; ++t)
+      if (const TreeRound *tr = round_of(t, r)) compress_at[at(r, t)] = compressions, compressions += tr->compresses;
+    push(THMM_COMPRESSION, begin, compressions - begin);
+  }
+  round_offsets.push_back(static_cast<Index>(batches.size()));
+
+  // Edge e of tree t is global edge e + (first(t) - t): one root per tree.
+  std::vector<std::size_t> edge_base(trees);
+  for (std::size_t t = 0; t < trees; ++t) edge_base[t] = first(t) - t;
+  plan.num_edges = n - trees;
+  plan.num_branches = rakes;
+  plan.edge_parents.resize(plan.num_edges);
+  plan.edge_children.resize(plan.num_edges);
+  plan.rakes.resize(rakes);
+  plan.combinations.resize(combinations);
+  plan.absorptions.resize(absorptions);
+  plan.compressions.resize(compressions);
+  parallel([&](std::size_t t) {
+    const Plan &p = local[t];
+    const Index node = first(t);
+    const Index edge = static_cast<Index>(edge_base[t]);
+    for (std::size_t e = 0; e < p.num_edges; ++e) {
+      plan.edge_parents[edge + e] = p.edge_parents[e] + node;
+      plan.edge_children[edge + e] = p.edge_children[e] + node;
+    }
+    for (std::size_t r = 0; r < rounds[t].size(); ++r) {
+      const TreeRound &tr = rounds[t][r];
+      // A round's branches are created by its rakes and consumed within it.
+      const auto branch = [&](Index b) { return static_cast<Index>(b - tr.rake_first + rake_at[at(r, t)]); };
+      for (std::uint32_t k = 0; k < tr.rakes; ++k) {
+        const thmm_rake &op = p.rakes[tr.rake_first + k];
+        plan.rakes[rake_at[at(r, t)] + k] = {op.leaf + node, op.parent + node, op.edge + edge, branch(op.branch)};
+      }
+      for (std::size_t s = 0; s < tr.stages.size(); ++s)
+        for (std::uint32_t k = 0; k < tr.stages[s].second; ++k) {
+          const thmm_branch_combination &op = p.combinations[tr.stages[s].first + k];
+          plan.combinations[stage_at[at(r, t) * max_stages + s] + k] = {branch(op.destination), branch(op.source),
+                                                                        op.stage};
+        }
+      for (std::uint32_t k = 0; k < tr.absorbs; ++k) {
+        const thmm_branch_absorption &op = p.absorptions[tr.absorb_first + k];
+        plan.absorptions[absorb_at[at(r, t)] + k] = {op.parent + node, branch(op.branch), op.round};
+      }
+      for (std::uint32_t k = 0; k < tr.compresses; ++k) {
+        const thmm_compression &op = p.compressions[tr.compress_first + k];
+        const Index index = static_cast<Index>(compress_at[at(r, t)] + k);
+        plan.compressions[index] = {op.left_edge + edge, op.middle + node, op.right_edge + edge,
+                                    op.parent + node,    op.child + node,  index};
+      }
+    }
+  });
+  plan.batches = std::move(batches);
+  plan.round_batch_offsets = std::move(round_offsets);
+}
+
+}  // namespace
+
+Plan MakePlan(const std::int64_t *parents, std::size_t num_nodes) {
+  if (num_nodes == 0) throw std::invalid_argument("a plan needs at least one node");
+  if (num_nodes >= kNone) throw std::length_error("plan node count exceeds uint32_t");
+  const std::size_t n = num_nodes;
+  std::vector<Index> parent(n, kNone);
+  Plan plan;
+  plan.num_nodes = n;
+  for (std::size_t i = 0; i < n; ++i) {
+    const std::int64_t p = parents[i];
+    if (p == -1) {
+      plan.roots.push_back(static_cast<Index>(i));
+      continue;
+    }
+    RequireWith(p >= 0 && static_cast<std::size_t>(p) < n && static_cast<std::size_t>(p) != i,
+                [&] { return "parent of node " + std::to_string(i) + " is out of range"; });
+    parent[i] = static_cast<Index>(p);
+  }
+  Require(!plan.roots.empty(), "a plan needs at least one root (parent -1)");
+  Require(Acyclic(parent), "parent links contain a cycle");
+  // Forests of many trees in consecutive node ranges plan tree by tree in
+  // parallel (same plan as the single contraction).
+  constexpr std::size_t kParallelNodes = 1u << 16;
+  const unsigned threads = PlannerThreads();
+  if (threads > 1 && n >= kParallelNodes && plan.roots.size() > 1 && ConsecutiveTrees(parent, plan.roots))
+    ContractTrees(parent, plan, threads);
+  else
+    Contract(parent, plan);
   return plan;
 }
 
@@ -233,9 +454,9 @@ void Validate(const Plan &plan) {
   }
   for (std::size_t e = 0; e < e_count; ++e) {
     const Index p = plan.edge_parents[e], c = plan.edge_children[e];
-    Require(p < n && c < n && p != c, "edge " + std::to_string(e) + " has invalid endpoints");
-    Require(!is_root[c] && parent[c] == kNone,
-            "node " + std::to_string(c) + " is the child of zero or several edges");
+    RequireWith(p < n && c < n && p != c, [&] { return "edge " + std::to_string(e) + " has invalid endpoints"; });
+    RequireWith(!is_root[c] && parent[c] == kNone,
+                [&] { return "node " + std::to_string(c) + " is the child of zero or several edges"; });
     parent[c] = p;
   }
   Require(e_count + plan.roots.size() == n, "every non-root node needs exactly one edge");
@@ -259,7 +480,8 @@ void Validate(const Plan &plan) {
     touched.clear();
   };
   auto claim = [&](Index id, const char *what) {
-    Require(!mark[id], std::string("batch is not write-disjoint: ") + what + " " + std::to_string(id));
+    RequireWith(!mark[id],
+                [&] { return std::string("batch is not write-disjoint: ") + what + " " + std::to_string(id); });
     mark[id] = 1;
     touched.push_back(id);
   };
@@ -391,11 +613,11 @@ void Validate(const Plan &plan) {
     if (is_root[i]) {
       Require(alive[i] && child_count[i] == 0 && pending[i] == 0, "root is not fully contracted");
     } else {
-      Require(!alive[i], "node " + std::to_string(i) + " is never eliminated");
+      RequireWith(!alive[i], [&] { return "node " + std::to_string(i) + " is never eliminated"; });
     }
   }
   for (std::size_t b = 0; b < plan.num_branches; ++b)
-    Require(branch_state[b] == 2, "branch " + std::to_string(b) + " is never consumed");
+    RequireWith(branch_state[b] == 2, [&] { return "branch " + std::to_string(b) + " is never consumed"; });
   Require(!plan.round_batch_offsets.empty() && plan.round_batch_offsets.front() == 0 &&
               plan.round_batch_offsets.back() == plan.batches.size() &&
               std::is_sorted(plan.round_batch_offsets.begin(), plan.round_batch_offsets.end()),
