@@ -319,6 +319,36 @@ thmm_status thmm_branch_length_backward(thmm_workspace *workspace,
                                         const double *rate_matrix,
                                         const double **grad, size_t *count);
 
+/* No reference counterpart (the reference takes edge factors only as
+ * caller-staged values, include/tree_hmm/accelerator.h:86-96). Transition
+ * matrices of a rate matrix Q (host, FP64, row-major K x K) reversible with
+ * respect to the stationary law pi (host, FP64 [K], positive; detailed
+ * balance pi_i Q_ij = pi_j Q_ji within 1e-10 of the largest flux):
+ *   P_e = expm(Q t_e) = V diag(exp(lambda t_e)) V^-1,
+ * from the symmetric eigendecomposition of D^1/2 Q D^-1/2 (D = diag(pi)),
+ * for branch lengths t (host, FP64 [edges], finite, >= 0). Entries that
+ * round below zero are set to zero. invalid_argument otherwise.
+ *
+ * thmm_expm_reversible writes the matrices to out (host, FP64
+ * [edges][K][K]) on the host. thmm_categorical_set_branch_lengths forms
+ * them on the device, rounded to the workspace precision, as the edge
+ * factors of a categorical workspace (one length per plan edge, in edge
+ * order). The observations and the root and emission factors must already
+ * be resident (a call with THMM_UPDATE_ALL or THMM_UPDATE_FACTORS;
+ * logic_error otherwise); later calls with THMM_UPDATE_NONE use the new
+ * edges, THMM_UPDATE_FACTORS restages the host edges. It drops a held
+ * forward pass of the branch-length pair and blocks until the factors are
+ * written. */
+thmm_status thmm_expm_reversible(const double *rate_matrix,
+                                 const double *stationary, size_t states,
+                                 const double *branch_lengths, size_t edges,
+                                 double *out);
+thmm_status thmm_categorical_set_branch_lengths(thmm_workspace *workspace,
+                                                const double *rate_matrix,
+                                                const double *stationary,
+                                                const double *branch_lengths,
+                                                size_t edge_count);
+
 /* ------------------------------------------------------------------------ */
 /* Device-resident entry points                                             */
 /* ------------------------------------------------------------------------ */

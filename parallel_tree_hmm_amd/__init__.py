@@ -28,6 +28,7 @@ from .api import (  # noqa: F401
     Workspace,
     available,
     device_description,
+    expm_reversible,
     log_partition_function_prepared,
     make_plan,
     maximum_a_posteriori_prepared,

@@ -209,6 +209,8 @@ PROTOTYPES = {
         [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_size)],
     ),
     "thmm_branch_length_backward": (c_status, [c_void_p, c_void_p, c_void_p, P(c_void_p), P(c_size)]),
+    "thmm_expm_reversible": (c_status, [c_void_p, c_void_p, c_size, c_void_p, c_size, c_void_p]),
+    "thmm_categorical_set_branch_lengths": (c_status, [c_void_p, c_void_p, c_void_p, c_void_p, c_size]),
     "thmm_get_tuning": (c_status, [c_void_p, P(Tuning)]),
     "thmm_set_tuning": (c_status, [c_void_p, P(Tuning)]),
     "thmm_autotune_marginals_device": (

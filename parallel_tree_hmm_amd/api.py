@@ -533,6 +533,19 @@ class Workspace:
         )
         return _array(ptr, count.value, np.float64).copy()
 
+    def categorical_set_branch_lengths(self, rate_matrix, stationary, branch_lengths) -> None:
+        """Device edge factors P_e = expm(Q t_e) for a rate matrix Q
+        reversible with respect to `stationary`, one branch length per plan
+        edge; follow with calls using CategoricalInputUpdate.kNone."""
+        k = self.states
+        q = np.ascontiguousarray(np.asarray(rate_matrix, dtype=np.float64))
+        pi = np.ascontiguousarray(np.asarray(stationary, dtype=np.float64)).ravel()
+        t = np.ascontiguousarray(np.asarray(branch_lengths, dtype=np.float64)).ravel()
+        if q.shape != (k, k) or pi.size != k:
+            raise InvalidArgument(f"rate matrix must be {k} x {k} and the stationary law of length {k}")
+        _check(_lib().thmm_categorical_set_branch_lengths(self._handle, q.ctypes.data, pi.ctypes.data, t.ctypes.data,
+                                                          t.size))
+
     def log_partitions_f64(self) -> np.ndarray:
         ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
         _check(_lib().thmm_result_log_partitions_f64(self._handle, ctypes.byref(ptr), ctypes.byref(count)))
@@ -542,6 +555,20 @@ class Workspace:
 # ---------------------------------------------------------------------------
 # Prepared calls (free functions, as in gpu_backend_api.inc:97-142)
 # ---------------------------------------------------------------------------
+
+
+def expm_reversible(rate_matrix, stationary, branch_lengths) -> np.ndarray:
+    """P_e = expm(Q t_e), FP64 [edges, K, K], for a rate matrix Q reversible
+    with respect to `stationary` (host computation, thmm_expm_reversible)."""
+    q = np.ascontiguousarray(np.asarray(rate_matrix, dtype=np.float64))
+    pi = np.ascontiguousarray(np.asarray(stationary, dtype=np.float64)).ravel()
+    t = np.ascontiguousarray(np.asarray(branch_lengths, dtype=np.float64)).ravel()
+    k = pi.size
+    if q.shape != (k, k):
+        raise InvalidArgument(f"rate matrix must be {k} x {k}")
+    out = np.empty((t.size, k, k), np.float64)
+    _check(_lib().thmm_expm_reversible(q.ctypes.data, pi.ctypes.data, k, t.ctypes.data, t.size, out.ctypes.data))
+    return out
 
 
 def _buffer(array, dtype, name: str):

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "expm.h"
 #include "plan.h"
 #include "tree_hmm_amd.h"
 #include "workspace.h"
@@ -263,6 +264,22 @@ thmm_status thmm_branch_length_backward(thmm_workspace *workspace, const double 
     Require(grad && count, "argument is null");
     W(workspace).BranchLengthBackward(grad_log_z, rate_matrix, grad, count);
   });
+}
+
+thmm_status thmm_expm_reversible(const double *rate_matrix, const double *stationary, size_t states,
+                                 const double *branch_lengths, size_t edges, double *out) {
+  return Guard([&] {
+    Require(edges == 0 || out != nullptr, "output is null");
+    const thmm::ReversibleRates rates = thmm::DecomposeReversible(rate_matrix, stationary, states);
+    thmm::CheckBranchLengths(branch_lengths, edges);
+    thmm::ExpmReversible(rates, branch_lengths, edges, out);
+  });
+}
+
+thmm_status thmm_categorical_set_branch_lengths(thmm_workspace *workspace, const double *rate_matrix,
+                                                const double *stationary, const double *branch_lengths,
+                                                size_t edge_count) {
+  return Guard([&] { W(workspace).SetBranchLengths(rate_matrix, stationary, branch_lengths, edge_count); });
 }
 
 thmm_status thmm_get_tuning(const thmm_workspace *workspace, thmm_tuning *out) {

@@ -955,6 +955,39 @@ __global__ void BranchChainSums(const double *terms, double *grad, Dims d) {
 
 }  // namespace
 
+namespace {
+
+// One thread per (e, i, j); exp(lambda_k t_e) is recomputed per entry (K of
+// them), which is negligible next to the K-term sum at the state counts of
+// substitution models.
+template <typename T>
+__global__ void ExpmEdges(const double *V, const double *lambda, const double *V_inv, const double *t, T *edges,
+                          Dims d) {
+  const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (tid >= kk * d.E) return;
+  const std::uint64_t e = tid / kk;
+  const std::uint32_t i = static_cast<std::uint32_t>(tid % kk) / d.K, j = static_cast<std::uint32_t>(tid % kk) % d.K;
+  double sum = 0.0;
+  for (std::uint32_t k = 0; k < d.K; ++k) sum += V[i * d.K + k] * exp(lambda[k] * t[e]) * V_inv[k * d.K + j];
+  edges[tid] = static_cast<T>(fmax(sum, 0.0));
+}
+
+}  // namespace
+
+void LaunchExpmEdges(const Launch &l, const double *V, const double *lambda, const double *V_inv,
+                     const double *branch_lengths, void *edges) {
+  const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  if (entries == 0) return;
+  if (l.real == Real::kF32)
+    ExpmEdges<float><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+        V, lambda, V_inv, branch_lengths, static_cast<float *>(edges), l.dims);
+  else
+    ExpmEdges<double><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+        V, lambda, V_inv, branch_lengths, static_cast<double *>(edges), l.dims);
+  Check(hipGetLastError(), "expm edge launch");
+}
+
 void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *edges, const double *rate,
                              double *terms, double *grad) {
   const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;

@@ -192,6 +192,11 @@ void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out
 //   grad[e] = sum_{p,c} M_e[p][c] (Q P_e)[p][c] / P_e[p][c],
 // M_e = sum_b edge_out[b][e] (the adjoint of P_e, batch-summed), FP64.
 // `terms` is [E][K][K] FP64 scratch; entries with P_e[p][c] = 0 contribute 0.
+// edges[e][i][j] = max(0, sum_k V[i][k] exp(lambda_k t_e) V_inv[k][j]) in
+// FP64, rounded to launch.real: expm(Q t_e) from the eigendecomposition of a
+// reversible rate matrix (factors [K][K], [K], [K][K] and t [E], FP64).
+void LaunchExpmEdges(const Launch &l, const double *V, const double *lambda, const double *V_inv,
+                     const double *branch_lengths, void *edges);
 void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *edges, const double *rate,
                              double *terms, double *grad);
 // Widens `count` FP32 values to FP64.

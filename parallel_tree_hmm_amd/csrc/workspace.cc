@@ -1,6 +1,8 @@
 // Host executor: see workspace.h.
 #include "workspace.h"
 
+#include "expm.h"
+
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -171,6 +173,9 @@ struct Workspace::Impl {
   // branch-length chain (categorical marginal workspaces)
   DeviceBuffer chain_seed, chain_rate, chain_terms, chain_grad;
   PinnedBuffer host_chain_seed, host_chain_rate, host_chain_grad;
+  // device expm of branch lengths: V, lambda, V_inv, then t (FP64)
+  DeviceBuffer expm_factors;
+  PinnedBuffer host_expm_factors;
 
   ~Impl() { Clear(); }
 
@@ -190,11 +195,12 @@ struct Workspace::Impl {
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
                             &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials,
-                            &chains})
+                            &chains, &expm_factors})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
-                            &host_edge_out, &host_failure, &host_chain_seed, &host_chain_rate, &host_chain_grad})
+                            &host_edge_out, &host_failure, &host_chain_seed, &host_chain_rate, &host_chain_grad,
+                            &host_expm_factors})
       b->Release();
     for (hipEvent_t &e : events) {
       if (e != nullptr) static_cast<void>(hipEventDestroy(e));
@@ -1412,6 +1418,38 @@ void Workspace::CategoricalLogZForward(const thmm_categorical_model &model, thmm
   w.forward = Impl::Forward::kCategorical;
   *log_z = w.host_log_z.as<double>();
   *count = model.batch * w.Roots();
+}
+
+void Workspace::SetBranchLengths(const double *rate_matrix, const double *stationary, const double *branch_lengths,
+                                 std::size_t edge_count) {
+  Impl &w = *impl_;
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
+  if (!w.categorical) throw std::logic_error("branch lengths set the edge factors of a categorical workspace");
+  if (edge_count != w.E()) throw std::invalid_argument("one branch length per edge is required");
+  const ReversibleRates rates = DecomposeReversible(rate_matrix, stationary, w.states);
+  CheckBranchLengths(branch_lengths, edge_count);
+  if (!w.resident_factors)
+    throw std::logic_error("categorical factors are not resident; stage them with kAll or kFactors first");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
+  const std::size_t K = w.states, factors = 2 * K * K + K;
+  if (w.expm_factors.bytes < (factors + w.E()) * sizeof(double)) {
+    w.expm_factors.Allocate((factors + w.E()) * sizeof(double), "expm factors");
+    w.host_expm_factors.Allocate((factors + w.E()) * sizeof(double), "pinned expm factors");
+  }
+  double *host = w.host_expm_factors.as<double>();
+  std::copy(rates.V.begin(), rates.V.end(), host);
+  std::copy(rates.lambda.begin(), rates.lambda.end(), host + K * K);
+  std::copy(rates.V_inv.begin(), rates.V_inv.end(), host + K * K + K);
+  std::copy(branch_lengths, branch_lengths + edge_count, host + factors);
+  Hip(hipMemcpyAsync(w.expm_factors.ptr, host, (factors + edge_count) * sizeof(double), hipMemcpyHostToDevice,
+                     w.stream),
+      "GPU runtime MemcpyAsync expm factor upload");
+  const Launch l = w.MakeLaunch(w.real, w.capacity, w.stream);
+  const double *device = w.expm_factors.as<double>();
+  LaunchExpmEdges(l, device, device + K * K, device + K * K + K, device + factors, w.edges.ptr);
+  CheckLaunches();
+  Hip(hipStreamSynchronize(w.stream), "tree-HMM accelerator execution");
 }
 
 void Workspace::BranchLengthBackward(const double *grad_log_z, const double *rate_matrix, const double **grad,

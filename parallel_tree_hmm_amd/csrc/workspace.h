@@ -64,6 +64,11 @@ class Workspace {
   void LogPartitionsF64(const double **values, std::size_t *count);
   void CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
                               const double **log_z, std::size_t *count);
+  // Writes the categorical workspace's device edge factors as expm(Q t_e)
+  // (reversible Q with stationary law pi); needs resident observations and
+  // factors, drops a held forward pass.
+  void SetBranchLengths(const double *rate_matrix, const double *stationary, const double *branch_lengths,
+                        std::size_t edge_count);
   void BranchLengthBackward(const double *grad_log_z, const double *rate_matrix, const double **grad,
                             std::size_t *count);
 

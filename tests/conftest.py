@@ -158,6 +158,13 @@ def reversible_rate_matrix(rng, k: int) -> np.ndarray:
     return q / -np.sum(pi * np.diag(q))
 
 
+def stationary_of(q: np.ndarray) -> np.ndarray:
+    """pi with pi Q = 0 (the left null vector of the rate matrix), normalised."""
+    w, v = np.linalg.eig(q.T)
+    pi = np.real(v[:, np.argmin(np.abs(w))])
+    return pi / pi.sum()
+
+
 def transition_matrices(q: np.ndarray, t: np.ndarray) -> np.ndarray:
     """P_e = expm(Q t_e) in FP64 ([edge, parent, child])."""
     from scipy.linalg import expm

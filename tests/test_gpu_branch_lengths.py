@@ -3,7 +3,9 @@ thmm_branch_length_backward) against the oracle's chain over its edge
 marginals (the chain rule itself is pinned by finite differences in
 test_branch_length_chain.py), at a small size and at the full categorical
 workload: one balanced binary tree with 4,096 leaves (8,191 nodes), 1,024
-sites, K = 4, 5 % missing leaves, FP32 with FP64 log scales.
+sites, K = 4, 5 % missing leaves, FP32 with FP64 log scales. Edge factors
+formed on the device from branch lengths (thmm_categorical_set_branch_lengths)
+give what the same factors staged from the host give.
 
 Tolerances: FP64 workspaces 1e-9 relative; FP32 workspaces 1e-4 relative to
 the largest gradient entry (each entry sums FP32 edge marginals of every
@@ -16,7 +18,7 @@ import pytest
 
 import oracle
 import parallel_tree_hmm_amd as thmm
-from conftest import dense_categorical_nodes, phylo_model, transition_matrices
+from conftest import dense_categorical_nodes, phylo_model, stationary_of, transition_matrices
 from parallel_tree_hmm_amd.api import BatchedCategoricalModelView, CategoricalInputUpdate
 
 pytestmark = pytest.mark.gpu
@@ -156,3 +158,46 @@ def test_branch_length_gradient_full_workload(require_gpu):
     all32 = ws32.branch_length_backward(q, None)
     all64 = ws64.branch_length_backward(q, None)
     assert np.max(np.abs(all32 - all64)) <= 1e-4 * max(1.0, np.max(np.abs(all64)))
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_device_branch_lengths_match_staged_factors(require_gpu, precision):
+    rng = np.random.default_rng(44)
+    sites = 32
+    parents, leaves, obs, root, emission, q, t = phylo_model(rng, 6, sites)
+    pi = stationary_of(q)
+    plan = thmm.Plan(parents)
+    dtype = DTYPES[precision]
+    seeds = rng.normal(size=sites)
+
+    def staged(lengths):
+        ws = reserve(plan, leaves, sites, precision)
+        factors = thmm.expm_reversible(q, pi, lengths).astype(dtype)
+        model = BatchedCategoricalModelView(plan, 4, sites, 5, leaves, obs, root, emission, factors)
+        return ws.categorical_logz_forward(model), ws.branch_length_backward(q, seeds), model
+
+    ws = reserve(plan, leaves, sites, precision)
+    with pytest.raises(thmm.LogicError):  # nothing resident yet
+        ws.categorical_set_branch_lengths(q, pi, t)
+    first_log_z, first_grad, model = staged(t)
+    assert np.array_equal(ws.categorical_logz_forward(model), first_log_z)
+
+    longer = t * 1.3 + 0.01
+    ws.categorical_set_branch_lengths(q, pi, longer)
+    with pytest.raises(thmm.LogicError):  # the held forward pass was dropped
+        ws.branch_length_backward(q, seeds)
+    log_z = ws.categorical_logz_forward(model, CategoricalInputUpdate.kNone)
+    grad = ws.branch_length_backward(q, seeds)
+    want_log_z, want_grad, _ = staged(longer)
+    tol = 1e-12 if precision == "f64" else 1e-6
+    assert np.max(np.abs(log_z - want_log_z) / np.maximum(1.0, np.abs(want_log_z))) <= tol
+    assert np.max(np.abs(grad - want_grad)) <= 10 * tol * max(1.0, np.max(np.abs(want_grad)))
+    assert not np.allclose(log_z, first_log_z)
+
+    # kFactors restages the host edges (the first lengths' factors).
+    again = ws.categorical_logz_forward(model, CategoricalInputUpdate.kFactors)
+    assert np.array_equal(again, first_log_z)
+    with pytest.raises(thmm.InvalidArgument):
+        ws.categorical_set_branch_lengths(q, pi, longer[:-1])
+    with pytest.raises(thmm.InvalidArgument):
+        ws.categorical_set_branch_lengths(q, pi, -longer)
