@@ -408,7 +408,12 @@ thmm_status thmm_synchronize(thmm_workspace *workspace);
  * (64, 128 or 256); gather_chunk: rakes a team folds into one parent per
  * step of the fused forward pass (1..64); compress_depth: compression-tree
  * levels evaluated as one grouped product of up to 2^depth slots on the
- * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2).
+ * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2);
+ * rake_log_edges: in thmm_logz_forward_device / _backward_device with
+ * K = 8, 16, 32 or 64, the rake kernels read the log edge potentials in
+ * place (1, default; only the edges compressions read are staged, and the
+ * caller's log edges must stay unchanged until the backward call) or every
+ * edge is staged first (0); both give the same values.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
  * forward program, and a new compress_depth also drops a held
@@ -417,6 +422,7 @@ typedef struct thmm_tuning {
   uint32_t compress_items_forward, compress_items_reverse;
   uint32_t rake_block, gather_chunk;
   uint32_t compress_depth;
+  uint32_t rake_log_edges;
   double ms;
 } thmm_tuning;
 

@@ -120,6 +120,7 @@ class Tuning(ctypes.Structure):
         ("rake_block", c_u32),
         ("gather_chunk", c_u32),
         ("compress_depth", c_u32),
+        ("rake_log_edges", c_u32),
         ("ms", ctypes.c_double),
     ]
 

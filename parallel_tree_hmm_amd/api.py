@@ -468,7 +468,7 @@ class Workspace:
 
     # -- launch tuning -----------------------------------------------------
     _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk",
-                      "compress_depth")
+                      "compress_depth", "rake_log_edges")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()

@@ -67,7 +67,8 @@ thmm::Workspace &W(thmm_workspace *workspace) {
 }
 
 thmm_tuning ToC(const thmm::Tuning &t, double ms) {
-  return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, t.compress_depth, ms};
+  return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, t.compress_depth,
+          t.rake_log_edges, ms};
 }
 
 }  // namespace
@@ -298,6 +299,7 @@ thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning
     t.rake_block = tuning->rake_block;
     t.gather_chunk = tuning->gather_chunk;
     t.compress_depth = tuning->compress_depth;
+    t.rake_log_edges = tuning->rake_log_edges;
     W(workspace).SetTuning(t);
   });
 }

@@ -32,6 +32,81 @@ namespace {
 
 using namespace dev;
 
+// Cross-lane maxima and exchanges without an LDS round trip: DPP moves
+// inside a 16-lane row (quad permutations, then row rotations by 4 and 8
+// cover all 16 lanes), and ds_swizzle's bitmask mode for lane ^ 16 inside a
+// 32-lane half. No source lane is ever out of its row, so every lane reads
+// a valid value.
+constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowHalfMirror = 0x141, kRowRotate4 = 0x124, kRowRotate8 = 0x128;
+constexpr int kSwizzleXor16 = 0x401F;  // and_mask 0x1F, or_mask 0, xor_mask 0x10
+
+template <int kCtrl> __device__ inline float DppMove(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
+}
+template <int kCtrl> __device__ inline double DppMove(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits), kCtrl, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), kCtrl, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Maximum over the 16 lanes of each row (lane l and l ^ 1, ..., l ^ 8).
+__device__ inline float RowLanesMax(float v) {
+  v = fmaxf(v, DppMove<kQuadXor1>(v));
+  v = fmaxf(v, DppMove<kQuadXor2>(v));
+  v = fmaxf(v, DppMove<kRowRotate4>(v));
+  return fmaxf(v, DppMove<kRowRotate8>(v));
+}
+__device__ inline double RowLanesMax(double v) {
+  v = fmax(v, DppMove<kQuadXor1>(v));
+  v = fmax(v, DppMove<kQuadXor2>(v));
+  v = fmax(v, DppMove<kRowRotate4>(v));
+  return fmax(v, DppMove<kRowRotate8>(v));
+}
+
+// The value of lane l ^ 16.
+__device__ inline float Xor16(float v) {
+  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), kSwizzleXor16));
+}
+
+
+// Maximum over each group of kLpr consecutive lanes (kLpr = 2, 4, 8, 16,
+// aligned groups inside a 16-lane row): quad permutations, then the mirror
+// of each 8-lane half (lane i <-> 7 - i) or rotations by 4 and 8.
+template <int kLpr, typename V> __device__ inline V GroupLanesMax(V v) {
+  static_assert(kLpr == 1 || kLpr == 2 || kLpr == 4 || kLpr == 8 || kLpr == 16, "lane group");
+  if constexpr (kLpr >= 2) v = Max(v, DppMove<kQuadXor1>(v));
+  if constexpr (kLpr >= 4) v = Max(v, DppMove<kQuadXor2>(v));
+  if constexpr (kLpr == 8) v = Max(v, DppMove<kRowHalfMirror>(v));
+  if constexpr (kLpr == 16) {
+    v = Max(v, DppMove<kRowRotate4>(v));
+    v = Max(v, DppMove<kRowRotate8>(v));
+  }
+  return v;
+}
+
+// Log edge potentials read in place by the rake kernels: each row of the
+// team's quads becomes exp(x - shift), the shift being the row's maximum M,
+// or 0 for a row of -inf (which then stays 0); the values equal those of
+// the log-edge staging transform (ExpNonPositive) bit for bit. Returns M of
+// step `owned`'s row (-inf for a dead row).
+template <typename T> __device__ inline T RowShift(T top) { return top > NegInf<T>() ? top : T(0); }
+
+template <typename T, int kLpr, int kJ>
+__device__ inline double ExpLogRows(Quad<T> (&pq)[kJ], int owned) {
+  double own = 0.0;
+#pragma unroll
+  for (int j = 0; j < kJ; ++j) {
+    T top = Max(Max(pq[j].v[0], pq[j].v[1]), Max(pq[j].v[2], pq[j].v[3]));
+    top = GroupLanesMax<kLpr>(top);
+    const T shift = RowShift(top);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pq[j].v[i] = ExpNonPositive(static_cast<T>(pq[j].v[i] - shift));
+    if (j == owned) own = top > NegInf<T>() ? static_cast<double>(top) : kNegInf;
+  }
+  return own;
+}
+
 template <int K> struct Team {
   static constexpr int kLanesPerRow = K / 4;
   static constexpr int kLanes = K <= 4 ? 4 : (K <= 16 ? 16 : 64);
@@ -130,14 +205,24 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
   T e[4];
   TeamExp<T, kT, kLpr>(vq, l, e);
-  const T *P = Slot<T>(buf, d, op.slot, b);
+  // Log edges (see RakeGatherTeam): exp(x - shift) with the row shifts the
+  // forward pass stored, the values the forward pass used.
+  const bool log_edge = buf.log_edges != nullptr && op.slot < d.E;
+  const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
+                        : Slot<T>(buf, d, op.slot, b);
   T *adj_p = SlotAdjoint<T>(buf, d, op.slot, b);
   const T *adj_m = RakeAdjoint<T>(buf, d, op, b);
   double column[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {
     const std::uint32_t quad = static_cast<std::uint32_t>(j * kT + l) * 4;
-    const Quad<T> p = LoadQuadStream(P + quad);
+    Quad<T> p = LoadQuadStream(P + quad);
+    if (log_edge) {
+      const T shift =
+          static_cast<T>(buf.log_edge_shift[static_cast<std::uint64_t>(op.slot) * K + j * kRps + l / kLpr]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) p.v[i] = ExpNonPositive(static_cast<T>(p.v[i] - shift));
+    }
     T t[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) t[i] = p.v[i] * e[i];
@@ -217,12 +302,21 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
     if (i + 1 < chunk.count) next = g.rakes[chunk.first + i + 1];
     const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
     const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
-    const T *P = Slot<T>(buf, d, op.slot, b);
+    const bool log_edge = buf.log_edges != nullptr && op.slot < d.E;  // the same for the whole team
+    const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
+                          : Slot<T>(buf, d, op.slot, b);
     Quad<T> pq[kJ];
 #pragma unroll
     for (int j = 0; j < kJ; ++j) pq[j] = LoadQuadStream(P + static_cast<std::uint32_t>(j * kT + l) * 4);
     const double leaf_scale = buf.node_scale[leaf_at];
-    const double row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
+    double row_scale;
+    if (log_edge) {
+      row_scale = ExpLogRows<T, kLpr, kJ>(pq, owned);
+      if (row >= 0)
+        buf.log_edge_shift[static_cast<std::uint64_t>(op.slot) * K + row] = row_scale > kNegInf ? row_scale : 0.0;
+    } else {
+      row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
+    }
     T e[4];
     TeamExp<T, kT, kLpr>(vq, l, e);
     T mine = T(0);
@@ -283,44 +377,6 @@ __global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, s
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;  // wavefronts per block
-
-// Cross-lane maxima and exchanges without an LDS round trip: DPP moves
-// inside a 16-lane row (quad permutations, then row rotations by 4 and 8
-// cover all 16 lanes), and ds_swizzle's bitmask mode for lane ^ 16 inside a
-// 32-lane half. No source lane is ever out of its row, so every lane reads
-// a valid value.
-constexpr int kQuadXor1 = 0xB1, kQuadXor2 = 0x4E, kRowRotate4 = 0x124, kRowRotate8 = 0x128;
-constexpr int kSwizzleXor16 = 0x401F;  // and_mask 0x1F, or_mask 0, xor_mask 0x10
-
-template <int kCtrl> __device__ inline float DppMove(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
-}
-template <int kCtrl> __device__ inline double DppMove(double v) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits), kCtrl, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(bits >> 32), kCtrl, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-// Maximum over the 16 lanes of each row (lane l and l ^ 1, ..., l ^ 8).
-__device__ inline float RowLanesMax(float v) {
-  v = fmaxf(v, DppMove<kQuadXor1>(v));
-  v = fmaxf(v, DppMove<kQuadXor2>(v));
-  v = fmaxf(v, DppMove<kRowRotate4>(v));
-  return fmaxf(v, DppMove<kRowRotate8>(v));
-}
-__device__ inline double RowLanesMax(double v) {
-  v = fmax(v, DppMove<kQuadXor1>(v));
-  v = fmax(v, DppMove<kQuadXor2>(v));
-  v = fmax(v, DppMove<kRowRotate4>(v));
-  return fmax(v, DppMove<kRowRotate8>(v));
-}
-
-// The value of lane l ^ 16.
-__device__ inline float Xor16(float v) {
-  return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), kSwizzleXor16));
-}
-
 
 __device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
@@ -1180,7 +1236,7 @@ __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if constexpr (kMode == 3)
-        o.v[i] = live ? Exp(static_cast<Out>(w[i] - top)) : Out(0);
+        o.v[i] = live ? ExpNonPositive(static_cast<Out>(w[i] - top)) : Out(0);
       else
         o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
     }
@@ -1355,6 +1411,61 @@ bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t 
 }
 
 }  // namespace
+
+namespace {
+
+// Mode-3 staging (exp of each row shifted by its maximum, with that
+// maximum as the row's FP64 scale) of the listed edges only, in place:
+// row r of the launch is row r % K of edge ids[r / K].
+template <typename T, int K>
+__global__ __launch_bounds__(256) void StageLogRowsTeam(const T *log_edges, const std::uint32_t *ids,
+                                                        std::uint64_t rows, T *edges, double *scale) {
+  constexpr int kLanes = K / 4;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t r = tid / kLanes;
+  const int l = static_cast<int>(tid % kLanes);
+  if (r >= rows) return;
+  const std::uint64_t v = static_cast<std::uint64_t>(ids[r / K]) * K + r % K;
+  const Quad<T> q = LoadQuad(log_edges + v * K + l * 4);
+  T top = Max(Max(q.v[0], q.v[1]), Max(q.v[2], q.v[3]));
+#pragma unroll
+  for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
+  const bool live = top > NegInf<T>();
+  Quad<T> o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) o.v[i] = live ? ExpNonPositive(static_cast<T>(q.v[i] - top)) : T(0);
+  StoreQuad(edges + v * K + l * 4, o);
+  if (l == 0) scale[v] = live ? static_cast<double>(top) : kNegInf;
+}
+
+}  // namespace
+
+bool LaunchStageLogEdges(const Launch &l, const void *log_edges, const std::uint32_t *ids, std::uint32_t count,
+                         void *edges, double *edge_scale) {
+  const std::uint32_t K = l.dims.K;
+  if (K != 8 && K != 16 && K != 32 && K != 64) return false;
+  if (count == 0) return true;
+  const std::uint64_t rows = static_cast<std::uint64_t>(count) * K;
+  const unsigned grid = Blocks(rows * (K / 4));
+  const hipStream_t s = StreamOf(l);
+#define THMM_STAGE(T, KK)                                                                                        \
+  StageLogRowsTeam<T, KK><<<grid, kBlock, 0, s>>>(static_cast<const T *>(log_edges), ids, rows,                  \
+                                                  static_cast<T *>(edges), edge_scale)
+  if (l.real == Real::kF32) {
+    if (K == 8) THMM_STAGE(float, 8);
+    if (K == 16) THMM_STAGE(float, 16);
+    if (K == 32) THMM_STAGE(float, 32);
+    if (K == 64) THMM_STAGE(float, 64);
+  } else {
+    if (K == 8) THMM_STAGE(double, 8);
+    if (K == 16) THMM_STAGE(double, 16);
+    if (K == 32) THMM_STAGE(double, 32);
+    if (K == 64) THMM_STAGE(double, 64);
+  }
+#undef THMM_STAGE
+  RecordLaunchError(hipGetLastError(), "log edge staging launch");
+  return true;
+}
 
 bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
                          std::uint32_t X, std::uint32_t Y, double *scale) {

@@ -84,6 +84,13 @@ struct LinearBuffers {
   void *branch_adj;
   void *path_adj;
   void *edge_out;
+  // Log edge potentials [E][K][K] that the team rake kernels read in place
+  // (exp of the row-shifted values on load) for edge slots; null when every
+  // edge is staged in `edges`. Compressions always read `edges`. The forward
+  // rake kernel writes each row's shift (its maximum, 0 for a row of -inf)
+  // to log_edge_shift [E][K], which the reverse reads.
+  const void *log_edges;
+  double *log_edge_shift;
 };
 
 // Log-domain max-product state (FP64):
@@ -108,6 +115,7 @@ struct Tuning {
   std::uint32_t rake_block = 256;            // threads per block of the team rake kernels: 64, 128, 256
   std::uint32_t gather_chunk = 16;           // rakes per team in the fused forward: 1 .. 64
   std::uint32_t compress_depth = 2;          // compression-tree levels per grouped compression: 1 (none) .. 3
+  std::uint32_t rake_log_edges = 1;          // log-potential calls: rakes read log edges in place (1) or staged (0)
 };
 
 struct Launch {
@@ -129,6 +137,11 @@ void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes
                              double *node_scale);
 // Log edge potentials [E][K][K] -> row-max-shifted exponentials + FP64 row
 // scales [E][K].
+// The same transform for the edges listed in ids only (edges and scales
+// written in place); false when the state count has no team kernels (8, 16,
+// 32, 64), in which case nothing is launched.
+bool LaunchStageLogEdges(const Launch &l, const void *log_edges, const std::uint32_t *ids, std::uint32_t count,
+                         void *edges, double *edge_scale);
 void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges,
                              double *edge_scale);
 // Categorical observations -> internal [N][B][K] node factors, the product

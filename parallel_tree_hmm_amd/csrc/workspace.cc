@@ -133,6 +133,12 @@ struct Workspace::Impl {
   DeviceBuffer chains;
   std::vector<Level> chain_levels;
   std::vector<Level> grouped_levels;
+  // Log-potential calls: the edges that compressions read (the only ones
+  // staged when the rake kernels read log edges in place), the largest rake
+  // level, and the caller's log edges a held forward pass read in place.
+  DeviceBuffer compress_edges;
+  std::uint32_t compress_edge_count = 0, max_rake_level = 0;
+  const void *forward_log_edges = nullptr;
   // Instantiated hipGraphs of the device-resident entry points (reserve flag
   // THMM_RESERVE_GRAPH), one per entry point, keyed by its arguments.
   struct GraphKey {
@@ -195,7 +201,7 @@ struct Workspace::Impl {
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
                             &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials,
-                            &chains, &expm_factors})
+                            &chains, &expm_factors, &compress_edges})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
@@ -224,6 +230,8 @@ struct Workspace::Impl {
     fused.clear();
     chain_levels.clear();
     grouped_levels.clear();
+    compress_edge_count = max_rake_level = 0;
+    forward_log_edges = nullptr;
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
@@ -586,6 +594,17 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   Upload(w.combines, combines, w.stream, "topology upload");
   Upload(w.absorbs, absorbs, w.stream, "topology upload");
   Upload(w.compresses, compresses, w.stream, "topology upload");
+  {
+    std::vector<std::uint32_t> consumed;
+    for (const DeviceCompress &c : compresses)
+      for (std::uint32_t slot : {c.left, c.right})
+        if (slot < E) consumed.push_back(slot);
+    std::sort(consumed.begin(), consumed.end());
+    Upload(w.compress_edges, consumed, w.stream, "topology upload");
+    w.compress_edge_count = static_cast<std::uint32_t>(consumed.size());
+    for (const thmm_primitive_batch &b : plan.batches)
+      if (b.primitive == THMM_RAKE) w.max_rake_level = std::max(w.max_rake_level, b.count);
+  }
   Upload(w.roots, plan.roots, w.stream, "topology upload");
   std::vector<std::uint8_t> is_root(plan.num_nodes, 0);
   for (std::uint32_t r : plan.roots) is_root[r] = 1;
@@ -1559,6 +1578,7 @@ void ValidateTuning(const Tuning &t) {
     throw std::invalid_argument("rake block size must be 64, 128 or 256");
   if (t.gather_chunk < 1 || t.gather_chunk > 64) throw std::invalid_argument("gather chunk must be in [1, 64]");
   if (t.compress_depth < 1 || t.compress_depth > 3) throw std::invalid_argument("compression depth must be 1, 2 or 3");
+  if (t.rake_log_edges > 1) throw std::invalid_argument("rake_log_edges must be 0 or 1");
 }
 
 // Installs `t`; a new chunk size or compression depth rebuilds the fused
@@ -1653,14 +1673,28 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   w.forward = Impl::Forward::kNone;
   Impl::GraphKey key;
   key.args[0] = log_nodes, key.args[1] = log_edges, key.args[2] = log_z, key.batch = batch;
+  // With the fused forward the team rake kernels read the log edges in
+  // place and only the edges compressions read are staged.
+  const bool in_place = w.tuning.rake_log_edges != 0 && !w.fused.empty() &&
+                        static_cast<std::uint64_t>(w.max_rake_level) * batch * 64 < (1ull << 32) &&
+                        (reinterpret_cast<std::uintptr_t>(log_edges) & 15u) == 0;
+  key.flag = in_place ? 1 : 0;
+  w.forward_log_edges = nullptr;
   RunOrReplay(w, 2, key, ChooseStream(w, stream), [&](hipStream_t s) {
     const Launch l = w.MakeLaunch(w.real, batch, s);
     LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
-    LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
-    const LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+    LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+    if (in_place && LaunchStageLogEdges(l, log_edges, w.compress_edges.as<std::uint32_t>(), w.compress_edge_count,
+                                        w.edges.ptr, w.edge_scale.as<double>())) {
+      buf.log_edges = log_edges;
+      buf.log_edge_shift = w.edge_scale.as<double>();  // rows of rake edges; staged rows keep their scales
+    }
+    else
+      LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
     SumProductForward(w, l, buf);
     LaunchFinish(l, buf, log_z);
   });
+  if (in_place && (w.states == 8 || w.states == 16 || w.states == 32 || w.states == 64)) w.forward_log_edges = log_edges;
   w.last_batch = batch;
   w.forward = Impl::Forward::kLogPotentials;
 }
@@ -1678,12 +1712,23 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   Impl::GraphKey key;
   key.args[0] = grad, key.args[1] = grad_nodes, key.args[2] = grad_edges, key.batch = w.last_batch;
+  key.args[3] = w.forward_log_edges;
   key.flag = reduce_edges ? 1 : 0;
   RunOrReplay(w, 3, key, ChooseStream(w, stream), [&](hipStream_t s) {
     const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
     LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
     buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
     AdjointsInto(buf, w.last_batch, grad_nodes);
+    if (w.forward_log_edges != nullptr) {
+      // The team rake kernels need 16-byte aligned outputs; otherwise the
+      // rest of the edges are staged for the plan-level kernels.
+      const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
+      if (aligned(buf.edge_out) && aligned(buf.node_adj)) {
+        buf.log_edges = w.forward_log_edges;
+        buf.log_edge_shift = w.edge_scale.as<double>();
+      } else
+        LaunchInitEdgesFromLogs(l, w.forward_log_edges, w.edges.ptr, w.edge_scale.as<double>());
+    }
     LaunchSeedAdjoint(l, buf, grad);
     ReverseLevels(l, ReverseProgram(w, buf), buf);
     if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);

@@ -281,3 +281,58 @@ def test_unaligned_device_buffers_fall_back(require_gpu):
     (lz0, n0, e0), (lz1, n1, e1) = results
     assert np.max(np.abs(lz0 - lz1) / np.maximum(1, np.abs(lz0))) <= 1e-6
     assert np.max(np.abs(n0 - n1)) <= 1e-6 and np.max(np.abs(e0 - e1)) <= 1e-6
+
+
+@pytest.mark.parametrize("k,precision", [(8, "f32"), (16, "f32"), (32, "f32"), (64, "f32"), (16, "f64")])
+def test_log_edges_read_in_place(require_gpu, k, precision):
+    """The adjoint pair over log potentials with the rake kernels reading the
+    log edges in place (tuning rake_log_edges = 1; only the edges
+    compressions read are staged) gives the same bits as staging every edge
+    first (rake_log_edges = 0), also through graph replay; with unaligned
+    gradient outputs the backward stages the remaining edges itself. Values
+    against the FP64 oracle."""
+    import torch
+
+    rng = np.random.default_rng(k)
+    plan = thmm.Plan(forest([random_recursive(150, rng), caterpillar(121), path(60), star(40)]))
+    batch = 2
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    log_nodes = torch.tensor(rng.normal(size=(batch, plan.num_nodes, k)), dtype=dtype, device="cuda")
+    log_edges = torch.tensor(rng.normal(size=(plan.num_edges, k, k)), dtype=dtype, device="cuda")
+    log_edges[0, 1, :] = -float("inf")  # a dead row
+    grad = torch.tensor(rng.normal(size=batch * plan.roots.size), dtype=torch.float64, device="cuda")
+
+    def run(in_place, graphs=False, shift=0):
+        ws = thmm.Workspace()
+        ws.reserve_marginals(plan, k, batch, precision=precision, device_resident=True, graphs=graphs)
+        ws.set_tuning(rake_log_edges=in_place)
+        outs = []
+        for _ in range(2 if graphs else 1):
+            lz = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+            gn = torch.empty(batch * plan.num_nodes * k + shift, dtype=dtype, device="cuda")[shift:]
+            ge = torch.empty(batch * plan.num_edges * k * k + shift, dtype=dtype, device="cuda")[shift:]
+            ws.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), batch, lz.data_ptr(), 0)
+            ws.logz_backward_device(grad.data_ptr(), gn.data_ptr(), ge.data_ptr(), False, 0)
+            torch.cuda.synchronize()
+            outs.append((lz.cpu(), gn.cpu(), ge.cpu()))
+        return outs[-1]
+
+    staged, direct, replayed = run(0), run(1), run(1, graphs=True)
+    for a, b, c in zip(staged, direct, replayed):
+        assert torch.equal(a, b) and torch.equal(a, c)
+    unaligned = run(1, shift=1)
+    for a, b in zip(staged, unaligned):
+        assert float((a.double() - b.double()).abs().max()) <= (1e-6 if precision == "f32" else 1e-12)
+
+    nodes = np.exp(log_nodes.double().cpu().numpy())
+    edges = np.exp(log_edges.double().cpu().numpy())
+    olz, onm, oem = oracle.marginals_batched(plan, k, nodes, edges, threads=4)
+    g = grad.cpu().numpy().reshape(batch, -1)
+    per_node = g[:, root_index_of_nodes(plan)]
+    tol = 1e-5 if precision == "f32" else 1e-11
+    lz = direct[0].numpy().reshape(batch, -1)
+    assert np.all(np.abs(lz - olz) <= tol * np.maximum(1.0, np.abs(olz)))
+    gn = direct[1].double().numpy().reshape(batch, plan.num_nodes, k)
+    assert np.max(np.abs(gn - per_node[:, :, None] * onm)) <= 4 * tol
+    ge = direct[2].double().numpy().reshape(batch, plan.num_edges, k, k)
+    assert np.max(np.abs(ge - per_node[:, plan.edge_children][:, :, None, None] * oem)) <= 4 * tol
