@@ -49,6 +49,9 @@ class LogPartition(torch.autograd.Function):
         log_z = torch.empty((batch, roots), dtype=torch.float64, device=log_nodes.device)
         stream = torch.cuda.current_stream().cuda_stream
         workspace.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), batch, log_z.data_ptr(), stream)
+        # The backward pass reads the log edges again (the rake kernels take
+        # them in place): keep this exact tensor alive and unmodified.
+        ctx.save_for_backward(log_edges)
         ctx.generation = _claim(workspace)
         ctx.workspace = workspace
         ctx.reduce_edges = reduce_edges
@@ -58,6 +61,7 @@ class LogPartition(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_log_z: torch.Tensor):
         _require_latest(ctx.workspace, ctx.generation)
+        (_log_edges,) = ctx.saved_tensors  # checks that it was not modified in place
         node_shape, edge_shape, dtype = ctx.shapes
         device = grad_log_z.device
         grad = grad_log_z.to(torch.float64).contiguous()

@@ -82,3 +82,40 @@ def test_autograd_forest_roots(require_gpu):
     assert np.allclose(log_z.detach().cpu().numpy()[0], olz, rtol=0, atol=1e-11)
     assert np.allclose(log_nodes.grad.cpu().numpy()[0], nm, atol=1e-11)
     assert np.allclose(log_edges.grad.cpu().numpy(), em, atol=1e-11)
+
+
+def test_autograd_keeps_log_edges_for_backward(require_gpu):
+    """K = 16: the backward pass reads the log edges again (the rake kernels
+    take them in place), so the wrapper keeps the tensor it passed to the
+    forward call alive, here the contiguous copy of a transposed view, and
+    an in-place change of it before backward is refused."""
+    import torch
+
+    from conftest import forest
+    from parallel_tree_hmm_amd.autograd import log_partition, reserve_for
+
+    rng = np.random.default_rng(5)
+    plan = thmm.Plan(forest([caterpillar(61), random_recursive(80, rng)]))
+    k = 16
+    log_nodes = torch.tensor(rng.normal(size=(1, plan.num_nodes, k)), dtype=torch.float32, device="cuda",
+                             requires_grad=True)
+    base = torch.tensor(rng.normal(size=(plan.num_edges, k, k)), dtype=torch.float32, device="cuda",
+                        requires_grad=True)
+    ws = reserve_for(plan, k, 1, torch.float32)
+    view = base.transpose(1, 2)  # not contiguous: the wrapper passes a copy
+    log_z = log_partition(log_nodes, view, ws)
+    churn = [torch.full_like(base, 123.0) for _ in range(4)]  # reuses freed blocks, if any
+    log_z.sum().backward()
+    del churn
+    edges = np.exp(view.detach().double().cpu().numpy())
+    olz, nm, em = oracle.marginals(plan, k, np.exp(log_nodes.detach().double().cpu().numpy()[0]), edges)
+    assert np.max(np.abs(log_z.detach().cpu().numpy()[0] - olz) / np.maximum(1.0, np.abs(olz))) <= 1e-5
+    assert np.max(np.abs(log_nodes.grad.double().cpu().numpy()[0] - nm)) <= 2e-5
+    assert np.max(np.abs(base.grad.transpose(1, 2).double().cpu().numpy() - em)) <= 4e-5
+
+    shifted = base.detach().clone().requires_grad_()
+    inner = shifted * 1.0
+    again = log_partition(log_nodes.detach(), inner, ws)
+    inner.add_(1.0)
+    with pytest.raises(RuntimeError):
+        again.sum().backward()
