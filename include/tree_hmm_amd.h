@@ -141,7 +141,18 @@ enum {
    * again whenever the pointers, batch or reduction flag change; the replay
    * is enqueued on the call's stream. thmm_profile_marginals_device stays
    * eager. */
-  THMM_RESERVE_GRAPH = 16u
+  THMM_RESERVE_GRAPH = 16u,
+  /* No reference counterpart: thmm_marginals_device (and its profiling and
+   * autotuning forms) and thmm_logz_backward_device write edge outputs
+   * [B][E][K][K] as IEEE binary16 (2 bytes a value, round to nearest even;
+   * magnitudes below 2^-14 are subnormal, absolute resolution 2^-24; finite
+   * up to 65504) instead of the reservation's precision, halving the
+   * largest output stream. Node marginals, log partitions and all internal
+   * state keep the reservation's precision. Needs THMM_RESERVE_MARGINALS |
+   * THMM_RESERVE_DEVICE_RESIDENT; the edge output must be 2-byte aligned
+   * (16-byte aligned for the vectorised kernels); batch-reduced edge
+   * gradients are not available with it. */
+  THMM_RESERVE_HALF_EDGE_MARGINALS = 32u
 };
 
 /* tree_hmm::CategoricalInputUpdate (include/tree_hmm/accelerator.h:92-96). */

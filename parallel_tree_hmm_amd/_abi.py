@@ -36,6 +36,7 @@ THMM_RESERVE_SAMPLING = 2
 THMM_RESERVE_MARGINALS = 4
 THMM_RESERVE_DEVICE_RESIDENT = 8
 THMM_RESERVE_GRAPH = 16
+THMM_RESERVE_HALF_EDGE_MARGINALS = 32
 
 THMM_RAKE = 0
 THMM_BRANCH_COMBINATION = 1

@@ -344,12 +344,15 @@ class Workspace:
         self._reserve(plan, states, batch_capacity, _abi.THMM_RESERVE_SAMPLING, device, precision)
 
     def reserve_marginals(self, plan, states, batch_capacity, device=0, precision="f32", device_resident=False,
-                          graphs=False):
+                          graphs=False, half_edge_marginals=False):
         """ReserveMarginals. device_resident: no pinned host mirrors (device
         entry points only); graphs: device entry points replay a captured
-        hipGraph of their launch sequence."""
+        hipGraph of their launch sequence; half_edge_marginals (with
+        device_resident): the device entry points write edge marginals and
+        edge gradients as IEEE binary16."""
         flags = _abi.THMM_RESERVE_MARGINALS | (_abi.THMM_RESERVE_DEVICE_RESIDENT if device_resident else 0)
         flags |= _abi.THMM_RESERVE_GRAPH if graphs else 0
+        flags |= _abi.THMM_RESERVE_HALF_EDGE_MARGINALS if half_edge_marginals else 0
         self._reserve(plan, states, batch_capacity, flags, device, precision)
 
     def reserve_categorical(self, plan, states, batch_capacity, categories, observation_nodes, device=0, precision="f32"):

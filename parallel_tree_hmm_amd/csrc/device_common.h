@@ -142,6 +142,51 @@ template <typename T> __device__ inline void StoreQuadStream(T *p, const Quad<T>
   }
 }
 
+// Where a slot's adjoint goes: composed-path adjoints in the element type,
+// edge marginals in the element type or, with buf.edge_out_half, as IEEE
+// binary16 (rounded to nearest even; magnitudes below 2^-14 are stored as
+// subnormals, with an absolute resolution of 2^-24).
+typedef _Float16 Half4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct AdjointTarget {
+  T *wide;
+  _Float16 *half;
+};
+
+template <typename T>
+__device__ inline AdjointTarget<T> SlotAdjointTarget(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
+                                                     std::uint32_t b) {
+  if (slot < d.E && buf.edge_out_half) {
+    const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
+    return {nullptr, static_cast<_Float16 *>(buf.edge_out) + (static_cast<std::uint64_t>(b) * d.E + slot) * kk};
+  }
+  return {SlotAdjoint<T>(buf, d, slot, b), nullptr};
+}
+
+// Four consecutive adjoint values at element offset `at` (a multiple of 4).
+template <typename T, bool kStream = true>
+__device__ inline void StoreAdjointQuad(const AdjointTarget<T> &t, std::uint64_t at, const Quad<T> &q) {
+  if (t.half != nullptr) {
+    const Half4 h = {static_cast<_Float16>(q.v[0]), static_cast<_Float16>(q.v[1]), static_cast<_Float16>(q.v[2]),
+                     static_cast<_Float16>(q.v[3])};
+    if (kStream)
+      __builtin_nontemporal_store(h, reinterpret_cast<Half4 *>(t.half + at));
+    else
+      *reinterpret_cast<Half4 *>(t.half + at) = h;
+  } else if (kStream) {
+    StoreQuadStream(t.wide + at, q);
+  } else {
+    StoreQuad(t.wide + at, q);
+  }
+}
+
+template <typename T> __device__ inline void StoreAdjointValue(const AdjointTarget<T> &t, std::uint64_t at, T v) {
+  if (t.half != nullptr)
+    t.half[at] = static_cast<_Float16>(v);
+  else
+    t.wide[at] = v;
+}
+
 }  // namespace dev
 }  // namespace thmm
 

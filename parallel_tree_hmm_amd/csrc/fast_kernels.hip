@@ -210,7 +210,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   const bool log_edge = buf.log_edges != nullptr && op.slot < d.E;
   const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
                         : Slot<T>(buf, d, op.slot, b);
-  T *adj_p = SlotAdjoint<T>(buf, d, op.slot, b);
+  const AdjointTarget<T> adj_p = SlotAdjointTarget<T>(buf, d, op.slot, b);
   const T *adj_m = RakeAdjoint<T>(buf, d, op, b);
   double column[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
       out.v[i] = static_cast<T>(c);
       column[i] += c;
     }
-    StoreQuadStream(adj_p + quad, out);
+    StoreAdjointQuad(adj_p, quad, out);
   }
 #pragma unroll
   for (int o = kLpr; o < kT; o <<= 1)
@@ -588,8 +588,8 @@ __global__ __launch_bounds__(256) void CompressReverseMfma16(DeviceOps ops, Dims
       adj_right.v[r] = static_cast<float>(right);
       row_total += right;
     }
-    StoreQuadStream(SlotAdjoint<float>(buf, d, op.left, b) + (lane % 16) * 16 + col, adj_left);
-    StoreQuadStream(SlotAdjoint<float>(buf, d, op.right, b) + (lane % 16) * 16 + col, adj_right);
+    StoreAdjointQuad(SlotAdjointTarget<float>(buf, d, op.left, b), (lane % 16) * 16 + col, adj_left);
+    StoreAdjointQuad(SlotAdjointTarget<float>(buf, d, op.right, b), (lane % 16) * 16 + col, adj_right);
     row_total += __shfl_xor(row_total, 16, 64);
     row_total += __shfl_xor(row_total, 32, 64);
     if (lane < 16)
@@ -691,12 +691,13 @@ __global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d,
 // without 16-byte alignment (quad_edges == 0), then stored per element.
 __device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
                                        int lane, const Quad<float> &q, bool quad_edges) {
-  float *at = SlotAdjoint<float>(buf, d, slot, b) + (lane % 16) * 16 + 4 * (lane / 16);
+  const AdjointTarget<float> t = SlotAdjointTarget<float>(buf, d, slot, b);
+  const std::uint64_t at = (lane % 16) * 16 + 4 * (lane / 16);
   if (slot >= d.E || quad_edges) {
-    StoreQuadStream(at, q);
+    StoreAdjointQuad(t, at, q);
   } else {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) __builtin_nontemporal_store(q.v[r], at + r);
+    for (int r = 0; r < 4; ++r) StoreAdjointValue(t, at + r, q.v[r]);
   }
 }
 
@@ -1131,7 +1132,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   for (int Im = 0; Im < NB; ++Im) LoadRowBand<K, true>(R, Im, i, g, rr[Im]);
 
   // adj_L[p][m] = L'[p][m] g_p H[p][m], H^T = R Gh^T.
-  float *adj_left = SlotAdjoint<float>(buf, d, op.left, b);
+  const AdjointTarget<float> adj_left = SlotAdjointTarget<float>(buf, d, op.left, b);
 #pragma unroll 1
   for (int Ip = 0; Ip < NB; ++Ip) {
     float gb[NB][4], lb[NB][4];
@@ -1149,7 +1150,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
       for (int r = 0; r < 4; ++r)
         q.v[r] = static_cast<float>(static_cast<double>(lb[Im][r]) * static_cast<double>(e[Im][r]) * g_row *
                                     static_cast<double>(h[r]));
-      StoreQuadStream(adj_left + (16 * Ip + i) * K + 16 * Im + 4 * g, q);
+      StoreAdjointQuad(adj_left, (16 * Ip + i) * K + 16 * Im + 4 * g, q);
     }
   }
 
@@ -1161,7 +1162,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   for (int J = 0; J < NB; ++J)
 #pragma unroll
     for (int r = 0; r < 4; ++r) scale[J][r] = static_cast<float>(gs[16 * J + 4 * g + r] * inverse_gamma);
-  float *adj_right = SlotAdjoint<float>(buf, d, op.right, b);
+  const AdjointTarget<float> adj_right = SlotAdjointTarget<float>(buf, d, op.right, b);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
 #pragma unroll 1
   for (int Im = 0; Im < NB; ++Im) {
@@ -1186,7 +1187,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
         out.v[r] = static_cast<float>(v);
         row_total += v;
       }
-      StoreQuadStream(adj_right + (16 * Im + i) * K + 16 * Ic + 4 * g, out);
+      StoreAdjointQuad(adj_right, (16 * Im + i) * K + 16 * Ic + 4 * g, out);
     }
     row_total += __shfl_xor(row_total, 16, 64);
     row_total += __shfl_xor(row_total, 32, 64);

@@ -234,8 +234,10 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
     for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
     const T a = RakeAdjoint<T>(buf, d, op, b)[lane.s];
     g = (a != T(0) && acc > T(0)) ? static_cast<double>(a) / static_cast<double>(acc) : 0.0;
-    T *out = SlotAdjoint<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
-    for (std::uint32_t c = 0; c < d.K; ++c) out[c] = static_cast<T>(g * static_cast<double>(row[c] * Exp(v[c])));
+    const AdjointTarget<T> out = SlotAdjointTarget<T>(buf, d, op.slot, b);
+    const std::uint64_t at = static_cast<std::uint64_t>(lane.s) * d.K;
+    for (std::uint32_t c = 0; c < d.K; ++c)
+      StoreAdjointValue(out, at + c, static_cast<T>(g * static_cast<double>(row[c] * Exp(v[c]))));
   }
   double column = 0.0;
   const T e_s = lane.s < d.K ? Exp(v[lane.s]) : T(0);
@@ -317,12 +319,12 @@ __global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std
         gr[p * G + c] = (a != T(0) && acc[c] > T(0)) ? static_cast<double>(a) / static_cast<double>(acc[c]) : 0.0;
       }
     }
-    T *adj_left = SlotAdjoint<T>(buf, d, op.left, b) + p * d.K;
+    const AdjointTarget<T> adj_left = SlotAdjointTarget<T>(buf, d, op.left, b);
     for (std::uint32_t m = 0; m < d.K; ++m) {
       const T *row = R + static_cast<std::uint64_t>(m) * d.K;
       double h = 0.0;
       for (std::uint32_t c = 0; c < d.K; ++c) h += gr[p * G + c] * static_cast<double>(row[c]);
-      adj_left[m] = static_cast<T>(static_cast<double>(L[p * d.K + m] * e[m]) * h);
+      StoreAdjointValue(adj_left, p * d.K + m, static_cast<T>(static_cast<double>(L[p * d.K + m] * e[m]) * h));
     }
   }
   __syncthreads();
@@ -337,7 +339,7 @@ __global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std
       for (int c = 0; c < G; ++c)
         if (c < static_cast<int>(d.K)) r[c] += lp * gr[p * G + c];
     }
-    T *adj_right = SlotAdjoint<T>(buf, d, op.right, b) + m * d.K;
+    const AdjointTarget<T> adj_right = SlotAdjointTarget<T>(buf, d, op.right, b);
     const T *row = R + m * d.K;
     const double em = static_cast<double>(e[m]);
     double total = 0.0;
@@ -345,7 +347,7 @@ __global__ void CompressReverse(DeviceOps ops, Dims d, std::uint32_t offset, std
     for (int c = 0; c < G; ++c) {
       if (c < static_cast<int>(d.K)) {
         const double value = static_cast<double>(row[c]) * em * r[c];
-        adj_right[c] = static_cast<T>(value);
+        StoreAdjointValue(adj_right, m * d.K + c, static_cast<T>(value));
         total += value;
       }
     }

@@ -91,6 +91,9 @@ struct LinearBuffers {
   // to log_edge_shift [E][K], which the reverse reads.
   const void *log_edges;
   double *log_edge_shift;
+  // Edge marginals (edge_out) stored as IEEE binary16 instead of the
+  // workspace element type.
+  int edge_out_half;
 };
 
 // Log-domain max-product state (FP64):

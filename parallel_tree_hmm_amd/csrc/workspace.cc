@@ -235,6 +235,7 @@ struct Workspace::Impl {
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
+  bool half_edges() const { return (flags & kReserveHalfEdgeMarginals) != 0; }
   std::size_t N() const { return plan->num_nodes; }
   std::size_t E() const { return plan->num_edges; }
   std::size_t R() const { return plan->num_branches; }
@@ -265,7 +266,7 @@ struct Workspace::Impl {
   LinearBuffers Linear(const void *edge_values, const double *edge_row_scales) const {
     return {nodes.ptr, node_scale.as<double>(), edge_values, edge_row_scales, paths.ptr,
             path_scale.as<double>(), branches.ptr, branch_scale.as<double>(), node_adj.ptr, branch_adj.ptr,
-            path_adj.ptr, edge_out.ptr};
+            path_adj.ptr, edge_out.ptr, nullptr, nullptr, half_edges() ? 1 : 0};
   }
 
   LinearBuffers SamplingLinear() const {
@@ -539,6 +540,9 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   if (states == 0 || batch == 0) throw std::invalid_argument("accelerator state and batch counts must be nonzero");
   if (states > 64) throw std::invalid_argument("state count exceeds the accelerator limit of 64");
   if (real != Real::kF32 && real != Real::kF64) throw std::invalid_argument("unknown precision");
+  if ((flags & kReserveHalfEdgeMarginals) &&
+      !((flags & kReserveDeviceResident) && (flags & THMM_RESERVE_MARGINALS)))
+    throw std::invalid_argument("half-precision edge marginals need a device-resident marginals reservation");
   w.Clear();
   Hip(hipSetDevice(device), "GPU runtime SetDevice");
   w.plan = &plan;
@@ -1606,6 +1610,12 @@ void ApplyTuning(Impl &w, const Tuning &t) {
   if (w.fused.empty()) throw std::logic_error("the fused forward program could not be rebuilt");
 }
 
+// Binary16 edge outputs need 2-byte alignment.
+void CheckEdgeOutput(const Impl &w, const void *edge_out) {
+  if (w.half_edges() && (reinterpret_cast<std::uintptr_t>(edge_out) & 1u) != 0)
+    throw std::invalid_argument("half-precision edge output must be 2-byte aligned");
+}
+
 void CheckDevice(const Impl &w, std::size_t batch, const void *nodes, const void *edges) {
   if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
   if (w.categorical) throw std::invalid_argument("device-resident calls take dense potentials");
@@ -1642,6 +1652,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   RequireMarginals(w);
   if (log_z == nullptr || node_out == nullptr || (w.E() != 0 && edge_out == nullptr))
     throw std::invalid_argument("marginal output is null");
+  CheckEdgeOutput(w, edge_out);
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);
@@ -1707,6 +1718,7 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   RequireMarginals(w);
   if (grad_nodes == nullptr || (w.E() != 0 && grad_edges == nullptr))
     throw std::invalid_argument("gradient output is null");
+  CheckEdgeOutput(w, grad_edges);
   if (reduce_edges && w.E() != 0 && w.edge_out.ptr == nullptr)
     throw std::logic_error("batch-reduced edge gradients need workspace edge storage (no device-resident flag)");
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
@@ -1758,6 +1770,7 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
   RequireMarginals(w);
   if (log_z == nullptr || node_out == nullptr || (w.E() != 0 && edge_out == nullptr))
     throw std::invalid_argument("marginal output is null");
+  CheckEdgeOutput(w, edge_out);
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   w.forward = Impl::Forward::kNone;
   const hipStream_t s = ChooseStream(w, stream);

@@ -23,6 +23,13 @@ namespace thmm {
 constexpr unsigned kReserveDeviceResident = 8u;
 // Extra reserve flag: device-resident entry points replay captured hipGraphs.
 constexpr unsigned kReserveGraph = 16u;
+// Extra reserve flag: the device-resident marginal entry points write edge
+// marginals (and edge gradients) as IEEE binary16. Needs
+// kReserveDeviceResident and THMM_RESERVE_MARGINALS.
+constexpr unsigned kReserveHalfEdgeMarginals = 32u;
+static_assert(kReserveDeviceResident == THMM_RESERVE_DEVICE_RESIDENT && kReserveGraph == THMM_RESERVE_GRAPH &&
+                  kReserveHalfEdgeMarginals == THMM_RESERVE_HALF_EDGE_MARGINALS,
+              "reserve flags of the C interface");
 
 class Workspace {
  public:

@@ -63,6 +63,14 @@ def test_workspace_misuse_is_reported_without_a_gpu():
         ws.reserve_categorical(plan, 4, 2, 0, [1])  # categories in [1, 256]
     with pytest.raises(thmm.InvalidArgument):
         ws.reserve_categorical(plan, 4, 2, 3, [2, 1])  # strictly increasing
+    # Binary16 edge outputs only for device-resident marginal reservations.
+    with pytest.raises(thmm.InvalidArgument):
+        ws.reserve_marginals(plan, 4, 2, half_edge_marginals=True)
+    with pytest.raises(thmm.InvalidArgument):
+        ws.reserve(plan, 4, 2, flags=_abi.THMM_RESERVE_DEVICE_RESIDENT | _abi.THMM_RESERVE_HALF_EDGE_MARGINALS)
+    half_marginals = _abi.THMM_RESERVE_MARGINALS | _abi.THMM_RESERVE_DEVICE_RESIDENT | _abi.THMM_RESERVE_HALF_EDGE_MARGINALS
+    with pytest.raises(thmm.InvalidArgument):  # categorical reservations drop the device-resident flag
+        ws._reserve_categorical(plan, 4, 2, 3, [1], half_marginals, 0, "f32")
     ws.close()
 
 
