@@ -117,7 +117,8 @@ def concatenate(parts):
 # n - 1 middle vectors and writes one path; its reverse reads the slots, the
 # middles and the path adjoint and writes n slot adjoints and n - 1 middle
 # adjoints. Its levels report these as K-vectors moved per item (reverse
-# levels split into groups of at most 4 slots, primitive 7, and the rest).
+# levels split into groups of 2 slots, primitive 8, of 3 or 4, primitive 7,
+# and the rest).
 # ---------------------------------------------------------------------------
 
 
@@ -128,7 +129,7 @@ def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
                 5: 8 * k, 6: k * s}[primitive]
     if phase == 3:  # reverse
         return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s, 6: k * s,
-                7: k * s}[primitive]
+                7: k * s, 8: k * s}[primitive]
     return 0
 
 
@@ -146,6 +147,7 @@ KERNEL_NAMES = {
     (3, 3): "CompressReverse",
     (3, 6): "ChainReverse",
     (3, 7): "ChainReverseSmall",
+    (3, 8): "ChainReversePair",
 }
 
 

@@ -463,8 +463,8 @@ thmm_status thmm_autotune_marginals_device(
  * in phases 1 and 3, 6 for a level of grouped compressions (count = K-element
  * vectors read and written per batch item: (n + 1) K + n - 1 forward and
  * (2n + 1) K + 2 (n - 1) reverse for a group of n slots), and in phase 3, 7
- * for the part of such a level whose groups have at most 4 slots; otherwise
- * count is the level's op count. */
+ * for the part of such a level whose groups have 3 or 4 slots and 8 for the
+ * part whose groups have 2; otherwise count is the level's op count. */
 typedef struct thmm_launch_timing {
   uint32_t phase, primitive, count, batch;
   double ms;

@@ -441,7 +441,7 @@ __device__ inline double StepWeights16(float mid_own, double right_scale_own, in
                                       float *e_own) {
   const double w = static_cast<double>(mid_own) + right_scale_own;
   const double top = RowLanesMax(w);
-  *e_own = top > kNegInf ? expf(static_cast<float>(w - top)) : 0.0f;
+  *e_own = top > kNegInf ? ExpNonPositive(static_cast<float>(w - top)) : 0.0f;
 #pragma unroll
   for (int r = 0; r < 4; ++r) e[r] = __shfl(*e_own, 4 * (lane / 16) + r, 64);
   return top;
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d,
     const floatx4 s = Mfma(cols[j], scaled);
     const float top = RowMax16(s);
     const bool live = top > 0.0f && w_top > kNegInf;
-    const float inverse = live ? 1.0f / top : 0.0f;
+    const float inverse = live ? __builtin_amdgcn_rcpf(top) : 0.0f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = s[r] * inverse;
     const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(c.middles[j - 1]) * d.B + c.b];
@@ -768,7 +768,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     const floatx4 u = Mfma(scaled, right.part.col);  // column layout
     const float top = RowMax16(s);
     const bool live = top > 0.0f && w_top > kNegInf;
-    part.inverse = live ? 1.0f / top : 0.0f;
+    part.inverse = live ? __builtin_amdgcn_rcpf(top) : 0.0f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       part.row[r] = s[r] * part.inverse;
@@ -801,8 +801,8 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
       t = fmaxf(t, Xor16(t));
       t = fmaxf(t, __shfl_xor(t, 32, 64));
       const float u = RowLanesMax(t);
-      const float inverse_top = t > 0.0f ? 1.0f / t : 0.0f;
-      const float inverse_gamma = u > 0.0f ? 1.0f / u : 0.0f;
+      const float inverse_top = t > 0.0f ? __builtin_amdgcn_rcpf(t) : 0.0f;
+      const float inverse_gamma = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 0.0f;
 #pragma unroll
       for (int r = 0; r < 4; ++r) gh[r] = gr[r] * inverse_top, gt[r] = gr[r] * inverse_gamma;
       top = t, gamma = u;
@@ -880,7 +880,7 @@ template <int kN, bool kKeepWeights> __device__ inline void ReverseGroup(const G
   root.Reverse(g, adj);
 }
 
-// kMax: the largest group a launch may hold (4 or 8); it bounds the
+// kMax: the largest group a launch may hold (2, 4 or 8); it bounds the
 // registers and LDS tiles of the kernel.
 template <int kMax>
 __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
@@ -895,6 +895,10 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
   float adj[4];
   const std::uint64_t out_at = static_cast<std::uint64_t>(c.out - d.E) * d.B + c.b;
   LoadRows<true>(static_cast<const float *>(buf.path_adj) + out_at * 256, lane, adj);
+  if constexpr (kMax == 2) {
+    ReverseGroup<2, true>(g, adj);
+    return;
+  }
   switch (c.n) {
     case 2: ReverseGroup<2, (kMax <= 4)>(g, adj); break;
     case 3: ReverseGroup<3, (kMax <= 4)>(g, adj); break;
@@ -913,6 +917,14 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
   }
 }
 
+// Groups of exactly 2 slots.
+__global__ __launch_bounds__(256) void ChainReversePair(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                        std::uint32_t count, LinearBuffers buf, int quad_edges) {
+  __shared__ float tiles[kWaves][1][16 * kPad];
+  ChainReverseBody<2>(ops, d, offset, count, buf, quad_edges, tiles);
+}
+
+// Groups of 3 or 4 slots (the kernel handles 2 as well).
 __global__ __launch_bounds__(256) void ChainReverseSmall(DeviceOps ops, Dims d, std::uint32_t offset,
                                                          std::uint32_t count, LinearBuffers buf, int quad_edges) {
   __shared__ float tiles[kWaves][3][16 * kPad];
@@ -1320,7 +1332,7 @@ void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t off
 bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std::uint32_t offset,
                      std::uint32_t count, const LinearBuffers &buf) {
   const std::uint32_t K = l.dims.K;
-  if (primitive == kChainPrimitive || primitive == kSmallChainPrimitive) {
+  if (primitive == kChainPrimitive || primitive == kSmallChainPrimitive || primitive == kPairChainPrimitive) {
     // Built only for K = 16 FP32 workspaces; there is no generic fallback
     // (the inner compressions' slots are not materialised).
     if (K != 16 || l.real != Real::kF32 || l.ops.chains == nullptr) {
@@ -1331,8 +1343,10 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
     const hipStream_t s = StreamOf(l);
     if (reverse && primitive == kChainPrimitive)
       ChainReverseLarge<<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf, Aligned16(buf.edge_out));
-    else if (reverse)
+    else if (reverse && primitive == kSmallChainPrimitive)
       ChainReverseSmall<<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf, Aligned16(buf.edge_out));
+    else if (reverse)
+      ChainReversePair<<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf, Aligned16(buf.edge_out));
     else
       ChainForwardMfma16<kMaxChain><<<grid, kBlock, 0, s>>>(l.ops, l.dims, offset, count, buf);
     RecordLaunchError(hipGetLastError(), "grouped compression launch");

@@ -40,10 +40,12 @@ struct DeviceChain {
   std::uint32_t pad;
 };
 // Level primitives of grouped compressions (after the plan's 0..3 and the
-// fused forward's 4 and 5): 6 for groups of any size, 7 for a range of
-// groups of at most 4 slots (a reverse launch with fewer registers).
+// fused forward's 4 and 5): 6 for groups of any size, and two reverse
+// launches with fewer registers: 7 for a range of groups of 3 or 4 slots,
+// 8 for a range of groups of 2 slots.
 constexpr std::uint32_t kChainPrimitive = 6;
 constexpr std::uint32_t kSmallChainPrimitive = 7;
+constexpr std::uint32_t kPairChainPrimitive = 8;
 
 struct DeviceOps {
   const DeviceRake *rakes;

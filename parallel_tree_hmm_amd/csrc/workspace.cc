@@ -468,8 +468,10 @@ void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &c
   }
   std::vector<DeviceChain> chains;
   std::vector<Level> levels(plan.batches.size(), Level{kChainPrimitive, 0, 0});
-  // Per compression batch, groups of at most 4 slots first: the reverse
-  // pass launches them separately (kSmallChainPrimitive, fewer registers).
+  // Per compression batch, groups of 2 slots first, then those of 3 or 4:
+  // the reverse pass launches each range separately (kPairChainPrimitive,
+  // kSmallChainPrimitive: fewer registers, more resident wavefronts).
+  std::vector<Level> pairs(plan.batches.size(), Level{kPairChainPrimitive, 0, 0});
   std::vector<Level> small(plan.batches.size(), Level{kSmallChainPrimitive, 0, 0});
   const auto forward_work = [](std::uint32_t n) { return (n + 1) * 16 + (n - 1); };
   const auto reverse_work = [](std::uint32_t n) { return (2 * n + 1) * 16 + 2 * (n - 1); };
@@ -495,11 +497,15 @@ void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &c
     const auto first = chains.begin() + level.offset;
     const auto split =
         std::stable_partition(first, chains.end(), [](const DeviceChain &x) { return x.count <= 4; });
-    small[i].offset = level.offset;
-    small[i].count = static_cast<std::uint32_t>(split - first);
+    const auto two = std::stable_partition(first, split, [](const DeviceChain &x) { return x.count == 2; });
+    pairs[i].offset = level.offset;
+    pairs[i].count = static_cast<std::uint32_t>(two - first);
+    small[i].offset = pairs[i].offset + pairs[i].count;
+    small[i].count = static_cast<std::uint32_t>(split - two);
     for (auto it = first; it != split; ++it) {
-      small[i].forward_work += forward_work(it->count);
-      small[i].reverse_work += reverse_work(it->count);
+      Level &part = it < two ? pairs[i] : small[i];
+      part.forward_work += forward_work(it->count);
+      part.reverse_work += reverse_work(it->count);
     }
     // The kernels index (group, item) in 32 bits.
     if (static_cast<std::uint64_t>(level.count) * batch * 64 >= (1ull << 32)) return;
@@ -512,10 +518,13 @@ void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &c
       continue;
     }
     const Level &all = levels[i];
+    if (pairs[i].count != 0) w.grouped_levels.push_back(pairs[i]);
     if (small[i].count != 0) w.grouped_levels.push_back(small[i]);
-    if (all.count > small[i].count)
-      w.grouped_levels.push_back({kChainPrimitive, all.offset + small[i].count, all.count - small[i].count,
-                                  all.forward_work - small[i].forward_work, all.reverse_work - small[i].reverse_work});
+    const std::uint32_t held = pairs[i].count + small[i].count;
+    if (all.count > held)
+      w.grouped_levels.push_back({kChainPrimitive, all.offset + held, all.count - held,
+                                  all.forward_work - small[i].forward_work - pairs[i].forward_work,
+                                  all.reverse_work - small[i].reverse_work - pairs[i].reverse_work});
   }
   w.chain_levels = std::move(levels);
 }
@@ -849,7 +858,8 @@ void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const Line
     if (buf.branch_adj == nullptr &&
         (levels[i].primitive == THMM_BRANCH_COMBINATION || levels[i].primitive == THMM_BRANCH_ABSORPTION))
       continue;
-    const bool grouped = levels[i].primitive == kChainPrimitive || levels[i].primitive == kSmallChainPrimitive;
+    const bool grouped = levels[i].primitive == kChainPrimitive || levels[i].primitive == kSmallChainPrimitive ||
+                         levels[i].primitive == kPairChainPrimitive;
     if (profiler) profiler->Begin(3, levels[i].primitive, grouped ? levels[i].reverse_work : levels[i].count);
     LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
     if (profiler) profiler->End();

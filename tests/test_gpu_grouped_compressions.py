@@ -12,7 +12,8 @@ stored). Checked here:
   and backward call is refused;
 * the profiled grouped levels move exactly the K-vectors that a separate
   derivation of the groups from the plan arrays predicts (group sizes, their
-  levels, and the reverse launches split at 4 slots).
+  levels, and the reverse launches split into groups of 2, of 3 or 4, and
+  of more slots).
 """
 
 from __future__ import annotations
@@ -119,8 +120,8 @@ def test_adjoint_pair_at_every_depth(require_gpu):
 
 
 def expected_group_work(plan, depth):
-    """Per compression batch with groups: (forward, small reverse, large
-    reverse) K-vectors per item, from the plan arrays alone: slot renaming,
+    """Per compression batch with groups: (forward, pair reverse, small
+    reverse, large reverse) K-vectors per item, from the plan arrays alone: slot renaming,
     heights, then groups formed top-down (an input compression is taken in,
     left first, when its height is not a multiple of `depth` and the group
     stays within 2**depth slots; otherwise it starts its own group)."""
@@ -165,8 +166,9 @@ def expected_group_work(plan, depth):
         if n.size:
             forward = (n + 1) * K + n - 1
             reverse = (2 * n + 1) * K + 2 * (n - 1)
-            small = n <= 4
-            out.append((int(forward.sum()), int(reverse[small].sum()), int(reverse[~small].sum())))
+            pair, small = n == 2, (n == 3) | (n == 4)
+            out.append((int(forward.sum()), int(reverse[pair].sum()), int(reverse[small].sum()),
+                        int(reverse[n > 4].sum())))
     return out
 
 
@@ -189,14 +191,16 @@ def test_profiled_group_levels_match_plan(require_gpu, depth):
                                           em.data_ptr(), 0)
     forward = [count for phase, primitive, count, batch, ms in timings if phase == 1 and primitive == 6]
     reverse = [(primitive, count) for phase, primitive, count, batch, ms in timings
-               if phase == 3 and primitive in (6, 7)]
+               if phase == 3 and primitive in (6, 7, 8)]
     assert not any(primitive == 3 for phase, primitive, count, batch, ms in timings)
     expected = expected_group_work(plan, depth)
-    assert forward == [f for f, small, large in expected]
+    assert forward == [f for f, pair, small, large in expected]
     # Reverse: batches in reverse order, each as its groups of more than 4
-    # slots (primitive 6) then those of at most 4 (primitive 7).
+    # slots (primitive 6), then those of 3 or 4 (primitive 7), then those
+    # of 2 (primitive 8).
     want = []
-    for f, small, large in reversed(expected):
+    for f, pair, small, large in reversed(expected):
         want += [(6, large)] if large else []
         want += [(7, small)] if small else []
+        want += [(8, pair)] if pair else []
     assert reverse == want
