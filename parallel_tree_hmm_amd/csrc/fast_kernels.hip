@@ -970,7 +970,7 @@ __device__ inline double MiddleWeightsWide(const LinearBuffers &buf, const Dims 
   double top = w;
 #pragma unroll
   for (int x = 1; x < K; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
-  *e_own = top > kNegInf ? expf(static_cast<float>(w - top)) : 0.0f;
+  *e_own = top > kNegInf ? ExpNonPositive(static_cast<float>(w - top)) : 0.0f;
 #pragma unroll
   for (int J = 0; J < NB; ++J)
 #pragma unroll
@@ -1096,7 +1096,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
 
   // S (recomputed), Gr = adj/S, g_p = max |Gr| over row p, Gh = Gr/g_p -> LDS;
-  // Gamma = max_p g_p.
+  // Gamma = max_p g_p. A band's ratios are formed in FP32 (hardware
+  // reciprocal) when every one is finite and below 1e30 in magnitude, else
+  // (wave-uniform) again in FP64 from a second product.
   double gamma = 0.0;
   {
     float rc[NB][NB][4];
@@ -1111,25 +1113,53 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
       for (int J = 0; J < NB; ++J)
 #pragma unroll
         for (int r = 0; r < 4; ++r) lb[J][r] *= e[J][r];
-      double gr[NB][4];
-      double top = 0.0;
+      float gr[NB][4];
+      float top_f = 0.0f;
+      bool small = true;
 #pragma unroll
       for (int Ic = 0; Ic < NB; ++Ic) {
         const floatx4 s = TileProduct<K>(rc[Ic], lb);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          gr[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f) ? static_cast<double>(ab[Ic][r]) / static_cast<double>(s[r]) : 0.0;
-          top = fmax(top, fabs(gr[Ic][r]));
+          gr[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f) ? ab[Ic][r] * __builtin_amdgcn_rcpf(s[r]) : 0.0f;
+          small = small && fabsf(gr[Ic][r]) < 1e30f;
+          top_f = fmaxf(top_f, fabsf(gr[Ic][r]));
         }
       }
-      top = fmax(top, __shfl_xor(top, 16, 64));
-      top = fmax(top, __shfl_xor(top, 32, 64));
+      double top;
+      if (__all(small)) {
+        top_f = fmaxf(top_f, __shfl_xor(top_f, 16, 64));
+        top_f = fmaxf(top_f, __shfl_xor(top_f, 32, 64));
+        const float inverse = top_f > 0.0f ? __builtin_amdgcn_rcpf(top_f) : 0.0f;
+#pragma unroll
+        for (int Ic = 0; Ic < NB; ++Ic)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] = gr[Ic][r] * inverse;
+        top = top_f;
+      } else {
+        double wide[NB][4];
+        top = 0.0;
+#pragma unroll
+        for (int Ic = 0; Ic < NB; ++Ic) {
+          const floatx4 s = TileProduct<K>(rc[Ic], lb);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            wide[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f)
+                              ? static_cast<double>(ab[Ic][r]) / static_cast<double>(s[r])
+                              : 0.0;
+            top = fmax(top, fabs(wide[Ic][r]));
+          }
+        }
+        top = fmax(top, __shfl_xor(top, 16, 64));
+        top = fmax(top, __shfl_xor(top, 32, 64));
+#pragma unroll
+        for (int Ic = 0; Ic < NB; ++Ic)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] =
+                top > 0.0 ? static_cast<float>(wide[Ic][r] / top) : 0.0f;
+      }
       gamma = fmax(gamma, top);
-#pragma unroll
-      for (int Ic = 0; Ic < NB; ++Ic)
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] = top > 0.0 ? static_cast<float>(gr[Ic][r] / top) : 0.0f;
       if (g == 0) gs[16 * Ip + i] = top;
     }
   }
@@ -1154,14 +1184,23 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
       for (int r = 0; r < 4; ++r) gb[J][r] = G[(16 * Ip + i) * kStride + 16 * J + 4 * g + r];
     LoadRowBand<K>(L, Ip, i, g, lb);
     const double g_row = gs[16 * Ip + i];
+    // FP32 products when every row's g_p is below 1e30 (|H| <= K, L and e
+    // at most 1), else FP64.
+    const bool narrow = __all(g_row < 1e30);
+    const float g_row_f = static_cast<float>(g_row);
 #pragma unroll
     for (int Im = 0; Im < NB; ++Im) {
       const floatx4 h = TileProduct<K>(rr[Im], gb);  // H[16Ip + i][16Im + 4g + r]
       Quad<float> q;
+      if (narrow) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        q.v[r] = static_cast<float>(static_cast<double>(lb[Im][r]) * static_cast<double>(e[Im][r]) * g_row *
-                                    static_cast<double>(h[r]));
+        for (int r = 0; r < 4; ++r) q.v[r] = (lb[Im][r] * e[Im][r]) * (g_row_f * h[r]);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          q.v[r] = static_cast<float>(static_cast<double>(lb[Im][r]) * static_cast<double>(e[Im][r]) * g_row *
+                                      static_cast<double>(h[r]));
+      }
       StoreAdjointQuad(adj_left, (16 * Ip + i) * K + 16 * Im + 4 * g, q);
     }
   }
@@ -1176,7 +1215,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
     for (int r = 0; r < 4; ++r) scale[J][r] = static_cast<float>(gs[16 * J + 4 * g + r] * inverse_gamma);
   const AdjointTarget<float> adj_right = SlotAdjointTarget<float>(buf, d, op.right, b);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-#pragma unroll 1
+  // Unrolled: rr is indexed by Im and must stay in registers.
+#pragma unroll
   for (int Im = 0; Im < NB; ++Im) {
     float lc[NB][4];
     LoadColumnBand<K, true>(L, Im, i, g, lc);
@@ -1192,12 +1232,23 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
         for (int r = 0; r < 4; ++r) gt[J][r] = G[(16 * J + 4 * g + r) * kStride + 16 * Ic + i] * scale[J][r];
       const floatx4 q = TileProduct<K>(gt, lc);  // Q[16Im + i][16Ic + 4g + r]
       Quad<float> out;
+      if (gamma < 1e30) {  // wave-uniform; |Q| <= K, R and e at most 1
+        const float weight = e_row * static_cast<float>(gamma);
+        float quad_total = 0.0f;
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const double v = static_cast<double>(rr[Im][Ic][r]) * static_cast<double>(e_row) * gamma *
-                         static_cast<double>(q[r]);
-        out.v[r] = static_cast<float>(v);
-        row_total += v;
+        for (int r = 0; r < 4; ++r) {
+          out.v[r] = rr[Im][Ic][r] * weight * q[r];
+          quad_total += out.v[r];
+        }
+        row_total += static_cast<double>(quad_total);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double v = static_cast<double>(rr[Im][Ic][r]) * static_cast<double>(e_row) * gamma *
+                           static_cast<double>(q[r]);
+          out.v[r] = static_cast<float>(v);
+          row_total += v;
+        }
       }
       StoreAdjointQuad(adj_right, (16 * Im + i) * K + 16 * Ic + 4 * g, out);
     }
