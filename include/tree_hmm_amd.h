@@ -374,6 +374,14 @@ thmm_status thmm_categorical_set_branch_lengths(thmm_workspace *workspace,
 
 #define THMM_HIP_NULL_STREAM ((void *)(~(uintptr_t)0))
 
+/* Linear potentials (these two entry points) are taken as given: node
+ * vectors are moved to the log domain with an FP64 scale, but edge matrices
+ * enter the products unscaled, so with THMM_F32 a product of two edge
+ * entries (and K times it) must stay below the FP32 maximum (entries up to
+ * about 1e17 at K = 64). Wider ranges belong in the log-potential pair
+ * below, which normalises every edge row by its maximum first, or in
+ * THMM_F64. */
+
 /* Scaled log partitions: d_log_z is double [batch * roots]. */
 thmm_status thmm_log_partition_device(thmm_workspace *workspace,
                                       const void *d_node_potentials,

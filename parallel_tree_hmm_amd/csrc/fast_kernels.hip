@@ -385,6 +385,21 @@ __device__ inline floatx4 Mfma(const float (&a)[4], const float (&b)[4]) {
   return acc;
 }
 
+// The multiplier that normalises a row of values x <= top by its maximum
+// top (0 for a dead row). A reciprocal of a top below 2^-100 can overflow
+// or be flushed, so when any live row of the wavefront has one, the
+// wavefront first scales its values (scale_values(2^100)) and tops by 2^100,
+// which is exact and leaves every normalised value unchanged. Products here
+// are at most K, so the scaled values stay finite.
+template <typename F>
+__device__ inline float RowInverse(float top, bool live, F &&scale_values) {
+  if (__any(live && top < 0x1p-100f)) {
+    scale_values(0x1p100f);
+    top *= 0x1p100f;
+  }
+  return live ? __builtin_amdgcn_rcpf(top) : 0.0f;
+}
+
 // kStream: the kernel's last read of these values (non-temporal loads).
 template <bool kStream = false> __device__ inline void LoadColumns(const float *X, int lane, float (&out)[4]) {
 #pragma unroll
@@ -487,11 +502,12 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
     top = fmaxf(top, __shfl_xor(top, 32, 64));
     if (first + t >= total) continue;
     const bool live = top > 0.0f && w_top > kNegInf;
-    const float inverse = live ? 1.0f / top : 0.0f;
+    floatx4 x = s;
+    const float inverse = RowInverse(top, live, [&](float f) { x *= f; });
     const std::uint64_t out_at = static_cast<std::uint64_t>(o[t].op.out - d.E) * d.B + o[t].b;
     Quad<float> q;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) q.v[r] = s[r] * inverse;
+    for (int r = 0; r < 4; ++r) q.v[r] = x[r] * inverse;
     StoreQuadStream(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
     // top lies in (0, 16]: its FP32 logarithm is exact to an ulp of a number
     // of order one; the sum of scales stays FP64.
@@ -670,10 +686,10 @@ __global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d,
     float scaled[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) scaled[r] = acc[r] * e[r];
-    const floatx4 s = Mfma(cols[j], scaled);
+    floatx4 s = Mfma(cols[j], scaled);
     const float top = RowMax16(s);
     const bool live = top > 0.0f && w_top > kNegInf;
-    const float inverse = live ? __builtin_amdgcn_rcpf(top) : 0.0f;
+    const float inverse = RowInverse(top, live, [&](float f) { s *= f; });
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = s[r] * inverse;
     const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(c.middles[j - 1]) * d.B + c.b];
@@ -713,7 +729,7 @@ __device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, 
 struct Part {
   float row[4], col[4];  // normalised product: row layout, column layout
   double scale;          // FP64 log scale of row l%16 (Build only)
-  float inverse;         // normaliser of row l%16 (1 for a stored slot)
+  double top;            // row l%16 of the product is top * row (1 for a stored slot)
   float e_own;           // split's middle weight e[m], m = l%16
   float e[4];            // split's middle weights e[m], m = 4(l/16) + r (kept when kKeepWeights)
 };
@@ -738,7 +754,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     LoadRows<true>(x, g.lane, part.row);
     LoadColumns<false>(x, g.lane, part.col);
     part.scale = RowScale(g.buf, g.d, g.c.slots[kLo], g.c.b, g.lane % 16);
-    part.inverse = 1.0f;
+    part.top = 1.0;
   }
   __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
     Quad<float> q;
@@ -764,24 +780,30 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     float scaled[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) scaled[r] = left.part.row[r] * part.e[r];
-    const floatx4 s = Mfma(right.part.col, scaled);  // row layout
-    const floatx4 u = Mfma(scaled, right.part.col);  // column layout
+    floatx4 s = Mfma(right.part.col, scaled);  // row layout
+    floatx4 u = Mfma(scaled, right.part.col);  // column layout
     const float top = RowMax16(s);
     const bool live = top > 0.0f && w_top > kNegInf;
-    part.inverse = live ? __builtin_amdgcn_rcpf(top) : 0.0f;
+    const float inverse = RowInverse(top, live, [&](float f) {
+      s *= f;
+      u *= f;
+    });
+    part.top = static_cast<double>(top);  // unscaled
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      part.row[r] = s[r] * part.inverse;
-      part.col[r] = u[r] * __shfl(part.inverse, 4 * (lane / 16) + r, 64);
+      part.row[r] = s[r] * inverse;
+      part.col[r] = u[r] * __shfl(inverse, 4 * (lane / 16) + r, 64);
     }
     part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + static_cast<double>(logf(top)) : kNegInf;
   }
 
   // adj: marginal-form adjoint of this span's product, row layout. The
-  // ratios Gr = adj / S (S = P / normaliser, row by row) are formed in FP32
-  // when every one is finite and below 1e30 in magnitude, else (wave-uniform)
-  // in FP64 as in the binary compression adjoint; the row and global scales
-  // g_p and Gamma are applied in FP64 when they exceed 1e30.
+  // ratios Gr = adj / S (S = top * P, row by row) are formed in FP32 when
+  // every one is finite and below 1e30 in magnitude and every row maximum
+  // g_p is 0 or at least 2^-100 (so that its reciprocal is finite), else
+  // (wave-uniform) in FP64 as in the binary compression adjoint; the row
+  // and global scales g_p and Gamma are applied in FP64 when they exceed
+  // 1e30.
   __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
     const int lane = g.lane;
     float e[4];
@@ -789,17 +811,23 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     for (int r = 0; r < 4; ++r) e[r] = kKeepWeights ? part.e[r] : __shfl(part.e_own, 4 * (lane / 16) + r, 64);
     float gr[4];
     bool small = true;
+    const float inverse = __builtin_amdgcn_rcpf(static_cast<float>(part.top));
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      gr[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? adj[r] * part.inverse * __builtin_amdgcn_rcpf(part.row[r]) : 0.0f;
+      gr[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? adj[r] * inverse * __builtin_amdgcn_rcpf(part.row[r]) : 0.0f;
       small = small && fabsf(gr[r]) < 1e30f;
     }
     float gh[4], gt[4];
     double top, gamma;
-    if (__all(small)) {
-      float t = fmaxf(fmaxf(fabsf(gr[0]), fabsf(gr[1])), fmaxf(fabsf(gr[2]), fabsf(gr[3])));
+    bool fast = __all(small);
+    float t = 0.0f;
+    if (fast) {
+      t = fmaxf(fmaxf(fabsf(gr[0]), fabsf(gr[1])), fmaxf(fabsf(gr[2]), fabsf(gr[3])));
       t = fmaxf(t, Xor16(t));
       t = fmaxf(t, __shfl_xor(t, 32, 64));
+      fast = __all(t == 0.0f || t >= 0x1p-100f);  // then Gamma is 0 or at least 2^-100 as well
+    }
+    if (fast) {
       const float u = RowLanesMax(t);
       const float inverse_top = t > 0.0f ? __builtin_amdgcn_rcpf(t) : 0.0f;
       const float inverse_gamma = u > 0.0f ? __builtin_amdgcn_rcpf(u) : 0.0f;
@@ -811,10 +839,9 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
       top = 0.0;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        wide[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? static_cast<double>(adj[r]) *
-                                                               static_cast<double>(part.inverse) /
-                                                               static_cast<double>(part.row[r])
-                                                         : 0.0;
+        wide[r] = (adj[r] != 0.0f && part.row[r] > 0.0f)
+                      ? static_cast<double>(adj[r]) / (part.top * static_cast<double>(part.row[r]))
+                      : 0.0;
         top = fmax(top, fabs(wide[r]));
       }
       top = fmax(top, __shfl_xor(top, 16, 64));
@@ -1060,7 +1087,10 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
     top = fmaxf(top, __shfl_xor(top, 16, 64));
     top = fmaxf(top, __shfl_xor(top, 32, 64));
     const bool live = top > 0.0f && w_top > kNegInf;
-    const float inverse = live ? 1.0f / top : 0.0f;
+    const float inverse = RowInverse(top, live, [&](float f) {
+#pragma unroll
+      for (int Ic = 0; Ic < NB; ++Ic) s[Ic] *= f;
+    });
 #pragma unroll
     for (int Ic = 0; Ic < NB; ++Ic) {
       Quad<float> q;
@@ -1097,8 +1127,9 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 
   // S (recomputed), Gr = adj/S, g_p = max |Gr| over row p, Gh = Gr/g_p -> LDS;
   // Gamma = max_p g_p. A band's ratios are formed in FP32 (hardware
-  // reciprocal) when every one is finite and below 1e30 in magnitude, else
-  // (wave-uniform) again in FP64 from a second product.
+  // reciprocal) when every one is finite and below 1e30 in magnitude and
+  // every row maximum g_p is 0 or at least 2^-100 (a finite reciprocal),
+  // else (wave-uniform) again in FP64 from a second product.
   double gamma = 0.0;
   {
     float rc[NB][NB][4];
@@ -1127,9 +1158,13 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
         }
       }
       double top;
-      if (__all(small)) {
+      bool fast = __all(small);
+      if (fast) {
         top_f = fmaxf(top_f, __shfl_xor(top_f, 16, 64));
         top_f = fmaxf(top_f, __shfl_xor(top_f, 32, 64));
+        fast = __all(top_f == 0.0f || top_f >= 0x1p-100f);
+      }
+      if (fast) {
         const float inverse = top_f > 0.0f ? __builtin_amdgcn_rcpf(top_f) : 0.0f;
 #pragma unroll
         for (int Ic = 0; Ic < NB; ++Ic)
