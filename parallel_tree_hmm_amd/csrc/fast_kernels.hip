@@ -213,7 +213,15 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   const AdjointTarget<T> adj_p = SlotAdjointTarget<T>(buf, d, op.slot, b);
   const T *adj_m = RakeAdjoint<T>(buf, d, op, b);
   double column[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+  // FP32: a step's ratios a / part are taken with the hardware reciprocal
+  // and its column sums kept in FP32 when, on every lane, part is 0 or at
+  // least 2^-100 and |a| below 2^26 (so a / part stays below 2^126), else
+  // (wave-uniform) in FP64.
+  T narrow[4] = {T(0), T(0), T(0), T(0)};
+  constexpr bool kFloat = std::is_same<T, float>::value;
+  // Four steps unrolled at a time: at K = 64 a full unroll holds all 16
+  // steps' loads in registers (three wavefronts per SIMD).
+#pragma unroll 4
   for (int j = 0; j < kJ; ++j) {
     const std::uint32_t quad = static_cast<std::uint32_t>(j * kT + l) * 4;
     Quad<T> p = LoadQuadStream(P + quad);
@@ -230,16 +238,28 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
 #pragma unroll
     for (int o = kLpr / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kT);
     const T a = adj_m[j * kRps + l / kLpr];
-    const double g = (a != T(0) && part > T(0)) ? static_cast<double>(a) / static_cast<double>(part) : 0.0;
     Quad<T> out;
+    if (kFloat && __all(!(part > T(0) && part < T(0x1p-100f)) && fabs(a) < T(0x1p26f))) {
+      const T g = (a != T(0) && part > T(0)) ? a * static_cast<T>(__builtin_amdgcn_rcpf(static_cast<float>(part)))
+                                             : T(0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const double c = g * static_cast<double>(t[i]);
-      out.v[i] = static_cast<T>(c);
-      column[i] += c;
+      for (int i = 0; i < 4; ++i) {
+        out.v[i] = g * t[i];
+        narrow[i] += out.v[i];
+      }
+    } else {
+      const double g = (a != T(0) && part > T(0)) ? static_cast<double>(a) / static_cast<double>(part) : 0.0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const double c = g * static_cast<double>(t[i]);
+        out.v[i] = static_cast<T>(c);
+        column[i] += c;
+      }
     }
     StoreAdjointQuad(adj_p, quad, out);
   }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) column[i] += static_cast<double>(narrow[i]);
 #pragma unroll
   for (int o = kLpr; o < kT; o <<= 1)
 #pragma unroll
