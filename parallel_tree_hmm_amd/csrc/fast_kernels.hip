@@ -529,11 +529,11 @@ __global__ __launch_bounds__(256) void CompressForwardMfma16(DeviceOps ops, Dims
 #pragma unroll
     for (int r = 0; r < 4; ++r) q.v[r] = x[r] * inverse;
     StoreQuadStream(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
-    // top lies in (0, 16]: its FP32 logarithm is exact to an ulp of a number
-    // of order one; the sum of scales stays FP64.
+    // log top to about 6e-8 absolute whatever its magnitude; the sum of
+    // scales stays FP64.
     if (lane < 16)
       buf.path_scale[out_at * 16 + lane] =
-          live ? left_scale[t] + mid_scale[t] + w_top + static_cast<double>(logf(top)) : kNegInf;
+          live ? left_scale[t] + mid_scale[t] + w_top + LogPositive(top) : kNegInf;
   }
 }
 
@@ -713,7 +713,7 @@ __global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d,
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[r] = s[r] * inverse;
     const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(c.middles[j - 1]) * d.B + c.b];
-    scale = live ? scale + mid_scale + w_top + static_cast<double>(logf(top)) : kNegInf;
+    scale = live ? scale + mid_scale + w_top + LogPositive(top) : kNegInf;
   }
   const std::uint64_t out_at = static_cast<std::uint64_t>(c.out - d.E) * d.B + c.b;
   Quad<float> q;
@@ -814,7 +814,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
       part.row[r] = s[r] * inverse;
       part.col[r] = u[r] * __shfl(inverse, 4 * (lane / 16) + r, 64);
     }
-    part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + static_cast<double>(logf(top)) : kNegInf;
+    part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + LogPositive(top) : kNegInf;
   }
 
   // adj: marginal-form adjoint of this span's product, row layout. The
@@ -1120,7 +1120,7 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
     }
     if (g == 0)
       buf.path_scale[out_at * K + 16 * Ip + i] =
-          live ? left_scale + mid_scale + w_top + static_cast<double>(logf(top)) : kNegInf;
+          live ? left_scale + mid_scale + w_top + LogPositive(top) : kNegInf;
   }
 }
 
@@ -1143,80 +1143,77 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
   const float *adj_out = static_cast<const float *>(buf.path_adj) + out_at * K * K;
   float e[NB][4], e_own;
-  MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
+  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
+  const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(op.middle) * d.B + b];
 
-  // S (recomputed), Gr = adj/S, g_p = max |Gr| over row p, Gh = Gr/g_p -> LDS;
-  // Gamma = max_p g_p. A band's ratios are formed in FP32 (hardware
-  // reciprocal) when every one is finite and below 1e30 in magnitude and
-  // every row maximum g_p is 0 or at least 2^-100 (a finite reciprocal),
-  // else (wave-uniform) again in FP64 from a second product.
+  // S is not recomputed: the forward pass stored P = S / top_p row by row
+  // as the compression's output slot and log top_p inside its row scale
+  // (left scale + middle scale + W + log top_p, summed in that order).
+  // Gr' = adj/P = Gr top_p and g_p' = max_c |Gr'| give Gh = Gr/g_p =
+  // Gr'/g_p' (-> LDS) and g_p = g_p' / top_p (FP64); Gamma = max_p g_p. A
+  // band's ratios are formed in FP32 (hardware reciprocal) when every one
+  // is finite and below 1e30 in magnitude and every g_p' is 0 or at least
+  // 2^-100 (a finite reciprocal), else (wave-uniform) in FP64.
+  const float *P = static_cast<const float *>(buf.paths) + out_at * K * K;
   double gamma = 0.0;
-  {
-    float rc[NB][NB][4];
-#pragma unroll
-    for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K>(R, Ic, i, g, rc[Ic]);
 #pragma unroll 1
-    for (int Ip = 0; Ip < NB; ++Ip) {
-      float lb[NB][4], ab[NB][4];
-      LoadRowBand<K>(L, Ip, i, g, lb);
-      LoadRowBand<K, true>(adj_out, Ip, i, g, ab);
+  for (int Ip = 0; Ip < NB; ++Ip) {
+    float pb[NB][4], ab[NB][4];
+    LoadRowBand<K, true>(P, Ip, i, g, pb);
+    LoadRowBand<K, true>(adj_out, Ip, i, g, ab);
+    const int row = 16 * Ip + i;
+    const double path_scale = buf.path_scale[out_at * K + row];
+    const double left_scale = RowScale(buf, d, op.left, b, row);
+    float gr[NB][4];
+    float top_f = 0.0f;
+    bool small = true;
 #pragma unroll
-      for (int J = 0; J < NB; ++J)
+    for (int Ic = 0; Ic < NB; ++Ic)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) lb[J][r] *= e[J][r];
-      float gr[NB][4];
-      float top_f = 0.0f;
-      bool small = true;
+      for (int r = 0; r < 4; ++r) {
+        gr[Ic][r] = (ab[Ic][r] != 0.0f && pb[Ic][r] > 0.0f) ? ab[Ic][r] * __builtin_amdgcn_rcpf(pb[Ic][r]) : 0.0f;
+        small = small && fabsf(gr[Ic][r]) < 1e30f;
+        top_f = fmaxf(top_f, fabsf(gr[Ic][r]));
+      }
+    double top;  // g_p'
+    bool fast = __all(small);
+    if (fast) {
+      top_f = fmaxf(top_f, __shfl_xor(top_f, 16, 64));
+      top_f = fmaxf(top_f, __shfl_xor(top_f, 32, 64));
+      fast = __all(top_f == 0.0f || top_f >= 0x1p-100f);
+    }
+    if (fast) {
+      const float inverse = top_f > 0.0f ? __builtin_amdgcn_rcpf(top_f) : 0.0f;
 #pragma unroll
-      for (int Ic = 0; Ic < NB; ++Ic) {
-        const floatx4 s = TileProduct<K>(rc[Ic], lb);
+      for (int Ic = 0; Ic < NB; ++Ic)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) G[row * kStride + 16 * Ic + 4 * g + r] = gr[Ic][r] * inverse;
+      top = top_f;
+    } else {
+      double wide[NB][4];
+      top = 0.0;
+#pragma unroll
+      for (int Ic = 0; Ic < NB; ++Ic)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          gr[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f) ? ab[Ic][r] * __builtin_amdgcn_rcpf(s[r]) : 0.0f;
-          small = small && fabsf(gr[Ic][r]) < 1e30f;
-          top_f = fmaxf(top_f, fabsf(gr[Ic][r]));
+          wide[Ic][r] = (ab[Ic][r] != 0.0f && pb[Ic][r] > 0.0f)
+                            ? static_cast<double>(ab[Ic][r]) / static_cast<double>(pb[Ic][r])
+                            : 0.0;
+          top = fmax(top, fabs(wide[Ic][r]));
         }
-      }
-      double top;
-      bool fast = __all(small);
-      if (fast) {
-        top_f = fmaxf(top_f, __shfl_xor(top_f, 16, 64));
-        top_f = fmaxf(top_f, __shfl_xor(top_f, 32, 64));
-        fast = __all(top_f == 0.0f || top_f >= 0x1p-100f);
-      }
-      if (fast) {
-        const float inverse = top_f > 0.0f ? __builtin_amdgcn_rcpf(top_f) : 0.0f;
+      top = fmax(top, __shfl_xor(top, 16, 64));
+      top = fmax(top, __shfl_xor(top, 32, 64));
+      const double inverse = top > 0.0 ? 1.0 / top : 0.0;
 #pragma unroll
-        for (int Ic = 0; Ic < NB; ++Ic)
+      for (int Ic = 0; Ic < NB; ++Ic)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] = gr[Ic][r] * inverse;
-        top = top_f;
-      } else {
-        double wide[NB][4];
-        top = 0.0;
-#pragma unroll
-        for (int Ic = 0; Ic < NB; ++Ic) {
-          const floatx4 s = TileProduct<K>(rc[Ic], lb);
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            wide[Ic][r] = (ab[Ic][r] != 0.0f && s[r] > 0.0f)
-                              ? static_cast<double>(ab[Ic][r]) / static_cast<double>(s[r])
-                              : 0.0;
-            top = fmax(top, fabs(wide[Ic][r]));
-          }
-        }
-        top = fmax(top, __shfl_xor(top, 16, 64));
-        top = fmax(top, __shfl_xor(top, 32, 64));
-#pragma unroll
-        for (int Ic = 0; Ic < NB; ++Ic)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-            G[(16 * Ip + i) * kStride + 16 * Ic + 4 * g + r] =
-                top > 0.0 ? static_cast<float>(wide[Ic][r] / top) : 0.0f;
-      }
-      gamma = fmax(gamma, top);
-      if (g == 0) gs[16 * Ip + i] = top;
+        for (int r = 0; r < 4; ++r) G[row * kStride + 16 * Ic + 4 * g + r] = static_cast<float>(wide[Ic][r] * inverse);
     }
+    // A row with a nonzero ratio is live in the forward pass: its scales
+    // are finite.
+    const double g_row = top > 0.0 ? top / exp(path_scale - ((left_scale + mid_scale) + w_top)) : 0.0;
+    gamma = fmax(gamma, g_row);
+    if (g == 0) gs[row] = g_row;
   }
 #pragma unroll
   for (int x = 1; x < 16; x <<= 1) gamma = fmax(gamma, __shfl_xor(gamma, x, 64));
