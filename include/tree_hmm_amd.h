@@ -415,6 +415,34 @@ thmm_status thmm_logz_backward_device(thmm_workspace *workspace,
                                       int reduce_edges_over_batch,
                                       void *stream);
 
+/* thmm_maximum on device-resident dense potentials (a THMM_RESERVE_MAXIMUM
+ * reservation, device-resident or not): the same kernels and outputs as
+ * RunMaximumPrepared (src/gpu_backend_impl.inc:2405-2604), written to
+ * d_log_weights (double
+ * [batch * roots], -inf where no assignment has positive weight, which the
+ * host call reports as an error) and d_states (uint32 [batch, node]). */
+thmm_status thmm_maximum_device(thmm_workspace *workspace,
+                                const void *d_node_potentials,
+                                const void *d_edge_potentials, size_t batch,
+                                double *d_log_weights, uint32_t *d_states,
+                                void *stream);
+
+/* thmm_sample on device-resident dense potentials and variates (a
+ * THMM_RESERVE_SAMPLING reservation): the same kernels and draws as the
+ * sampling mode of RunPrepared (src/gpu_backend_impl.inc:2156-2404, draws
+ * at 2328-2384) from d_uniforms [batch, node] in the
+ * workspace precision. The variates are not checked on the host; any value
+ * gives in-range states (one at or above 1 takes the last supported state).
+ * d_log_z (double [batch * roots]) is -inf for a nonpositive partition
+ * function, and *d_failure (uint32, zeroed by the call) is set to 1 when a
+ * conditional has no support; the host call reports both as errors. */
+thmm_status thmm_sample_device(thmm_workspace *workspace,
+                               const void *d_node_potentials,
+                               const void *d_edge_potentials,
+                               const void *d_uniforms, size_t batch,
+                               double *d_log_z, uint32_t *d_states,
+                               uint32_t *d_failure, void *stream);
+
 /* Blocks until the workspace stream has drained. */
 thmm_status thmm_synchronize(thmm_workspace *workspace);
 

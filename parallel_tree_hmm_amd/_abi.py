@@ -226,6 +226,11 @@ PROTOTYPES = {
     ),
     "thmm_logz_forward_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
     "thmm_logz_backward_device": (c_status, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "thmm_maximum_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p]),
+    "thmm_sample_device": (
+        c_status,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
     "thmm_synchronize": (c_status, [c_void_p]),
     "thmm_profile_marginals_device": (
         c_status,

@@ -455,6 +455,26 @@ class Workspace:
             )
         )
 
+    def maximum_device(self, nodes_ptr, edges_ptr, batch, log_weights_ptr, states_ptr, stream=None):
+        """thmm_maximum_device: MAP log weights (double) and states (uint32
+        [batch, node]) into device buffers."""
+        _check(
+            _lib().thmm_maximum_device(
+                self._handle, nodes_ptr, edges_ptr, batch, log_weights_ptr, states_ptr, _stream(stream)
+            )
+        )
+
+    def sample_device(self, nodes_ptr, edges_ptr, uniforms_ptr, batch, log_z_ptr, states_ptr, failure_ptr,
+                      stream=None):
+        """thmm_sample_device: posterior states (uint32 [batch, node]) from
+        device variates; log Z (double) and a uint32 failure flag."""
+        _check(
+            _lib().thmm_sample_device(
+                self._handle, nodes_ptr, edges_ptr, uniforms_ptr, batch, log_z_ptr, states_ptr, failure_ptr,
+                _stream(stream)
+            )
+        )
+
     def profile_marginals_device(self, nodes_ptr, edges_ptr, batch, log_z_ptr, node_out_ptr, edge_out_ptr, stream=None):
         """thmm_marginals_device with per-launch HIP event timing; returns a
         list of (phase, primitive, count, batch, ms) tuples."""

@@ -346,6 +346,23 @@ thmm_status thmm_logz_backward_device(thmm_workspace *workspace, const double *d
   });
 }
 
+thmm_status thmm_maximum_device(thmm_workspace *workspace, const void *d_node_potentials,
+                                const void *d_edge_potentials, size_t batch, double *d_log_weights,
+                                uint32_t *d_states, void *stream) {
+  return Guard([&] {
+    W(workspace).MaximumDevice(d_node_potentials, d_edge_potentials, batch, d_log_weights, d_states, stream);
+  });
+}
+
+thmm_status thmm_sample_device(thmm_workspace *workspace, const void *d_node_potentials, const void *d_edge_potentials,
+                               const void *d_uniforms, size_t batch, double *d_log_z, uint32_t *d_states,
+                               uint32_t *d_failure, void *stream) {
+  return Guard([&] {
+    W(workspace).SampleDevice(d_node_potentials, d_edge_potentials, d_uniforms, batch, d_log_z, d_states, d_failure,
+                              stream);
+  });
+}
+
 thmm_status thmm_profile_marginals_device(thmm_workspace *workspace, const void *d_node_potentials,
                                           const void *d_edge_potentials, size_t batch, double *d_log_z,
                                           void *d_node_marginals, void *d_edge_marginals, void *stream,

@@ -749,7 +749,7 @@ __device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, 
 struct Part {
   float row[4], col[4];  // normalised product: row layout, column layout
   double scale;          // FP64 log scale of row l%16 (Build only)
-  double top;            // row l%16 of the product is top * row (1 for a stored slot)
+  float top;             // row l%16 of the product is top * row (1 for a stored slot)
   float e_own;           // split's middle weight e[m], m = l%16
   float e[4];            // split's middle weights e[m], m = 4(l/16) + r (kept when kKeepWeights)
 };
@@ -774,7 +774,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     LoadRows<true>(x, g.lane, part.row);
     LoadColumns<false>(x, g.lane, part.col);
     part.scale = RowScale(g.buf, g.d, g.c.slots[kLo], g.c.b, g.lane % 16);
-    part.top = 1.0;
+    part.top = 1.0f;
   }
   __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
     Quad<float> q;
@@ -808,7 +808,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
       s *= f;
       u *= f;
     });
-    part.top = static_cast<double>(top);  // unscaled
+    part.top = top;  // unscaled
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       part.row[r] = s[r] * inverse;
@@ -831,7 +831,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     for (int r = 0; r < 4; ++r) e[r] = kKeepWeights ? part.e[r] : __shfl(part.e_own, 4 * (lane / 16) + r, 64);
     float gr[4];
     bool small = true;
-    const float inverse = __builtin_amdgcn_rcpf(static_cast<float>(part.top));
+    const float inverse = __builtin_amdgcn_rcpf(part.top);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       gr[r] = (adj[r] != 0.0f && part.row[r] > 0.0f) ? adj[r] * inverse * __builtin_amdgcn_rcpf(part.row[r]) : 0.0f;
@@ -860,7 +860,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         wide[r] = (adj[r] != 0.0f && part.row[r] > 0.0f)
-                      ? static_cast<double>(adj[r]) / (part.top * static_cast<double>(part.row[r]))
+                      ? static_cast<double>(adj[r]) / (static_cast<double>(part.top) * static_cast<double>(part.row[r]))
                       : 0.0;
         top = fmax(top, fabs(wide[r]));
       }

@@ -269,8 +269,9 @@ struct Workspace::Impl {
             path_adj.ptr, edge_out.ptr, nullptr, nullptr, half_edges() ? 1 : 0};
   }
 
-  LinearBuffers SamplingLinear() const {
-    if (real == Real::kF64) return Linear(edges.ptr, nullptr);
+  // `fp64_edges`: the edge values of an FP64 workspace's sampling call.
+  LinearBuffers SamplingLinear(const void *fp64_edges = nullptr) const {
+    if (real == Real::kF64) return Linear(fp64_edges != nullptr ? fp64_edges : edges.ptr, nullptr);
     return {s_nodes.ptr, s_node_scale.as<double>(), s_edges.ptr, nullptr, s_paths.ptr, s_path_scale.as<double>(),
             s_branches.ptr, s_branch_scale.as<double>(), nullptr, nullptr, nullptr, nullptr};
   }
@@ -932,10 +933,12 @@ struct Source {
 };
 
 // Writes the internal node state (max-shifted logs + scales, element type
-// l.real) of a call from its staged inputs.
-void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale) {
+// l.real) of a call from its staged inputs, or for a dense call from
+// `dense_nodes` ([B][N][K], the workspace element type) when given.
+void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale,
+               const void *dense_nodes = nullptr) {
   if (src.categorical == nullptr) {
-    LaunchInitNodes(l, w.real, w.input_nodes.ptr, nodes, scale);
+    LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale);
   } else {
     LaunchInitCategorical(l, w.real, false, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
@@ -1001,12 +1004,27 @@ void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   CheckLaunches();
 }
 
-void RunMaximum(Impl &w, std::size_t batch, const Source &src) {
-  const Launch l = w.MakeLaunch(w.real, batch, w.stream);
+// Where a MAP or sampling call reads its dense inputs and writes its
+// outputs: the staged workspace buffers, or a device-resident caller's.
+struct DenseCall {
+  const void *nodes, *edges, *uniforms;
+  double *log_z;             // log partitions, or MAP log weights
+  std::uint32_t *assignment;  // [B][N]
+  std::uint32_t *failure;
+  hipStream_t stream;
+};
+
+DenseCall StagedCall(Impl &w, double *log_z) {
+  return {w.input_nodes.ptr, w.edges.ptr, w.uniforms.ptr, log_z, w.assignment.as<std::uint32_t>(),
+          w.failure.as<std::uint32_t>(), w.stream};
+}
+
+void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &call) {
+  const Launch l = w.MakeLaunch(w.real, batch, call.stream);
   // Logs of the raw staged factors (bit-exact with the oracle's log of the
   // same values), internal layout.
   if (src.categorical == nullptr) {
-    LaunchMaxInit(l, w.real, w.input_nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
+    LaunchMaxInit(l, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
   } else {
     LaunchInitCategorical(l, w.real, true, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
@@ -1021,31 +1039,38 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src) {
   }
   const MaxBuffers buf = w.Max();
   for (const Level &level : w.levels) LaunchMaxLevel(l, level.primitive, level.offset, level.count, buf);
-  LaunchMaxFinish(l, buf, w.log_weight.as<double>(), w.assignment.as<std::uint32_t>());
+  LaunchMaxFinish(l, buf, call.log_z, call.assignment);
   for (std::size_t i = w.levels.size(); i-- > 0;)
-    LaunchMaxExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf,
-                         w.assignment.as<std::uint32_t>());
+    LaunchMaxExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf, call.assignment);
+  CheckLaunches();
+}
+
+void RunMaximum(Impl &w, std::size_t batch, const Source &src) {
+  RunMaximum(w, batch, src, StagedCall(w, w.log_weight.as<double>()));
+}
+
+void RunSample(Impl &w, std::size_t batch, const Source &src, const DenseCall &call) {
+  const Launch l = w.MakeLaunch(Real::kF64, batch, call.stream);
+  if (w.real == Real::kF32) {
+    // FP64 sampling state from the float factors.
+    InitState(w, src, l, w.s_nodes.ptr, w.s_node_scale.as<double>(), call.nodes);
+    LaunchWidenF32(l, static_cast<const float *>(call.edges), w.s_edges.as<double>(), w.E() * w.KK());
+  } else {
+    InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), call.nodes);
+  }
+  Hip(hipMemsetAsync(call.failure, 0, sizeof(std::uint32_t), call.stream), "GPU runtime MemsetAsync sampling status");
+  const LinearBuffers buf = w.SamplingLinear(call.edges);
+  ForwardLevels(l, w.levels, buf);
+  LaunchFinish(l, buf, call.log_z);
+  LaunchSampleSeed(l, buf, w.real, call.uniforms, call.assignment, call.failure);
+  for (std::size_t i = w.levels.size(); i-- > 0;)
+    LaunchSampleExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf, w.real,
+                            call.uniforms, call.assignment, call.failure);
   CheckLaunches();
 }
 
 void RunSample(Impl &w, std::size_t batch, const Source &src) {
-  const Launch l = w.MakeLaunch(Real::kF64, batch, w.stream);
-  if (w.real == Real::kF32) {
-    // FP64 sampling state from the staged float factors.
-    InitState(w, src, l, w.s_nodes.ptr, w.s_node_scale.as<double>());
-    LaunchWidenF32(l, w.edges.as<float>(), w.s_edges.as<double>(), w.E() * w.KK());
-  } else {
-    InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
-  }
-  Hip(hipMemsetAsync(w.failure.ptr, 0, sizeof(std::uint32_t), w.stream), "GPU runtime MemsetAsync sampling status");
-  const LinearBuffers buf = w.SamplingLinear();
-  ForwardLevels(l, w.levels, buf);
-  LaunchFinish(l, buf, w.log_z.as<double>());
-  LaunchSampleSeed(l, buf, w.real, w.uniforms.ptr, w.assignment.as<std::uint32_t>(), w.failure.as<std::uint32_t>());
-  for (std::size_t i = w.levels.size(); i-- > 0;)
-    LaunchSampleExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf, w.real,
-                            w.uniforms.ptr, w.assignment.as<std::uint32_t>(), w.failure.as<std::uint32_t>());
-  CheckLaunches();
+  RunSample(w, batch, src, StagedCall(w, w.log_z.as<double>()));
 }
 
 void ValidateUniforms(const Impl &w, const void *uniforms, std::size_t count, std::size_t batch) {
@@ -1683,6 +1708,30 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   key.args[0] = nodes, key.args[1] = edges, key.args[2] = log_z, key.args[3] = node_out, key.args[4] = edge_out;
   key.batch = batch;
   RunOrReplay(w, 1, key, s, enqueue);
+}
+
+void Workspace::MaximumDevice(const void *nodes, const void *edges, std::size_t batch, double *log_weights,
+                              std::uint32_t *states, void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, nodes, edges);
+  RequireMaximum(w);
+  if (log_weights == nullptr || states == nullptr) throw std::invalid_argument("MAP output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
+  RunMaximum(w, batch, Source{}, DenseCall{nodes, edges, nullptr, log_weights, states, nullptr, ChooseStream(w, stream)});
+}
+
+void Workspace::SampleDevice(const void *nodes, const void *edges, const void *uniforms, std::size_t batch,
+                             double *log_z, std::uint32_t *states, std::uint32_t *failure, void *stream) {
+  Impl &w = *impl_;
+  CheckDevice(w, batch, nodes, edges);
+  if (!(w.flags & THMM_RESERVE_SAMPLING))
+    throw std::invalid_argument("prepared accelerator posterior sampling requires ReserveSampling");
+  if (uniforms == nullptr || log_z == nullptr || states == nullptr || failure == nullptr)
+    throw std::invalid_argument("sampling input or output is null");
+  Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
+  w.forward = Impl::Forward::kNone;
+  RunSample(w, batch, Source{}, DenseCall{nodes, edges, uniforms, log_z, states, failure, ChooseStream(w, stream)});
 }
 
 void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,

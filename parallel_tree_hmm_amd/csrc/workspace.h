@@ -89,6 +89,12 @@ class Workspace {
                          void *stream);
   void LogZBackwardDevice(const double *grad, void *grad_nodes, void *grad_edges, bool reduce_edges,
                           void *stream);
+  // MAP and posterior sampling on device-resident dense potentials; the
+  // outputs are device buffers, written without a host synchronisation.
+  void MaximumDevice(const void *nodes, const void *edges, std::size_t batch, double *log_weights,
+                     std::uint32_t *states, void *stream);
+  void SampleDevice(const void *nodes, const void *edges, const void *uniforms, std::size_t batch, double *log_z,
+                    std::uint32_t *states, std::uint32_t *failure, void *stream);
   void Synchronize();
 
   // Launch variants (see Tuning in launch.h).

@@ -1,0 +1,181 @@
+"""Device-resident MAP and posterior sampling (thmm_maximum_device,
+thmm_sample_device): the kernels of the host-staged calls on caller device
+buffers. Checked here, for FP32 and FP64 workspaces:
+
+* states are those of the host-staged call bit for bit, and those of the
+  FP64 oracle (MAP exactly; samples away from CDF boundaries, as in the host
+  tests); MAP log weights and log Z are the host call's values (the host
+  call rounds them once to the workspace precision);
+* a tree whose potentials leave no positive weight gives -inf and, for
+  sampling, the failure flag, where the host call raises; the other trees'
+  results are unaffected;
+* variates outside [0, 1), which the host call rejects, still give states
+  in range.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+import parallel_tree_hmm_amd as thmm
+from parallel_tree_hmm_amd import _abi
+from conftest import caterpillar, forest, path, random_potentials, random_recursive, star
+
+pytestmark = pytest.mark.gpu
+
+
+def make_plan(rng):
+    return thmm.Plan(forest([random_recursive(300, rng), caterpillar(201), path(150), star(80)]))
+
+
+def device_inputs(torch, nodes, edges, precision):
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    return torch.tensor(nodes, dtype=dtype, device="cuda"), torch.tensor(edges, dtype=dtype, device="cuda")
+
+
+def as_precision(x, precision):
+    return np.asarray(x, np.float32) if precision == "f32" else np.asarray(x, np.float64)
+
+
+def reserve_device(plan, k, batch, precision, flag):
+    ws = thmm.Workspace()
+    ws.reserve(plan, k, batch, precision=precision, flags=flag | _abi.THMM_RESERVE_DEVICE_RESIDENT)
+    return ws
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k", [4, 16])
+def test_maximum_device_matches_host_and_oracle(require_gpu, precision, k):
+    import torch
+
+    rng = np.random.default_rng(5 * k + (precision == "f64"))
+    plan = make_plan(rng)
+    batch = 3
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    host_ws = thmm.Workspace()
+    host_ws.reserve_maximum(plan, k, batch, precision=precision)
+    host = thmm.maximum_a_posteriori_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), host_ws)
+    ws = reserve_device(plan, k, batch, precision, _abi.THMM_RESERVE_MAXIMUM)
+    d_nodes, d_edges = device_inputs(torch, nodes, edges, precision)
+    weights = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+    states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+    ws.maximum_device(d_nodes.data_ptr(), d_edges.data_ptr(), batch, weights.data_ptr(), states.data_ptr(), 0)
+    torch.cuda.synchronize()
+    got_states = states.cpu().numpy().view(np.uint32)
+    got_weights = weights.cpu().numpy()
+    assert np.array_equal(got_states.reshape(-1), np.asarray(host.states).reshape(-1))
+    assert np.array_equal(got_weights.astype(nodes.dtype), np.asarray(host.log_weights).reshape(-1))
+    for b in range(batch):
+        lw, expected = oracle.maximum(plan, k, nodes[b].astype(np.float64), edges.astype(np.float64))
+        assert np.array_equal(got_states[b], expected)
+        assert np.all(np.abs(got_weights.reshape(batch, -1)[b] - lw) <= 1e-12 * np.maximum(1, np.abs(lw)))
+    host_ws.close()
+    ws.close()
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k", [4, 16])
+def test_sample_device_matches_host_and_oracle(require_gpu, precision, k):
+    import torch
+
+    rng = np.random.default_rng(9 * k + (precision == "f64"))
+    plan = make_plan(rng)
+    batch = 3
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    u = as_precision(rng.random((batch, plan.num_nodes)), precision)
+    host_ws = thmm.Workspace()
+    host_ws.reserve_sampling(plan, k, batch, precision=precision)
+    host = thmm.posterior_sample_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges), u, host_ws)
+    host_states = np.asarray(host.states).reshape(batch, -1).copy()
+    host_log_z = host_ws.log_partitions_f64()[: batch * plan.roots.size].copy()
+    ws = reserve_device(plan, k, batch, precision, _abi.THMM_RESERVE_SAMPLING)
+    d_nodes, d_edges = device_inputs(torch, nodes, edges, precision)
+    d_u = torch.tensor(u, device="cuda")
+    log_z = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+    states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+    failure = torch.full((1,), 7, dtype=torch.int32, device="cuda")
+    ws.sample_device(d_nodes.data_ptr(), d_edges.data_ptr(), d_u.data_ptr(), batch, log_z.data_ptr(),
+                     states.data_ptr(), failure.data_ptr(), 0)
+    torch.cuda.synchronize()
+    got = states.cpu().numpy().view(np.uint32)
+    assert int(failure.item()) == 0
+    assert np.array_equal(got, host_states)
+    assert np.array_equal(log_z.cpu().numpy(), host_log_z)
+    for b in range(batch):
+        expected, dist = oracle.sample(plan, k, nodes[b].astype(np.float64), edges.astype(np.float64),
+                                       u[b].astype(np.float64))
+        if np.min(dist) >= 1e-12:
+            assert np.array_equal(got[b], expected)
+    host_ws.close()
+    ws.close()
+
+
+def test_device_calls_report_unsupported_trees_in_their_outputs(require_gpu):
+    import torch
+
+    rng = np.random.default_rng(2)
+    plan = make_plan(rng)
+    k, batch = 4, 2
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k)
+    nodes, edges = nodes.astype(np.float32), edges.astype(np.float32)
+    # Item 1: the first tree's root has no supported state.
+    root = int(plan.roots[0])
+    nodes[1, root, :] = 0.0
+    d_nodes, d_edges = device_inputs(torch, nodes, edges, "f32")
+    roots = plan.roots.size
+
+    map_ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_MAXIMUM)
+    weights = torch.empty(batch * roots, dtype=torch.float64, device="cuda")
+    states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+    map_ws.maximum_device(d_nodes.data_ptr(), d_edges.data_ptr(), batch, weights.data_ptr(), states.data_ptr(), 0)
+    torch.cuda.synchronize()
+    w = weights.cpu().numpy().reshape(batch, roots)
+    assert np.isneginf(w[1, 0]) and np.all(np.isfinite(w[0])) and np.all(np.isfinite(w[1, 1:]))
+    assert np.all(states.cpu().numpy().view(np.uint32) < k)
+
+    sample_ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_SAMPLING)
+    # Variates outside [0, 1) are taken as given: states stay in range.
+    u = rng.random((batch, plan.num_nodes)).astype(np.float32)
+    u[0, ::7] = 1.5
+    u[0, 3::7] = -0.5
+    d_u = torch.tensor(u, device="cuda")
+    log_z = torch.empty(batch * roots, dtype=torch.float64, device="cuda")
+    failure = torch.zeros(1, dtype=torch.int32, device="cuda")
+    sample_ws.sample_device(d_nodes.data_ptr(), d_edges.data_ptr(), d_u.data_ptr(), batch, log_z.data_ptr(),
+                            states.data_ptr(), failure.data_ptr(), 0)
+    torch.cuda.synchronize()
+    lz = log_z.cpu().numpy().reshape(batch, roots)
+    assert np.isneginf(lz[1, 0]) and np.all(np.isfinite(lz[0]))
+    assert int(failure.item()) == 1
+    assert np.all(states.cpu().numpy().view(np.uint32) < k)
+
+    # The host-staged calls raise for the same inputs.
+    host_ws = thmm.Workspace()
+    host_ws.reserve_sampling(plan, k, batch, precision="f32")
+    with pytest.raises(thmm.DomainError):
+        thmm.posterior_sample_prepared(thmm.BatchedModelView(plan, k, batch, nodes, edges),
+                                       np.clip(u, 0.0, 0.999), host_ws)
+    for ws in (map_ws, sample_ws, host_ws):
+        ws.close()
+
+
+def test_device_calls_need_their_reservations(require_gpu):
+    import torch
+
+    plan = thmm.Plan(path(20))
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, 4, 1, precision="f32", device_resident=True)
+    x = torch.ones((1, plan.num_nodes, 4), device="cuda")
+    e = torch.ones((plan.num_edges, 4, 4), device="cuda")
+    out = torch.empty(plan.num_nodes, dtype=torch.int32, device="cuda")
+    lz = torch.empty(1, dtype=torch.float64, device="cuda")
+    with pytest.raises(thmm.InvalidArgument):
+        ws.maximum_device(x.data_ptr(), e.data_ptr(), 1, lz.data_ptr(), out.data_ptr(), 0)
+    with pytest.raises(thmm.InvalidArgument):
+        ws.sample_device(x.data_ptr(), e.data_ptr(), x.data_ptr(), 1, lz.data_ptr(), out.data_ptr(), out.data_ptr(), 0)
+    torch.cuda.synchronize()
+    ws.close()
