@@ -194,6 +194,54 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup, tune=T
     return ws, ms, timings, log_z, node_out, edge_out, tuning
 
 
+def run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, gen):
+    """MAP (thmm_maximum_device) and posterior sampling (thmm_sample_device,
+    one U[0,1) variate per node) on the same resident FP32 inputs, each in
+    its own device-resident workspace; both run in FP64 internally. Times
+    `steps` calls after `warmup`, and checks that repeated calls give the
+    same states and that sampled states lie in range."""
+    n = plan.num_nodes
+    roots = plan.roots.size
+    out = {}
+    stream = torch.cuda.current_stream()
+    for name, flag in (("map", thmm._abi.THMM_RESERVE_MAXIMUM), ("sample", thmm._abi.THMM_RESERVE_SAMPLING)):
+        ws = thmm.Workspace()
+        ws.reserve(plan, k, 1, precision="f32", flags=flag | thmm._abi.THMM_RESERVE_DEVICE_RESIDENT)
+        log_z = torch.empty(roots, dtype=torch.float64, device="cuda")
+        states = torch.empty(n, dtype=torch.int32, device="cuda")
+        if name == "map":
+            call = lambda: ws.maximum_device(nodes.data_ptr(), edges.data_ptr(), 1, log_z.data_ptr(),  # noqa: E731
+                                             states.data_ptr(), stream.cuda_stream)
+        else:
+            uniforms = torch.rand(n, device="cuda", dtype=torch.float32, generator=gen)
+            failure = torch.zeros(1, dtype=torch.int32, device="cuda")
+            call = lambda: ws.sample_device(nodes.data_ptr(), edges.data_ptr(), uniforms.data_ptr(), 1,  # noqa: E731
+                                            log_z.data_ptr(), states.data_ptr(), failure.data_ptr(),
+                                            stream.cuda_stream)
+        for _ in range(max(1, args.warmup)):
+            call()
+        torch.cuda.synchronize()
+        first = states.clone()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.steps):
+            call()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / args.steps
+        entry = {"ms_per_step": round(ms, 4), "value": n / (ms * 1e-3), "unit": "nodes/s",
+                 "repeatable": bool(torch.equal(first, states)),
+                 "states_in_range": bool(((states >= 0) & (states < k)).all()),
+                 "log_z_finite": bool(torch.isfinite(log_z).all())}
+        if name == "sample":
+            entry["failure_flag"] = int(failure.item())
+        out[name] = entry
+        ws.close()
+        del ws, log_z, states
+        torch.cuda.empty_cache()
+    return out
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
@@ -405,6 +453,7 @@ def main() -> int:
     ap.add_argument("--cpu-nodes", type=int, default=400_000)
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--no-half", action="store_true", help="skip the binary16 edge-marginal leg")
+    ap.add_argument("--no-map", action="store_true", help="skip the device MAP and sampling legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (the reference point that scales HBM counters)")
     args = ap.parse_args()
@@ -531,6 +580,11 @@ def main() -> int:
     del node_out, edge_out, log_z
     torch.cuda.empty_cache()
 
+    decode = None
+    if not args.no_map:
+        decode = run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, gen)
+        log(f"device MAP and sampling: {decode}")
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -588,6 +642,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
         "half_edge_outputs": half,
+        "map_and_sampling": decode,
         "parity": parity,
     }
     print(json.dumps(result), flush=True)

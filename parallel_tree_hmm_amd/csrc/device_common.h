@@ -142,6 +142,14 @@ template <typename T> __device__ inline void StoreQuadStream(T *p, const Quad<T>
   }
 }
 
+// A dispatch's grid size is a 32-bit count of work-items (blocks times
+// their threads). Kernels over arrays that can exceed it run a capped grid
+// (GridStrideBlocks) and step through the array by the grid's size.
+__device__ inline std::uint64_t GridThread() {
+  return static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+}
+__device__ inline std::uint64_t GridStride() { return static_cast<std::uint64_t>(gridDim.x) * blockDim.x; }
+
 // Where a slot's adjoint goes: composed-path adjoints in the element type,
 // edge marginals in the element type or, with buf.edge_out_half, as IEEE
 // binary16 (rounded to nearest even; magnitudes below 2^-14 are stored as

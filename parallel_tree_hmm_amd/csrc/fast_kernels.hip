@@ -1328,37 +1328,39 @@ template <typename In, typename Out, int kMode, int K>
 __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
                                                      double *scale) {
   constexpr int kLanes = K / 4;
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::uint64_t v = tid / kLanes;
-  const int l = static_cast<int>(tid % kLanes);
-  if (v >= static_cast<std::uint64_t>(X) * Y) return;
-  const std::uint64_t x = v / Y, y = v % Y;
-  const std::uint64_t to = y * X + x;
-  const Quad<In> q = LoadQuad(in + v * K + l * 4);
-  // Logs are taken in the wider of the two element types.
-  using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
-  Quad<Out> o;
-  if constexpr (kMode == 0) {
+  // Grid-stride: the stride is a whole number of teams, so a team's lanes
+  // stay together.
+  for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(X) * Y * kLanes; tid += GridStride()) {
+    const std::uint64_t v = tid / kLanes;
+    const int l = static_cast<int>(tid % kLanes);
+    const std::uint64_t x = v / Y, y = v % Y;
+    const std::uint64_t to = y * X + x;
+    const Quad<In> q = LoadQuad(in + v * K + l * 4);
+    // Logs are taken in the wider of the two element types.
+    using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
+    Quad<Out> o;
+    if constexpr (kMode == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) o.v[i] = static_cast<Out>(q.v[i]);
-  } else {
-    W w[4];
+      for (int i = 0; i < 4; ++i) o.v[i] = static_cast<Out>(q.v[i]);
+    } else {
+      W w[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = kMode == 1 ? LogOf(static_cast<W>(q.v[i])) : static_cast<W>(q.v[i]);
-    W top = Max(Max(w[0], w[1]), Max(w[2], w[3]));
+      for (int i = 0; i < 4; ++i) w[i] = kMode == 1 ? LogOf(static_cast<W>(q.v[i])) : static_cast<W>(q.v[i]);
+      W top = Max(Max(w[0], w[1]), Max(w[2], w[3]));
 #pragma unroll
-    for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
-    const bool live = top > NegInf<W>();
+      for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
+      const bool live = top > NegInf<W>();
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if constexpr (kMode == 3)
-        o.v[i] = live ? ExpNonPositive(static_cast<Out>(w[i] - top)) : Out(0);
-      else
-        o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+      for (int i = 0; i < 4; ++i) {
+        if constexpr (kMode == 3)
+          o.v[i] = live ? ExpNonPositive(static_cast<Out>(w[i] - top)) : Out(0);
+        else
+          o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+      }
+      if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
     }
-    if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
+    StoreQuad(out + to * K + l * 4, o);
   }
-  StoreQuad(out + to * K + l * 4, o);
 }
 
 constexpr unsigned kBlock = 256;
@@ -1390,6 +1392,11 @@ bool FusedForwardSupported(std::uint32_t states) {
 void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
   if (count == 0) return;
+  if (static_cast<std::uint64_t>(count) * l.dims.B * 64 >= kMaxLaunchItems) {
+    InOpChunks(offset, count, l.dims.B,
+               [&](std::uint32_t first, std::uint32_t ops) { LaunchGatherChunks(l, g, first, ops, buf); });
+    return;
+  }
   const std::uint32_t K = l.dims.K;
   const unsigned block = l.tuning.rake_block;
   const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * (K >= 32 ? 64 : 16), block);
@@ -1413,6 +1420,11 @@ void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offse
 void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
                           const LinearBuffers &buf) {
   if (count == 0) return;
+  if (static_cast<std::uint64_t>(count) * l.dims.B * 64 >= kMaxLaunchItems) {
+    InOpChunks(offset, count, l.dims.B,
+               [&](std::uint32_t first, std::uint32_t ops) { LaunchGatherSegments(l, g, first, ops, buf); });
+    return;
+  }
   const std::uint32_t K = l.dims.K;
   const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * K);
   const hipStream_t s = StreamOf(l);
@@ -1512,8 +1524,7 @@ namespace {
 template <typename In, typename Out, int kMode>
 bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
   const std::uint64_t threads = static_cast<std::uint64_t>(X) * Y * (l.dims.K / 4);
-  if (threads / kBlock >= (1ull << 31)) return false;  // grid size limit
-  const unsigned grid = Blocks(threads);
+  const unsigned grid = GridStrideBlocks(threads, kBlock);
   const auto *i = static_cast<const In *>(in);
   auto *o = static_cast<Out *>(out);
   switch (l.dims.K) {
@@ -1539,21 +1550,21 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void StageLogRowsTeam(const T *log_edges, const std::uint32_t *ids,
                                                         std::uint64_t rows, T *edges, double *scale) {
   constexpr int kLanes = K / 4;
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  const std::uint64_t r = tid / kLanes;
-  const int l = static_cast<int>(tid % kLanes);
-  if (r >= rows) return;
-  const std::uint64_t v = static_cast<std::uint64_t>(ids[r / K]) * K + r % K;
-  const Quad<T> q = LoadQuad(log_edges + v * K + l * 4);
-  T top = Max(Max(q.v[0], q.v[1]), Max(q.v[2], q.v[3]));
+  for (std::uint64_t tid = GridThread(); tid < rows * kLanes; tid += GridStride()) {  // whole teams per stride
+    const std::uint64_t r = tid / kLanes;
+    const int l = static_cast<int>(tid % kLanes);
+    const std::uint64_t v = static_cast<std::uint64_t>(ids[r / K]) * K + r % K;
+    const Quad<T> q = LoadQuad(log_edges + v * K + l * 4);
+    T top = Max(Max(q.v[0], q.v[1]), Max(q.v[2], q.v[3]));
 #pragma unroll
-  for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
-  const bool live = top > NegInf<T>();
-  Quad<T> o;
+    for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
+    const bool live = top > NegInf<T>();
+    Quad<T> o;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) o.v[i] = live ? ExpNonPositive(static_cast<T>(q.v[i] - top)) : T(0);
-  StoreQuad(edges + v * K + l * 4, o);
-  if (l == 0) scale[v] = live ? static_cast<double>(top) : kNegInf;
+    for (int i = 0; i < 4; ++i) o.v[i] = live ? ExpNonPositive(static_cast<T>(q.v[i] - top)) : T(0);
+    StoreQuad(edges + v * K + l * 4, o);
+    if (l == 0) scale[v] = live ? static_cast<double>(top) : kNegInf;
+  }
 }
 
 }  // namespace
@@ -1564,7 +1575,7 @@ bool LaunchStageLogEdges(const Launch &l, const void *log_edges, const std::uint
   if (K != 8 && K != 16 && K != 32 && K != 64) return false;
   if (count == 0) return true;
   const std::uint64_t rows = static_cast<std::uint64_t>(count) * K;
-  const unsigned grid = Blocks(rows * (K / 4));
+  const unsigned grid = GridStrideBlocks(rows * (K / 4), kBlock);
   const hipStream_t s = StreamOf(l);
 #define THMM_STAGE(T, KK)                                                                                        \
   StageLogRowsTeam<T, KK><<<grid, kBlock, 0, s>>>(static_cast<const T *>(log_edges), ids, rows,                  \

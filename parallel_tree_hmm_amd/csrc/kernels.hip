@@ -371,38 +371,44 @@ __global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::u
   __shared__ double tile[64][65];
   const std::uint32_t ty = 64 / K;  // y-vectors per tile row
   const std::uint32_t row = ty * K; // elements per tile row
-  const std::uint64_t x0 = static_cast<std::uint64_t>(blockIdx.x % x_tiles) * 64;
-  const std::uint64_t y0 = static_cast<std::uint64_t>(blockIdx.x / x_tiles) * ty;
-  for (std::uint32_t i = threadIdx.x; i < 64 * row; i += blockDim.x) {
-    const std::uint32_t x = i / row, j = i % row;
-    const std::uint64_t gx = x0 + x;
-    if (gx < X && y0 * K + j < static_cast<std::uint64_t>(Y) * K) {
-      const double value = static_cast<double>(in[(gx * Y + y0) * K + j]);
-      tile[x][j] = kMode == 1 ? (value > 0.0 ? log(value) : kNegInf) : value;
+  // Block-stride over the tiles (a dispatch's grid size is a 32-bit count
+  // of work-items); every thread of a block runs the same iterations.
+  const std::uint64_t tiles = static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty);
+  for (std::uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    const std::uint64_t x0 = (t % x_tiles) * 64;
+    const std::uint64_t y0 = (t / x_tiles) * ty;
+    for (std::uint32_t i = threadIdx.x; i < 64 * row; i += blockDim.x) {
+      const std::uint32_t x = i / row, j = i % row;
+      const std::uint64_t gx = x0 + x;
+      if (gx < X && y0 * K + j < static_cast<std::uint64_t>(Y) * K) {
+        const double value = static_cast<double>(in[(gx * Y + y0) * K + j]);
+        tile[x][j] = kMode == 1 ? (value > 0.0 ? log(value) : kNegInf) : value;
+      }
     }
-  }
-  __syncthreads();
-  if (kMode == 0) {
-    const std::uint32_t xk = 64 * K;
-    for (std::uint32_t i = threadIdx.x; i < ty * xk; i += blockDim.x) {
-      const std::uint32_t y = i / xk, r = i % xk;
-      const std::uint32_t x = r / K, k = r % K;
-      const std::uint64_t gy = y0 + y, gx = x0 + x;
-      if (gy < Y && gx < X) out[(gy * X + gx) * K + k] = static_cast<Out>(tile[x][y * K + k]);
+    __syncthreads();
+    if (kMode == 0) {
+      const std::uint32_t xk = 64 * K;
+      for (std::uint32_t i = threadIdx.x; i < ty * xk; i += blockDim.x) {
+        const std::uint32_t y = i / xk, r = i % xk;
+        const std::uint32_t x = r / K, k = r % K;
+        const std::uint64_t gy = y0 + y, gx = x0 + x;
+        if (gy < Y && gx < X) out[(gy * X + gx) * K + k] = static_cast<Out>(tile[x][y * K + k]);
+      }
+    } else {
+      // one thread per output vector
+      for (std::uint32_t i = threadIdx.x; i < 64 * ty; i += blockDim.x) {
+        const std::uint32_t y = i / 64, x = i % 64;
+        const std::uint64_t gy = y0 + y, gx = x0 + x;
+        if (gy >= Y || gx >= X) continue;
+        double top = kNegInf;
+        for (std::uint32_t k = 0; k < K; ++k) top = fmax(top, tile[x][y * K + k]);
+        const bool live = top > kNegInf;
+        Out *o = out + (gy * X + gx) * K;
+        for (std::uint32_t k = 0; k < K; ++k) o[k] = live ? static_cast<Out>(tile[x][y * K + k] - top) : NegInf<Out>();
+        scale[gy * X + gx] = top;
+      }
     }
-  } else {
-    // one thread per output vector
-    for (std::uint32_t i = threadIdx.x; i < 64 * ty; i += blockDim.x) {
-      const std::uint32_t y = i / 64, x = i % 64;
-      const std::uint64_t gy = y0 + y, gx = x0 + x;
-      if (gy >= Y || gx >= X) continue;
-      double top = kNegInf;
-      for (std::uint32_t k = 0; k < K; ++k) top = fmax(top, tile[x][y * K + k]);
-      const bool live = top > kNegInf;
-      Out *o = out + (gy * X + gx) * K;
-      for (std::uint32_t k = 0; k < K; ++k) o[k] = live ? static_cast<Out>(tile[x][y * K + k] - top) : NegInf<Out>();
-      scale[gy * X + gx] = top;
-    }
+    __syncthreads();  // the tile is refilled by the next iteration
   }
 }
 
@@ -410,14 +416,16 @@ __global__ void TransposeVectors(const In *in, Out *out, std::uint32_t X, std::u
 // scales [E][K].
 template <typename T>
 __global__ void InitEdgesFromLogs(const T *in, T *edges, double *scale, Dims d) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<std::uint64_t>(d.E) * d.K) return;
-  const T *x = in + tid * d.K;
-  T top = NegInf<T>();
-  for (std::uint32_t c = 0; c < d.K; ++c) top = Max(top, x[c]);
-  const bool live = top > NegInf<T>();
-  for (std::uint32_t c = 0; c < d.K; ++c) edges[tid * d.K + c] = live ? static_cast<T>(exp(static_cast<double>(x[c] - top))) : T(0);
-  scale[tid] = live ? static_cast<double>(top) : kNegInf;
+  const std::uint64_t rows = static_cast<std::uint64_t>(d.E) * d.K;
+  for (std::uint64_t tid = GridThread(); tid < rows; tid += GridStride()) {
+    const T *x = in + tid * d.K;
+    T top = NegInf<T>();
+    for (std::uint32_t c = 0; c < d.K; ++c) top = Max(top, x[c]);
+    const bool live = top > NegInf<T>();
+    for (std::uint32_t c = 0; c < d.K; ++c)
+      edges[tid * d.K + c] = live ? static_cast<T>(exp(static_cast<double>(x[c] - top))) : T(0);
+    scale[tid] = live ? static_cast<double>(top) : kNegInf;
+  }
 }
 
 // Categorical node factors (src/gpu_backend_impl.inc:326-382 semantics):
@@ -461,18 +469,17 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
 template <typename T>
 __global__ void ReduceEdges(const T *edge_out, T *out, Dims d) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::uint64_t total = static_cast<std::uint64_t>(d.E) * kk;
-  if (tid >= total) return;
-  double sum = 0.0;
-  for (std::uint32_t b = 0; b < d.B; ++b) sum += static_cast<double>(edge_out[b * total + tid]);
-  out[tid] = static_cast<T>(sum);
+  for (std::uint64_t tid = GridThread(); tid < total; tid += GridStride()) {
+    double sum = 0.0;
+    for (std::uint32_t b = 0; b < d.B; ++b) sum += static_cast<double>(edge_out[b * total + tid]);
+    out[tid] = static_cast<T>(sum);
+  }
 }
 
 template <typename In, typename Out>
 __global__ void Convert(const In *in, Out *out, std::uint64_t count) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid < count) out[tid] = static_cast<Out>(in[tid]);
+  for (std::uint64_t tid = GridThread(); tid < count; tid += GridStride()) out[tid] = static_cast<Out>(in[tid]);
 }
 
 // ------------------------------------------------------------------------
@@ -483,18 +490,19 @@ __global__ void Convert(const In *in, Out *out, std::uint64_t count) {
 // the edges (src/inference.cc:95-98).
 template <typename In>
 __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
   const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
-  if (tid < node_values) {
-    const std::uint64_t s = tid % d.K, nb = tid / d.K;
-    const std::uint64_t b = nb % d.B, n = nb / d.B;
-    const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
-    nodes[tid] = x > 0.0 ? log(x) : kNegInf;
-  } else if (tid < node_values + edge_values) {
-    const std::uint64_t i = tid - node_values;
-    const double x = static_cast<double>(edges_in[i]);
-    edges[i] = x > 0.0 ? log(x) : kNegInf;
+  for (std::uint64_t tid = GridThread(); tid < node_values + edge_values; tid += GridStride()) {
+    if (tid < node_values) {
+      const std::uint64_t s = tid % d.K, nb = tid / d.K;
+      const std::uint64_t b = nb % d.B, n = nb / d.B;
+      const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
+      nodes[tid] = x > 0.0 ? log(x) : kNegInf;
+    } else {
+      const std::uint64_t i = tid - node_values;
+      const double x = static_cast<double>(edges_in[i]);
+      edges[i] = x > 0.0 ? log(x) : kNegInf;
+    }
   }
 }
 
@@ -776,8 +784,9 @@ void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X
   if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale)) return;
   const std::uint32_t ty = 64 / l.dims.K;
   const std::uint32_t x_tiles = (X + 63) / 64;
-  const unsigned grid = static_cast<unsigned>(static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty));
-  if (grid == 0) return;
+  const std::uint64_t tiles = static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty);
+  if (tiles == 0) return;
+  const unsigned grid = static_cast<unsigned>(tiles < (1u << 20) ? tiles : (1u << 20));
   TransposeVectors<In, Out, kMode><<<grid, kThreads, 0, StreamOf(l)>>>(
       static_cast<const In *>(in), static_cast<Out *>(out), X, Y, l.dims.K, scale, x_tiles);
 }
@@ -830,7 +839,7 @@ void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges
     Check(hipGetLastError(), "log edge initialisation launch");
     return;
   }
-  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.E) * l.dims.K, kThreads);
+  const unsigned grid = GridStrideBlocks(static_cast<std::uint64_t>(l.dims.E) * l.dims.K, kThreads);
   if (l.real == Real::kF32)
     InitEdgesFromLogs<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(log_input),
                                                                 static_cast<float *>(edges), edge_scale, l.dims);
@@ -877,12 +886,14 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const s
 
 void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
-  if (count == 0 || LaunchFastLevel(l, false, primitive, offset, count, buf)) return;
-  if (l.real == Real::kF32) {
-    THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, offset, count, buf)));
-  } else {
-    THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<double, G>(l, primitive, offset, count, buf)));
-  }
+  InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
+    if (LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
+    if (l.real == Real::kF32) {
+      THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, first, ops, buf)));
+    } else {
+      THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<double, G>(l, primitive, first, ops, buf)));
+    }
+  });
 }
 
 void LaunchFinish(const Launch &l, const LinearBuffers &buf, double *log_z) {
@@ -905,12 +916,14 @@ void LaunchSeedAdjoint(const Launch &l, const LinearBuffers &buf, const double *
 
 void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
-  if (count == 0 || LaunchFastLevel(l, true, primitive, offset, count, buf)) return;
-  if (l.real == Real::kF32) {
-    THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<float, G>(l, primitive, offset, count, buf)));
-  } else {
-    THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<double, G>(l, primitive, offset, count, buf)));
-  }
+  InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
+    if (LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
+    if (l.real == Real::kF32) {
+      THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<float, G>(l, primitive, first, ops, buf)));
+    } else {
+      THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<double, G>(l, primitive, first, ops, buf)));
+    }
+  });
 }
 
 void LaunchNodeAdjointOut(const Launch &l, const void *node_adj, void *out) {
@@ -929,21 +942,22 @@ namespace {
 template <typename T>
 __global__ void BranchChainTerms(const T *edge_out, const T *edges, const double *rate, double *terms, Dims d) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<std::uint64_t>(d.E) * kk) return;
-  const std::uint64_t e = tid / kk;
-  const std::uint32_t p = static_cast<std::uint32_t>((tid % kk) / d.K), c = static_cast<std::uint32_t>(tid % d.K);
-  double mass = 0.0;
-  for (std::uint32_t b = 0; b < d.B; ++b) mass += static_cast<double>(edge_out[static_cast<std::uint64_t>(b) * d.E * kk + tid]);
-  const T *P = edges + e * kk;
-  const double value = static_cast<double>(P[p * d.K + c]);
-  double term = 0.0;
-  if (value > 0.0 && mass != 0.0) {
-    double slope = 0.0;
-    for (std::uint32_t j = 0; j < d.K; ++j) slope += rate[p * d.K + j] * static_cast<double>(P[j * d.K + c]);
-    term = mass * slope / value;
+  for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(d.E) * kk; tid += GridStride()) {
+    const std::uint64_t e = tid / kk;
+    const std::uint32_t p = static_cast<std::uint32_t>((tid % kk) / d.K), c = static_cast<std::uint32_t>(tid % d.K);
+    double mass = 0.0;
+    for (std::uint32_t b = 0; b < d.B; ++b)
+      mass += static_cast<double>(edge_out[static_cast<std::uint64_t>(b) * d.E * kk + tid]);
+    const T *P = edges + e * kk;
+    const double value = static_cast<double>(P[p * d.K + c]);
+    double term = 0.0;
+    if (value > 0.0 && mass != 0.0) {
+      double slope = 0.0;
+      for (std::uint32_t j = 0; j < d.K; ++j) slope += rate[p * d.K + j] * static_cast<double>(P[j * d.K + c]);
+      term = mass * slope / value;
+    }
+    terms[tid] = term;
   }
-  terms[tid] = term;
 }
 
 __global__ void BranchChainSums(const double *terms, double *grad, Dims d) {
@@ -966,13 +980,13 @@ template <typename T>
 __global__ void ExpmEdges(const double *V, const double *lambda, const double *V_inv, const double *t, T *edges,
                           Dims d) {
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= kk * d.E) return;
-  const std::uint64_t e = tid / kk;
-  const std::uint32_t i = static_cast<std::uint32_t>(tid % kk) / d.K, j = static_cast<std::uint32_t>(tid % kk) % d.K;
-  double sum = 0.0;
-  for (std::uint32_t k = 0; k < d.K; ++k) sum += V[i * d.K + k] * exp(lambda[k] * t[e]) * V_inv[k * d.K + j];
-  edges[tid] = static_cast<T>(fmax(sum, 0.0));
+  for (std::uint64_t tid = GridThread(); tid < kk * d.E; tid += GridStride()) {
+    const std::uint64_t e = tid / kk;
+    const std::uint32_t i = static_cast<std::uint32_t>(tid % kk) / d.K, j = static_cast<std::uint32_t>(tid % kk) % d.K;
+    double sum = 0.0;
+    for (std::uint32_t k = 0; k < d.K; ++k) sum += V[i * d.K + k] * exp(lambda[k] * t[e]) * V_inv[k * d.K + j];
+    edges[tid] = static_cast<T>(fmax(sum, 0.0));
+  }
 }
 
 }  // namespace
@@ -982,10 +996,10 @@ void LaunchExpmEdges(const Launch &l, const double *V, const double *lambda, con
   const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
   if (entries == 0) return;
   if (l.real == Real::kF32)
-    ExpmEdges<float><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+    ExpmEdges<float><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
         V, lambda, V_inv, branch_lengths, static_cast<float *>(edges), l.dims);
   else
-    ExpmEdges<double><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+    ExpmEdges<double><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
         V, lambda, V_inv, branch_lengths, static_cast<double *>(edges), l.dims);
   Check(hipGetLastError(), "expm edge launch");
 }
@@ -995,10 +1009,10 @@ void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *
   const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
   if (entries == 0) return;
   if (l.real == Real::kF32)
-    BranchChainTerms<float><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+    BranchChainTerms<float><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
         static_cast<const float *>(edge_out), static_cast<const float *>(edges), rate, terms, l.dims);
   else
-    BranchChainTerms<double><<<BlocksFor(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
+    BranchChainTerms<double><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
         static_cast<const double *>(edge_out), static_cast<const double *>(edges), rate, terms, l.dims);
   BranchChainSums<<<BlocksFor(l.dims.E, kThreads), kThreads, 0, StreamOf(l)>>>(terms, grad, l.dims);
   Check(hipGetLastError(), "branch-length chain launch");
@@ -1007,7 +1021,7 @@ void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *
 void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out) {
   const std::uint64_t total = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
   if (total == 0) return;
-  const unsigned grid = BlocksFor(total, kThreads);
+  const unsigned grid = GridStrideBlocks(total, kThreads);
   if (l.real == Real::kF32)
     ReduceEdges<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(edge_out),
                                                           static_cast<float *>(out), l.dims);
@@ -1019,13 +1033,13 @@ void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out
 
 void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t count) {
   if (count == 0) return;
-  Convert<float, double><<<BlocksFor(count, kThreads), kThreads, 0, StreamOf(l)>>>(in, out, count);
+  Convert<float, double><<<GridStrideBlocks(count, kThreads), kThreads, 0, StreamOf(l)>>>(in, out, count);
   Check(hipGetLastError(), "widen launch");
 }
 
 void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t count) {
   if (count == 0) return;
-  const unsigned grid = BlocksFor(count, kThreads);
+  const unsigned grid = GridStrideBlocks(count, kThreads);
   if (l.real == Real::kF32)
     Convert<double, float><<<grid, kThreads, 0, StreamOf(l)>>>(in, static_cast<float *>(out), count);
   else
@@ -1037,7 +1051,7 @@ void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const vo
                    double *edges) {
   const std::uint64_t total = static_cast<std::uint64_t>(l.dims.N) * l.dims.B * l.dims.K +
                               static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
-  const unsigned grid = BlocksFor(total, kThreads);
+  const unsigned grid = GridStrideBlocks(total, kThreads);
   if (in_real == Real::kF32)
     MaxInit<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
                                                       static_cast<const float *>(edges_in), nodes, edges, l.dims);
@@ -1049,8 +1063,9 @@ void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const vo
 
 void LaunchMaxLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                     const MaxBuffers &buf) {
-  if (count == 0) return;
-  THMM_WITH_GROUP(l.dims.K, (MaxLevelT<G>(l, primitive, offset, count, buf)));
+  InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
+    THMM_WITH_GROUP(l.dims.K, (MaxLevelT<G>(l, primitive, first, ops, buf)));
+  });
 }
 
 void LaunchMaxFinish(const Launch &l, const MaxBuffers &buf, double *log_weight, std::uint32_t *assignment) {

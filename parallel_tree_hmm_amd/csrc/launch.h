@@ -47,6 +47,27 @@ constexpr std::uint32_t kChainPrimitive = 6;
 constexpr std::uint32_t kSmallChainPrimitive = 7;
 constexpr std::uint32_t kPairChainPrimitive = 8;
 
+// A dispatch's grid size is a 32-bit count of work-items. Kernels that step
+// through their arrays by the grid's size take at most 2^20 blocks;
+// level launches split their op range so that each dispatch stays below
+// 2^31 work-items (kMaxLaunchItems) at up to 64 lanes per (op, item).
+inline unsigned GridStrideBlocks(std::uint64_t threads, unsigned block) {
+  const std::uint64_t blocks = (threads + block - 1) / block;
+  return static_cast<unsigned>(blocks < (1u << 20) ? blocks : (1u << 20));
+}
+constexpr std::uint64_t kMaxLaunchItems = std::uint64_t{1} << 31;
+
+// Runs launch(offset, count) over consecutive sub-ranges of the level's ops
+// with at most kMaxLaunchItems / (batch * 64) ops each (the ops of a level
+// are independent and the kernels address them as offset + index).
+template <typename F>
+inline void InOpChunks(std::uint32_t offset, std::uint32_t count, std::uint64_t batch, F &&launch) {
+  const std::uint64_t per = kMaxLaunchItems / (batch * 64);
+  const std::uint32_t step = static_cast<std::uint32_t>(per == 0 ? 1 : (per < count ? per : count));
+  for (std::uint32_t done = 0; done < count; done += step)
+    launch(offset + done, count - done < step ? count - done : step);
+}
+
 struct DeviceOps {
   const DeviceRake *rakes;
   const DeviceCombine *combines;
