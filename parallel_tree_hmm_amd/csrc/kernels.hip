@@ -582,6 +582,105 @@ __global__ void MaxCompress(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
   }
 }
 
+// Team forms of MaxRake and MaxCompress for K = 8 and 16 (FP64 state, the
+// same association and strict '>' argmax, so bit-identical results): a
+// lane owns a quad of four consecutive columns of one row, loads and stores
+// move whole quads, and choices are stored four at a time.
+
+// Combines two (value, index) argmax partials of one row: the larger value,
+// and on a tie the lower index, as an ascending strict '>' scan does.
+__device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double other, std::uint32_t other_arg) {
+  if (other > best || (other == best && other_arg < arg)) best = other, arg = other_arg;
+}
+
+// One rake row per kQ = K/4 consecutive lanes; K rows per (op, item).
+template <int K>
+__global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                   MaxBuffers buf) {
+  constexpr int kQ = K / 4, kLanes = K * kQ;
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t item = tid / kLanes;
+  const int l = static_cast<int>(tid % kLanes);
+  if (item >= static_cast<std::uint64_t>(count) * d.B) return;  // whole teams
+  const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
+  const DeviceRake op = ops.rakes[offset + item / d.B];
+  const int p = l / kQ, c0 = (l % kQ) * 4;
+  const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
+  const Quad<double> row = LoadQuad(MaxSlot(buf, d, op.slot, b) + p * K + c0);
+  double best = row.v[0] + v.v[0];
+  std::uint32_t arg = c0;
+#pragma unroll
+  for (int c = 1; c < 4; ++c) {
+    const double candidate = row.v[c] + v.v[c];
+    if (candidate > best) best = candidate, arg = c0 + c;
+  }
+#pragma unroll
+  for (int o = 1; o < kQ; o <<= 1) {
+    const double other = __shfl_xor(best, o, kQ);
+    const std::uint32_t other_arg = __shfl_xor(arg, o, kQ);
+    ArgMaxMerge(best, arg, other, other_arg);
+  }
+  if (c0 == 0) {
+    const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + p;
+    buf.branches[at] = best;
+    buf.rake_choice[at] = static_cast<std::uint8_t>(arg);
+  }
+}
+
+// K*K/4 lanes per (op, item), 256 / (K*K/4) items per block: L + mid and R
+// are staged in LDS with quad loads, then each lane scans the K middle
+// states for its quad of outputs.
+template <int K>
+__global__ __launch_bounds__(256) void MaxCompressTeam(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                       std::uint32_t count, MaxBuffers buf) {
+  constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
+  __shared__ double left_mid[kItems][K * K];  // L[p][m] + mid[m]
+  __shared__ double right[kItems][K * K];
+  const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  DeviceCompress op{};
+  std::uint32_t b = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.compresses[offset + item / d.B];
+    const double *L = MaxSlot(buf, d, op.left, b);
+    const double *R = MaxSlot(buf, d, op.right, b);
+    const double *mid = buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K;
+    const Quad<double> lq = LoadQuad(L + 4 * l), rq = LoadQuad(R + 4 * l);
+    const Quad<double> mq = LoadQuad(mid + (4 * l) % K);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      left_mid[t][4 * l + i] = lq.v[i] + mq.v[i];
+      right[t][4 * l + i] = rq.v[i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int p = l / kQ, c0 = (l % kQ) * 4;
+  double best[4];
+  std::uint32_t arg[4] = {0, 0, 0, 0};
+  const double lm0 = left_mid[t][p * K];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) best[c] = lm0 + right[t][c0 + c];
+#pragma unroll
+  for (int m = 1; m < K; ++m) {
+    const double lm = left_mid[t][p * K + m];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double candidate = lm + right[t][m * K + c0 + c];
+      if (candidate > best[c]) best[c] = candidate, arg[c] = m;
+    }
+  }
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  Quad<double> q;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) q.v[c] = best[c];
+  StoreQuad(buf.paths + out_at * K * K + p * K + c0, q);
+  const std::uint32_t packed = arg[0] | (arg[1] << 8) | (arg[2] << 16) | (arg[3] << 24);
+  *reinterpret_cast<std::uint32_t *>(buf.compress_choice + out_at * K * K + p * K + c0) = packed;
+}
+
 // src/inference.cc:641-649
 __global__ void MaxFinish(DeviceOps ops, Dims d, MaxBuffers buf, double *log_weight, std::uint32_t *assignment) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -753,6 +852,25 @@ void ReverseLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
 template <int G>
 void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                const MaxBuffers &buf) {
+  if (l.dims.K == 8 || l.dims.K == 16) {
+    const std::uint64_t lanes = static_cast<std::uint64_t>(count) * l.dims.B * l.dims.K * l.dims.K / 4;
+    if (primitive == 0) {
+      if (l.dims.K == 8)
+        MaxRakeTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else
+        MaxRakeTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      Check(hipGetLastError(), "max-product level launch");
+      return;
+    }
+    if (primitive == 3) {
+      if (l.dims.K == 8)
+        MaxCompressTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else
+        MaxCompressTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      Check(hipGetLastError(), "max-product level launch");
+      return;
+    }
+  }
   const unsigned block = GroupBlock(G);
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
   switch (primitive) {
