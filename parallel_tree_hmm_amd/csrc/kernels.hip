@@ -180,6 +180,69 @@ __global__ void CompressForward(DeviceOps ops, Dims d, std::uint32_t offset, std
            : kNegInf;
 }
 
+// Team form of CompressForward for FP64 state at K = 8 and 16: K*K/4 lanes
+// per (op, item), each owning a quad of four consecutive columns of one
+// output row. The middle weights, L diag(e) and R are staged in LDS with
+// quad loads; rows are reduced across the K/4 lanes that hold them. The
+// arithmetic is CompressForward's, in the same order.
+template <int K>
+__global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                           std::uint32_t count, LinearBuffers buf) {
+  constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
+  __shared__ double weights[kItems][K];      // mid[m] + r_m
+  __shared__ double left[kItems][K * K];     // L[p][m] e[m]
+  __shared__ double right[kItems][K * K];
+  const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  DeviceCompress op{};
+  std::uint32_t b = 0;
+  std::uint64_t mid_at = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.compresses[offset + item / d.B];
+    mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+    if (l < K) weights[t][l] = static_cast<const double *>(buf.nodes)[mid_at * K + l] + RowScale(buf, d, op.right, b, l);
+  }
+  __syncthreads();
+  double top_w = kNegInf;
+  if (active) {
+    for (int m = 0; m < K; ++m) top_w = fmax(top_w, weights[t][m]);
+    const Quad<double> lq = LoadQuad(Slot<double>(buf, d, op.left, b) + 4 * l);
+    const Quad<double> rq = LoadQuad(Slot<double>(buf, d, op.right, b) + 4 * l);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = (4 * l + i) % K;
+      const double e = top_w > kNegInf ? exp(weights[t][m] - top_w) : 0.0;
+      left[t][4 * l + i] = lq.v[i] * e;
+      right[t][4 * l + i] = rq.v[i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const int p = l / kQ, c0 = (l % kQ) * 4;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int m = 0; m < K; ++m) {
+    const double lm = left[t][p * K + m];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
+  }
+  double top = fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
+#pragma unroll
+  for (int o = 1; o < kQ; o <<= 1) top = fmax(top, __shfl_xor(top, o, kQ));
+  const bool live = top > 0.0 && top_w > kNegInf;
+  const double inverse = live ? 1.0 / top : 0.0;
+  const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+  Quad<double> q;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) q.v[c] = acc[c] * inverse;
+  StoreQuad(static_cast<double *>(buf.paths) + out_at * K * K + p * K + c0, q);
+  if (c0 == 0)
+    buf.path_scale[out_at * K + p] =
+        live ? RowScale(buf, d, op.left, b, p) + buf.node_scale[mid_at] + top_w + log(top) : kNegInf;
+}
+
 // log Z = scale(root) + log sum exp(root), in FP64 (src/inference.cc:778-784).
 template <typename T>
 __global__ void Finish(DeviceOps ops, Dims d, LinearBuffers buf, double *log_z) {
@@ -1006,6 +1069,17 @@ void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     if (LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
+    // FP64 compressions at K = 8 and 16: the team kernel (16-byte aligned slots).
+    if (primitive == 3 && l.real == Real::kF64 && (l.dims.K == 8 || l.dims.K == 16) &&
+        (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+      const unsigned grid = BlocksFor(static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4, 256);
+      if (l.dims.K == 8)
+        CompressForwardTeam<8><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
+      else
+        CompressForwardTeam<16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
+      Check(hipGetLastError(), "compression level launch");
+      return;
+    }
     if (l.real == Real::kF32) {
       THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, first, ops, buf)));
     } else {
