@@ -824,46 +824,85 @@ __global__ void SampleSeed(DeviceOps ops, Dims d, LinearBuffers buf, const U *un
   assignment[at] = s >= d.K ? 0 : s;
 }
 
-template <typename U>
-__global__ void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf,
-                                  const U *uniforms, std::uint32_t *assignment, std::uint32_t *failure) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
-  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
-  const DeviceRake op = ops.rakes[offset + tid / d.B];
-  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
-  const double *row = Slot<double>(buf, d, op.slot, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
-  const double *v = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
-  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t c) { return row[c] * exp(v[c]); },
-                                     static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.leaf]));
+// The expansions draw each child's state given its parent's. G lanes take
+// one (op, item): lane c forms the weight of state c with the same
+// expression as a one-thread scan would, and the team's first lane draws
+// with InverseCdf over the weights staged in LDS, so the draws do not depend
+// on the team layout. Work per op is then K parallel exponentials and one
+// short sequential scan.
+template <typename U, int G>
+__global__ __launch_bounds__(256) void SampleExpandRakes(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                         std::uint32_t count, LinearBuffers buf, const U *uniforms,
+                                                         std::uint32_t *assignment, std::uint32_t *failure) {
+  __shared__ double weights[256];
+  const std::uint64_t item = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+  const std::uint32_t c = threadIdx.x % G;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  double w = 0.0;
+  DeviceRake op{};
+  std::uint32_t b = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.rakes[offset + item / d.B];
+    if (c < d.K) {
+      const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+      const double *row = Slot<double>(buf, d, op.slot, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
+      const double *v = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
+      w = row[c] * exp(v[c]);
+    }
+  }
+  weights[threadIdx.x] = w;
+  __syncthreads();
+  if (!active || c != 0) return;
+  const double *team = weights + threadIdx.x;
+  const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + op.leaf;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return team[i]; }, static_cast<double>(uniforms[at]));
   if (s >= d.K) *failure = 1;
-  a[op.leaf] = s >= d.K ? 0 : s;
+  assignment[at] = s >= d.K ? 0 : s;
 }
 
-template <typename U>
-__global__ void SampleExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
-                                         LinearBuffers buf, const U *uniforms, std::uint32_t *assignment,
-                                         std::uint32_t *failure) {
-  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (tid >= static_cast<std::uint64_t>(count) * d.B) return;
-  const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
-  const DeviceCompress op = ops.compresses[offset + tid / d.B];
-  std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
-  const std::uint32_t cs = a[op.child];
-  const double *L = Slot<double>(buf, d, op.left, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
-  const double *R = Slot<double>(buf, d, op.right, b);
-  const double *mid = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.middle) * d.B + b) * d.K;
-  double top = kNegInf;
-  for (std::uint32_t m = 0; m < d.K; ++m) top = fmax(top, mid[m] + RowScale(buf, d, op.right, b, m));
-  const std::uint32_t s = InverseCdf(
-      d.K,
-      [&](std::uint32_t m) {
-        if (!(top > kNegInf)) return 0.0;
-        return L[m] * exp(mid[m] + RowScale(buf, d, op.right, b, m) - top) * R[static_cast<std::uint64_t>(m) * d.K + cs];
-      },
-      static_cast<double>(uniforms[static_cast<std::uint64_t>(b) * d.N + op.middle]));
+template <typename U, int G>
+__global__ __launch_bounds__(256) void SampleExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                                std::uint32_t count, LinearBuffers buf,
+                                                                const U *uniforms, std::uint32_t *assignment,
+                                                                std::uint32_t *failure) {
+  __shared__ double middle[256];   // mid[m] + r_m
+  __shared__ double weights[256];
+  const std::uint64_t item = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / G;
+  const std::uint32_t m = threadIdx.x % G;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  DeviceCompress op{};
+  std::uint32_t b = 0;
+  double x = kNegInf;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.compresses[offset + item / d.B];
+    if (m < d.K)
+      x = static_cast<const double *>(buf.nodes)[(static_cast<std::uint64_t>(op.middle) * d.B + b) * d.K + m] +
+          RowScale(buf, d, op.right, b, m);
+  }
+  middle[threadIdx.x] = x;
+  __syncthreads();
+  const std::uint32_t base = threadIdx.x - m;
+  double w = 0.0;
+  if (active && m < d.K) {
+    double top = kNegInf;
+    for (std::uint32_t i = 0; i < d.K; ++i) top = fmax(top, middle[base + i]);
+    if (top > kNegInf) {
+      const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+      const double *L = Slot<double>(buf, d, op.left, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
+      const double *R = Slot<double>(buf, d, op.right, b);
+      w = L[m] * exp(x - top) * R[static_cast<std::uint64_t>(m) * d.K + a[op.child]];
+    }
+  }
+  weights[threadIdx.x] = w;
+  __syncthreads();
+  if (!active || m != 0) return;
+  const double *team = weights + threadIdx.x;
+  const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + op.middle;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return team[i]; }, static_cast<double>(uniforms[at]));
   if (s >= d.K) *failure = 1;
-  a[op.middle] = s >= d.K ? 0 : s;
+  assignment[at] = s >= d.K ? 0 : s;
 }
 
 // ------------------------------------------------------------------------
@@ -910,6 +949,18 @@ void ReverseLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
     }
   }
   Check(hipGetLastError(), "reverse level launch");
+}
+
+template <typename U, int G>
+void SampleExpandT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
+                   const LinearBuffers &buf, const U *uniforms, std::uint32_t *assignment, std::uint32_t *failure) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, 256);
+  if (primitive == 0)
+    SampleExpandRakes<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms, assignment,
+                                                         failure);
+  else
+    SampleExpandCompressions<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
+                                                                assignment, failure);
 }
 
 template <int G>
@@ -1293,24 +1344,15 @@ void LaunchSampleExpandLevel(const Launch &l, std::uint32_t primitive, std::uint
                              const LinearBuffers &buf, Real in_real, const void *uniforms, std::uint32_t *assignment,
                              std::uint32_t *failure) {
   if (count == 0 || (primitive != 0 && primitive != 3)) return;
-  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, kThreads);
-  if (in_real == Real::kF32) {
-    const float *u = static_cast<const float *>(uniforms);
-    if (primitive == 0)
-      SampleExpandRakes<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment,
-                                                                   failure);
-    else
-      SampleExpandCompressions<float><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u,
-                                                                          assignment, failure);
-  } else {
-    const double *u = static_cast<const double *>(uniforms);
-    if (primitive == 0)
-      SampleExpandRakes<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u, assignment,
-                                                                    failure);
-    else
-      SampleExpandCompressions<double><<<grid, kThreads, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, u,
-                                                                           assignment, failure);
-  }
+  InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
+    if (in_real == Real::kF32) {
+      THMM_WITH_GROUP(l.dims.K, (SampleExpandT<float, G>(l, primitive, first, ops, buf,
+                                                         static_cast<const float *>(uniforms), assignment, failure)));
+    } else {
+      THMM_WITH_GROUP(l.dims.K, (SampleExpandT<double, G>(l, primitive, first, ops, buf,
+                                                          static_cast<const double *>(uniforms), assignment, failure)));
+    }
+  });
   Check(hipGetLastError(), "sample expansion launch");
 }
 
