@@ -650,6 +650,29 @@ __global__ void MaxCompress(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
 // lane owns a quad of four consecutive columns of one row, loads and stores
 // move whole quads, and choices are stored four at a time.
 
+__device__ inline double LogFactor(double x) { return x > 0.0 ? log(x) : kNegInf; }
+
+// Four consecutive log factors of a slot from element `at` (a multiple of
+// 4): edge slots from the raw potentials when given, logged here.
+__device__ inline Quad<double> MaxQuad(const MaxBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                                       std::uint64_t at) {
+  if (slot < d.E && buf.raw_edges != nullptr) {
+    const std::uint64_t i = static_cast<std::uint64_t>(slot) * d.K * d.K + at;
+    Quad<double> q;
+    if (buf.raw_edges_f32) {
+      const Quad<float> f = LoadQuad(static_cast<const float *>(buf.raw_edges) + i);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(static_cast<double>(f.v[r]));
+    } else {
+      const Quad<double> f = LoadQuad(static_cast<const double *>(buf.raw_edges) + i);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(f.v[r]);
+    }
+    return q;
+  }
+  return LoadQuad(MaxSlot(buf, d, slot, b) + at);
+}
+
 // Combines two (value, index) argmax partials of one row: the larger value,
 // and on a tie the lower index, as an ascending strict '>' scan does.
 __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double other, std::uint32_t other_arg) {
@@ -669,7 +692,7 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
   const DeviceRake op = ops.rakes[offset + item / d.B];
   const int p = l / kQ, c0 = (l % kQ) * 4;
   const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
-  const Quad<double> row = LoadQuad(MaxSlot(buf, d, op.slot, b) + p * K + c0);
+  const Quad<double> row = MaxQuad(buf, d, op.slot, b, p * K + c0);
   double best = row.v[0] + v.v[0];
   std::uint32_t arg = c0;
 #pragma unroll
@@ -707,10 +730,8 @@ __global__ __launch_bounds__(256) void MaxCompressTeam(DeviceOps ops, Dims d, st
   if (active) {
     b = static_cast<std::uint32_t>(item % d.B);
     op = ops.compresses[offset + item / d.B];
-    const double *L = MaxSlot(buf, d, op.left, b);
-    const double *R = MaxSlot(buf, d, op.right, b);
     const double *mid = buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K;
-    const Quad<double> lq = LoadQuad(L + 4 * l), rq = LoadQuad(R + 4 * l);
+    const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l), rq = MaxQuad(buf, d, op.right, b, 4 * l);
     const Quad<double> mq = LoadQuad(mid + (4 * l) % K);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {

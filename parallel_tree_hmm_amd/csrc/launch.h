@@ -129,6 +129,11 @@ struct MaxBuffers {
   double *branches;
   std::uint8_t *rake_choice;
   std::uint8_t *compress_choice;
+  // Edge potentials [E][K][K] in the workspace precision (16-byte aligned)
+  // that the K = 8 and 16 kernels log on load (x > 0 ? log x : -inf, as the
+  // log transform into `edges` does); null when edges come from `edges`.
+  const void *raw_edges;
+  int raw_edges_f32;
 };
 
 enum class Real { kF32 = 0, kF64 = 1 };

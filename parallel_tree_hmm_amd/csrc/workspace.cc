@@ -278,7 +278,7 @@ struct Workspace::Impl {
 
   MaxBuffers Max() const {
     return {m_nodes.as<double>(), m_edges.as<double>(), m_paths.as<double>(), m_branches.as<double>(),
-            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>()};
+            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>(), nullptr, 0};
   }
 };
 
@@ -1021,10 +1021,18 @@ DenseCall StagedCall(Impl &w, double *log_z) {
 
 void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &call) {
   const Launch l = w.MakeLaunch(w.real, batch, call.stream);
+  // At K = 8 and 16 the level kernels log the edge factors on load from
+  // 16-byte aligned raw potentials; the log transform then covers the nodes
+  // only (a launch without edges).
+  const void *raw_edges = src.categorical == nullptr ? call.edges : w.edges.ptr;
+  const bool edges_on_load = (w.states == 8 || w.states == 16) && w.E() != 0 &&
+                             (reinterpret_cast<std::uintptr_t>(raw_edges) & 15u) == 0;
+  Launch init = l;
+  if (edges_on_load) init.dims.E = 0;
   // Logs of the raw staged factors (bit-exact with the oracle's log of the
   // same values), internal layout.
   if (src.categorical == nullptr) {
-    LaunchMaxInit(l, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
+    LaunchMaxInit(init, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
   } else {
     LaunchInitCategorical(l, w.real, true, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
@@ -1032,12 +1040,16 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
                           w.root_potential.ptr, w.emissions.ptr, w.nodes.ptr, nullptr);
     // The raw products are internal [N][B][K]: present them to MaxInit as a
     // [1][N*B][K] "public" array so that its transpose is the identity.
-    Launch as_public = l;
+    Launch as_public = init;
     as_public.dims.B = 1;
     as_public.dims.N = static_cast<std::uint32_t>(w.N() * batch);
     LaunchMaxInit(as_public, w.real, w.nodes.ptr, w.edges.ptr, w.m_nodes.as<double>(), w.m_edges.as<double>());
   }
-  const MaxBuffers buf = w.Max();
+  MaxBuffers buf = w.Max();
+  if (edges_on_load) {
+    buf.raw_edges = raw_edges;
+    buf.raw_edges_f32 = w.real == Real::kF32 ? 1 : 0;
+  }
   for (const Level &level : w.levels) LaunchMaxLevel(l, level.primitive, level.offset, level.count, buf);
   LaunchMaxFinish(l, buf, call.log_z, call.assignment);
   for (std::size_t i = w.levels.size(); i-- > 0;)
