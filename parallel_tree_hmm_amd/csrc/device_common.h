@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "launch.h"
 
@@ -149,6 +150,35 @@ __device__ inline std::uint64_t GridThread() {
   return static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
 }
 __device__ inline std::uint64_t GridStride() { return static_cast<std::uint64_t>(gridDim.x) * blockDim.x; }
+
+// Four consecutive values of a slot from element `at` (a multiple of 4) and
+// one value at `at`: edge slots widened from FP32 when buf.edges_f32 is set.
+template <typename T>
+__device__ inline Quad<T> LoadSlotQuad(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                                       std::uint64_t at) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (slot < d.E && buf.edges_f32) {
+      const Quad<float> f =
+          LoadQuad(static_cast<const float *>(buf.edges) + static_cast<std::uint64_t>(slot) * d.K * d.K + at);
+      Quad<double> q;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) q.v[r] = static_cast<double>(f.v[r]);
+      return q;
+    }
+  }
+  return LoadQuad(Slot<T>(buf, d, slot, b) + at);
+}
+
+template <typename T>
+__device__ inline T SlotValue(const LinearBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
+                              std::uint64_t at) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (slot < d.E && buf.edges_f32)
+      return static_cast<double>(
+          static_cast<const float *>(buf.edges)[static_cast<std::uint64_t>(slot) * d.K * d.K + at]);
+  }
+  return Slot<T>(buf, d, slot, b)[at];
+}
 
 // Where a slot's adjoint goes: composed-path adjoints in the element type,
 // edge marginals in the element type or, with buf.edge_out_half, as IEEE

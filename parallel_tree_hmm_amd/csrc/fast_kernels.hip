@@ -158,14 +158,13 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
   const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
   T e[4];
   TeamExp<T, kT, kLpr>(vq, l, e);
-  const T *P = Slot<T>(buf, d, op.slot, b);
   // After the row reduction all kLpr lanes of a row hold its sum; the lane
   // with l % kLpr == j % kLpr takes the logarithm, so each row's log is
   // formed once and the team's rows are stored by one contiguous store.
   T sums[kJ];
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {
-    const Quad<T> p = LoadQuad(P + static_cast<std::uint32_t>(j * kT + l) * 4);
+    const Quad<T> p = LoadSlotQuad<T>(buf, d, op.slot, b, static_cast<std::uint32_t>(j * kT + l) * 4);
     T part = p.v[0] * e[0] + p.v[1] * e[1] + p.v[2] * e[2] + p.v[3] * e[3];
 #pragma unroll
     for (int o = kLpr / 2; o > 0; o >>= 1) part += __shfl_xor(part, o, kT);

@@ -208,8 +208,8 @@ __global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d
   double top_w = kNegInf;
   if (active) {
     for (int m = 0; m < K; ++m) top_w = fmax(top_w, weights[t][m]);
-    const Quad<double> lq = LoadQuad(Slot<double>(buf, d, op.left, b) + 4 * l);
-    const Quad<double> rq = LoadQuad(Slot<double>(buf, d, op.right, b) + 4 * l);
+    const Quad<double> lq = LoadSlotQuad<double>(buf, d, op.left, b, 4 * l);
+    const Quad<double> rq = LoadSlotQuad<double>(buf, d, op.right, b, 4 * l);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = (4 * l + i) % K;
@@ -867,9 +867,9 @@ __global__ __launch_bounds__(256) void SampleExpandRakes(DeviceOps ops, Dims d, 
     op = ops.rakes[offset + item / d.B];
     if (c < d.K) {
       const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
-      const double *row = Slot<double>(buf, d, op.slot, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
+      const double row_c = SlotValue<double>(buf, d, op.slot, b, static_cast<std::uint64_t>(a[op.parent]) * d.K + c);
       const double *v = static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * d.K;
-      w = row[c] * exp(v[c]);
+      w = row_c * exp(v[c]);
     }
   }
   weights[threadIdx.x] = w;
@@ -911,9 +911,10 @@ __global__ __launch_bounds__(256) void SampleExpandCompressions(DeviceOps ops, D
     for (std::uint32_t i = 0; i < d.K; ++i) top = fmax(top, middle[base + i]);
     if (top > kNegInf) {
       const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
-      const double *L = Slot<double>(buf, d, op.left, b) + static_cast<std::uint64_t>(a[op.parent]) * d.K;
-      const double *R = Slot<double>(buf, d, op.right, b);
-      w = L[m] * exp(x - top) * R[static_cast<std::uint64_t>(m) * d.K + a[op.child]];
+      const double left = SlotValue<double>(buf, d, op.left, b, static_cast<std::uint64_t>(a[op.parent]) * d.K + m);
+      const double right =
+          SlotValue<double>(buf, d, op.right, b, static_cast<std::uint64_t>(m) * d.K + a[op.child]);
+      w = left * exp(x - top) * right;
     }
   }
   weights[threadIdx.x] = w;
@@ -1150,6 +1151,10 @@ void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
       else
         CompressForwardTeam<16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
       Check(hipGetLastError(), "compression level launch");
+      return;
+    }
+    if (buf.edges_f32 && (primitive == 0 || primitive == 3)) {
+      RecordLaunchError(hipErrorInvalidValue, "FP32 edges in an FP64 pass need the K = 8 and 16 level kernels");
       return;
     }
     if (l.real == Real::kF32) {

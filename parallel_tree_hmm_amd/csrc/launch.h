@@ -117,6 +117,10 @@ struct LinearBuffers {
   // Edge marginals (edge_out) stored as IEEE binary16 instead of the
   // workspace element type.
   int edge_out_half;
+  // Edge slots hold FP32 values in an FP64 pass (the sampling state of an
+  // FP32 workspace): the K = 8 and 16 kernels widen them on load, which is
+  // exact. Only those kernels may run with it set.
+  int edges_f32;
 };
 
 // Log-domain max-product state (FP64):

@@ -1063,15 +1063,24 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src) {
 
 void RunSample(Impl &w, std::size_t batch, const Source &src, const DenseCall &call) {
   const Launch l = w.MakeLaunch(Real::kF64, batch, call.stream);
+  // At K = 8 and 16 the FP64 level kernels widen 16-byte aligned FP32 edges
+  // as they load them; otherwise the edges are widened into s_edges first.
+  const bool widen_on_load = w.real == Real::kF32 && (w.states == 8 || w.states == 16) &&
+                             (reinterpret_cast<std::uintptr_t>(call.edges) & 15u) == 0;
   if (w.real == Real::kF32) {
     // FP64 sampling state from the float factors.
     InitState(w, src, l, w.s_nodes.ptr, w.s_node_scale.as<double>(), call.nodes);
-    LaunchWidenF32(l, static_cast<const float *>(call.edges), w.s_edges.as<double>(), w.E() * w.KK());
+    if (!widen_on_load)
+      LaunchWidenF32(l, static_cast<const float *>(call.edges), w.s_edges.as<double>(), w.E() * w.KK());
   } else {
     InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), call.nodes);
   }
   Hip(hipMemsetAsync(call.failure, 0, sizeof(std::uint32_t), call.stream), "GPU runtime MemsetAsync sampling status");
-  const LinearBuffers buf = w.SamplingLinear(call.edges);
+  LinearBuffers buf = w.SamplingLinear(call.edges);
+  if (widen_on_load) {
+    buf.edges = call.edges;
+    buf.edges_f32 = 1;
+  }
   ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, call.log_z);
   LaunchSampleSeed(l, buf, w.real, call.uniforms, call.assignment, call.failure);
