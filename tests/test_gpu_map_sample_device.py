@@ -10,7 +10,10 @@ buffers. Checked here, for FP32 and FP64 workspaces:
   sampling, the failure flag, where the host call raises; the other trees'
   results are unaffected;
 * variates outside [0, 1), which the host call rejects, still give states
-  in range.
+  in range;
+* at K = 8 and 16, the edge logs (MAP) and FP64 widening (sampling) taken
+  on load equal, bit for bit, the whole-array passes that edges off
+  16-byte alignment take.
 """
 
 from __future__ import annotations
@@ -179,3 +182,42 @@ def test_device_calls_need_their_reservations(require_gpu):
         ws.sample_device(x.data_ptr(), e.data_ptr(), x.data_ptr(), 1, lz.data_ptr(), out.data_ptr(), out.data_ptr(), 0)
     torch.cuda.synchronize()
     ws.close()
+
+
+@pytest.mark.parametrize("k", [8, 16])
+def test_on_load_edge_transforms_equal_the_whole_array_passes(require_gpu, k):
+    """At K = 8 and 16 the MAP level kernels log FP32 edge values as they
+    load them, and the sampling kernels widen them to FP64; edges off 16-byte
+    alignment take the log / widening pass into FP64 copies instead. Both
+    give the same bits."""
+    import torch
+
+    rng = np.random.default_rng(40 + k)
+    plan = make_plan(rng)
+    batch = 2
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    d_nodes = torch.tensor(nodes, dtype=torch.float32, device="cuda")
+    store = torch.empty(edges.size + 1, dtype=torch.float32, device="cuda")
+    store[1:] = torch.tensor(edges.reshape(-1), dtype=torch.float32, device="cuda")
+    aligned = torch.tensor(edges, dtype=torch.float32, device="cuda")
+    u = torch.tensor(rng.random((batch, plan.num_nodes)), dtype=torch.float32, device="cuda")
+    results = {}
+    for name, ptr in (("on load", aligned.data_ptr()), ("pass", store[1:].data_ptr())):
+        out = []
+        ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_MAXIMUM)
+        weights = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+        states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+        ws.maximum_device(d_nodes.data_ptr(), ptr, batch, weights.data_ptr(), states.data_ptr(), 0)
+        torch.cuda.synchronize()
+        out += [weights.cpu().numpy(), states.cpu().numpy()]
+        ws.close()
+        ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_SAMPLING)
+        failure = torch.zeros(1, dtype=torch.int32, device="cuda")
+        ws.sample_device(d_nodes.data_ptr(), ptr, u.data_ptr(), batch, weights.data_ptr(), states.data_ptr(),
+                         failure.data_ptr(), 0)
+        torch.cuda.synchronize()
+        out += [weights.cpu().numpy(), states.cpu().numpy(), int(failure.item())]
+        ws.close()
+        results[name] = out
+    for a, b in zip(results["on load"], results["pass"]):
+        assert np.array_equal(a, b)
