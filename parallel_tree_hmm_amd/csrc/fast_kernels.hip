@@ -1002,17 +1002,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void C
 // and S, H, Q come out in row layout: rows reduce over 4 lanes, stores are
 // 16-byte quads. Gh/Gt pass through one LDS tile per wavefront.
 
-// Middle weights: lane l forms e[m] for m = l % K (*e_own) and gathers those
-// of m = 16J + 4g + r into e[J][r]; returns W (-inf: no live state).
+// Middle weights: lane l forms e[m] for m = l % K (*e_own), given the scale
+// of row m of the right slot, and gathers those of m = 16J + 4g + r into
+// e[J][r]; returns W (-inf: no live state).
 template <int K>
 __device__ inline double MiddleWeightsWide(const LinearBuffers &buf, const Dims &d, const DeviceCompress &op,
-                                           std::uint32_t b, int lane, float (&e)[K / 16][4], float *e_own) {
+                                           std::uint32_t b, int lane, double right_scale, float (&e)[K / 16][4],
+                                           float *e_own) {
   constexpr int NB = K / 16;
   const int g = lane / 16;
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
   const int m = lane % K;
-  const double w = static_cast<double>(static_cast<const float *>(buf.nodes)[mid_at * K + m]) +
-                   RowScale(buf, d, op.right, b, m);
+  const double w = static_cast<double>(static_cast<const float *>(buf.nodes)[mid_at * K + m]) + right_scale;
   double top = w;
 #pragma unroll
   for (int x = 1; x < K; x <<= 1) top = fmax(top, __shfl_xor(top, x, 64));
@@ -1047,6 +1048,22 @@ __device__ inline void LoadColumnBand(const float *X, int I, int i, int g, float
     }
 }
 
+// The row-shifted exponential of a log edge value, as edge staging forms it:
+// exp(x - m) in a row of maximum m, 0 in a row of -inf.
+__device__ inline float ShiftedExp(float x, float m) { return m > NegInf<float>() ? ExpNonPositive(x - m) : 0.0f; }
+
+// A compression input that is an edge read as log potentials in place.
+__device__ inline bool LogEdgeInput(const LinearBuffers &buf, const Dims &d, std::uint32_t slot) {
+  return buf.compress_log_edges != 0 && slot < d.E;
+}
+
+__device__ inline const float *CompressInput(const LinearBuffers &buf, const Dims &d, std::uint32_t slot,
+                                             std::uint32_t b) {
+  return LogEdgeInput(buf, d, slot)
+             ? static_cast<const float *>(buf.log_edges) + static_cast<std::uint64_t>(slot) * d.K * d.K
+             : Slot<float>(buf, d, slot, b);
+}
+
 // The op of a wavefront's item with its fields in scalar registers.
 __device__ inline DeviceCompress UniformOp(const DeviceOps &ops, std::uint32_t index) {
   const DeviceCompress op = ops.compresses[index];
@@ -1071,18 +1088,45 @@ template <int K>
 __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(DeviceOps ops, Dims d, std::uint32_t offset,
                                                                            std::uint32_t count, LinearBuffers buf) {
   constexpr int NB = K / 16;
-  const int lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
-  const std::uint32_t item = Uniform(blockIdx.x * kWideWaves + threadIdx.x / 64);
-  if (item >= count * d.B) return;
+  __shared__ float right_shift[kWideWaves][K];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
+  const std::uint32_t item = Uniform(blockIdx.x * kWideWaves + wave);
+  if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
   const std::uint32_t b = item % d.B;
   const DeviceCompress op = UniformOp(ops, offset + item / d.B);
-  const float *L = Slot<float>(buf, d, op.left, b);
-  const float *R = Slot<float>(buf, d, op.right, b);
-  float e[NB][4], e_own;
-  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
+  const bool log_left = LogEdgeInput(buf, d, op.left), log_right = LogEdgeInput(buf, d, op.right);
+  const float *L = CompressInput(buf, d, op.left, b);
+  const float *R = CompressInput(buf, d, op.right, b);
   float rc[NB][NB][4];  // R column layout: rc[Ic] is the band Ic of R^T
 #pragma unroll
   for (int Ic = 0; Ic < NB; ++Ic) LoadColumnBand<K, true>(R, Ic, i, g, rc[Ic]);
+  double right_scale;  // of row lane % K of R
+  if (log_right) {
+    // Row 16J + 4g + r of R lies in the 16 lanes of group g, NB values each.
+#pragma unroll
+    for (int J = 0; J < NB; ++J)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float top = rc[0][J][r];
+#pragma unroll
+        for (int Ic = 1; Ic < NB; ++Ic) top = fmaxf(top, rc[Ic][J][r]);
+#pragma unroll
+        for (int x = 1; x < 16; x <<= 1) top = fmaxf(top, __shfl_xor(top, x, 64));
+#pragma unroll
+        for (int Ic = 0; Ic < NB; ++Ic) rc[Ic][J][r] = ShiftedExp(rc[Ic][J][r], top);
+        if (i == 0) right_shift[wave][16 * J + 4 * g + r] = top;
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float top = right_shift[wave][lane % K];
+    right_scale = top > NegInf<float>() ? static_cast<double>(top) : kNegInf;
+    if (lane < K) buf.log_edge_shift[static_cast<std::uint64_t>(op.right) * K + lane] = right_scale;
+  } else {
+    right_scale = RowScale(buf, d, op.right, b, lane % K);
+  }
+  float e[NB][4], e_own;
+  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, lane, right_scale, e, &e_own);
   const std::uint64_t mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
   const double mid_scale = buf.node_scale[mid_at];
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
@@ -1091,7 +1135,24 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
   for (int Ip = 0; Ip < NB; ++Ip) {
     float lb[NB][4];
     LoadRowBand<K, true>(L, Ip, i, g, lb);
-    const double left_scale = RowScale(buf, d, op.left, b, 16 * Ip + i);
+    double left_scale;
+    if (log_left) {  // row 16Ip + i lies in lanes i, i + 16, i + 32, i + 48
+      float top = lb[0][0];
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) top = fmaxf(top, lb[J][r]);
+      top = fmaxf(top, __shfl_xor(top, 16, 64));
+      top = fmaxf(top, __shfl_xor(top, 32, 64));
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lb[J][r] = ShiftedExp(lb[J][r], top);
+      left_scale = top > NegInf<float>() ? static_cast<double>(top) : kNegInf;
+      if (g == 0) buf.log_edge_shift[static_cast<std::uint64_t>(op.left) * K + 16 * Ip + i] = left_scale;
+    } else {
+      left_scale = RowScale(buf, d, op.left, b, 16 * Ip + i);
+    }
 #pragma unroll
     for (int J = 0; J < NB; ++J)
 #pragma unroll
@@ -1130,19 +1191,25 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   constexpr int kStride = K + 1;  // LDS row stride (floats)
   __shared__ float tiles[kWideWaves][K * kStride];
   __shared__ double row_ratio[kWideWaves][K];
+  __shared__ float left_shift[kWideWaves][K];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64, i = lane % 16, g = lane / 16;
   const std::uint32_t item = Uniform(blockIdx.x * kWideWaves + wave);
   if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
   float *G = tiles[wave];
   double *gs = row_ratio[wave];
+  float *ls = left_shift[wave];
   const std::uint32_t b = item % d.B;
   const DeviceCompress op = UniformOp(ops, offset + item / d.B);
-  const float *L = Slot<float>(buf, d, op.left, b);
-  const float *R = Slot<float>(buf, d, op.right, b);
+  // Log edge inputs are shifted by the row maxima the forward pass stored
+  // as their row scales, which gives back the forward's values bit for bit.
+  const bool log_left = LogEdgeInput(buf, d, op.left), log_right = LogEdgeInput(buf, d, op.right);
+  const float *L = CompressInput(buf, d, op.left, b);
+  const float *R = CompressInput(buf, d, op.right, b);
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
   const float *adj_out = static_cast<const float *>(buf.path_adj) + out_at * K * K;
   float e[NB][4], e_own;
-  const double w_top = MiddleWeightsWide<K>(buf, d, op, b, lane, e, &e_own);
+  const double w_top =
+      MiddleWeightsWide<K>(buf, d, op, b, lane, RowScale(buf, d, op.right, b, lane % K), e, &e_own);
   const double mid_scale = buf.node_scale[static_cast<std::uint64_t>(op.middle) * d.B + b];
 
   // S is not recomputed: the forward pass stored P = S / top_p row by row
@@ -1212,7 +1279,10 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
     // are finite.
     const double g_row = top > 0.0 ? top / exp(path_scale - ((left_scale + mid_scale) + w_top)) : 0.0;
     gamma = fmax(gamma, g_row);
-    if (g == 0) gs[row] = g_row;
+    if (g == 0) {
+      gs[row] = g_row;
+      ls[row] = static_cast<float>(left_scale);  // exact: an FP32 maximum or -inf
+    }
   }
 #pragma unroll
   for (int x = 1; x < 16; x <<= 1) gamma = fmax(gamma, __shfl_xor(gamma, x, 64));
@@ -1222,7 +1292,16 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 
   float rr[NB][NB][4];  // R row layout: rr[Im] is the band Im of R
 #pragma unroll
-  for (int Im = 0; Im < NB; ++Im) LoadRowBand<K, true>(R, Im, i, g, rr[Im]);
+  for (int Im = 0; Im < NB; ++Im) {
+    LoadRowBand<K, true>(R, Im, i, g, rr[Im]);
+    if (log_right) {
+      const float top = static_cast<float>(RowScale(buf, d, op.right, b, 16 * Im + i));
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rr[Im][J][r] = ShiftedExp(rr[Im][J][r], top);
+    }
+  }
 
   // adj_L[p][m] = L'[p][m] g_p H[p][m], H^T = R Gh^T.
   const AdjointTarget<float> adj_left = SlotAdjointTarget<float>(buf, d, op.left, b);
@@ -1234,6 +1313,13 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 #pragma unroll
       for (int r = 0; r < 4; ++r) gb[J][r] = G[(16 * Ip + i) * kStride + 16 * J + 4 * g + r];
     LoadRowBand<K>(L, Ip, i, g, lb);
+    if (log_left) {
+      const float top = ls[16 * Ip + i];
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lb[J][r] = ShiftedExp(lb[J][r], top);
+    }
     const double g_row = gs[16 * Ip + i];
     // FP32 products when every row's g_p is below 1e30 (|H| <= K, L and e
     // at most 1), else FP64.
@@ -1271,6 +1357,12 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
   for (int Im = 0; Im < NB; ++Im) {
     float lc[NB][4];
     LoadColumnBand<K, true>(L, Im, i, g, lc);
+    if (log_left) {  // lc[J][r] is in row 16J + 4g + r of L
+#pragma unroll
+      for (int J = 0; J < NB; ++J)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lc[J][r] = ShiftedExp(lc[J][r], ls[16 * J + 4 * g + r]);
+    }
     // e[m] for m = 16Im + i (the row this lane holds of Q and adj_R).
     const float e_row = __shfl(e_own, 16 * Im + i, 64);
     double row_total = 0.0;

@@ -1157,6 +1157,10 @@ void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
       RecordLaunchError(hipErrorInvalidValue, "FP32 edges in an FP64 pass need the K = 8 and 16 level kernels");
       return;
     }
+    if (buf.compress_log_edges && primitive == 3) {
+      RecordLaunchError(hipErrorInvalidValue, "log edge compression inputs need the K = 32 and 64 MFMA kernels");
+      return;
+    }
     if (l.real == Real::kF32) {
       THMM_WITH_GROUP(l.dims.K, (ForwardLevelT<float, G>(l, primitive, first, ops, buf)));
     } else {
@@ -1187,6 +1191,10 @@ void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     if (LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
+    if (buf.compress_log_edges && primitive == 3) {
+      RecordLaunchError(hipErrorInvalidValue, "log edge compression inputs need the K = 32 and 64 MFMA kernels");
+      return;
+    }
     if (l.real == Real::kF32) {
       THMM_WITH_GROUP(l.dims.K, (ReverseLevelT<float, G>(l, primitive, first, ops, buf)));
     } else {

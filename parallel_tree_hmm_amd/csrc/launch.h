@@ -109,9 +109,9 @@ struct LinearBuffers {
   void *edge_out;
   // Log edge potentials [E][K][K] that the team rake kernels read in place
   // (exp of the row-shifted values on load) for edge slots; null when every
-  // edge is staged in `edges`. Compressions always read `edges`. The forward
-  // rake kernel writes each row's shift (its maximum, 0 for a row of -inf)
-  // to log_edge_shift [E][K], which the reverse reads.
+  // edge is staged in `edges`. The forward rake kernel writes each row's
+  // shift (its maximum, 0 for a row of -inf) to log_edge_shift [E][K], which
+  // the reverse reads.
   const void *log_edges;
   double *log_edge_shift;
   // Edge marginals (edge_out) stored as IEEE binary16 instead of the
@@ -121,6 +121,13 @@ struct LinearBuffers {
   // FP32 workspace): the K = 8 and 16 kernels widen them on load, which is
   // exact. Only those kernels may run with it set.
   int edges_f32;
+  // Compressions read their edge slots from log_edges too (the K = 32 and 64
+  // FP32 MFMA kernels; no other compression kernel may run with it set).
+  // The forward one writes each row's shift (its maximum, -inf for a row of
+  // -inf) to log_edge_shift, which must then alias edge_scale, and the
+  // reverse reads it back through edge_scale. Otherwise compressions read
+  // `edges`.
+  int compress_log_edges;
 };
 
 // Log-domain max-product state (FP64):

@@ -1755,6 +1755,14 @@ void Workspace::SampleDevice(const void *nodes, const void *edges, const void *u
   RunSample(w, batch, Source{}, DenseCall{nodes, edges, uniforms, log_z, states, failure, ChooseStream(w, stream)});
 }
 
+namespace {
+
+// The log-potential pair's compressions read edge slots as log potentials
+// in place (the K = 32 and 64 FP32 MFMA kernels) instead of staged.
+bool CompressionsReadLogEdges(const Impl &w) { return w.real == Real::kF32 && (w.states == 32 || w.states == 64); }
+
+}  // namespace
+
 void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
                                   void *stream) {
   Impl &w = *impl_;
@@ -1765,20 +1773,24 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   Impl::GraphKey key;
   key.args[0] = log_nodes, key.args[1] = log_edges, key.args[2] = log_z, key.batch = batch;
   // With the fused forward the team rake kernels read the log edges in
-  // place and only the edges compressions read are staged.
+  // place and only the edges compressions read are staged; at K = 32 and 64
+  // (FP32) the MFMA compressions read theirs in place as well.
   const bool in_place = w.tuning.rake_log_edges != 0 && !w.fused.empty() &&
                         static_cast<std::uint64_t>(w.max_rake_level) * batch * 64 < (1ull << 32) &&
                         (reinterpret_cast<std::uintptr_t>(log_edges) & 15u) == 0;
+  const bool compress_in_place = in_place && CompressionsReadLogEdges(w);
   key.flag = in_place ? 1 : 0;
   w.forward_log_edges = nullptr;
   RunOrReplay(w, 2, key, ChooseStream(w, stream), [&](hipStream_t s) {
     const Launch l = w.MakeLaunch(w.real, batch, s);
     LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
     LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-    if (in_place && LaunchStageLogEdges(l, log_edges, w.compress_edges.as<std::uint32_t>(), w.compress_edge_count,
-                                        w.edges.ptr, w.edge_scale.as<double>())) {
+    if (in_place && (compress_in_place ||
+                     LaunchStageLogEdges(l, log_edges, w.compress_edges.as<std::uint32_t>(), w.compress_edge_count,
+                                         w.edges.ptr, w.edge_scale.as<double>()))) {
       buf.log_edges = log_edges;
       buf.log_edge_shift = w.edge_scale.as<double>();  // rows of rake edges; staged rows keep their scales
+      buf.compress_log_edges = compress_in_place ? 1 : 0;
     }
     else
       LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
@@ -1818,6 +1830,7 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
       if (aligned(buf.edge_out) && aligned(buf.node_adj)) {
         buf.log_edges = w.forward_log_edges;
         buf.log_edge_shift = w.edge_scale.as<double>();
+        buf.compress_log_edges = CompressionsReadLogEdges(w) ? 1 : 0;
       } else
         LaunchInitEdgesFromLogs(l, w.forward_log_edges, w.edges.ptr, w.edge_scale.as<double>());
     }

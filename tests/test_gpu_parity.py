@@ -286,11 +286,13 @@ def test_unaligned_device_buffers_fall_back(require_gpu):
 @pytest.mark.parametrize("k,precision", [(8, "f32"), (16, "f32"), (32, "f32"), (64, "f32"), (16, "f64")])
 def test_log_edges_read_in_place(require_gpu, k, precision):
     """The adjoint pair over log potentials with the rake kernels reading the
-    log edges in place (tuning rake_log_edges = 1; only the edges
-    compressions read are staged) gives the same bits as staging every edge
-    first (rake_log_edges = 0), also through graph replay; with unaligned
-    gradient outputs the backward stages the remaining edges itself. Values
-    against the FP64 oracle."""
+    log edges in place (tuning rake_log_edges = 1; at K = 8 and 16 and FP64
+    only the edges compressions read are staged, at K = 32 and 64 FP32 the
+    compressions read theirs in place too) gives the same bits as staging
+    every edge first (rake_log_edges = 0), also through graph replay; with
+    unaligned gradient outputs the backward stages the edges itself. Dead
+    rows (all -inf) sit in a rake edge and in both inputs of the first
+    compression. Values against the FP64 oracle."""
     import torch
 
     rng = np.random.default_rng(k)
@@ -300,6 +302,9 @@ def test_log_edges_read_in_place(require_gpu, k, precision):
     log_nodes = torch.tensor(rng.normal(size=(batch, plan.num_nodes, k)), dtype=dtype, device="cuda")
     log_edges = torch.tensor(rng.normal(size=(plan.num_edges, k, k)), dtype=dtype, device="cuda")
     log_edges[0, 1, :] = -float("inf")  # a dead row
+    first = plan.compressions[0]  # reads two edges, not composed paths
+    log_edges[int(first["left_edge"]), 2, :] = -float("inf")
+    log_edges[int(first["right_edge"]), 3, :] = -float("inf")
     grad = torch.tensor(rng.normal(size=batch * plan.roots.size), dtype=torch.float64, device="cuda")
 
     def run(in_place, graphs=False, shift=0):
