@@ -391,11 +391,12 @@ def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / args.steps
     value = n / (ms * 1e-3)
-    # Algorithmic bytes of one step (values only, s = 4): log-edge staging
-    # (read + write), node staging, then per rake / compression the forward
-    # and reverse figures of op_bytes (kind 0 rakes, 3 compressions).
+    # Algorithmic bytes of one step (values only, s = 4): node staging (read
+    # + write), then per rake / compression the forward and reverse figures
+    # of op_bytes (kind 0 rakes, 3 compressions). Log edges are read where
+    # those figures read an edge: rakes and compressions take them in place.
     s4, rakes = 4, n - c - plan.roots.size
-    step_bytes = (2 * e * k * k + 2 * n * k) * s4 + rakes * (op_bytes(1, 0, k, s4) + op_bytes(3, 0, k, s4)) + c * (
+    step_bytes = 2 * n * k * s4 + rakes * (op_bytes(1, 0, k, s4) + op_bytes(3, 0, k, s4)) + c * (
         op_bytes(1, 3, k, s4) + op_bytes(3, 3, k, s4)
     )
     achieved = step_bytes / (ms * 1e-3) / 1e9

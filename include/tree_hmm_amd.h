@@ -458,9 +458,11 @@ thmm_status thmm_synchronize(thmm_workspace *workspace);
  * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2);
  * rake_log_edges: in thmm_logz_forward_device / _backward_device with
  * K = 8, 16, 32 or 64, the rake kernels read the log edge potentials in
- * place (1, default; only the edges compressions read are staged, and the
- * caller's log edges must stay unchanged until the backward call) or every
- * edge is staged first (0); both give the same values.
+ * place (1, default; at K = 8 and 16 and in FP64 only the edges
+ * compressions read are staged, at K = 32 and 64 in FP32 the compressions
+ * read theirs in place too; the caller's log edges must stay unchanged
+ * until the backward call) or every edge is staged first (0); both give the
+ * same values.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
  * forward program, and a new compress_depth also drops a held
