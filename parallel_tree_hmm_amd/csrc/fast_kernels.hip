@@ -187,8 +187,9 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
 
 // Rake adjoint (src/inference.cc:488-518): with t = P[p][c] exp(v[c]) and
 // m[p] = sum_c t, adj_P = adj_m[p] t / m[p] (ratio in FP64) and
-// adj_v[c] = sum_p adj_P[p][c].
-template <typename T, int K>
+// adj_v[c] = sum_p adj_P[p][c]. kLogEdges: buf.log_edges may be set (the
+// variant without it holds fewer registers).
+template <typename T, int K, bool kLogEdges>
 __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
                                                        std::uint32_t count, LinearBuffers buf) {
   using TM = Team<K>;
@@ -206,7 +207,7 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   TeamExp<T, kT, kLpr>(vq, l, e);
   // Log edges (see RakeGatherTeam): exp(x - shift) with the row shifts the
   // forward pass stored, the values the forward pass used.
-  const bool log_edge = buf.log_edges != nullptr && op.slot < d.E;
+  const bool log_edge = kLogEdges && buf.log_edges != nullptr && op.slot < d.E;
   const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
                         : Slot<T>(buf, d, op.slot, b);
   const AdjointTarget<T> adj_p = SlotAdjointTarget<T>(buf, d, op.slot, b);
@@ -800,19 +801,22 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
 #pragma unroll
     for (int r = 0; r < 4; ++r) scaled[r] = left.part.row[r] * part.e[r];
     floatx4 s = Mfma(right.part.col, scaled);  // row layout
-    floatx4 u = Mfma(scaled, right.part.col);  // column layout
     const float top = RowMax16(s);
     const bool live = top > 0.0f && w_top > kNegInf;
-    const float inverse = RowInverse(top, live, [&](float f) {
-      s *= f;
-      u *= f;
-    });
+    const float inverse = RowInverse(top, live, [&](float f) { s *= f; });
     part.top = top;  // unscaled
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      part.row[r] = s[r] * inverse;
-      part.col[r] = u[r] * __shfl(inverse, 4 * (lane / 16) + r, 64);
-    }
+    for (int r = 0; r < 4; ++r) part.row[r] = s[r] * inverse;
+    // The column layout is the row layout transposed through this split's
+    // LDS tile (Reverse reuses the tile later; one wavefront, in order).
+    float *tile = g.tiles[kMid - 1];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) tile[(lane % 16) * kPad + 4 * (lane / 16) + r] = part.row[r];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int r = 0; r < 4; ++r) part.col[r] = tile[(4 * (lane / 16) + r) * kPad + lane % 16];
     part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + LogPositive(top) : kNegInf;
   }
 
@@ -1468,8 +1472,10 @@ template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, 
                                                  const LinearBuffers &buf) {
   const unsigned block = l.tuning.rake_block;
   const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * Team<K>::kLanes, block);
-  if (reverse)
-    RakeReverseTeam<T, K><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+  if (reverse && buf.log_edges != nullptr)
+    RakeReverseTeam<T, K, true><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+  else if (reverse)
+    RakeReverseTeam<T, K, false><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
   else
     RakeForwardTeam<T, K><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
 }
