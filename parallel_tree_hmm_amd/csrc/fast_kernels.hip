@@ -1125,6 +1125,8 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressForwardMfmaWide(Devic
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     const float top = right_shift[wave][lane % K];
     right_scale = top > NegInf<float>() ? static_cast<double>(top) : kNegInf;
+    // Edges are shared by the batch: the op's items at other batch indices
+    // store the same values here, and nothing reads them in this pass.
     if (lane < K) buf.log_edge_shift[static_cast<std::uint64_t>(op.right) * K + lane] = right_scale;
   } else {
     right_scale = RowScale(buf, d, op.right, b, lane % K);

@@ -290,7 +290,8 @@ def test_log_edges_read_in_place(require_gpu, k, precision):
     only the edges compressions read are staged, at K = 32 and 64 FP32 the
     compressions read theirs in place too) gives the same bits as staging
     every edge first (rake_log_edges = 0), also through graph replay; with
-    unaligned gradient outputs the backward stages the edges itself. Dead
+    unaligned gradient outputs the backward stages the edges itself, and
+    with unaligned log edges the forward stages every edge. Dead
     rows (all -inf) sit in a rake edge and in both inputs of the first
     compression. Values against the FP64 oracle."""
     import torch
@@ -307,7 +308,14 @@ def test_log_edges_read_in_place(require_gpu, k, precision):
     log_edges[int(first["right_edge"]), 3, :] = -float("inf")
     grad = torch.tensor(rng.normal(size=batch * plan.roots.size), dtype=torch.float64, device="cuda")
 
-    def run(in_place, graphs=False, shift=0):
+    # The same log edges 4 or 8 bytes past 16-byte alignment: the forward
+    # then stages every edge.
+    edge_storage = torch.empty(log_edges.numel() + 1, dtype=dtype, device="cuda")
+    shifted_edges = edge_storage[1:]
+    shifted_edges.copy_(log_edges.reshape(-1))
+    assert shifted_edges.data_ptr() % 16 != 0
+
+    def run(in_place, graphs=False, shift=0, edges=log_edges):
         ws = thmm.Workspace()
         ws.reserve_marginals(plan, k, batch, precision=precision, device_resident=True, graphs=graphs)
         ws.set_tuning(rake_log_edges=in_place)
@@ -316,7 +324,7 @@ def test_log_edges_read_in_place(require_gpu, k, precision):
             lz = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
             gn = torch.empty(batch * plan.num_nodes * k + shift, dtype=dtype, device="cuda")[shift:]
             ge = torch.empty(batch * plan.num_edges * k * k + shift, dtype=dtype, device="cuda")[shift:]
-            ws.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), batch, lz.data_ptr(), 0)
+            ws.logz_forward_device(log_nodes.data_ptr(), edges.data_ptr(), batch, lz.data_ptr(), 0)
             ws.logz_backward_device(grad.data_ptr(), gn.data_ptr(), ge.data_ptr(), False, 0)
             torch.cuda.synchronize()
             outs.append((lz.cpu(), gn.cpu(), ge.cpu()))
@@ -325,9 +333,14 @@ def test_log_edges_read_in_place(require_gpu, k, precision):
     staged, direct, replayed = run(0), run(1), run(1, graphs=True)
     for a, b, c in zip(staged, direct, replayed):
         assert torch.equal(a, b) and torch.equal(a, c)
-    unaligned = run(1, shift=1)
-    for a, b in zip(staged, unaligned):
-        assert float((a.double() - b.double()).abs().max()) <= (1e-6 if precision == "f32" else 1e-12)
+    # Other kernels (and, for unaligned log edges, the generic staging
+    # kernel) round differently: log Z within a relative, gradients within
+    # an absolute FP32 tolerance.
+    close = 1e-6 if precision == "f32" else 1e-12
+    for other in (run(1, shift=1), run(1, edges=shifted_edges)):
+        assert float(((staged[0] - other[0]).abs() / staged[0].abs().clamp(min=1.0)).max()) <= close
+        for a, b in zip(staged[1:], other[1:]):
+            assert float((a.double() - b.double()).abs().max()) <= close
 
     nodes = np.exp(log_nodes.double().cpu().numpy())
     edges = np.exp(log_edges.double().cpu().numpy())
