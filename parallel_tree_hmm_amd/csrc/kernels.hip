@@ -679,7 +679,62 @@ __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double othe
   if (other > best || (other == best && other_arg < arg)) best = other, arg = other_arg;
 }
 
+// Linear-domain screen of one rake row of raw edge factors f against the
+// leaf's log vector v (lanes of kQ hold the row's K columns, 4 each):
+// returns the mask of this lane's columns c whose log-domain value
+// log f_c + v_c may be the row's largest, so that only those are logged.
+//
+// l_c = f_c 2^((v_c - vmax) log2 e) is formed with the hardware FP32 exp2
+// of the fractional part and an exact power of two (relative error below
+// 2^-20 where the weight is normal; columns with v_c - vmax below -762
+// get weight 0).
+// With lmax the row's largest l_c, a column below lmax (1 - 2^-12) has
+// log f_c + v_c below the winner's by more than 2^-13 in exact arithmetic,
+// which the FP64 rounding of either sum (below 2^-30 at |v| < 2^30) cannot
+// close: it neither wins nor ties. Rows where that bound does not hold
+// log every column: vmax not finite or |vmax| >= 2^30, a factor above
+// 2^128 (FP64 inputs) or not a number, or lmax < 2^-880 (a weight in the
+// subnormal range has lost precision, and with f <= 2^128 such a column is
+// below 2^-894 in exact arithmetic).
+template <int kQ>
+__device__ inline unsigned ScreenRakeRow(const double (&f)[4], const Quad<double> &v) {
+  double vmax = fmax(fmax(v.v[0], v.v[1]), fmax(v.v[2], v.v[3]));
+  // 0 <= f <= 2^128 for every column, NaN included (a failed comparison).
+  double wild = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) wild = (f[c] >= 0.0 && f[c] <= 0x1p128) ? wild : 1.0;
+#pragma unroll
+  for (int o = 1; o < kQ; o <<= 1) {
+    vmax = fmax(vmax, __shfl_xor(vmax, o, kQ));
+    wild = fmax(wild, __shfl_xor(wild, o, kQ));
+  }
+  const bool bounded = vmax > kNegInf && fabs(vmax) < 0x1p30 && wild == 0.0;
+  double lin[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double y = (v.v[c] - vmax) * 1.4426950408889634;
+    double w = 0.0;
+    if (bounded && v.v[c] > kNegInf && y > -1100.0) {
+      const double n = floor(y);
+      w = ldexp(static_cast<double>(__builtin_amdgcn_exp2f(static_cast<float>(y - n))), static_cast<int>(n));
+    }
+    lin[c] = f[c] > 0.0 ? f[c] * w : 0.0;
+  }
+  double lmax = fmax(fmax(lin[0], lin[1]), fmax(lin[2], lin[3]));
+#pragma unroll
+  for (int o = 1; o < kQ; o <<= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, kQ));
+  if (!bounded || !(lmax >= 0x1p-880)) return 0xFu;
+  const double bar = lmax * (1.0 - 0x1p-12);
+  unsigned mask = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    if (lin[c] >= bar) mask |= 1u << c;
+  return mask;
+}
+
 // One rake row per kQ = K/4 consecutive lanes; K rows per (op, item).
+// Raw edge factors are screened (ScreenRakeRow) before they are logged;
+// the values compared are the same FP64 sums as without the screen.
 template <int K>
 __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
                                                    MaxBuffers buf) {
@@ -692,13 +747,33 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
   const DeviceRake op = ops.rakes[offset + item / d.B];
   const int p = l / kQ, c0 = (l % kQ) * 4;
   const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
-  const Quad<double> row = MaxQuad(buf, d, op.slot, b, p * K + c0);
-  double best = row.v[0] + v.v[0];
+  double best = kNegInf;  // the strict '>' scan from -inf: a row of -inf keeps its first column
   std::uint32_t arg = c0;
+  if (op.slot < d.E && buf.raw_edges != nullptr) {
+    const std::uint64_t at = static_cast<std::uint64_t>(op.slot) * K * K + p * K + c0;
+    double f[4];
+    if (buf.raw_edges_f32) {
+      const Quad<float> q = LoadQuad(static_cast<const float *>(buf.raw_edges) + at);
 #pragma unroll
-  for (int c = 1; c < 4; ++c) {
-    const double candidate = row.v[c] + v.v[c];
-    if (candidate > best) best = candidate, arg = c0 + c;
+      for (int c = 0; c < 4; ++c) f[c] = static_cast<double>(q.v[c]);
+    } else {
+      const Quad<double> q = LoadQuad(static_cast<const double *>(buf.raw_edges) + at);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) f[c] = q.v[c];
+    }
+    // Ascending columns: the strict '>' keeps the lower index on a tie.
+    for (unsigned mask = ScreenRakeRow<kQ>(f, v); mask != 0; mask &= mask - 1) {
+      const int c = __builtin_ctz(mask);
+      const double candidate = LogFactor(f[c]) + v.v[c];
+      if (candidate > best) best = candidate, arg = c0 + c;
+    }
+  } else {
+    const Quad<double> row = LoadQuad(MaxSlot(buf, d, op.slot, b) + p * K + c0);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double candidate = row.v[c] + v.v[c];
+      if (candidate > best) best = candidate, arg = c0 + c;
+    }
   }
 #pragma unroll
   for (int o = 1; o < kQ; o <<= 1) {

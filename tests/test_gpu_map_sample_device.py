@@ -13,7 +13,8 @@ buffers. Checked here, for FP32 and FP64 workspaces:
   in range;
 * at K = 8 and 16, the edge logs (MAP) and FP64 widening (sampling) taken
   on load equal, bit for bit, the whole-array passes that edges off
-  16-byte alignment take.
+  16-byte alignment take, also where the MAP rake kernel's linear-domain
+  screen meets ties, zeros and extreme magnitudes.
 """
 
 from __future__ import annotations
@@ -221,3 +222,57 @@ def test_on_load_edge_transforms_equal_the_whole_array_passes(require_gpu, k):
         results[name] = out
     for a, b in zip(results["on load"], results["pass"]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("k,precision", [(8, "f32"), (16, "f32"), (16, "f64")])
+def test_screened_rake_rows_match_the_logged_rows(require_gpu, k, precision):
+    """The MAP rake kernel screens raw edge rows in the linear domain and
+    logs only the columns that can win. On inputs built to stress the screen
+    its results equal, bit for bit, those of the unscreened kernel (edges off
+    16-byte alignment are logged whole first), and its states the oracle's.
+    The inputs hold exact ties (duplicated states and columns), zeros,
+    subnormal and near-maximal FP32 factors, leaf vectors spanning thousands
+    of nats (a star's centre raked after absorbing extreme leaves), and, for
+    FP64, factors above 2^128 (rows logged whole)."""
+    import torch
+
+    rng = np.random.default_rng(60 + k + (precision == "f64"))
+    plan = thmm.Plan(forest([random_recursive(300, rng), caterpillar(201), star(120), path(40)]))
+    batch = 2
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    n_tie = rng.random(plan.num_nodes) < 0.3
+    nodes[:, n_tie, 1] = nodes[:, n_tie, 0]
+    wide = rng.random(plan.num_nodes) < 0.2
+    nodes[:, wide, :] *= np.exp(rng.choice([-80.0, 80.0], size=(batch, int(wide.sum()), k)))
+    e_tie = rng.random(plan.num_edges) < 0.3
+    edges[e_tie, :, 1] = edges[e_tie, :, 0]
+    picks = rng.random(edges.shape)
+    edges[picks < 0.02] = 1e-40 if precision == "f32" else 1e-300
+    edges[(picks >= 0.02) & (picks < 0.04)] = 3e38 if precision == "f32" else 1e300
+    if precision == "f64":
+        edges[(picks >= 0.04) & (picks < 0.05)] = 1e45
+    nodes, edges = as_precision(nodes, precision), as_precision(edges, precision)
+    assert np.all(np.isfinite(nodes)) and np.all(np.isfinite(edges))
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    d_nodes = torch.tensor(nodes, dtype=dtype, device="cuda")
+    aligned = torch.tensor(edges, dtype=dtype, device="cuda")
+    store = torch.empty(edges.size + 1, dtype=dtype, device="cuda")
+    store[1:] = aligned.reshape(-1)
+    results = {}
+    for name, ptr in (("screened", aligned.data_ptr()), ("logged", store[1:].data_ptr())):
+        ws = reserve_device(plan, k, batch, precision, _abi.THMM_RESERVE_MAXIMUM)
+        weights = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+        states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+        ws.maximum_device(d_nodes.data_ptr(), ptr, batch, weights.data_ptr(), states.data_ptr(), 0)
+        torch.cuda.synchronize()
+        results[name] = (weights.cpu().numpy(), states.cpu().numpy().view(np.uint32))
+        ws.close()
+    assert np.array_equal(results["screened"][0], results["logged"][0])
+    assert np.array_equal(results["screened"][1], results["logged"][1])
+    got_weights = results["screened"][0].reshape(batch, -1)
+    for b in range(batch):
+        lw, expected = oracle.maximum(plan, k, nodes[b].astype(np.float64), edges.astype(np.float64))
+        assert np.array_equal(results["screened"][1][b], expected)
+        finite = np.isfinite(lw)
+        assert np.array_equal(np.isfinite(got_weights[b]), finite)
+        assert np.all(np.abs(got_weights[b][finite] - lw[finite]) <= 1e-12 * np.maximum(1, np.abs(lw[finite])))
