@@ -957,6 +957,42 @@ __global__ __launch_bounds__(256) void SampleExpandRakes(DeviceOps ops, Dims d, 
   assignment[at] = s >= d.K ? 0 : s;
 }
 
+// K = 8 and 16 with 16-byte aligned slots: K/4 lanes per op, one quad of
+// columns each, so a wavefront holds four times as many ops (their chains
+// of dependent loads: op, parent state, row) as with a lane per column. The
+// weights are the same expression and the draw the same scan.
+template <typename U, int K>
+__global__ __launch_bounds__(256) void SampleExpandRakesQuad(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                             std::uint32_t count, LinearBuffers buf,
+                                                             const U *uniforms, std::uint32_t *assignment,
+                                                             std::uint32_t *failure) {
+  constexpr int kT = K / 4;
+  __shared__ double weights[256 * 4];
+  const std::uint64_t item = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kT;
+  const int q = static_cast<int>(threadIdx.x % kT);
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  double *team = weights + (threadIdx.x - q) * 4;  // the op's K weights
+  DeviceRake op{};
+  std::uint32_t b = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.rakes[offset + item / d.B];
+    const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+    const Quad<double> row =
+        LoadSlotQuad<double>(buf, d, op.slot, b, static_cast<std::uint64_t>(a[op.parent]) * K + 4 * q);
+    const Quad<double> v =
+        LoadQuad(static_cast<const double *>(buf.nodes) + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + 4 * q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) team[4 * q + r] = row.v[r] * exp(v.v[r]);
+  }
+  __syncthreads();
+  if (!active || q != 0) return;
+  const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + op.leaf;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return team[i]; }, static_cast<double>(uniforms[at]));
+  if (s >= d.K) *failure = 1;
+  assignment[at] = s >= d.K ? 0 : s;
+}
+
 template <typename U, int G>
 __global__ __launch_bounds__(256) void SampleExpandCompressions(DeviceOps ops, Dims d, std::uint32_t offset,
                                                                 std::uint32_t count, LinearBuffers buf,
@@ -996,6 +1032,56 @@ __global__ __launch_bounds__(256) void SampleExpandCompressions(DeviceOps ops, D
   __syncthreads();
   if (!active || m != 0) return;
   const double *team = weights + threadIdx.x;
+  const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + op.middle;
+  const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return team[i]; }, static_cast<double>(uniforms[at]));
+  if (s >= d.K) *failure = 1;
+  assignment[at] = s >= d.K ? 0 : s;
+}
+
+// The compression counterpart of SampleExpandRakesQuad: K/4 lanes per op,
+// four middle states each; the same weights (left * exp(x - top) * right)
+// and the same scan as SampleExpandCompressions.
+template <typename U, int K>
+__global__ __launch_bounds__(256) void SampleExpandCompressionsQuad(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                                    std::uint32_t count, LinearBuffers buf,
+                                                                    const U *uniforms, std::uint32_t *assignment,
+                                                                    std::uint32_t *failure) {
+  constexpr int kT = K / 4;
+  __shared__ double weights[256 * 4];
+  const std::uint64_t item = (static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kT;
+  const int q = static_cast<int>(threadIdx.x % kT);
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;  // the same for a whole team
+  double *team = weights + (threadIdx.x - q) * 4;
+  DeviceCompress op{};
+  std::uint32_t b = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.compresses[offset + item / d.B];
+    const Quad<double> mid = LoadQuad(static_cast<const double *>(buf.nodes) +
+                                      (static_cast<std::uint64_t>(op.middle) * d.B + b) * K + 4 * q);
+    double x[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = mid.v[r] + RowScale(buf, d, op.right, b, 4 * q + r);
+    double top = fmax(fmax(x[0], x[1]), fmax(x[2], x[3]));
+#pragma unroll
+    for (int o = 1; o < kT; o <<= 1) top = fmax(top, __shfl_xor(top, o, kT));
+    double w[4] = {0.0, 0.0, 0.0, 0.0};
+    if (top > kNegInf) {
+      const std::uint32_t *a = assignment + static_cast<std::uint64_t>(b) * d.N;
+      const Quad<double> left =
+          LoadSlotQuad<double>(buf, d, op.left, b, static_cast<std::uint64_t>(a[op.parent]) * K + 4 * q);
+      const std::uint32_t child = a[op.child];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const double right = SlotValue<double>(buf, d, op.right, b, static_cast<std::uint64_t>(4 * q + r) * K + child);
+        w[r] = left.v[r] * exp(x[r] - top) * right;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) team[4 * q + r] = w[r];
+  }
+  __syncthreads();
+  if (!active || q != 0) return;
   const std::uint64_t at = static_cast<std::uint64_t>(b) * d.N + op.middle;
   const std::uint32_t s = InverseCdf(d.K, [&](std::uint32_t i) { return team[i]; }, static_cast<double>(uniforms[at]));
   if (s >= d.K) *failure = 1;
@@ -1052,10 +1138,27 @@ template <typename U, int G>
 void SampleExpandT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                    const LinearBuffers &buf, const U *uniforms, std::uint32_t *assignment, std::uint32_t *failure) {
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, 256);
-  if (primitive == 0)
+  const std::uint32_t K = l.dims.K;
+  if (primitive == 0 && (K == 8 || K == 16) && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+    const unsigned quad_grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * (K / 4), 256);
+    if (K == 8)
+      SampleExpandRakesQuad<U, 8><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
+                                                                     assignment, failure);
+    else
+      SampleExpandRakesQuad<U, 16><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
+                                                                      assignment, failure);
+  } else if (primitive == 0) {
     SampleExpandRakes<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms, assignment,
                                                          failure);
-  else
+  } else if ((K == 8 || K == 16) && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+    const unsigned quad_grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * (K / 4), 256);
+    if (K == 8)
+      SampleExpandCompressionsQuad<U, 8><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf,
+                                                                            uniforms, assignment, failure);
+    else
+      SampleExpandCompressionsQuad<U, 16><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf,
+                                                                             uniforms, assignment, failure);
+  } else
     SampleExpandCompressions<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
                                                                 assignment, failure);
 }
