@@ -1540,6 +1540,15 @@ void LaunchMaxExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_
   Check(hipGetLastError(), "MAP expansion launch");
 }
 
+namespace {
+__global__ void ClearWord(std::uint32_t *word) { *word = 0; }
+}  // namespace
+
+void LaunchClearWord(const Launch &l, std::uint32_t *word) {
+  ClearWord<<<1, 1, 0, StreamOf(l)>>>(word);
+  Check(hipGetLastError(), "flag clear launch");
+}
+
 void LaunchSampleSeed(const Launch &l, const LinearBuffers &buf, Real in_real, const void *uniforms,
                       std::uint32_t *assignment, std::uint32_t *failure) {
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.roots) * l.dims.B, kThreads);

@@ -273,6 +273,9 @@ void LaunchMaxExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_
 // ---- posterior sampling (FP64 scaled state) ----
 // uniforms [B][N] (element type in_real), assignment [B][N]. A root with no
 // supported state sets *failure.
+// Sets one device word to zero with a kernel (not a memset node when a
+// caller captures the stream into a graph).
+void LaunchClearWord(const Launch &l, std::uint32_t *word);
 void LaunchSampleSeed(const Launch &l, const LinearBuffers &buf, Real in_real,
                       const void *uniforms, std::uint32_t *assignment, std::uint32_t *failure);
 void LaunchSampleExpandLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,

@@ -1075,7 +1075,7 @@ void RunSample(Impl &w, std::size_t batch, const Source &src, const DenseCall &c
   } else {
     InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), call.nodes);
   }
-  Hip(hipMemsetAsync(call.failure, 0, sizeof(std::uint32_t), call.stream), "GPU runtime MemsetAsync sampling status");
+  LaunchClearWord(l, call.failure);
   LinearBuffers buf = w.SamplingLinear(call.edges);
   if (widen_on_load) {
     buf.edges = call.edges;

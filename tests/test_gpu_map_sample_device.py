@@ -14,7 +14,9 @@ buffers. Checked here, for FP32 and FP64 workspaces:
 * at K = 8 and 16, the edge logs (MAP) and FP64 widening (sampling) taken
   on load equal, bit for bit, the whole-array passes that edges off
   16-byte alignment take, also where the MAP rake kernel's linear-domain
-  screen meets ties, zeros and extreme magnitudes.
+  screen meets ties, zeros and extreme magnitudes;
+* a caller's graph capture of the sampling call replays to the eager
+  results (the failure flag is cleared by a kernel, not a memset node).
 """
 
 from __future__ import annotations
@@ -276,3 +278,51 @@ def test_screened_rake_rows_match_the_logged_rows(require_gpu, k, precision):
         finite = np.isfinite(lw)
         assert np.array_equal(np.isfinite(got_weights[b]), finite)
         assert np.all(np.abs(got_weights[b][finite] - lw[finite]) <= 1e-12 * np.maximum(1, np.abs(lw[finite])))
+
+
+def test_sample_device_replays_in_a_caller_graph(require_gpu):
+    """A caller may capture thmm_sample_device on its stream into a graph of
+    its own. The call clears the failure flag with a kernel (no memset node
+    in the caller's graph), so every replay, after the flag, states and log Z
+    are overwritten eagerly, gives the eager call's results and a cleared
+    flag."""
+    import torch
+
+    rng = np.random.default_rng(71)
+    plan = make_plan(rng)
+    k, batch = 16, 2
+    nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
+    d_nodes, d_edges = device_inputs(torch, nodes, edges, "f32")
+    u = torch.tensor(rng.random((batch, plan.num_nodes)), dtype=torch.float32, device="cuda")
+    ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_SAMPLING)
+    log_z = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+    states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
+    failure = torch.ones(1, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+
+    def call():
+        ws.sample_device(d_nodes.data_ptr(), d_edges.data_ptr(), u.data_ptr(), batch, log_z.data_ptr(),
+                         states.data_ptr(), failure.data_ptr(), stream.cuda_stream)
+
+    with torch.cuda.stream(stream):
+        call()
+    torch.cuda.synchronize()
+    want = (log_z.cpu().numpy().copy(), states.cpu().numpy().copy())
+    assert int(failure.item()) == 0
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=stream):
+        call()
+    torch.cuda.synchronize()
+    for replay in range(3):
+        failure.fill_(1)
+        states.fill_(-1)
+        log_z.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert int(failure.item()) == 0, replay
+        assert np.array_equal(log_z.cpu().numpy(), want[0]), replay
+        assert np.array_equal(states.cpu().numpy(), want[1]), replay
+    del graph
+    ws.close()
