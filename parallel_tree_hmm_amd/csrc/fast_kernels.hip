@@ -298,7 +298,10 @@ __device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int
   if (lane == 0) buf.node_scale[at] = live ? top : kNegInf;
 }
 
-template <typename T, int K>
+// kLogEdges: buf.log_edges may be set (the variant without it holds fewer
+// registers). FP32 edge slots of an FP64 pass (buf.edges_f32) are widened
+// as they load, which is exact.
+template <typename T, int K, bool kLogEdges>
 __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::uint32_t offset, std::uint32_t count,
                                                       LinearBuffers buf) {
   using TM = Team<K>;
@@ -322,12 +325,24 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
     if (i + 1 < chunk.count) next = g.rakes[chunk.first + i + 1];
     const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
     const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
-    const bool log_edge = buf.log_edges != nullptr && op.slot < d.E;  // the same for the whole team
+    const bool log_edge = kLogEdges && buf.log_edges != nullptr && op.slot < d.E;  // the same for the whole team
     const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
                           : Slot<T>(buf, d, op.slot, b);
     Quad<T> pq[kJ];
+    bool widen = false;
+    if constexpr (std::is_same<T, double>::value) widen = buf.edges_f32 != 0 && op.slot < d.E;
+    if (widen) {
+      const float *F = static_cast<const float *>(buf.edges) + static_cast<std::uint64_t>(op.slot) * K * K;
 #pragma unroll
-    for (int j = 0; j < kJ; ++j) pq[j] = LoadQuadStream(P + static_cast<std::uint32_t>(j * kT + l) * 4);
+      for (int j = 0; j < kJ; ++j) {
+        const Quad<float> f = LoadQuadStream(F + static_cast<std::uint32_t>(j * kT + l) * 4);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) pq[j].v[i] = static_cast<T>(f.v[i]);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) pq[j] = LoadQuadStream(P + static_cast<std::uint32_t>(j * kT + l) * 4);
+    }
     const double leaf_scale = buf.node_scale[leaf_at];
     double row_scale;
     if (log_edge) {
@@ -1500,7 +1515,13 @@ void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offse
   const unsigned block = l.tuning.rake_block;
   const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * (K >= 32 ? 64 : 16), block);
   const hipStream_t s = StreamOf(l);
-#define THMM_GATHER(T, KK) RakeGatherTeam<T, KK><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf)
+#define THMM_GATHER(T, KK)                                                                  \
+  do {                                                                                      \
+    if (buf.log_edges != nullptr)                                                           \
+      RakeGatherTeam<T, KK, true><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf);    \
+    else                                                                                    \
+      RakeGatherTeam<T, KK, false><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf);   \
+  } while (0)
   if (l.real == Real::kF32) {
     if (K == 8) THMM_GATHER(float, 8);
     if (K == 16) THMM_GATHER(float, 16);

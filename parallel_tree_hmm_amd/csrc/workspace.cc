@@ -125,6 +125,9 @@ struct Workspace::Impl {
     std::uint32_t work;                 // rakes (kind 0), partials (kind 1), compressions (kind 3), K-vectors (6)
   };
   std::vector<FusedStep> fused;
+  // The same program with every compression level plain (kind 3): the FP64
+  // sampling forward, which has no grouped compressions.
+  std::vector<FusedStep> fused_plain;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
   // Grouped compressions (K = 16 FP32 with the fused forward): the level of
   // each plan batch that holds compressions (count 0: none rooted there),
@@ -228,6 +231,7 @@ struct Workspace::Impl {
     levels.clear();
     rake_adjoint_from_parent = false;
     fused.clear();
+    fused_plain.clear();
     chain_levels.clear();
     grouped_levels.clear();
     compress_edge_count = max_rake_level = 0;
@@ -325,7 +329,7 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
           absorb_round[j] = static_cast<std::uint32_t>(r);
 
   const std::uint32_t chunk_size = w.tuning.gather_chunk;
-  std::vector<Impl::FusedStep> steps;
+  std::vector<Impl::FusedStep> steps, plain;
   std::vector<DeviceRake> sorted;
   std::vector<DeviceGatherChunk> chunks;
   std::vector<DeviceGatherSegment> segments;
@@ -362,9 +366,11 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
       if (!direct) open.push_back({parent, first_partial, partials - first_partial});
       i = end;
     }
-    if (chunks.size() > chunk_offset)
+    if (chunks.size() > chunk_offset) {
       steps.push_back({0, chunk_offset, static_cast<std::uint32_t>(chunks.size() - chunk_offset),
                        static_cast<std::uint32_t>(keyed.size())});
+      plain.push_back(steps.back());
+    }
     // Reduction levels: runs of up to kGatherFanIn partials per team; a
     // parent whose partials fit one run is absorbed at that level.
     while (!open.empty()) {
@@ -383,10 +389,12 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
         next.push_back({parent, next_first, partials - next_first});
       }
       steps.push_back({1, segment_offset, static_cast<std::uint32_t>(segments.size() - segment_offset), parts});
+      plain.push_back(steps.back());
       open = std::move(next);
     }
     for (std::size_t i = rounds[r].first; i < rounds[r].second; ++i) {
       if (plan.batches[i].primitive != THMM_COMPRESSION) continue;
+      plain.push_back({3, plan.batches[i].offset, plan.batches[i].count, plan.batches[i].count});
       if (w.chain_levels.empty()) {
         steps.push_back({3, plan.batches[i].offset, plan.batches[i].count, plan.batches[i].count});
       } else if (w.chain_levels[i].count != 0) {
@@ -407,6 +415,7 @@ void BuildFusedForward(Impl &w, const Plan &plan, const std::vector<DeviceRake> 
   // The uploads read the local vectors above.
   Hip(hipStreamSynchronize(w.stream), "fused topology upload");
   w.fused = std::move(steps);
+  w.fused_plain = std::move(plain);
 }
 
 // Groups the renamed compressions for the K = 16 FP32 MFMA path. A
@@ -536,6 +545,7 @@ void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &c
 void BuildPrograms(Impl &w, const Plan &plan, const std::vector<DeviceRake> &rakes,
                    const std::vector<DeviceCompress> &compresses, std::size_t batch) {
   w.fused.clear();
+  w.fused_plain.clear();
   BuildChains(w, plan, compresses, batch);
   BuildFusedForward(w, plan, rakes, batch);
   if (w.fused.empty()) {
@@ -823,13 +833,10 @@ bool UsesFusedForward(const Impl &w, const LinearBuffers &buf) {
 
 // The sum-product forward pass: the fused program when the workspace has
 // one, else the plan's levels.
-void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
-  if (!UsesFusedForward(w, buf)) {
-    ForwardLevels(l, w.levels, buf, profiler);
-    return;
-  }
+void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, const Launch &l,
+                   const LinearBuffers &buf, Profiler *profiler = nullptr) {
   const GatherOps g = w.Gather();
-  for (const Impl::FusedStep &step : w.fused) {
+  for (const Impl::FusedStep &step : steps) {
     // Profiled as primitive 4 (fused rake chunks, count = rakes), 5 (partial
     // absorptions, count = partials), a compression level, or a grouped
     // compression level (kChainPrimitive, count = K-vectors moved per item).
@@ -843,6 +850,13 @@ void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf,
       LaunchForwardLevel(l, step.kind, step.offset, step.count, buf);
     if (profiler) profiler->End();
   }
+}
+
+void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
+  if (UsesFusedForward(w, buf))
+    RunFusedSteps(w, w.fused, l, buf, profiler);
+  else
+    ForwardLevels(l, w.levels, buf, profiler);
 }
 
 // The levels whose reverse matches the forward pass SumProductForward ran
@@ -1081,7 +1095,12 @@ void RunSample(Impl &w, std::size_t batch, const Source &src, const DenseCall &c
     buf.edges = call.edges;
     buf.edges_f32 = 1;
   }
-  ForwardLevels(l, w.levels, buf);
+  // The fused program with plain compression levels; its gather kernels
+  // widen FP32 edges on load too.
+  if (UsesFusedForward(w, buf))
+    RunFusedSteps(w, w.fused_plain, l, buf);
+  else
+    ForwardLevels(l, w.levels, buf);
   LaunchFinish(l, buf, call.log_z);
   LaunchSampleSeed(l, buf, w.real, call.uniforms, call.assignment, call.failure);
   for (std::size_t i = w.levels.size(); i-- > 0;)
