@@ -15,8 +15,8 @@ buffers. Checked here, for FP32 and FP64 workspaces:
   on load equal, bit for bit, the whole-array passes that edges off
   16-byte alignment take, also where the MAP rake kernel's linear-domain
   screen meets ties, zeros and extreme magnitudes;
-* a caller's graph capture of the sampling call replays to the eager
-  results (the failure flag is cleared by a kernel, not a memset node).
+* a caller's graph capture of either call replays to the eager results
+  (sampling clears its failure flag with a kernel, not a memset node).
 """
 
 from __future__ import annotations
@@ -280,12 +280,13 @@ def test_screened_rake_rows_match_the_logged_rows(require_gpu, k, precision):
         assert np.all(np.abs(got_weights[b][finite] - lw[finite]) <= 1e-12 * np.maximum(1, np.abs(lw[finite])))
 
 
-def test_sample_device_replays_in_a_caller_graph(require_gpu):
-    """A caller may capture thmm_sample_device on its stream into a graph of
-    its own. The call clears the failure flag with a kernel (no memset node
-    in the caller's graph), so every replay, after the flag, states and log Z
-    are overwritten eagerly, gives the eager call's results and a cleared
-    flag."""
+@pytest.mark.parametrize("kind", ["map", "sample"])
+def test_device_calls_replay_in_a_caller_graph(require_gpu, kind):
+    """A caller may capture thmm_maximum_device or thmm_sample_device on its
+    stream into a graph of its own: both enqueue kernels only (sampling
+    clears its failure flag with a kernel, not a memset node). Every replay,
+    after the outputs are overwritten eagerly, gives the eager call's
+    results (and a cleared flag)."""
     import torch
 
     rng = np.random.default_rng(71)
@@ -294,16 +295,22 @@ def test_sample_device_replays_in_a_caller_graph(require_gpu):
     nodes, edges = random_potentials(rng, batch, plan.num_nodes, plan.num_edges, k, zero_fraction=0.1, sigma=2.0)
     d_nodes, d_edges = device_inputs(torch, nodes, edges, "f32")
     u = torch.tensor(rng.random((batch, plan.num_nodes)), dtype=torch.float32, device="cuda")
-    ws = reserve_device(plan, k, batch, "f32", _abi.THMM_RESERVE_SAMPLING)
+    flag = _abi.THMM_RESERVE_MAXIMUM if kind == "map" else _abi.THMM_RESERVE_SAMPLING
+    ws = reserve_device(plan, k, batch, "f32", flag)
     log_z = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
     states = torch.empty((batch, plan.num_nodes), dtype=torch.int32, device="cuda")
-    failure = torch.ones(1, dtype=torch.int32, device="cuda")
+    failure = torch.ones(1, dtype=torch.int32, device="cuda") if kind == "sample" else torch.zeros(1, dtype=torch.int32,
+                                                                                                   device="cuda")
     stream = torch.cuda.Stream()
     stream.wait_stream(torch.cuda.current_stream())
 
     def call():
-        ws.sample_device(d_nodes.data_ptr(), d_edges.data_ptr(), u.data_ptr(), batch, log_z.data_ptr(),
-                         states.data_ptr(), failure.data_ptr(), stream.cuda_stream)
+        if kind == "map":
+            ws.maximum_device(d_nodes.data_ptr(), d_edges.data_ptr(), batch, log_z.data_ptr(), states.data_ptr(),
+                              stream.cuda_stream)
+        else:
+            ws.sample_device(d_nodes.data_ptr(), d_edges.data_ptr(), u.data_ptr(), batch, log_z.data_ptr(),
+                             states.data_ptr(), failure.data_ptr(), stream.cuda_stream)
 
     with torch.cuda.stream(stream):
         call()
@@ -315,7 +322,7 @@ def test_sample_device_replays_in_a_caller_graph(require_gpu):
         call()
     torch.cuda.synchronize()
     for replay in range(3):
-        failure.fill_(1)
+        failure.fill_(1 if kind == "sample" else 0)
         states.fill_(-1)
         log_z.fill_(float("nan"))
         torch.cuda.synchronize()
