@@ -684,47 +684,47 @@ __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double othe
 // returns the mask of this lane's columns c whose log-domain value
 // log f_c + v_c may be the row's largest, so that only those are logged.
 //
-// l_c = f_c 2^((v_c - vmax) log2 e) is formed with the hardware FP32 exp2
-// of the fractional part and an exact power of two (relative error below
-// 2^-20 where the weight is normal; columns with v_c - vmax below -762
-// get weight 0).
-// With lmax the row's largest l_c, a column below lmax (1 - 2^-12) has
-// log f_c + v_c below the winner's by more than 2^-13 in exact arithmetic,
-// which the FP64 rounding of either sum (below 2^-30 at |v| < 2^30) cannot
-// close: it neither wins nor ties. Rows where that bound does not hold
-// log every column: vmax not finite or |vmax| >= 2^30, a factor above
-// 2^128 (FP64 inputs) or not a number, or lmax < 2^-880 (a weight in the
-// subnormal range has lost precision, and with f <= 2^128 such a column is
-// below 2^-894 in exact arithmetic).
+// l_c = f_c 2^y_c, y_c = (v_c - vmax) log2 e, is formed in FP32: y_c from
+// the FP64 difference rounded to FP32 (absolute error below 2^-15 where
+// y_c >= -254, the only columns with l_c >= 2^-126 when f <= 2^126), then
+// ldexp(f_c exp2(y_c - n), n) with n = floor(y_c) and the hardware exp2.
+// Normal values carry a relative error below 2^-15, and no weight
+// underflows on its own: an l_c flushed to 0 is below 2^-126 in exact
+// arithmetic. With lmax the row's largest l_c >= 2^-100, a column below
+// lmax (1 - 2^-12) has log f_c + v_c below the winner's by more than 2^-13
+// in exact arithmetic. The FP64 rounding of either sum (below 2^-21 at
+// |v| < 2^30) cannot close that gap: it neither wins nor ties. Rows outside
+// these conditions log every column: vmax not finite or |vmax| >= 2^30,
+// a factor above 2^126 or not a number, or lmax < 2^-100.
 template <int kQ>
 __device__ inline unsigned ScreenRakeRow(const double (&f)[4], const Quad<double> &v) {
   double vmax = fmax(fmax(v.v[0], v.v[1]), fmax(v.v[2], v.v[3]));
-  // 0 <= f <= 2^128 for every column, NaN included (a failed comparison).
-  double wild = 0.0;
+  // 0 <= f <= 2^126 for every column, NaN included (a failed comparison).
+  float wild = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) wild = (f[c] >= 0.0 && f[c] <= 0x1p128) ? wild : 1.0;
+  for (int c = 0; c < 4; ++c) wild = (f[c] >= 0.0 && f[c] <= 0x1p126) ? wild : 1.0f;
 #pragma unroll
   for (int o = 1; o < kQ; o <<= 1) {
     vmax = fmax(vmax, __shfl_xor(vmax, o, kQ));
-    wild = fmax(wild, __shfl_xor(wild, o, kQ));
+    wild = fmaxf(wild, __shfl_xor(wild, o, kQ));
   }
-  const bool bounded = vmax > kNegInf && fabs(vmax) < 0x1p30 && wild == 0.0;
-  double lin[4];
+  const bool bounded = vmax > kNegInf && fabs(vmax) < 0x1p30 && wild == 0.0f;
+  float lin[4];
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
-    const double y = (v.v[c] - vmax) * 1.4426950408889634;
-    double w = 0.0;
-    if (bounded && v.v[c] > kNegInf && y > -1100.0) {
-      const double n = floor(y);
-      w = ldexp(static_cast<double>(__builtin_amdgcn_exp2f(static_cast<float>(y - n))), static_cast<int>(n));
+    const float y = static_cast<float>(v.v[c] - vmax) * 1.44269504f;
+    float value = 0.0f;
+    if (bounded && y > -300.0f) {  // false for -inf and NaN
+      const float n = floorf(y);
+      value = ldexpf(static_cast<float>(f[c]) * __builtin_amdgcn_exp2f(y - n), static_cast<int>(n));
     }
-    lin[c] = f[c] > 0.0 ? f[c] * w : 0.0;
+    lin[c] = value;
   }
-  double lmax = fmax(fmax(lin[0], lin[1]), fmax(lin[2], lin[3]));
+  float lmax = fmaxf(fmaxf(lin[0], lin[1]), fmaxf(lin[2], lin[3]));
 #pragma unroll
-  for (int o = 1; o < kQ; o <<= 1) lmax = fmax(lmax, __shfl_xor(lmax, o, kQ));
-  if (!bounded || !(lmax >= 0x1p-880)) return 0xFu;
-  const double bar = lmax * (1.0 - 0x1p-12);
+  for (int o = 1; o < kQ; o <<= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, o, kQ));
+  if (!bounded || !(lmax >= 0x1p-100f)) return 0xFu;
+  const float bar = lmax * (1.0f - 0x1p-12f);
   unsigned mask = 0;
 #pragma unroll
   for (int c = 0; c < 4; ++c)
