@@ -1095,9 +1095,11 @@ void RunSample(Impl &w, std::size_t batch, const Source &src, const DenseCall &c
     buf.edges = call.edges;
     buf.edges_f32 = 1;
   }
-  // The fused program with plain compression levels; its gather kernels
-  // widen FP32 edges on load too.
-  if (UsesFusedForward(w, buf))
+  // The fused program with plain compression levels (its gather kernels
+  // widen FP32 edges on load too) up to K = 32. At K = 64 the FP64 gather
+  // holds 216 registers (two wavefronts per SIMD) and measured no faster
+  // than the plan's levels.
+  if (w.states <= 32 && UsesFusedForward(w, buf))
     RunFusedSteps(w, w.fused_plain, l, buf);
   else
     ForwardLevels(l, w.levels, buf);
