@@ -1139,24 +1139,31 @@ void SampleExpandT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
                    const LinearBuffers &buf, const U *uniforms, std::uint32_t *assignment, std::uint32_t *failure) {
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, 256);
   const std::uint32_t K = l.dims.K;
-  if (primitive == 0 && (K == 8 || K == 16) && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+  const bool quads = (K == 8 || K == 16 || K == 32) && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0;
+  if (primitive == 0 && quads) {
     const unsigned quad_grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * (K / 4), 256);
     if (K == 8)
       SampleExpandRakesQuad<U, 8><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
                                                                      assignment, failure);
-    else
+    else if (K == 16)
       SampleExpandRakesQuad<U, 16><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
+                                                                      assignment, failure);
+    else
+      SampleExpandRakesQuad<U, 32><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
                                                                       assignment, failure);
   } else if (primitive == 0) {
     SampleExpandRakes<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms, assignment,
                                                          failure);
-  } else if ((K == 8 || K == 16) && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+  } else if (quads) {
     const unsigned quad_grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * (K / 4), 256);
     if (K == 8)
       SampleExpandCompressionsQuad<U, 8><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf,
                                                                             uniforms, assignment, failure);
-    else
+    else if (K == 16)
       SampleExpandCompressionsQuad<U, 16><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf,
+                                                                             uniforms, assignment, failure);
+    else
+      SampleExpandCompressionsQuad<U, 32><<<quad_grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf,
                                                                              uniforms, assignment, failure);
   } else
     SampleExpandCompressions<U, G><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf, uniforms,
@@ -1166,21 +1173,28 @@ void SampleExpandT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
 template <int G>
 void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                const MaxBuffers &buf) {
-  if (l.dims.K == 8 || l.dims.K == 16) {
-    const std::uint64_t lanes = static_cast<std::uint64_t>(count) * l.dims.B * l.dims.K * l.dims.K / 4;
+  // Team kernels: K*K/4 lanes per (op, item), 256 at K = 32, so a launch
+  // there must stay below 2^31 work-items itself (the level chunks allow 64
+  // lanes per item).
+  const std::uint64_t lanes = static_cast<std::uint64_t>(count) * l.dims.B * l.dims.K * l.dims.K / 4;
+  if ((l.dims.K == 8 || l.dims.K == 16 || l.dims.K == 32) && lanes < kMaxLaunchItems) {
     if (primitive == 0) {
       if (l.dims.K == 8)
         MaxRakeTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
-      else
+      else if (l.dims.K == 16)
         MaxRakeTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else
+        MaxRakeTeam<32><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
       Check(hipGetLastError(), "max-product level launch");
       return;
     }
     if (primitive == 3) {
       if (l.dims.K == 8)
         MaxCompressTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
-      else
+      else if (l.dims.K == 16)
         MaxCompressTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else
+        MaxCompressTeam<32><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
       Check(hipGetLastError(), "max-product level launch");
       return;
     }
@@ -1320,14 +1334,18 @@ void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     if (LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
-    // FP64 compressions at K = 8 and 16: the team kernel (16-byte aligned slots).
-    if (primitive == 3 && l.real == Real::kF64 && (l.dims.K == 8 || l.dims.K == 16) &&
-        (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
-      const unsigned grid = BlocksFor(static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4, 256);
+    // FP64 compressions at K = 8, 16 and 32: the team kernel (16-byte
+    // aligned slots; K*K/4 lanes per item, below 2^31 work-items).
+    const std::uint64_t team_lanes = static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4;
+    if (primitive == 3 && l.real == Real::kF64 && (l.dims.K == 8 || l.dims.K == 16 || l.dims.K == 32) &&
+        team_lanes < kMaxLaunchItems && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
+      const unsigned grid = BlocksFor(team_lanes, 256);
       if (l.dims.K == 8)
         CompressForwardTeam<8><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
-      else
+      else if (l.dims.K == 16)
         CompressForwardTeam<16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
+      else
+        CompressForwardTeam<32><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
       Check(hipGetLastError(), "compression level launch");
       return;
     }
