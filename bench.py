@@ -63,7 +63,8 @@ def star_tree(n):
 
 
 def forest_workload(trees: int, low: int, high: int, seed: int, scale: float = 1.0):
-    """Per-tree parent arrays of the uneven mixed-topology forest."""
+    """Per-tree parent arrays of the uneven mixed-topology forest (numpy's
+    generator; the benchmark itself uses forest_parts)."""
     rng = np.random.default_rng(seed)
     count = max(3, int(round(trees * scale)))
     sizes = np.floor(np.exp(rng.uniform(math.log(low), math.log(high + 1), size=count))).astype(np.int64)
@@ -74,14 +75,179 @@ def forest_workload(trees: int, low: int, high: int, seed: int, scale: float = 1
     return parts
 
 
-def yule_tree(leaves: int, rng):
-    """Random full binary tree: split a uniform random leaf until `leaves`."""
+# ---------------------------------------------------------------------------
+# Portable synthetic inputs: SplitMix64 (Steele, Lea and Flood, "Fast
+# splittable pseudorandom number generators", OOPSLA 2014) -> uniform doubles
+# (x >> 11) * 2^-53 (the conversion of tests/inference_test.cc:252-257),
+# Box-Muller normals and -log(1 - u) exponentials. Output i (0-based) of a
+# stream is the mix of seed + (i + 1) * gamma, so any range of a stream can
+# be generated on its own, on the host (numpy uint64) or on the device
+# (torch int64, whose multiplication wraps modulo 2^64 like uint64).
+# The seed of stream s of config c is 0x54484D4D_0000_000c with s in bits
+# 16..31 (stream 0 is the plain config seed).
+# ---------------------------------------------------------------------------
+
+_GAMMA = 0x9E3779B97F4A7C15
+_MIX1 = 0xBF58476D1CE4E5B9
+_MIX2 = 0x94D049BB133111EB
+
+STREAM_TOPOLOGY, STREAM_NODES, STREAM_EDGES, STREAM_VARIATES, STREAM_ZEROS = 0, 1, 2, 3, 4
+STREAM_OBSERVATIONS, STREAM_MISSING, STREAM_ROOT, STREAM_RATES, STREAM_LENGTHS = 5, 6, 7, 8, 9
+
+
+def config_seed(config: int, stream: int = 0) -> int:
+    return 0x54484D4D00000000 | (stream << 16) | config
+
+
+def uniforms_host(seed: int, start: int, count: int) -> np.ndarray:
+    """Outputs start .. start + count - 1 of a SplitMix64 stream as doubles in [0, 1)."""
+    i = np.arange(start + 1, start + count + 1, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = i * np.uint64(_GAMMA) + np.uint64(seed)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(_MIX1)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(_MIX2)
+        z ^= z >> np.uint64(31)
+    return (z >> np.uint64(11)).astype(np.float64) * 2.0**-53
+
+
+def _signed(x: int) -> int:
+    return x - (1 << 64) if x >= 1 << 63 else x
+
+
+def uniforms_device(torch, seed: int, start: int, count: int, device="cuda"):
+    """uniforms_host on the device (float64 tensor), the same values."""
+    z = torch.arange(start + 1, start + count + 1, dtype=torch.int64, device=device)
+    z.mul_(_signed(_GAMMA)).add_(_signed(seed))
+
+    def srl(x, s):  # logical right shift of the int64 bit pattern
+        return (x >> s) & ((1 << (64 - s)) - 1)
+
+    z ^= srl(z, 30)
+    z.mul_(_signed(_MIX1))
+    z ^= srl(z, 27)
+    z.mul_(_signed(_MIX2))
+    z ^= srl(z, 31)
+    return srl(z, 11).double().mul_(2.0**-53)
+
+
+def normals_host(seed: int, start: int, count: int) -> np.ndarray:
+    """Box-Muller normals; normal i uses uniforms start + 2*(i//2) and +1
+    (start even): r = sqrt(-2 log(1 - u0)), theta = 2 pi u1, the pair is
+    (r cos theta, r sin theta)."""
+    pairs = (count + 1) // 2
+    u = uniforms_host(seed, start, 2 * pairs).reshape(pairs, 2)
+    r = np.sqrt(-2.0 * np.log1p(-u[:, 0]))
+    t = 2.0 * math.pi * u[:, 1]
+    return np.stack((r * np.cos(t), r * np.sin(t)), 1).reshape(-1)[:count]
+
+
+def normals_device(torch, seed: int, start: int, count: int, device="cuda"):
+    pairs = (count + 1) // 2
+    u = uniforms_device(torch, seed, start, 2 * pairs, device).view(pairs, 2)
+    r = torch.sqrt(-2.0 * torch.log1p(-u[:, 0]))
+    t = (2.0 * math.pi) * u[:, 1]
+    return torch.stack((r * torch.cos(t), r * torch.sin(t)), 1).view(-1)[:count]
+
+
+def check_device_stream(torch, device="cuda") -> None:
+    """The device generator reproduces the host one (bit patterns of the
+    uniforms and normals within rounding of the device's libm)."""
+    seed = config_seed(4, STREAM_EDGES)
+    host = uniforms_host(seed, 12345, 1 << 16)
+    dev = uniforms_device(torch, seed, 12345, 1 << 16, device).cpu().numpy()
+    if not np.array_equal(host, dev):
+        raise RuntimeError("device SplitMix64 stream differs from the host stream")
+    hn = normals_host(seed, 0, 1 << 12)
+    dn = normals_device(torch, seed, 0, 1 << 12, device).cpu().numpy()
+    if np.max(np.abs(hn - dn)) > 1e-12:
+        raise RuntimeError("device Box-Muller normals differ from the host ones")
+
+
+def variates_device(torch, seed: int, count: int, dtype=None, device="cuda"):
+    """Sampling variates: uniforms truncated to 24 bits, floor(u 2^24) 2^-24,
+    which FP32 holds exactly, so every variate stays below 1 in FP32 and in
+    FP64 alike."""
+    u = uniforms_device(torch, seed, 0, count, device).mul_(2.0**24).floor_().mul_(2.0**-24)
+    return u.to(dtype or torch.float32)
+
+
+_CHUNK = 1 << 26  # values per device generation pass (even)
+
+
+def fill_lognormal_device(torch, out, seed: int, sigma: float = 1.0, log: bool = False):
+    """out.view(-1)[i] = exp(sigma * normal_i) (or sigma * normal_i with
+    log=True), rounded once from FP64 to out's type."""
+    flat = out.view(-1)
+    for i in range(0, flat.numel(), _CHUNK):
+        n = min(_CHUNK, flat.numel() - i)
+        v = normals_device(torch, seed, i, n, flat.device).mul_(sigma)
+        flat[i:i + n] = v if log else v.exp_()
+        del v
+    return out
+
+
+def fill_dirichlet_rows_device(torch, out, seed: int):
+    """Rows of the last dimension ~ Dirichlet(1): Exp(1) = -log(1 - u) draws
+    normalised by their row sum in FP64, rounded once to out's type."""
+    k = out.shape[-1]
+    rows = out.view(-1, k)
+    per = max(1, _CHUNK // k)
+    for r in range(0, rows.shape[0], per):
+        m = min(per, rows.shape[0] - r)
+        v = uniforms_device(torch, seed, r * k, m * k, rows.device).view(m, k)
+        v = torch.log1p(v.neg_()).neg_()
+        rows[r:r + m] = v.div_(v.sum(1, keepdim=True))
+        del v
+    return out
+
+
+def forest_parts(config: int = 4, trees: int = 4096, low: int = 16, high: int = 65536, scale: float = 1.0):
+    """The uneven forest from SplitMix64 stream 0 of the config: sizes
+    floor(exp(ln low + u (ln(high + 1) - ln low))) from uniforms 0 .. T-1,
+    topology t % 3 (random recursive, caterpillar, star), and the random
+    recursive trees' parents parent(i) = floor(u i) from the uniforms that
+    follow, tree after tree."""
+    seed = config_seed(config, STREAM_TOPOLOGY)
+    count = max(3, int(round(trees * scale)))
+    u = uniforms_host(seed, 0, count)
+    sizes = np.floor(np.exp(math.log(low) + u * (math.log(high + 1) - math.log(low)))).astype(np.int64)
+    parts, cursor = [], count
+    for t, n in enumerate(sizes):
+        kind = t % 3
+        if kind == 0:
+            p = np.empty(n, np.int64)
+            p[0] = -1
+            if n > 1:
+                p[1:] = np.floor(uniforms_host(seed, cursor, n - 1) * np.arange(1, n)).astype(np.int64)
+            cursor += n - 1
+            parts.append(p)
+        else:
+            parts.append(caterpillar_tree(n) if kind == 1 else star_tree(n))
+    return parts
+
+
+def recursive_trees(config: int, trees: int, n: int):
+    """`trees` random recursive trees of n nodes from stream 0 of the config."""
+    seed = config_seed(config, STREAM_TOPOLOGY)
+    u = uniforms_host(seed, 0, trees * (n - 1)).reshape(trees, n - 1)
+    out = []
+    for t in range(trees):
+        p = np.empty(n, np.int64)
+        p[0] = -1
+        p[1:] = np.floor(u[t] * np.arange(1, n)).astype(np.int64)
+        out.append(p)
+    return out
+
+
+def yule_tree(leaves: int, uniforms):
+    """Random full binary tree: split a uniform random leaf until `leaves`;
+    split j takes leaf floor(u_j * current leaves) of the list of leaves."""
     parents = np.empty(2 * leaves - 1, np.int64)
     parents[0] = -1
     current = [0]
     size = 1
-    for draw in rng.integers(0, 1 << 62, size=leaves - 1):
-        i = int(draw % len(current))
+    for u in uniforms[: leaves - 1].tolist():
+        i = int(u * len(current))
         leaf = current[i]
         parents[size] = leaf
         parents[size + 1] = leaf
@@ -194,7 +360,7 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup, tune=T
     return ws, ms, timings, log_z, node_out, edge_out, tuning
 
 
-def run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, gen):
+def run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, variate_seed):
     """MAP (thmm_maximum_device) and posterior sampling (thmm_sample_device,
     one U[0,1) variate per node) on the same resident FP32 inputs, each in
     its own device-resident workspace; both run in FP64 internally. Times
@@ -213,7 +379,7 @@ def run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, gen):
             call = lambda: ws.maximum_device(nodes.data_ptr(), edges.data_ptr(), 1, log_z.data_ptr(),  # noqa: E731
                                              states.data_ptr(), stream.cuda_stream)
         else:
-            uniforms = torch.rand(n, device="cuda", dtype=torch.float32, generator=gen)
+            uniforms = variates_device(torch, variate_seed, n)
             failure = torch.zeros(1, dtype=torch.int32, device="cuda")
             call = lambda: ws.sample_device(nodes.data_ptr(), edges.data_ptr(), uniforms.data_ptr(), 1,  # noqa: E731
                                             log_z.data_ptr(), states.data_ptr(), failure.data_ptr(),
@@ -304,9 +470,54 @@ def tree_inputs(nodes, edges, offsets, t):
     return nodes[lo:hi].double().cpu().numpy(), edges[elo:ehi].double().cpu().numpy()
 
 
-def cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, budget_nodes, threads):
-    """Times the FP64 oracle on a bounded sample of the trees (one tree per
-    task, `threads` host threads)."""
+def host_cpus():
+    """The host's CPUs: what this process may use (its affinity mask), what
+    the machine reports, and the model name."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        import subprocess
+
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, timeout=10).stdout.strip())
+    except Exception:
+        nproc = usable
+    return {"nproc": nproc, "os_cpu_count": os.cpu_count(), "affinity": usable, "model": model}
+
+
+# One GPU's share of a GPU box's host is 16 CPUs, whatever the machine
+# reports; the CPU leg uses that many threads (fewer if the process may use
+# fewer).
+CPU_SHARE = 16
+
+
+def timed_median(fn, repeats: int = 5):
+    """One warm-up call, then the median wall time of `repeats` calls."""
+    fn()
+    times = []
+    for _ in range(repeats):
+        start = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - start)
+    return float(np.median(times)), times
+
+
+def cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, budget_nodes, repeats=5):
+    """Times the FP64 oracle (log Z + node and edge marginals) on a bounded
+    sample of whole trees, one tree per task over CPU_SHARE host threads:
+    one warm-up pass, then the median of `repeats` passes (the method of
+    benchmarks/inference_benchmark.cc:19-36,86-121)."""
+    host = host_cpus()
+    threads = max(1, min(CPU_SHARE, host["affinity"]))
     order = np.random.default_rng(7).permutation(len(parts))
     chosen, total = [], 0
     for t in order:
@@ -323,11 +534,12 @@ def cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, budget_nodes, th
         t, nn, ee = job
         return oracle.marginals(plans[t], k, nn, ee)
 
-    start = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as pool:
-        list(pool.map(work, jobs))
-    seconds = time.perf_counter() - start
-    return total / seconds, total, len(chosen), seconds
+        seconds, times = timed_median(lambda: list(pool.map(work, jobs)), repeats)
+    return {"value": total / seconds, "unit": "nodes/s", "cores": threads, "kind": "port",
+            "sample": f"{len(chosen)} whole trees of the forest, {total} nodes, FP64 oracle log Z + node and edge "
+                      f"marginals; median of {repeats} passes after one warm-up ({seconds:.2f}s per pass)",
+            "host": host, "pass_seconds": [round(x, 3) for x in times]}
 
 
 def parity_sample(thmm, oracle, parts, offsets, nodes, edges, k, log_z, node_out, edge_out, picks):
@@ -356,19 +568,17 @@ def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
     edge log potential through the forward/backward pair. A step is one
     forward + backward over the forest."""
     k = 64
-    seed = 0x54484D4D00000005
-    rng = np.random.default_rng(seed)
     trees = max(1, int(round(256 * args.scale)))
     t0 = time.perf_counter()
-    parts = [yule_tree(8192, rng) for _ in range(trees)]
+    topology = uniforms_host(config_seed(5, STREAM_TOPOLOGY), 0, trees * 8191).reshape(trees, 8191)
+    parts = [yule_tree(8192, topology[t]) for t in range(trees)]
     parents, offsets = concatenate(parts)
     plan = thmm.Plan(parents)
     n, e, c = plan.num_nodes, plan.num_edges, plan.num_compressions
     log(f"forest: {trees} Yule trees, {n} nodes, {c} compressions; built and planned in {time.perf_counter() - t0:.1f}s")
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed & 0x7FFFFFFFFFFFFFFF)
-    log_nodes = torch.randn((1, n, k), device="cuda", generator=gen)
-    log_edges = torch.randn((e, k, k), device="cuda", generator=gen)
+    check_device_stream(torch)
+    log_nodes = fill_lognormal_device(torch, torch.empty((1, n, k), device="cuda"), config_seed(5, STREAM_NODES), log=True)
+    log_edges = fill_lognormal_device(torch, torch.empty((e, k, k), device="cuda"), config_seed(5, STREAM_EDGES), log=True)
     grad_nodes = torch.empty((1, n, k), device="cuda")
     grad_edges = torch.empty((e, k, k), device="cuda")
     log_z = torch.empty(plan.roots.size, dtype=torch.float64, device="cuda")
@@ -429,7 +639,7 @@ def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (torch device RNG: log node and edge potentials N(0,1))",
+        "data": "synthetic (SplitMix64 / Box-Muller, seed 0x54484D4D00000005: log node and edge potentials N(0,1))",
         "config": {"workload": f"{trees} random full binary (Yule) trees of 16383 nodes, K=64, FP32 log Z + gradients of "
                    "all node and edge log potentials", "nodes": int(n), "compressions": int(c), "scale": args.scale},
         "roofline": {"kernel": "whole step", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -509,21 +719,18 @@ def main() -> int:
         torch.cuda.empty_cache()
 
     k = 16
-    seed = 0x54484D4D00000004
     t0 = time.perf_counter()
-    parts = forest_workload(4096, 16, 65536, seed, args.scale)
+    parts = forest_parts(4, scale=args.scale)
     parents, offsets = concatenate(parts)
     plan = thmm.Plan(parents)
     plan_s = time.perf_counter() - t0
     n, e = plan.num_nodes, plan.num_edges
     log(f"forest: {len(parts)} trees, {n} nodes, {plan.num_rounds} rounds, {plan.primitive_batches.size} levels, "
-        f"{plan.num_compressions} compressions; planned in {plan_s:.1f}s")
+        f"{plan.num_compressions} compressions; built and planned in {plan_s:.1f}s")
 
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(seed & 0x7FFFFFFFFFFFFFFF)
-    nodes = torch.randn((n, k), device="cuda", dtype=torch.float32, generator=gen).exp_()
-    edges = torch.empty((e, k, k), device="cuda", dtype=torch.float32).exponential_(1.0, generator=gen)
-    edges /= edges.sum(dim=2, keepdim=True)
+    check_device_stream(torch)
+    nodes = fill_lognormal_device(torch, torch.empty((n, k), device="cuda"), config_seed(4, STREAM_NODES))
+    edges = fill_dirichlet_rows_device(torch, torch.empty((e, k, k), device="cuda"), config_seed(4, STREAM_EDGES))
     torch.cuda.synchronize()
 
     ws, ms, timings, log_z, node_out, edge_out, tuning = run_gpu(
@@ -583,16 +790,13 @@ def main() -> int:
 
     decode = None
     if not args.no_map:
-        decode = run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, gen)
+        decode = run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, config_seed(4, STREAM_VARIATES))
         log(f"device MAP and sampling: {decode}")
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        rate, total, count, seconds = cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, args.cpu_nodes, threads)
-        cpu = {"value": rate, "unit": "nodes/s", "cores": threads, "kind": "port",
-               "sample": f"{count} whole trees, {total} nodes, FP64 oracle log Z + marginals, {seconds:.1f}s wall"}
-        log(f"CPU oracle: {rate:.4g} nodes/s on {threads} threads")
+        cpu = cpu_baseline(thmm, oracle, parts, offsets, nodes, edges, k, args.cpu_nodes)
+        log(f"CPU oracle: {cpu}")
 
     fp64 = None
     if not args.no_fp64:
@@ -628,7 +832,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (torch device RNG: node potentials exp(N(0,1)), edge rows Dirichlet(1))",
+        "data": "synthetic (SplitMix64 / Box-Muller, seed 0x54484D4D00000004: node potentials exp(N(0,1)), "
+                "edge rows Dirichlet(1), one variate per node)",
         "config": {
             "workload": "uneven forest: 4096 trees, N log-uniform [16, 65536], random recursive / caterpillar / star, "
             "K=16, FP32 log Z + adjoint (node and edge marginals)",

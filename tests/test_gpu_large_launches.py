@@ -129,3 +129,77 @@ def test_level_split_into_launches_on_a_large_star(require_gpu):
     rows = edge_out.double().sum(dim=3)  # [b, l, s]: rows sum to the root marginal
     assert (rows - node_out[:, :1, :].double()).abs().max().item() <= 1e-5
     ws.close()
+
+
+def test_k32_team_kernels_and_their_large_launch_forms_agree_with_the_oracle(require_gpu):
+    """At K = 32 the max-product rake and compression kernels and the FP64
+    compression forward of sampling use 256 lanes per (op, item); a level of
+    2^23 (op, item) pairs would pass 2^31 work-items, so those launches keep
+    the group-per-item kernels. A broom (2^15 arms root -> a -> b -> leaf: one
+    level of 2^15 rakes, one of 2^15 compressions, one of 2^15 rakes into the
+    root) at batch 256 takes that route; the same items at batch 2 take the
+    team kernels. States and draws of both are bit-exact against the oracle
+    for two items, and the log weights and log partitions agree."""
+    import oracle
+    import torch
+
+    arms, k, batch = 1 << 15, 32, 256
+    parents = np.full(1 + 3 * arms, -1, np.int64)
+    a = 1 + 3 * np.arange(arms)
+    parents[a], parents[a + 1], parents[a + 2] = 0, a, a + 1
+    plan = thmm.Plan(parents)
+    assert arms * batch * k * k // 4 >= 2**31
+    n, e = plan.num_nodes, plan.num_edges
+    gen = torch.Generator(device="cuda").manual_seed(32)
+    nodes = torch.exp(torch.randn((batch, n, k), device="cuda", generator=gen) * 2.0)
+    edges = torch.exp(torch.randn((e, k, k), device="cuda", generator=gen))
+    uniforms = torch.rand((batch, n), device="cuda", generator=gen)
+    uniforms = (uniforms * 2.0**24).floor() * 2.0**-24  # exact in FP32, below 1
+    host_edges = edges.double().cpu().numpy()
+    items = (0, 1)
+    host_nodes = [nodes[b].double().cpu().numpy() for b in items]
+    host_u = [uniforms[b].double().cpu().numpy() for b in items]
+    results = {}
+    for size in (batch, 2):
+        ws = device_reservation(plan, k, size, _abi.THMM_RESERVE_MAXIMUM)
+        weights = torch.empty(size, dtype=torch.float64, device="cuda")
+        states = torch.empty((size, n), dtype=torch.int32, device="cuda")
+        ws.maximum_device(nodes.data_ptr(), edges.data_ptr(), size, weights.data_ptr(), states.data_ptr(), 0)
+        torch.cuda.synchronize()
+        ws.close()
+        del ws
+        results[("map", size)] = (weights[:2].cpu().numpy(), states[:2].cpu().numpy().astype(np.uint32))
+        del weights, states
+        torch.cuda.empty_cache()
+        ws = device_reservation(plan, k, size, _abi.THMM_RESERVE_SAMPLING)
+        log_z = torch.empty(size, dtype=torch.float64, device="cuda")
+        states = torch.empty((size, n), dtype=torch.int32, device="cuda")
+        failure = torch.ones(1, dtype=torch.int32, device="cuda")
+        ws.sample_device(nodes.data_ptr(), edges.data_ptr(), uniforms.data_ptr(), size, log_z.data_ptr(),
+                         states.data_ptr(), failure.data_ptr(), 0)
+        torch.cuda.synchronize()
+        ws.close()
+        del ws
+        assert int(failure.item()) == 0
+        results[("sample", size)] = (log_z[:2].cpu().numpy(), states[:2].cpu().numpy().astype(np.uint32))
+        del log_z, states
+        torch.cuda.empty_cache()
+        print(f"batch {size}: MAP and sampling done", flush=True)
+    for i, b in enumerate(items):
+        lw, expected = oracle.maximum(plan, k, host_nodes[i], host_edges)
+        for size in (batch, 2):
+            weights, states = results[("map", size)]
+            assert np.array_equal(states[i], expected), f"MAP states of item {b} at batch {size}"
+            assert abs(weights[i] - lw[0]) <= 1e-12 * abs(lw[0])
+        drawn, dist = oracle.sample(plan, k, host_nodes[i], host_edges, host_u[i])
+        near = int((dist < 1e-12).sum())
+        print(f"item {b}: {near} variates within 1e-12 of a boundary", flush=True)
+        if near:
+            continue  # reported separately, not compared
+        for size in (batch, 2):
+            log_z, states = results[("sample", size)]
+            assert np.array_equal(states[i], drawn), f"draws of item {b} at batch {size}"
+
+    assert np.array_equal(results[("map", batch)][0], results[("map", 2)][0])
+    lz_big, lz_small = results[("sample", batch)][0], results[("sample", 2)][0]
+    assert np.max(np.abs(lz_big - lz_small) / np.abs(lz_small)) <= 1e-13

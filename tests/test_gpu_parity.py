@@ -90,8 +90,23 @@ def test_log_partition_and_marginals(require_gpu, precision, k, topo, n, batch):
     assert lp.values.shape == (batch * roots,)
 
 
+# MAP and sampling: every case above plus the K >= 32 team and group-per-item
+# kernels on raked subtrees, multi-child absorptions, forests and
+# batch-varying items.
+DECODE_CASES = CASES + [
+    (32, "forest", 200, 3),
+    (32, "recursive", 150, 3),
+    (32, "star", 120, 4),
+    (48, "recursive", 60, 2),
+    (48, "forest", 80, 2),
+    (64, "forest", 80, 2),
+    (64, "star", 60, 3),
+    (64, "path", 50, 2),
+]
+
+
 @pytest.mark.parametrize("precision", ["f32", "f64"])
-@pytest.mark.parametrize("k,topo,n,batch", [c for c in CASES if c[0] <= 33])
+@pytest.mark.parametrize("k,topo,n,batch", DECODE_CASES)
 def test_maximum_matches_oracle(require_gpu, precision, k, topo, n, batch):
     rng = np.random.default_rng(7 * k + n)
     plan = thmm.Plan(topology(topo, n, rng))
@@ -109,7 +124,7 @@ def test_maximum_matches_oracle(require_gpu, precision, k, topo, n, batch):
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
-@pytest.mark.parametrize("k,topo,n,batch", [c for c in CASES if c[0] <= 33])
+@pytest.mark.parametrize("k,topo,n,batch", DECODE_CASES)
 def test_samples_match_oracle(require_gpu, precision, k, topo, n, batch):
     rng = np.random.default_rng(11 * k + n)
     plan = thmm.Plan(topology(topo, n, rng))
