@@ -443,6 +443,29 @@ thmm_status thmm_sample_device(thmm_workspace *workspace,
                                double *d_log_z, uint32_t *d_states,
                                uint32_t *d_failure, void *stream);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). thmm_log_partition_device without a workspace: the same
+ * inputs ([batch, node, state] node and [edge, parent, child] edge
+ * potentials in HBM, precision's element type), the same output (d_log_z,
+ * double [batch * roots]) from the plan's levels, with every buffer beyond
+ * the inputs and outputs (the renamed ops, node, branch and path state) in
+ * one library-owned static device array of THMM_STATIC_SCRATCH_BYTES per
+ * GPU, which that GPU holds until the process ends. The call allocates no
+ * device memory; thmm_static_scratch_bytes gives the bytes a shape needs,
+ * and a shape above the maximum is a length_error. The topology is copied
+ * from the plan (host memory) on `stream` (NULL or THMM_HIP_NULL_STREAM:
+ * HIP's null stream); `device` is made current. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (other streams, other
+ * threads) race, so serialise them. */
+#define THMM_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+thmm_status thmm_static_scratch_bytes(const thmm_plan *plan, size_t states, size_t batch,
+                                      thmm_precision precision, size_t *bytes);
+thmm_status thmm_log_partition_device_static(const thmm_plan *plan, size_t states, size_t batch,
+                                             thmm_precision precision, const void *d_node_potentials,
+                                             const void *d_edge_potentials, double *d_log_z, void *stream,
+                                             int device);
+
 /* Blocks until the workspace stream has drained. */
 thmm_status thmm_synchronize(thmm_workspace *workspace);
 

@@ -29,11 +29,13 @@ from .api import (  # noqa: F401
     available,
     device_description,
     expm_reversible,
+    log_partition_device_static,
     log_partition_function_prepared,
     make_plan,
     maximum_a_posteriori_prepared,
     partition_function_prepared,
     posterior_marginals_prepared,
     posterior_sample_prepared,
+    static_scratch_bytes,
 )
 from ._abi import library_path  # noqa: F401

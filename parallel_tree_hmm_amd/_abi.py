@@ -37,6 +37,7 @@ THMM_RESERVE_MARGINALS = 4
 THMM_RESERVE_DEVICE_RESIDENT = 8
 THMM_RESERVE_GRAPH = 16
 THMM_RESERVE_HALF_EDGE_MARGINALS = 32
+THMM_STATIC_SCRATCH_BYTES = 64 << 20
 
 THMM_RAKE = 0
 THMM_BRANCH_COMBINATION = 1
@@ -220,6 +221,11 @@ PROTOTYPES = {
         [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p, P(Tuning)],
     ),
     "thmm_log_partition_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
+    "thmm_static_scratch_bytes": (c_status, [c_void_p, c_size, c_size, c_int, P(c_size)]),
+    "thmm_log_partition_device_static": (
+        c_status,
+        [c_void_p, c_size, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int],
+    ),
     "thmm_marginals_device": (
         c_status,
         [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p, c_void_p, c_void_p],

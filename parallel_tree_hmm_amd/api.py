@@ -268,6 +268,28 @@ _PRECISIONS = {"f32": (_abi.THMM_F32, np.float32), "f64": (_abi.THMM_F64, np.flo
 _HIP_NULL_STREAM = ctypes.c_void_p(2**64 - 1)  # THMM_HIP_NULL_STREAM
 
 
+def static_scratch_bytes(plan: Plan, states: int, batch: int, precision: str = "f32") -> int:
+    """Bytes of contraction state log_partition_device_static needs for this
+    shape (at most _abi.THMM_STATIC_SCRATCH_BYTES are available)."""
+    out = ctypes.c_size_t()
+    _check(_lib().thmm_static_scratch_bytes(plan._handle, states, batch, _PRECISIONS[precision][0], ctypes.byref(out)))
+    return out.value
+
+
+def log_partition_device_static(plan: Plan, states: int, batch: int, nodes_ptr: int, edges_ptr: int, log_z_ptr: int,
+                                precision: str = "f32", stream=0, device: int = 0) -> None:
+    """thmm_log_partition_device_static: log Z of device-resident potentials
+    with no workspace and no device allocation (state in the library's static
+    per-GPU array; calls that overlap on one GPU race). stream 0 or None is
+    HIP's null stream."""
+    _check(
+        _lib().thmm_log_partition_device_static(
+            plan._handle, states, batch, _PRECISIONS[precision][0], nodes_ptr, edges_ptr, log_z_ptr, stream or None,
+            device
+        )
+    )
+
+
 def _stream(stream):
     """Maps a hipStream_t handle to the C-ABI convention (NULL means the
     workspace's own stream, so HIP's null stream 0 gets its sentinel)."""

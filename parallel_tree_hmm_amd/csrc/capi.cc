@@ -10,11 +10,15 @@
 
 #include "expm.h"
 #include "plan.h"
+#include "static_scratch.h"
 #include "tree_hmm_amd.h"
 #include "workspace.h"
 
 struct thmm_plan {
   thmm::Plan plan;
+  // Renamed ops for the workspace-free entry point (formed once here, so
+  // that entry point allocates nothing).
+  thmm::RenamedOps renamed;
 };
 
 struct thmm_workspace {
@@ -81,7 +85,8 @@ thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes, thmm_plan
   return Guard([&] {
     Require(out != nullptr, "output pointer is null");
     Require(parents != nullptr || num_nodes == 0, "parents are null");
-    auto *plan = new thmm_plan{thmm::MakePlan(parents, num_nodes)};
+    auto *plan = new thmm_plan{thmm::MakePlan(parents, num_nodes), {}};
+    plan->renamed = thmm::RenameSlots(plan->plan);
     *out = plan;
   });
 }
@@ -89,7 +94,8 @@ thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes, thmm_plan
 thmm_status thmm_plan_create_from_arrays(const thmm_plan_arrays *arrays, thmm_plan **out) {
   return Guard([&] {
     Require(out != nullptr && arrays != nullptr, "argument is null");
-    auto *plan = new thmm_plan{thmm::PlanFromArrays(*arrays)};
+    auto *plan = new thmm_plan{thmm::PlanFromArrays(*arrays), {}};
+    plan->renamed = thmm::RenameSlots(plan->plan);
     *out = plan;
   });
 }
@@ -320,6 +326,25 @@ thmm_status thmm_autotune_marginals_device(thmm_workspace *workspace, const void
 thmm_status thmm_log_partition_device(thmm_workspace *workspace, const void *d_node_potentials,
                                       const void *d_edge_potentials, size_t batch, double *d_log_z, void *stream) {
   return Guard([&] { W(workspace).LogPartitionDevice(d_node_potentials, d_edge_potentials, batch, d_log_z, stream); });
+}
+
+thmm_status thmm_static_scratch_bytes(const thmm_plan *plan, size_t states, size_t batch,
+                                      thmm_precision precision, size_t *bytes) {
+  return Guard([&] {
+    Require(plan != nullptr && bytes != nullptr, "argument is null");
+    *bytes = thmm::StaticScratchBytes(plan->plan, states, batch, RealOf(precision));
+  });
+}
+
+thmm_status thmm_log_partition_device_static(const thmm_plan *plan, size_t states, size_t batch,
+                                             thmm_precision precision, const void *d_node_potentials,
+                                             const void *d_edge_potentials, double *d_log_z, void *stream,
+                                             int device) {
+  return Guard([&] {
+    Require(plan != nullptr, "plan is null");
+    thmm::LogPartitionStatic(plan->plan, plan->renamed, states, batch, RealOf(precision), d_node_potentials,
+                             d_edge_potentials, d_log_z, stream, device);
+  });
 }
 
 thmm_status thmm_marginals_device(thmm_workspace *workspace, const void *d_node_potentials,

@@ -624,6 +624,48 @@ void Validate(const Plan &plan) {
           "round offsets do not partition the batches");
 }
 
+// Replays the plan's edge slots: a compression's composed path lives in slot
+// E + c, and later ops that name its left edge read that slot. Renaming
+// happens after each whole batch: ops of one batch never read a slot another
+// op of the same batch writes.
+RenamedOps RenameSlots(const Plan &plan) {
+  const std::size_t E = plan.num_edges;
+  std::vector<Index> current(E);
+  for (std::size_t e = 0; e < E; ++e) current[e] = static_cast<Index>(e);
+  RenamedOps out;
+  out.rakes.resize(plan.rakes.size());
+  out.combines.resize(plan.combinations.size());
+  out.absorbs.resize(plan.absorptions.size());
+  out.compresses.resize(plan.compressions.size());
+  for (const thmm_primitive_batch &b : plan.batches) {
+    for (std::uint32_t i = b.offset; i < b.offset + b.count; ++i) {
+      switch (b.primitive) {
+        case THMM_RAKE: {
+          const thmm_rake &op = plan.rakes[i];
+          out.rakes[i] = {op.leaf, current[op.edge], op.branch, op.parent};
+          break;
+        }
+        case THMM_BRANCH_COMBINATION:
+          out.combines[i] = {plan.combinations[i].destination, plan.combinations[i].source};
+          break;
+        case THMM_BRANCH_ABSORPTION:
+          out.absorbs[i] = {plan.absorptions[i].parent, plan.absorptions[i].branch};
+          break;
+        case THMM_COMPRESSION: {
+          const thmm_compression &op = plan.compressions[i];
+          out.compresses[i] = {current[op.left_edge], op.middle, current[op.right_edge], static_cast<Index>(E + i),
+                               op.parent, op.child, 0, 0};
+          break;
+        }
+      }
+    }
+    if (b.primitive == THMM_COMPRESSION)
+      for (std::uint32_t i = b.offset; i < b.offset + b.count; ++i)
+        current[plan.compressions[i].left_edge] = static_cast<Index>(E + i);
+  }
+  return out;
+}
+
 Plan PlanFromArrays(const thmm_plan_arrays &a) {
   Plan plan;
   if (a.num_nodes == 0 || a.num_nodes >= kNone)

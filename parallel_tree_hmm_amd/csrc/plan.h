@@ -29,6 +29,7 @@
 #include <string>
 #include <vector>
 
+#include "launch.h"
 #include "tree_hmm_amd.h"
 
 namespace thmm {
@@ -55,6 +56,17 @@ struct Plan {
   }
   thmm_plan_arrays Arrays() const;
 };
+
+// The plan's ops with renamed path slots (launch.h): slots 0..E-1 are the
+// caller's edge potentials and compression c writes the fresh slot E + c
+// instead of overwriting its left edge, as src/inference.cc:191-214 does.
+struct RenamedOps {
+  std::vector<DeviceRake> rakes;
+  std::vector<DeviceCombine> combines;
+  std::vector<DeviceAbsorb> absorbs;
+  std::vector<DeviceCompress> compresses;
+};
+RenamedOps RenameSlots(const Plan &plan);
 
 // Builds a plan from a parent array (-1 = root). Throws invalid_argument for
 // out-of-range parents or cycles and length_error beyond 2^32 - 2 nodes.

@@ -577,46 +577,14 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   Hip(hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking), "GPU runtime StreamCreateWithFlags");
   for (hipEvent_t &e : w.events) Hip(hipEventCreate(&e), "GPU runtime EventCreate");
 
-  // Slot renaming: replay the plan's edge slots and give compression c the
-  // fresh slot E + c.
+  // Slot renaming: compression c writes the fresh slot E + c.
   const std::size_t E = plan.num_edges;
-  std::vector<std::uint32_t> current(E);
-  for (std::size_t e = 0; e < E; ++e) current[e] = static_cast<std::uint32_t>(e);
-  std::vector<DeviceRake> rakes(plan.rakes.size());
-  std::vector<DeviceCombine> combines(plan.combinations.size());
-  std::vector<DeviceAbsorb> absorbs(plan.absorptions.size());
-  std::vector<DeviceCompress> compresses(plan.compressions.size());
-  for (const thmm_primitive_batch &batch_desc : plan.batches) {
-    for (std::uint32_t i = batch_desc.offset; i < batch_desc.offset + batch_desc.count; ++i) {
-      switch (batch_desc.primitive) {
-        case THMM_RAKE: {
-          const thmm_rake &op = plan.rakes[i];
-          rakes[i] = {op.leaf, current[op.edge], op.branch, op.parent};
-          break;
-        }
-        case THMM_BRANCH_COMBINATION:
-          combines[i] = {plan.combinations[i].destination, plan.combinations[i].source};
-          break;
-        case THMM_BRANCH_ABSORPTION:
-          absorbs[i] = {plan.absorptions[i].parent, plan.absorptions[i].branch};
-          break;
-        case THMM_COMPRESSION: {
-          const thmm_compression &op = plan.compressions[i];
-          const std::uint32_t out = static_cast<std::uint32_t>(E + i);
-          compresses[i] = {current[op.left_edge], op.middle, current[op.right_edge], out, op.parent, op.child, 0, 0};
-          break;
-        }
-      }
-    }
-    // Renaming happens after the whole batch: ops of one batch never read a
-    // slot another op of the same batch writes.
-    if (batch_desc.primitive == THMM_COMPRESSION)
-      for (std::uint32_t i = batch_desc.offset; i < batch_desc.offset + batch_desc.count; ++i)
-        current[plan.compressions[i].left_edge] = static_cast<std::uint32_t>(E + i);
-  }
+  RenamedOps renamed = RenameSlots(plan);
+  const std::vector<DeviceRake> &rakes = renamed.rakes;
+  const std::vector<DeviceCompress> &compresses = renamed.compresses;
   Upload(w.rakes, rakes, w.stream, "topology upload");
-  Upload(w.combines, combines, w.stream, "topology upload");
-  Upload(w.absorbs, absorbs, w.stream, "topology upload");
+  Upload(w.combines, renamed.combines, w.stream, "topology upload");
+  Upload(w.absorbs, renamed.absorbs, w.stream, "topology upload");
   Upload(w.compresses, compresses, w.stream, "topology upload");
   {
     std::vector<std::uint32_t> consumed;
