@@ -420,15 +420,76 @@ def measured_traffic(kernel: str):
     return None
 
 
-def roofline_from(timings, k, s):
+class SharedEdgeReads:
+    """For a batch of items that share one set of edge potentials: per plan
+    level, how many of its edge-slot reads name a caller edge (read once per
+    op for the whole batch) rather than a composed path (one per item). A
+    compression's path replaces its left edge's slot (src/inference.cc:
+    191-214), which is replayed here."""
+
+    def __init__(self, plan):
+        e = plan.num_edges
+        current = np.arange(e, dtype=np.int64)  # < e: the caller's edge
+        rounds = plan.round_batch_offsets
+        self.rake_rounds, self.rake_levels, self.compress_levels = [], [], []
+        for r in range(plan.num_rounds):
+            rakes = shared = 0
+            for i in range(int(rounds[r]), int(rounds[r + 1])):
+                b = plan.primitive_batches[i]
+                lo, hi = int(b["offset"]), int(b["offset"]) + int(b["count"])
+                if b["primitive"] == 0:
+                    sh = int(np.count_nonzero(current[plan.rakes["edge"][lo:hi].astype(np.int64)] < e))
+                    self.rake_levels.append((hi - lo, sh))
+                    rakes, shared = rakes + hi - lo, shared + sh
+                elif b["primitive"] == 3:
+                    ops = plan.compressions[lo:hi]
+                    sh = int(np.count_nonzero(current[ops["left_edge"].astype(np.int64)] < e)) + int(
+                        np.count_nonzero(current[ops["right_edge"].astype(np.int64)] < e))
+                    self.compress_levels.append((hi - lo, sh))
+                    current[ops["left_edge"].astype(np.int64)] = e + np.arange(lo, hi)
+            if rakes:
+                self.rake_rounds.append((rakes, shared))
+
+    def bytes_of(self, timings, k, s):
+        """Algorithmic bytes of each profiled launch, in order."""
+        kk = k * k
+        rounds, rakes, compresses = iter(self.rake_rounds), iter(self.rake_levels), iter(self.compress_levels)
+        back_rakes, back_compresses = iter(self.rake_levels[::-1]), iter(self.compress_levels[::-1])
+        out = []
+        for phase, primitive, count, batch, _ in timings:
+            if phase == 1 and primitive in (0, 4):
+                c, sh = next(rounds if primitive == 4 else rakes)
+                out.append(batch * c * 2 * k * s + (sh + (c - sh) * batch) * kk * s)
+            elif phase == 1 and primitive == 3:
+                c, sh = next(compresses)
+                out.append(batch * c * (kk + k) * s + (sh + (2 * c - sh) * batch) * kk * s)
+            elif phase == 3 and primitive == 0:
+                c, sh = next(back_rakes)
+                out.append(batch * c * (kk + 3 * k) * s + (sh + (c - sh) * batch) * kk * s)
+            elif phase == 3 and primitive == 3:
+                c, sh = next(back_compresses)
+                out.append(batch * c * (3 * kk + 2 * k) * s + (sh + (2 * c - sh) * batch) * kk * s)
+            elif phase in (1, 3):
+                out.append(op_bytes(phase, primitive, k, s) * count * batch)
+            else:
+                out.append(0)
+        return out
+
+
+def roofline_from(timings, k, s, reads=None):
+    """The profiled kernel family with the most time, its algorithmic GB/s
+    and the breakdown; `reads` (SharedEdgeReads) counts edges shared by a
+    batch once per op."""
     totals = {}
-    for phase, primitive, count, batch, ms in timings:
+    nbytes_of = reads.bytes_of(timings, k, s) if reads is not None else [
+        op_bytes(ph, pr, k, s) * c * b if ph in (1, 3) else 0 for ph, pr, c, b, _ in timings]
+    for (phase, primitive, count, batch, ms), nb in zip(timings, nbytes_of):
         if phase not in (1, 3):
             continue
         key = (phase, primitive)
         t = totals.setdefault(key, [0.0, 0, 0])
         t[0] += ms
-        t[1] += op_bytes(phase, primitive, k, s) * count * batch
+        t[1] += nb
         t[2] += 1
     key = max(totals, key=lambda x: totals[x][0])
     ms, nbytes, launches = totals[key]
@@ -562,6 +623,339 @@ def parity_sample(thmm, oracle, parts, offsets, nodes, edges, k, log_z, node_out
 # ---------------------------------------------------------------------------
 
 
+def time_calls(torch, call, steps, warmup):
+    """Event-timed milliseconds per call of `call` on torch's current stream
+    (warm-up calls first, untimed)."""
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        call()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def marginal_pass(thmm, torch, plan, k, batch, precision, nodes, edges, graphs=False):
+    """A device-resident marginal workspace, its outputs and a call."""
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    ws = thmm.Workspace()
+    ws.reserve_marginals(plan, k, batch, precision=precision, device_resident=True, graphs=graphs)
+    out = {"log_z": torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda"),
+           "nodes": torch.empty((batch, plan.num_nodes, k), dtype=dtype, device="cuda"),
+           "edges": torch.empty((batch, plan.num_edges, k, k), dtype=dtype, device="cuda")}
+    stream = torch.cuda.current_stream().cuda_stream
+    args = (nodes.data_ptr(), edges.data_ptr(), batch, out["log_z"].data_ptr(), out["nodes"].data_ptr(),
+            out["edges"].data_ptr(), stream)
+    return ws, out, (lambda: ws.marginals_device(*args)), args
+
+
+def result_line(metric, value, args, ms, dtype, data, config, **extra):
+    line = {"metric": metric, "value": value, "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype, "data": data, "config": config}
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def cpu_median(label, total_nodes, fn, repeats=5):
+    host = host_cpus()
+    seconds, times = timed_median(fn, repeats)
+    return {"value": total_nodes / seconds, "unit": "nodes/s", "cores": max(1, min(CPU_SHARE, host["affinity"])),
+            "kind": "port", "sample": f"{label}; median of {repeats} passes after one warm-up ({seconds:.3f}s per pass)",
+            "host": host, "pass_seconds": [round(x, 4) for x in times]}
+
+
+def run_recursive_fp64(args, thmm, oracle, torch) -> int:
+    """64 random recursive trees of 1,023 nodes (parent(i) ~ U{0..i-1}), K = 4,
+    plain FP64, as one forest (per-tree edges): log Z + node and edge
+    marginals per step, replayed as a captured hipGraph."""
+    k, trees, n_tree = 4, 64, 1023
+    parts = recursive_trees(1, trees, n_tree)
+    parents, offsets = concatenate(parts)
+    plan = thmm.Plan(parents)
+    n, e = plan.num_nodes, plan.num_edges
+    check_device_stream(torch)
+    nodes = fill_lognormal_device(torch, torch.empty((1, n, k), dtype=torch.float64, device="cuda"),
+                                  config_seed(1, STREAM_NODES))
+    edges = fill_dirichlet_rows_device(torch, torch.empty((e, k, k), dtype=torch.float64, device="cuda"),
+                                       config_seed(1, STREAM_EDGES))
+    ws, out, call, margs = marginal_pass(thmm, torch, plan, k, 1, "f64", nodes, edges, graphs=True)
+    ms = time_calls(torch, call, args.steps, args.warmup)
+    wse, oute, calle, eargs = marginal_pass(thmm, torch, plan, k, 1, "f64", nodes, edges)
+    ms_eager = time_calls(torch, calle, args.steps, args.warmup)
+    timings = wse.profile_marginals_device(*eargs)
+    roofline = roofline_from(timings, k, 8)
+    # Parity: all 64 trees against the FP64 oracle.
+    lz = out["log_z"].cpu().numpy()
+    nn, ee = nodes[0].cpu().numpy(), edges.cpu().numpy()
+    gn, ge = out["nodes"][0].cpu().numpy(), out["edges"][0].cpu().numpy()
+    tree_plans = [thmm.Plan(p) for p in parts]
+    worst = {"log_z_rel": 0.0, "node_abs": 0.0, "edge_abs": 0.0}
+
+    def tree_job(t):
+        lo, hi, elo, ehi = tree_slices(offsets, t)
+        return t, oracle.marginals(tree_plans[t], k, nn[lo:hi], ee[elo:ehi])
+
+    with ThreadPoolExecutor(max_workers=max(1, min(CPU_SHARE, host_cpus()["affinity"]))) as pool:
+        for t, (olz, onm, oem) in pool.map(tree_job, range(trees)):
+            lo, hi, elo, ehi = tree_slices(offsets, t)
+            worst["log_z_rel"] = max(worst["log_z_rel"], abs(lz[t] - olz[0]) / max(1.0, abs(olz[0])))
+            worst["node_abs"] = max(worst["node_abs"], float(np.max(np.abs(gn[lo:hi] - onm))))
+            worst["edge_abs"] = max(worst["edge_abs"], float(np.max(np.abs(ge[elo:ehi] - oem))))
+        cpu = None
+        if not args.no_cpu:
+            cpu = cpu_median(f"all {trees} trees ({n} nodes), FP64 oracle log Z + node and edge marginals, one tree "
+                             "per task", n, lambda: list(pool.map(tree_job, range(trees))))
+    worst["trees_checked"] = trees
+    log(f"K=4 FP64 forest: graph replay {ms:.4f} ms, eager {ms_eager:.4f} ms; parity {worst}")
+    ws.close()
+    wse.close()
+    return result_line(
+        "nodes/s log Z + adjoint, plain FP64 (64 random recursive trees)", n / (ms * 1e-3), args, ms, "f64",
+        "synthetic (SplitMix64 / Box-Muller, seed 0x54484D4D00000001: parent(i) = floor(u i), node potentials "
+        "exp(N(0,1)), edge rows Dirichlet(1))",
+        {"workload": "64 random recursive trees of 1023 nodes, K=4, plain FP64, one forest (per-tree edges), "
+         "log Z + node and edge marginals, hipGraph replay", "nodes": int(n), "levels": int(plan.primitive_batches.size)},
+        roofline=roofline, cpu_baseline=cpu, eager_ms_per_step=round(ms_eager, 4), parity=worst)
+
+
+def run_long_chains(args, thmm, oracle, torch) -> int:
+    """16 path graphs of 100,000 nodes, K = 8, FP32, one topology with 16
+    batch items sharing the edges: node potentials exp(N(0, 2)) with each
+    entry set to 0 with probability 0.1 (a node whose entries all drew 0
+    keeps its first state at 1), edges exp(N(0, 1)). A step is log Z + node
+    and edge marginals; MAP and posterior sampling (one variate per node)
+    are timed on the same inputs."""
+    k, batch, n_path = 8, 16, 100_000
+    plan = thmm.Plan(np.arange(-1, n_path - 1, dtype=np.int64))
+    n, e = plan.num_nodes, plan.num_edges
+    check_device_stream(torch)
+    nodes = fill_lognormal_device(torch, torch.empty((batch, n, k), device="cuda"), config_seed(2, STREAM_NODES),
+                                  sigma=2.0)
+    zero = uniforms_device(torch, config_seed(2, STREAM_ZEROS), 0, batch * n * k).view(batch, n, k) < 0.1
+    nodes[zero] = 0.0
+    dead = (nodes == 0).all(dim=2)
+    nodes[..., 0][dead] = 1.0
+    del zero
+    edges = fill_lognormal_device(torch, torch.empty((e, k, k), device="cuda"), config_seed(2, STREAM_EDGES))
+    ws, out, call, margs = marginal_pass(thmm, torch, plan, k, batch, "f32", nodes, edges)
+    ms = time_calls(torch, call, args.steps, args.warmup)
+    timings = ws.profile_marginals_device(*margs)
+    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan))
+    lz = out["log_z"].cpu().numpy()
+    # Parity: two whole items against the FP64 oracle; every item's
+    # normalisation.
+    worst = {"log_z_rel": 0.0, "node_abs": 0.0, "edge_abs": 0.0}
+    ee = edges.double().cpu().numpy()
+    for b in (0, batch - 1):
+        olz, onm, oem = oracle.marginals(plan, k, nodes[b].double().cpu().numpy(), ee)
+        worst["log_z_rel"] = max(worst["log_z_rel"], abs(lz[b] - olz[0]) / max(1.0, abs(olz[0])))
+        worst["node_abs"] = max(worst["node_abs"], float(np.max(np.abs(out["nodes"][b].double().cpu().numpy() - onm))))
+        worst["edge_abs"] = max(worst["edge_abs"], float(np.max(np.abs(out["edges"][b].double().cpu().numpy() - oem))))
+    worst["node_normalisation_abs"] = float((out["nodes"].double().sum(2) - 1.0).abs().max())
+    worst["items_checked"] = 2
+    ws.close()
+    del ws, out
+    torch.cuda.empty_cache()
+    # MAP and sampling on the same inputs (device entry points, FP64 inside).
+    decode = {}
+    stream = torch.cuda.current_stream().cuda_stream
+    uniforms = variates_device(torch, config_seed(2, STREAM_VARIATES), batch * n).view(batch, n)
+    for name, flag in (("map", thmm._abi.THMM_RESERVE_MAXIMUM), ("sample", thmm._abi.THMM_RESERVE_SAMPLING)):
+        wd = thmm.Workspace()
+        wd.reserve(plan, k, batch, precision="f32", flags=flag | thmm._abi.THMM_RESERVE_DEVICE_RESIDENT)
+        lzd = torch.empty(batch, dtype=torch.float64, device="cuda")
+        states = torch.empty((batch, n), dtype=torch.int32, device="cuda")
+        failure = torch.zeros(1, dtype=torch.int32, device="cuda")
+        if name == "map":
+            dcall = lambda: wd.maximum_device(nodes.data_ptr(), edges.data_ptr(), batch, lzd.data_ptr(),  # noqa: E731
+                                              states.data_ptr(), stream)
+        else:
+            dcall = lambda: wd.sample_device(nodes.data_ptr(), edges.data_ptr(), uniforms.data_ptr(), batch,  # noqa: E731
+                                             lzd.data_ptr(), states.data_ptr(), failure.data_ptr(), stream)
+        dms = time_calls(torch, dcall, args.steps, args.warmup)
+        got = states[0].cpu().numpy().astype(np.uint32)
+        nn0 = nodes[0].double().cpu().numpy()
+        if name == "map":
+            _, expected = oracle.maximum(plan, k, nn0, ee)
+            exact = bool(np.array_equal(got, expected))
+        else:
+            expected, dist = oracle.sample(plan, k, nn0, ee, uniforms[0].double().cpu().numpy())
+            exact = bool(np.array_equal(got, expected)) if np.min(dist) >= 1e-12 else "near a CDF boundary"
+        decode[name] = {"ms_per_step": round(dms, 4), "value": batch * n / (dms * 1e-3), "unit": "nodes/s",
+                        "item0_bit_exact_vs_oracle": exact, "failure_flag": int(failure.item())}
+        wd.close()
+        del wd
+        torch.cuda.empty_cache()
+    log(f"K=8 chains: marginals {ms:.3f} ms; {decode}; parity {worst}")
+    cpu = None
+    if not args.no_cpu:
+        host_nodes = nodes.double().cpu().numpy()
+        threads = max(1, min(CPU_SHARE, host_cpus()["affinity"]))
+        cpu = cpu_median(f"all {batch} items ({batch * n} nodes), FP64 oracle log Z + node and edge marginals, "
+                         f"one item per thread", batch * n,
+                         lambda: oracle.marginals_batched(plan, k, host_nodes, ee, threads=threads))
+    return result_line(
+        "nodes/s log Z + adjoint, FP32 (16 long chains)", batch * n / (ms * 1e-3), args, ms, "f32",
+        "synthetic (SplitMix64 / Box-Muller, seed 0x54484D4D00000002: node potentials exp(N(0,2)) with 10% of "
+        "entries 0, edges exp(N(0,1)), one variate per node)",
+        {"workload": "16 path graphs of 100000 nodes (one topology, batch 16, shared edges), K=8, FP32 log Z + node "
+         "and edge marginals", "nodes": int(batch * n), "levels": int(plan.primitive_batches.size)},
+        roofline=roofline, cpu_baseline=cpu, map_and_sampling=decode, parity=worst)
+
+
+def phylo_inputs(sites: int, depth: int = 12):
+    """The categorical model of config 3 from its SplitMix64 streams:
+    balanced binary tree of 2^(depth+1) - 1 nodes, leaves observed;
+    observations floor(4 u), coded 4 (missing, an all-ones emission row)
+    where a second uniform is below 0.05; root ~ Dirichlet(1); emission
+    [I_4; 1^T]; Q = S Pi with S_ij ~ Exp(1) symmetric, pi ~ Dirichlet(1),
+    rows summing to 0, unit mean rate; t_e ~ Exp(mean 0.1)."""
+    k = 4
+    n = 2 ** (depth + 1) - 1
+    parents = np.arange(n, dtype=np.int64) - 1
+    parents //= 2
+    parents[0] = -1
+    leaves = np.arange(n // 2, n, dtype=np.uint32)
+    u = uniforms_host(config_seed(3, STREAM_OBSERVATIONS), 0, sites * leaves.size).reshape(sites, leaves.size)
+    obs = np.floor(u * k).astype(np.uint8)
+    obs[uniforms_host(config_seed(3, STREAM_MISSING), 0, obs.size).reshape(obs.shape) < 0.05] = k
+    r = -np.log1p(-uniforms_host(config_seed(3, STREAM_ROOT), 0, k))
+    root = r / r.sum()
+    emission = np.vstack([np.eye(k), np.ones((1, k))])
+    draws = -np.log1p(-uniforms_host(config_seed(3, STREAM_RATES), 0, k * k + k))
+    sym = draws[: k * k].reshape(k, k)
+    sym = np.triu(sym, 1)
+    sym = sym + sym.T
+    pi = draws[k * k:] / draws[k * k:].sum()
+    q = sym * pi[None, :]
+    np.fill_diagonal(q, -q.sum(axis=1))
+    q /= -np.sum(pi * np.diag(q))
+    t = -0.1 * np.log1p(-uniforms_host(config_seed(3, STREAM_LENGTHS), 0, n - 1))
+    return parents, leaves, obs, root, emission, q, pi, t
+
+
+def run_phylogenetic(args, thmm, oracle, torch) -> int:
+    """1,024 sites on one balanced binary tree of 4,096 leaves (8,191 nodes),
+    K = 4, FP32 categorical model. A step sets the branch lengths (edge
+    factors expm(Q t_e) formed on the device), runs the forward half with the
+    observations and factors resident (kNone) and the backward half giving
+    d (sum of log Z) / d t_e. The kFactors form (host expm, edges restaged)
+    is timed too."""
+    sites, k, categories = 1024, 4, 5
+    parents, leaves, obs, root, emission, q, pi, t = phylo_inputs(sites)
+    plan = thmm.Plan(parents)
+    n, e = plan.num_nodes, plan.num_edges
+    factors = thmm.expm_reversible(q, pi, t)
+    model = thmm.BatchedCategoricalModelView(plan, k, sites, categories, leaves, obs, root.astype(np.float32),
+                                             emission.astype(np.float32), factors.astype(np.float32))
+    ws = thmm.Workspace()
+    ws.reserve_categorical_marginals(plan, k, sites, categories, leaves, precision="f32")
+    ws.categorical_logz_forward(model)  # stages everything (kAll)
+    ws.branch_length_backward(q, None)
+    lengths = [t * (1.0 + 0.01 * i) for i in range(max(args.steps, args.warmup) + 1)]
+
+    def step(i):
+        ws.categorical_set_branch_lengths(q, pi, lengths[i])
+        ws.categorical_logz_forward(model, thmm.CategoricalInputUpdate.kNone)
+        return ws.branch_length_backward(q, None)
+
+    for i in range(args.warmup):
+        step(i)
+    start = time.perf_counter()
+    for i in range(args.steps):
+        grad = step(i)
+    ms = (time.perf_counter() - start) / args.steps * 1e3
+    start = time.perf_counter()
+    for i in range(args.steps):
+        ws.categorical_logz_forward(model, thmm.CategoricalInputUpdate.kFactors)
+        ws.branch_length_backward(q, None)
+    ms_factors = (time.perf_counter() - start) / args.steps * 1e3
+    # Parity: the last step's lengths. log Z of 16 sites against the oracle,
+    # the FP32 gradient against the FP64 form and against central finite
+    # differences of the FP64 form's summed log Z on 4 edges.
+    last = lengths[args.steps - 1]
+    lz32 = ws.categorical_logz_forward(model, thmm.CategoricalInputUpdate.kNone)
+    ws.categorical_set_branch_lengths(q, pi, last)
+    lz32 = ws.categorical_logz_forward(model, thmm.CategoricalInputUpdate.kNone)
+    grad32 = ws.branch_length_backward(q, None)
+    ws64 = thmm.Workspace()
+    ws64.reserve_categorical_marginals(plan, k, sites, categories, leaves, precision="f64")
+    model64 = thmm.BatchedCategoricalModelView(plan, k, sites, categories, leaves, obs, root, emission,
+                                               thmm.expm_reversible(q, pi, last))
+    lz64 = ws64.categorical_logz_forward(model64)
+    grad64 = ws64.branch_length_backward(q, None)
+    fd, picks = [], [0, e // 3, e // 2, e - 1]
+    for j in picks:
+        h = 1e-5 * max(last[j], 1e-3)
+        vals = []
+        for sign in (1.0, -1.0):
+            tt = last.copy()
+            tt[j] += sign * h
+            m = thmm.BatchedCategoricalModelView(plan, k, sites, categories, leaves, obs, root, emission,
+                                                 thmm.expm_reversible(q, pi, tt))
+            vals.append(float(np.sum(ws64.categorical_logz_forward(m))))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    picked = np.arange(0, sites, sites // 16)
+    nodes = np.ones((picked.size, n, k))
+    nodes[:, 0, :] = root.astype(np.float32)
+    nodes[:, leaves, :] *= emission[obs[picked]]
+    p32 = thmm.expm_reversible(q, pi, last).astype(np.float32).astype(np.float64)
+    olz, _, _ = oracle.marginals_batched(plan, k, nodes, p32, threads=max(1, min(CPU_SHARE, host_cpus()["affinity"])))
+    parity = {
+        "log_z_rel_vs_oracle_16_sites": float(np.max(np.abs(lz32[picked] - olz[:, 0]) / np.maximum(1, np.abs(olz[:, 0])))),
+        "log_z_rel_fp32_vs_fp64": float(np.max(np.abs(lz32 - lz64) / np.maximum(1, np.abs(lz64)))),
+        "grad_rel_fp32_vs_fp64": float(np.max(np.abs(grad32 - grad64)) / max(1.0, np.max(np.abs(grad64)))),
+        "grad_fp64_vs_central_fd_rel": float(np.max(np.abs(grad64[picks] - fd) / np.maximum(1.0, np.abs(fd)))),
+        "grad_fp32_vs_central_fd_rel": float(np.max(np.abs(grad32[picks] - fd) / np.maximum(1.0, np.abs(fd)))),
+    }
+    ws64.close()
+    # Roofline of the dominant level kernel: the same levels profiled on the
+    # dense form of the model (node factors materialised [B][N][K] FP32).
+    dense = torch.ones((sites, n, k), device="cuda")
+    dense[:, 0, :] = torch.from_numpy(root.astype(np.float32)).cuda()
+    dense[:, torch.from_numpy(leaves.astype(np.int64)).cuda(), :] *= torch.from_numpy(
+        emission.astype(np.float32)[obs]).cuda()
+    dedges = torch.from_numpy(p32.astype(np.float32)).cuda()
+    wsd, outd, calld, dargs = marginal_pass(thmm, torch, plan, k, sites, "f32", dense, dedges)
+    calld()
+    timings = wsd.profile_marginals_device(*dargs)
+    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan))
+    wsd.close()
+    log(f"categorical K=4: {ms:.3f} ms/step (kFactors {ms_factors:.3f}); parity {parity}")
+    cpu = None
+    if not args.no_cpu:
+        sample = 128
+        cn = np.ones((sample, n, k))
+        cn[:, 0, :] = root
+        cn[:, leaves, :] *= emission[obs[:sample]]
+        pf = thmm.expm_reversible(q, pi, last)
+        ratio = np.matmul(q, pf) / pf
+        threads = max(1, min(CPU_SHARE, host_cpus()["affinity"]))
+
+        def chain():
+            _, _, em = oracle.marginals_batched(plan, k, cn, pf, threads=threads)
+            return np.einsum("bepc,epc->e", em, ratio)
+
+        cpu = cpu_median(f"{sample} of the {sites} sites ({sample * n} node-sites), FP64 oracle log Z + marginals + "
+                         "branch-length chain, one site per thread", sample * n, chain)
+    ws.close()
+    return result_line(
+        "nodes/s log Z + d log Z/d t_e, FP32 categorical (1024 sites x 8191 nodes)", sites * n / (ms * 1e-3), args, ms,
+        "f32", "synthetic (SplitMix64, seed 0x54484D4D00000003: observations uniform in {0..3} with 5% missing, root "
+        "Dirichlet(1), emission [I;1], reversible Q, t_e ~ Exp(mean 0.1))",
+        {"workload": "1024 sites on one balanced binary tree of 4096 leaves (8191 nodes), K=4, FP32 categorical; step = "
+         "device expm of the branch lengths + forward (kNone) + backward to d log Z/d t_e (host wall clock, "
+         "synchronous calls)", "nodes": int(sites * n), "levels": int(plan.primitive_batches.size)},
+        roofline=roofline, cpu_baseline=cpu, kfactors_ms_per_step=round(ms_factors, 4), parity=parity,
+        grad_norm=float(np.linalg.norm(grad)))
+
+
 def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
     """Random full binary (Yule) trees of 16,383 nodes, K = 64, log node and
     edge potentials N(0, 1), FP32: log Z plus the gradients of every node and
@@ -656,7 +1050,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=4, choices=[4, 5],
+    ap.add_argument("--config", type=int, default=4, choices=[1, 2, 3, 4, 5],
                     help="4: the uneven K=16 forest (default); 5: the K=64 Yule forest through the adjoint pair")
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the trees (quick runs)")
     ap.add_argument("--no-fp64", action="store_true")
@@ -688,6 +1082,8 @@ def main() -> int:
     log(f"device: {thmm.device_description(0)}")
     if args.config == 5:
         return run_dense_state_adjoint(args, thmm, oracle, torch)
+    if args.config in (1, 2, 3):
+        return {1: run_recursive_fp64, 2: run_long_chains, 3: run_phylogenetic}[args.config](args, thmm, oracle, torch)
 
     if args.copy_calibration:
         # One plain device-to-device copy of known size: reading FETCH_SIZE and
