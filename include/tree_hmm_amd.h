@@ -469,6 +469,26 @@ thmm_status thmm_log_partition_device_static(const thmm_plan *plan, size_t state
 /* Blocks until the workspace stream has drained. */
 thmm_status thmm_synchronize(thmm_workspace *workspace);
 
+/* No reference counterpart (the reference stores every tape and path,
+ * src/gpu_backend_impl.inc:1913-1966). A budget, in bytes, for the path
+ * storage of the workspace's next reservations: the composed paths, their
+ * FP64 row scales and, for marginal reservations, their adjoints (0, the
+ * default: no budget). When a reservation's path storage would exceed it,
+ * a batch-1 forest whose trees occupy consecutive node ranges (device-
+ * resident, without MAP or sampling) runs in groups of consecutive trees
+ * that reuse one path store, each group's storage within the budget;
+ * anything else is a length_error, and so is a single tree above the
+ * budget. In tree groups thmm_marginals_device (and thmm_log_partition_device)
+ * runs each group's forward and reverse pass in turn, and
+ * thmm_logz_backward_device recomputes each group's paths from the stored
+ * node state and the caller's log edges (which must stay unchanged between
+ * the forward and the backward call) before the group's reverse pass; the
+ * results are bit-equal to an unbudgeted workspace's. thmm_path_storage
+ * reports the groups of the current reservation (0: not grouped) and the
+ * bytes of path storage it holds. */
+thmm_status thmm_set_path_budget(thmm_workspace *workspace, size_t bytes);
+thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, size_t *bytes);
+
 /* ------------------------------------------------------------------------ */
 /* Launch tuning                                                            */
 /* ------------------------------------------------------------------------ */

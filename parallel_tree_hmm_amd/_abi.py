@@ -222,6 +222,8 @@ PROTOTYPES = {
     ),
     "thmm_log_partition_device": (c_status, [c_void_p, c_void_p, c_void_p, c_size, c_void_p, c_void_p]),
     "thmm_static_scratch_bytes": (c_status, [c_void_p, c_size, c_size, c_int, P(c_size)]),
+    "thmm_set_path_budget": (c_status, [c_void_p, c_size]),
+    "thmm_path_storage": (c_status, [c_void_p, P(c_size), P(c_size)]),
     "thmm_log_partition_device_static": (
         c_status,
         [c_void_p, c_size, c_size, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int],

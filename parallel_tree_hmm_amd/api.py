@@ -356,6 +356,18 @@ class Workspace:
         self._plan, self.dtype, self.states, self.capacity = plan, dtype, states, batch_capacity
         self.categories, self.observation_nodes = categories, obs
 
+    def set_path_budget(self, nbytes: int) -> None:
+        """Path-storage budget (bytes) for the next reservations; 0 = none.
+        Forests above it run in tree groups (see thmm_set_path_budget)."""
+        _check(_lib().thmm_set_path_budget(self._handle, int(nbytes)))
+
+    def path_storage(self) -> tuple:
+        """(tree groups of the reservation, 0 if not grouped; bytes of path
+        storage held)."""
+        groups, nbytes = ctypes.c_size_t(), ctypes.c_size_t()
+        _check(_lib().thmm_path_storage(self._handle, ctypes.byref(groups), ctypes.byref(nbytes)))
+        return groups.value, nbytes.value
+
     def reserve(self, plan, states, batch_capacity, device=0, precision="f32", flags=0):
         self._reserve(plan, states, batch_capacity, flags, device, precision)
 

@@ -328,6 +328,17 @@ thmm_status thmm_log_partition_device(thmm_workspace *workspace, const void *d_n
   return Guard([&] { W(workspace).LogPartitionDevice(d_node_potentials, d_edge_potentials, batch, d_log_z, stream); });
 }
 
+thmm_status thmm_set_path_budget(thmm_workspace *workspace, size_t bytes) {
+  return Guard([&] { W(workspace).SetPathBudget(bytes); });
+}
+
+thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, size_t *bytes) {
+  return Guard([&] {
+    Require(workspace != nullptr && groups != nullptr && bytes != nullptr, "argument is null");
+    workspace->workspace.PathStorage(groups, bytes);
+  });
+}
+
 thmm_status thmm_static_scratch_bytes(const thmm_plan *plan, size_t states, size_t batch,
                                       thmm_precision precision, size_t *bytes) {
   return Guard([&] {

@@ -53,15 +53,23 @@ std::size_t RealSize(Real real) { return real == Real::kF32 ? sizeof(float) : si
 struct DeviceBuffer {
   void *ptr = nullptr;
   std::size_t bytes = 0;
+  bool owned = true;  // false: a view into memory another buffer owns
   void Allocate(std::size_t size, const char *what) {
     Release();
     Hip(hipMalloc(&ptr, std::max<std::size_t>(size, 1)), what);
     bytes = size;
   }
+  void View(void *at, std::size_t size) noexcept {
+    Release();
+    ptr = at;
+    bytes = size;
+    owned = false;
+  }
   void Release() noexcept {
-    if (ptr != nullptr) static_cast<void>(hipFree(ptr));
+    if (ptr != nullptr && owned) static_cast<void>(hipFree(ptr));
     ptr = nullptr;
     bytes = 0;
+    owned = true;
   }
   template <typename V> V *as() const { return static_cast<V *>(ptr); }
   ~DeviceBuffer() { Release(); }
@@ -142,6 +150,10 @@ struct Workspace::Impl {
   DeviceBuffer compress_edges;
   std::uint32_t compress_edge_count = 0, max_rake_level = 0;
   const void *forward_log_edges = nullptr;
+  // The log edges of the held forward pass and how it read them (a tree
+  // group's backward half stages them again to recompute its paths).
+  const void *forward_log_source = nullptr;
+  bool forward_in_place = false, forward_compress_in_place = false;
   // Instantiated hipGraphs of the device-resident entry points (reserve flag
   // THMM_RESERVE_GRAPH), one per entry point, keyed by its arguments.
   struct GraphKey {
@@ -162,6 +174,25 @@ struct Workspace::Impl {
   };
   GraphCache graphs[4];
   Tuning tuning;
+
+  // Budget for path storage (paths, their row scales and path adjoints) in
+  // bytes, 0 for none; set before a reservation and kept across them.
+  std::size_t path_budget = 0;
+  // A forest whose path storage exceeds the budget runs in groups of
+  // consecutive trees. Each group is a workspace reserved on the group's own
+  // plan (local node, edge and root numbering) whose node state and edge row
+  // shifts are views into this workspace's buffers, and whose paths, path
+  // adjoints, branches and staged edges are views into one arena that every
+  // group reuses in turn.
+  struct TreeGroup {
+    std::unique_ptr<Plan> plan;
+    std::unique_ptr<Impl> w;
+    std::size_t first_node = 0, first_edge = 0, first_root = 0;
+  };
+  std::vector<TreeGroup> groups;
+  DeviceBuffer arena;
+  // This workspace is a group of another one: its state buffers are views.
+  bool group_member = false;
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
@@ -195,6 +226,9 @@ struct Workspace::Impl {
       static_cast<void>(hipDeviceSynchronize());  // graph replays may run on other streams
     }
     for (GraphCache &g : graphs) g.Release();
+    groups.clear();
+    arena.Release();
+    group_member = false;
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
                             &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
@@ -236,6 +270,8 @@ struct Workspace::Impl {
     grouped_levels.clear();
     compress_edge_count = max_rake_level = 0;
     forward_log_edges = nullptr;
+    forward_log_source = nullptr;
+    forward_in_place = forward_compress_in_place = false;
   }
 
   bool device_resident() const { return (flags & kReserveDeviceResident) != 0; }
@@ -555,15 +591,22 @@ void BuildPrograms(Impl &w, const Plan &plan, const std::vector<DeviceRake> &rak
   }
 }
 
+void ReserveGroups(Impl &w, const Plan &plan, const void *identity, std::size_t states, Real real, unsigned flags,
+                   int device, std::size_t slot_bytes);
+
+// `group_member`: reserve a tree group of another workspace (ops and
+// programs only; the owner assigns the state buffers as views).
 void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t states, std::size_t batch,
-                   Real real, unsigned flags, int device) {
+                   Real real, unsigned flags, int device, bool group_member = false) {
   if (states == 0 || batch == 0) throw std::invalid_argument("accelerator state and batch counts must be nonzero");
   if (states > 64) throw std::invalid_argument("state count exceeds the accelerator limit of 64");
   if (real != Real::kF32 && real != Real::kF64) throw std::invalid_argument("unknown precision");
   if ((flags & kReserveHalfEdgeMarginals) &&
       !((flags & kReserveDeviceResident) && (flags & THMM_RESERVE_MARGINALS)))
     throw std::invalid_argument("half-precision edge marginals need a device-resident marginals reservation");
+  const std::size_t budget = w.path_budget;
   w.Clear();
+  w.path_budget = budget;
   Hip(hipSetDevice(device), "GPU runtime SetDevice");
   w.plan = &plan;
   w.identity = identity;
@@ -626,8 +669,34 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   const std::size_t edge_values = Product({E, KK}, "edge inputs");
   const std::size_t path_values = Product({C, batch, KK}, "path workspace");
   const std::size_t branch_values = Product({R, batch, K}, "branch workspace");
+  w.group_member = group_member;
+  if (group_member) {
+    Hip(hipStreamSynchronize(w.stream), "topology upload");
+    return;
+  }
+  // Path storage per compression slot and item: the path, its row scales
+  // and, for marginals, its adjoint.
+  const std::size_t slot_bytes =
+      KK * S + K * sizeof(double) + ((flags & THMM_RESERVE_MARGINALS) ? KK * S : 0);
+  const std::size_t path_bytes = Product({C, batch, slot_bytes}, "path storage");
+  if (w.path_budget != 0 && path_bytes > w.path_budget) {
+    if (batch != 1 || !resident || (flags & (THMM_RESERVE_MAXIMUM | THMM_RESERVE_SAMPLING)) || roots < 2)
+      throw std::length_error("path storage of " + std::to_string(path_bytes) + " bytes exceeds the budget of " +
+                              std::to_string(w.path_budget) +
+                              " bytes; only a device-resident batch-1 forest reservation without MAP or sampling "
+                              "can run in tree groups");
+    w.nodes.Allocate(node_values * S, "node storage");
+    w.node_scale.Allocate(N * batch * sizeof(double), "node scales");
+    w.edge_scale.Allocate(E * K * sizeof(double), "edge row scales");
+    w.log_z.Allocate(batch * roots * sizeof(double), "log partitions");
+    ReserveGroups(w, plan, identity, states, real, flags, device, slot_bytes);
+    Hip(hipStreamSynchronize(w.stream), "topology upload");
+    return;
+  }
 
-  w.edges.Allocate(edge_values * S, "edge storage");
+  // Device-resident reservations stage edges only for log-potential calls
+  // that need it (EnsureEdgeStaging).
+  if (!resident) w.edges.Allocate(edge_values * S, "edge storage");
   w.edge_scale.Allocate(E * K * sizeof(double), "edge row scales");
   w.nodes.Allocate(node_values * S, "node storage");
   w.node_scale.Allocate(N * batch * sizeof(double), "node scales");
@@ -683,6 +752,104 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
     }
   }
   Hip(hipStreamSynchronize(w.stream), "topology upload");
+}
+
+// Splits a forest whose trees occupy consecutive node ranges into groups of
+// consecutive trees whose path storage fits w.path_budget, reserves each
+// group on its own plan and assigns its buffers (see Impl::TreeGroup). The
+// group plans are planned from the groups' parent arrays: the planner
+// contracts the trees of a forest independently, so each group plan holds
+// the same ops, in the same order, as the forest plan holds for those trees
+// (checked here through the edge arrays), and every kernel computes each op
+// on its own, so grouped calls give the values of ungrouped ones bit for
+// bit.
+void ReserveGroups(Impl &w, const Plan &plan, const void *identity, std::size_t states, Real real, unsigned flags,
+                   int device, std::size_t slot_bytes) {
+  const std::size_t N = plan.num_nodes, E = plan.num_edges, T = plan.roots.size(), K = states, KK = K * K;
+  const std::size_t S = RealSize(real);
+  const auto too_big = [&](const std::string &why) {
+    throw std::length_error("path storage exceeds the budget of " + std::to_string(w.path_budget) + " bytes and " +
+                            why);
+  };
+  if (plan.roots.front() != 0) too_big("the forest does not start with a root");
+  // Tree of every node: tree t spans [roots[t], roots[t + 1]).
+  std::vector<std::uint32_t> tree(N);
+  for (std::size_t t = 0; t < T; ++t) {
+    const std::size_t end = t + 1 < T ? plan.roots[t + 1] : N;
+    std::fill(tree.begin() + plan.roots[t], tree.begin() + static_cast<std::ptrdiff_t>(end), static_cast<std::uint32_t>(t));
+  }
+  std::vector<std::int64_t> parent(N, -1);
+  for (std::size_t e = 0; e < E; ++e) {
+    if (tree[plan.edge_parents[e]] != tree[plan.edge_children[e]])
+      too_big("its trees do not occupy consecutive node ranges (tree groups need them)");
+    parent[plan.edge_children[e]] = plan.edge_parents[e];
+  }
+  std::vector<std::size_t> per_tree(T, 0);
+  for (const thmm_compression &c : plan.compressions) ++per_tree[tree[c.middle]];
+  // Greedy groups of consecutive trees.
+  std::vector<std::size_t> starts;
+  std::size_t used = 0;
+  for (std::size_t t = 0; t < T; ++t) {
+    const std::size_t need = per_tree[t] * slot_bytes;
+    if (need > w.path_budget)
+      too_big("tree " + std::to_string(t) + " alone needs " + std::to_string(need) + " bytes of path storage");
+    if (starts.empty() || used + need > w.path_budget) starts.push_back(t), used = 0;
+    used += need;
+  }
+  starts.push_back(T);
+  // Reserve the groups; the arena holds the largest group's pieces.
+  const bool marginals = (flags & THMM_RESERVE_MARGINALS) != 0;
+  const bool stage = !(real == Real::kF32 && (K == 32 || K == 64));
+  std::size_t max_c = 0, max_r = 0, max_e = 0;
+  bool branch_adjoints = false;
+  for (std::size_t g = 0; g + 1 < starts.size(); ++g) {
+    const std::size_t t0 = starts[g], t1 = starts[g + 1];
+    const std::size_t n0 = plan.roots[t0], n1 = t1 < T ? plan.roots[t1] : N;
+    std::vector<std::int64_t> local(n1 - n0);
+    for (std::size_t i = n0; i < n1; ++i) local[i - n0] = parent[i] < 0 ? -1 : parent[i] - static_cast<std::int64_t>(n0);
+    Impl::TreeGroup group;
+    group.plan = std::make_unique<Plan>(MakePlan(local.data(), local.size()));
+    group.first_node = n0;
+    group.first_edge = n0 - t0;
+    group.first_root = t0;
+    const Plan &sub = *group.plan;
+    bool same = sub.roots.size() == t1 - t0 && sub.num_edges == (n1 - n0) - (t1 - t0);
+    for (std::size_t e = 0; same && e < sub.num_edges; ++e)
+      same = plan.edge_children[group.first_edge + e] == n0 + sub.edge_children[e] &&
+             plan.edge_parents[group.first_edge + e] == n0 + sub.edge_parents[e];
+    if (!same) throw std::logic_error("a tree group's plan does not number its edges as the forest plan does");
+    group.w = std::make_unique<Impl>();
+    ReserveCommon(*group.w, sub, identity, states, 1, real, flags & ~kReserveGraph, device, true);
+    max_c = std::max(max_c, sub.num_compressions());
+    max_r = std::max(max_r, sub.num_branches);
+    max_e = std::max(max_e, sub.num_edges);
+    branch_adjoints = branch_adjoints || (marginals && !group.w->rake_adjoint_from_parent);
+    w.groups.push_back(std::move(group));
+  }
+  const auto round = [](std::size_t bytes) { return (bytes + 255) & ~std::size_t{255}; };
+  const std::size_t paths = round(max_c * KK * S), path_scale = round(max_c * K * sizeof(double));
+  const std::size_t path_adj = marginals ? round(max_c * KK * S) : 0;
+  const std::size_t branches = round(max_r * K * S), branch_scale = round(max_r * sizeof(double));
+  const std::size_t branch_adj = branch_adjoints ? round(max_r * K * S) : 0;
+  const std::size_t staging = stage ? round(max_e * KK * S) : 0;
+  w.arena.Allocate(paths + path_scale + path_adj + branches + branch_scale + branch_adj + staging, "tree group arena");
+  for (Impl::TreeGroup &group : w.groups) {
+    Impl &gw = *group.w;
+    char *at = static_cast<char *>(w.arena.ptr);
+    const std::size_t n = gw.N(), e = gw.E();
+    gw.nodes.View(static_cast<char *>(w.nodes.ptr) + group.first_node * K * S, n * K * S);
+    gw.node_scale.View(w.node_scale.as<double>() + group.first_node, n * sizeof(double));
+    gw.edge_scale.View(w.edge_scale.as<double>() + group.first_edge * K, e * K * sizeof(double));
+    gw.paths.View(at, paths), at += paths;
+    gw.path_scale.View(at, path_scale), at += path_scale;
+    if (marginals) gw.path_adj.View(at, path_adj);
+    at += path_adj;
+    gw.branches.View(at, branches), at += branches;
+    gw.branch_scale.View(at, branch_scale), at += branch_scale;
+    if (marginals && !gw.rake_adjoint_from_parent) gw.branch_adj.View(at, branch_adj);
+    at += branch_adj;
+    if (stage) gw.edges.View(at, staging);
+  }
 }
 
 void RequireHost(const Impl &w) {
@@ -1109,8 +1276,10 @@ void Workspace::Reserve(const Plan &plan, const void *identity, std::size_t stat
   try {
     ReserveCommon(w, plan, identity, states, batch, real, flags, device);
     const std::size_t node_values = Product({batch, plan.num_nodes, states}, "node inputs");
-    w.input_nodes.Allocate(node_values * w.S(), "node inputs");
-    if (!w.device_resident()) w.host_nodes.Allocate(node_values * w.S(), "pinned node inputs");
+    if (!w.device_resident()) {
+      w.input_nodes.Allocate(node_values * w.S(), "node inputs");
+      w.host_nodes.Allocate(node_values * w.S(), "pinned node inputs");
+    }
   } catch (...) {
     w.Clear();
     throw;
@@ -1619,6 +1788,31 @@ void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *ed
   if (p) p->End();
 }
 
+// Byte offset of a group's first value in a forest-wide array of per-node
+// (K values), per-edge (K * K) or per-root (one) records.
+const char *At(const void *base, std::size_t first, std::size_t values, std::size_t size) {
+  return static_cast<const char *>(base) + first * values * size;
+}
+char *At(void *base, std::size_t first, std::size_t values, std::size_t size) {
+  return static_cast<char *>(base) + first * values * size;
+}
+
+// thmm_marginals_device's launches, group after group for a workspace in
+// tree groups (each group's forward, log Z, seed and reverse in turn, so
+// nothing is recomputed).
+void EnqueueMarginalsAll(Impl &w, hipStream_t on, const void *nodes, const void *edges, std::size_t batch,
+                         double *log_z, void *node_out, void *edge_out, Profiler *p) {
+  if (w.groups.empty()) {
+    EnqueueMarginals(w, on, nodes, edges, batch, log_z, node_out, edge_out, p);
+    return;
+  }
+  const std::size_t K = w.states, S = w.S(), out_s = w.half_edges() ? 2 : S;
+  for (Impl::TreeGroup &g : w.groups)
+    EnqueueMarginals(*g.w, on, At(nodes, g.first_node, K, S), At(edges, g.first_edge, K * K, S), 1,
+                     log_z + g.first_root, At(node_out, g.first_node, K, S), At(edge_out, g.first_edge, K * K, out_s),
+                     p);
+}
+
 void ValidateTuning(const Tuning &t) {
   if (t.compress_items_forward < 1 || t.compress_items_forward > 2 || t.compress_items_reverse < 1 ||
       t.compress_items_reverse > 2)
@@ -1636,6 +1830,7 @@ void ValidateTuning(const Tuning &t) {
 // drops a held forward pass: its reverse needs the groups it ran.
 void ApplyTuning(Impl &w, const Tuning &t) {
   ValidateTuning(t);
+  for (Impl::TreeGroup &g : w.groups) ApplyTuning(*g.w, t);
   const bool rebuild =
       !w.fused.empty() && (t.gather_chunk != w.tuning.gather_chunk || t.compress_depth != w.tuning.compress_depth);
   if (t.compress_depth != w.tuning.compress_depth) w.forward = Impl::Forward::kNone;
@@ -1680,12 +1875,19 @@ void Workspace::LogPartitionDevice(const void *nodes, const void *edges, std::si
   w.forward = Impl::Forward::kNone;
   Impl::GraphKey key;
   key.args[0] = nodes, key.args[1] = edges, key.args[2] = log_z, key.batch = batch;
-  RunOrReplay(w, 0, key, ChooseStream(w, stream), [&](hipStream_t s) {
-    const Launch l = w.MakeLaunch(w.real, batch, s);
-    LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-    const LinearBuffers buf = w.Linear(edges, nullptr);
-    SumProductForward(w, l, buf);
+  const auto forward = [](Impl &v, hipStream_t s, const void *nodes, const void *edges, std::size_t batch,
+                          double *log_z) {
+    const Launch l = v.MakeLaunch(v.real, batch, s);
+    LaunchInitNodes(l, v.real, nodes, v.nodes.ptr, v.node_scale.as<double>());
+    const LinearBuffers buf = v.Linear(edges, nullptr);
+    SumProductForward(v, l, buf);
     LaunchFinish(l, buf, log_z);
+  };
+  RunOrReplay(w, 0, key, ChooseStream(w, stream), [&](hipStream_t s) {
+    if (w.groups.empty()) return forward(w, s, nodes, edges, batch, log_z);
+    for (Impl::TreeGroup &g : w.groups)
+      forward(*g.w, s, At(nodes, g.first_node, w.states, w.S()), At(edges, g.first_edge, w.KK(), w.S()), 1,
+              log_z + g.first_root);
   });
 }
 
@@ -1706,7 +1908,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
   if (timings != nullptr) profiler = std::make_unique<Profiler>(s, static_cast<std::uint32_t>(batch));
   Profiler *p = profiler.get();
   const auto enqueue = [&](hipStream_t on) {
-    EnqueueMarginals(w, on, nodes, edges, batch, log_z, node_out, edge_out, p);
+    EnqueueMarginalsAll(w, on, nodes, edges, batch, log_z, node_out, edge_out, p);
   };
   if (p) {
     enqueue(s);
@@ -1750,6 +1952,115 @@ namespace {
 // in place (the K = 32 and 64 FP32 MFMA kernels) instead of staged.
 bool CompressionsReadLogEdges(const Impl &w) { return w.real == Real::kF32 && (w.states == 32 || w.states == 64); }
 
+// How a log-potential forward pass reads its edges: with the fused forward
+// the team rake kernels read the log edges in place and only the edges
+// compressions read are staged; at K = 32 and 64 (FP32) the MFMA
+// compressions read theirs in place as well.
+struct LogMode {
+  bool in_place = false, compress_in_place = false;
+};
+
+LogMode LogModeOf(const Impl &w, const void *log_edges, std::size_t batch) {
+  LogMode m;
+  m.in_place = w.tuning.rake_log_edges != 0 && !w.fused.empty() &&
+               static_cast<std::uint64_t>(w.max_rake_level) * batch * 64 < (1ull << 32) &&
+               (reinterpret_cast<std::uintptr_t>(log_edges) & 15u) == 0;
+  m.compress_in_place = m.in_place && CompressionsReadLogEdges(w);
+  return m;
+}
+
+// The staged edge buffer, allocated when a device-resident log-potential
+// call first needs it.
+void EnsureEdgeStaging(Impl &w) {
+  if (w.edges.ptr != nullptr || w.E() == 0) return;
+  if (w.group_member)
+    throw std::logic_error(
+        "a workspace in tree groups reads log edges in place at K = 32 and 64: keep rake_log_edges = 1 and "
+        "16-byte aligned log edges and gradients");
+  w.edges.Allocate(Product({w.E(), w.KK(), w.S()}, "edge storage"), "edge storage");
+}
+
+// Stages what a log-potential forward pass reads of its edges and returns
+// the buffers its levels use.
+LinearBuffers LogEdgeBuffers(Impl &w, const Launch &l, const void *log_edges, const LogMode &m) {
+  LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+  if (m.in_place && (m.compress_in_place ||
+                     LaunchStageLogEdges(l, log_edges, w.compress_edges.as<std::uint32_t>(), w.compress_edge_count,
+                                         w.edges.ptr, w.edge_scale.as<double>()))) {
+    buf.log_edges = log_edges;
+    buf.log_edge_shift = w.edge_scale.as<double>();  // rows of rake edges; staged rows keep their scales
+    buf.compress_log_edges = m.compress_in_place ? 1 : 0;
+  } else {
+    LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
+  }
+  return buf;
+}
+
+void EnqueueLogZForward(Impl &w, hipStream_t s, const void *log_nodes, const void *log_edges, std::size_t batch,
+                        double *log_z, const LogMode &m) {
+  const Launch l = w.MakeLaunch(w.real, batch, s);
+  LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
+  const LinearBuffers buf = LogEdgeBuffers(w, l, log_edges, m);
+  SumProductForward(w, l, buf);
+  LaunchFinish(l, buf, log_z);
+}
+
+// Records what the backward half needs of a forward pass.
+void HoldLogForward(Impl &w, const void *log_edges, std::size_t batch, const LogMode &m) {
+  w.forward_log_edges =
+      m.in_place && (w.states == 8 || w.states == 16 || w.states == 32 || w.states == 64) ? log_edges : nullptr;
+  w.forward_log_source = log_edges;
+  w.forward_in_place = m.in_place;
+  w.forward_compress_in_place = m.compress_in_place;
+  w.last_batch = batch;
+}
+
+// The compression levels of the forward pass that SumProductForward ran for
+// these buffers. They rewrite every stored path from its inputs (edges,
+// earlier paths and final node values), the values the forward pass wrote,
+// bit for bit; rakes and absorptions are not repeated (they add into nodes).
+void RecomputePaths(const Impl &w, const Launch &l, const LinearBuffers &buf) {
+  if (UsesFusedForward(w, buf)) {
+    for (const Impl::FusedStep &step : w.fused)
+      if (step.kind == 3 || step.kind == kChainPrimitive) LaunchForwardLevel(l, step.kind, step.offset, step.count, buf);
+  } else {
+    for (const Level &level : w.levels)
+      if (level.primitive == THMM_COMPRESSION) LaunchForwardLevel(l, level.primitive, level.offset, level.count, buf);
+  }
+}
+
+bool Aligned16(const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; }
+
+// Whether the backward half stages every edge again (unaligned outputs keep
+// the plan-level kernels, which read staged edges).
+bool BackwardRestages(const Impl &w, const void *grad_nodes, const void *grad_edges, bool reduce_edges) {
+  const void *edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
+  const void *node_adj = w.last_batch == 1 ? grad_nodes : w.node_adj.ptr;
+  return w.forward_log_edges != nullptr && !(Aligned16(edge_out) && Aligned16(node_adj));
+}
+
+void EnqueueLogZBackward(Impl &w, hipStream_t s, const double *grad, void *grad_nodes, void *grad_edges,
+                         bool reduce_edges) {
+  const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
+  LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
+  buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
+  AdjointsInto(buf, w.last_batch, grad_nodes);
+  if (w.forward_log_edges != nullptr) {
+    // The team rake kernels need 16-byte aligned outputs; otherwise the
+    // rest of the edges are staged for the plan-level kernels.
+    if (Aligned16(buf.edge_out) && Aligned16(buf.node_adj)) {
+      buf.log_edges = w.forward_log_edges;
+      buf.log_edge_shift = w.edge_scale.as<double>();
+      buf.compress_log_edges = CompressionsReadLogEdges(w) ? 1 : 0;
+    } else
+      LaunchInitEdgesFromLogs(l, w.forward_log_edges, w.edges.ptr, w.edge_scale.as<double>());
+  }
+  LaunchSeedAdjoint(l, buf, grad);
+  ReverseLevels(l, ReverseProgram(w, buf), buf);
+  if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
+  if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
+}
+
 }  // namespace
 
 void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, std::size_t batch, double *log_z,
@@ -1761,33 +2072,37 @@ void Workspace::LogZForwardDevice(const void *log_nodes, const void *log_edges, 
   w.forward = Impl::Forward::kNone;
   Impl::GraphKey key;
   key.args[0] = log_nodes, key.args[1] = log_edges, key.args[2] = log_z, key.batch = batch;
-  // With the fused forward the team rake kernels read the log edges in
-  // place and only the edges compressions read are staged; at K = 32 and 64
-  // (FP32) the MFMA compressions read theirs in place as well.
-  const bool in_place = w.tuning.rake_log_edges != 0 && !w.fused.empty() &&
-                        static_cast<std::uint64_t>(w.max_rake_level) * batch * 64 < (1ull << 32) &&
-                        (reinterpret_cast<std::uintptr_t>(log_edges) & 15u) == 0;
-  const bool compress_in_place = in_place && CompressionsReadLogEdges(w);
-  key.flag = in_place ? 1 : 0;
+  const std::size_t K = w.states, S = w.S();
+  if (!w.groups.empty()) {
+    // Tree groups: each group's forward pass in turn; only node state (and
+    // edge row shifts) outlive a group, the backward half recomputes paths.
+    std::vector<LogMode> modes;
+    for (Impl::TreeGroup &g : w.groups) {
+      modes.push_back(LogModeOf(*g.w, At(log_edges, g.first_edge, K * K, S), 1));
+      if (!modes.back().compress_in_place) EnsureEdgeStaging(*g.w);
+    }
+    RunOrReplay(w, 2, key, ChooseStream(w, stream), [&](hipStream_t s) {
+      for (std::size_t i = 0; i < w.groups.size(); ++i) {
+        const Impl::TreeGroup &g = w.groups[i];
+        EnqueueLogZForward(*g.w, s, At(log_nodes, g.first_node, K, S), At(log_edges, g.first_edge, K * K, S), 1,
+                           log_z + g.first_root, modes[i]);
+      }
+    });
+    for (std::size_t i = 0; i < w.groups.size(); ++i)
+      HoldLogForward(*w.groups[i].w, At(log_edges, w.groups[i].first_edge, K * K, S), 1, modes[i]);
+    HoldLogForward(w, log_edges, batch, LogMode{});
+    w.forward_log_edges = log_edges;  // names the forward pass in the backward half's graph key
+    w.forward = Impl::Forward::kLogPotentials;
+    return;
+  }
+  const LogMode mode = LogModeOf(w, log_edges, batch);
+  if (!mode.compress_in_place) EnsureEdgeStaging(w);
+  key.flag = mode.in_place ? 1 : 0;
   w.forward_log_edges = nullptr;
   RunOrReplay(w, 2, key, ChooseStream(w, stream), [&](hipStream_t s) {
-    const Launch l = w.MakeLaunch(w.real, batch, s);
-    LaunchInitNodesFromLogs(l, log_nodes, w.nodes.ptr, w.node_scale.as<double>());
-    LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-    if (in_place && (compress_in_place ||
-                     LaunchStageLogEdges(l, log_edges, w.compress_edges.as<std::uint32_t>(), w.compress_edge_count,
-                                         w.edges.ptr, w.edge_scale.as<double>()))) {
-      buf.log_edges = log_edges;
-      buf.log_edge_shift = w.edge_scale.as<double>();  // rows of rake edges; staged rows keep their scales
-      buf.compress_log_edges = compress_in_place ? 1 : 0;
-    }
-    else
-      LaunchInitEdgesFromLogs(l, log_edges, w.edges.ptr, w.edge_scale.as<double>());
-    SumProductForward(w, l, buf);
-    LaunchFinish(l, buf, log_z);
+    EnqueueLogZForward(w, s, log_nodes, log_edges, batch, log_z, mode);
   });
-  if (in_place && (w.states == 8 || w.states == 16 || w.states == 32 || w.states == 64)) w.forward_log_edges = log_edges;
-  w.last_batch = batch;
+  HoldLogForward(w, log_edges, batch, mode);
   w.forward = Impl::Forward::kLogPotentials;
 }
 
@@ -1807,26 +2122,29 @@ void Workspace::LogZBackwardDevice(const double *grad, void *grad_nodes, void *g
   key.args[0] = grad, key.args[1] = grad_nodes, key.args[2] = grad_edges, key.batch = w.last_batch;
   key.args[3] = w.forward_log_edges;
   key.flag = reduce_edges ? 1 : 0;
+  const std::size_t K = w.states, S = w.S(), out_s = w.half_edges() ? 2 : S;
+  if (!w.groups.empty()) {
+    // Per group: stage its edges again as its forward pass did, recompute
+    // its paths, then its reverse pass.
+    for (Impl::TreeGroup &g : w.groups)
+      if (BackwardRestages(*g.w, At(grad_nodes, g.first_node, K, S), At(grad_edges, g.first_edge, K * K, out_s),
+                           false))
+        EnsureEdgeStaging(*g.w);
+    RunOrReplay(w, 3, key, ChooseStream(w, stream), [&](hipStream_t s) {
+      for (Impl::TreeGroup &g : w.groups) {
+        Impl &v = *g.w;
+        const Launch l = v.MakeLaunch(v.real, 1, s);
+        const LogMode mode{v.forward_in_place, v.forward_compress_in_place};
+        RecomputePaths(v, l, LogEdgeBuffers(v, l, v.forward_log_source, mode));
+        EnqueueLogZBackward(v, s, grad != nullptr ? grad + g.first_root : nullptr,
+                            At(grad_nodes, g.first_node, K, S), At(grad_edges, g.first_edge, K * K, out_s), false);
+      }
+    });
+    return;
+  }
+  if (BackwardRestages(w, grad_nodes, grad_edges, reduce_edges)) EnsureEdgeStaging(w);
   RunOrReplay(w, 3, key, ChooseStream(w, stream), [&](hipStream_t s) {
-    const Launch l = w.MakeLaunch(w.real, w.last_batch, s);
-    LinearBuffers buf = w.Linear(w.edges.ptr, w.edge_scale.as<double>());
-    buf.edge_out = reduce_edges ? w.edge_out.ptr : grad_edges;
-    AdjointsInto(buf, w.last_batch, grad_nodes);
-    if (w.forward_log_edges != nullptr) {
-      // The team rake kernels need 16-byte aligned outputs; otherwise the
-      // rest of the edges are staged for the plan-level kernels.
-      const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
-      if (aligned(buf.edge_out) && aligned(buf.node_adj)) {
-        buf.log_edges = w.forward_log_edges;
-        buf.log_edge_shift = w.edge_scale.as<double>();
-        buf.compress_log_edges = CompressionsReadLogEdges(w) ? 1 : 0;
-      } else
-        LaunchInitEdgesFromLogs(l, w.forward_log_edges, w.edges.ptr, w.edge_scale.as<double>());
-    }
-    LaunchSeedAdjoint(l, buf, grad);
-    ReverseLevels(l, ReverseProgram(w, buf), buf);
-    if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
-    if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
+    EnqueueLogZBackward(w, s, grad, grad_nodes, grad_edges, reduce_edges);
   });
 }
 
@@ -1863,7 +2181,7 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
     float best = std::numeric_limits<float>::infinity();
     for (int rep = 0; rep < 3; ++rep) {
       Hip(hipEventRecord(start, s), "GPU runtime EventRecord");
-      EnqueueMarginals(w, s, nodes, edges, batch, log_z, node_out, edge_out, nullptr);
+      EnqueueMarginalsAll(w, s, nodes, edges, batch, log_z, node_out, edge_out, nullptr);
       Hip(hipEventRecord(stop, s), "GPU runtime EventRecord");
       Hip(hipEventSynchronize(stop), "tree-HMM accelerator execution");
       CheckLaunches();
@@ -1904,6 +2222,18 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
   static_cast<void>(hipEventDestroy(start));
   static_cast<void>(hipEventDestroy(stop));
   return best_ms;
+}
+
+void Workspace::SetPathBudget(std::size_t bytes) { impl_->path_budget = bytes; }
+
+void Workspace::PathStorage(std::size_t *groups, std::size_t *bytes) const {
+  const Impl &w = *impl_;
+  if (w.plan == nullptr) throw std::logic_error("accelerator Workspace::Reserve must precede this call");
+  *groups = w.groups.size();
+  *bytes = w.groups.empty() ? w.paths.bytes + w.path_scale.bytes + w.path_adj.bytes : [&] {
+    const Impl &g = *w.groups.front().w;
+    return g.paths.bytes + g.path_scale.bytes + g.path_adj.bytes;
+  }();
 }
 
 void Workspace::Synchronize() {

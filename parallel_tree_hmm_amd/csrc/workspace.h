@@ -97,6 +97,12 @@ class Workspace {
                     std::uint32_t *states, std::uint32_t *failure, void *stream);
   void Synchronize();
 
+  // Path-storage budget for the next reservations (0: none) and, after a
+  // reservation, the tree groups it runs in (0: not grouped) and the bytes
+  // of path storage it holds.
+  void SetPathBudget(std::size_t bytes);
+  void PathStorage(std::size_t *groups, std::size_t *bytes) const;
+
   // Launch variants (see Tuning in launch.h).
   void SetTuning(const Tuning &t);
   Tuning GetTuning() const;
