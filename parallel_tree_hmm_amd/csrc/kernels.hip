@@ -180,18 +180,19 @@ __global__ void CompressForward(DeviceOps ops, Dims d, std::uint32_t offset, std
            : kNegInf;
 }
 
-// Team form of CompressForward for FP64 state at K = 8 and 16: K*K/4 lanes
-// per (op, item), each owning a quad of four consecutive columns of one
-// output row. The middle weights, L diag(e) and R are staged in LDS with
-// quad loads; rows are reduced across the K/4 lanes that hold them. The
-// arithmetic is CompressForward's, in the same order.
-template <int K>
+// Team form of CompressForward at K = 4, 8, 16 and 32: K*K/4 lanes per
+// (op, item), each owning a quad of four consecutive columns of one output
+// row. The middle weights, L diag(e) and R are staged in LDS with quad
+// loads; rows are reduced across the K/4 lanes that hold them. The
+// arithmetic is CompressForward's, in the same order and element type, so
+// the results are bitwise those of the generic kernel.
+template <typename T, int K>
 __global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d, std::uint32_t offset,
                                                            std::uint32_t count, LinearBuffers buf) {
   constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
-  __shared__ double weights[kItems][K];      // mid[m] + r_m
-  __shared__ double left[kItems][K * K];     // L[p][m] e[m]
-  __shared__ double right[kItems][K * K];
+  __shared__ double weights[kItems][K];  // mid[m] + r_m
+  __shared__ T left[kItems][K * K];      // L[p][m] e[m]
+  __shared__ T right[kItems][K * K];
   const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
   const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
   const bool active = item < static_cast<std::uint64_t>(count) * d.B;
@@ -202,18 +203,20 @@ __global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d
     b = static_cast<std::uint32_t>(item % d.B);
     op = ops.compresses[offset + item / d.B];
     mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
-    if (l < K) weights[t][l] = static_cast<const double *>(buf.nodes)[mid_at * K + l] + RowScale(buf, d, op.right, b, l);
+    if (l < K)
+      weights[t][l] = static_cast<double>(static_cast<const T *>(buf.nodes)[mid_at * K + l]) +
+                      RowScale(buf, d, op.right, b, l);
   }
   __syncthreads();
   double top_w = kNegInf;
   if (active) {
     for (int m = 0; m < K; ++m) top_w = fmax(top_w, weights[t][m]);
-    const Quad<double> lq = LoadSlotQuad<double>(buf, d, op.left, b, 4 * l);
-    const Quad<double> rq = LoadSlotQuad<double>(buf, d, op.right, b, 4 * l);
+    const Quad<T> lq = LoadSlotQuad<T>(buf, d, op.left, b, 4 * l);
+    const Quad<T> rq = LoadSlotQuad<T>(buf, d, op.right, b, 4 * l);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int m = (4 * l + i) % K;
-      const double e = top_w > kNegInf ? exp(weights[t][m] - top_w) : 0.0;
+      const T e = top_w > kNegInf ? static_cast<T>(exp(weights[t][m] - top_w)) : T(0);
       left[t][4 * l + i] = lq.v[i] * e;
       right[t][4 * l + i] = rq.v[i];
     }
@@ -221,26 +224,134 @@ __global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d
   __syncthreads();
   if (!active) return;
   const int p = l / kQ, c0 = (l % kQ) * 4;
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  T acc[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int m = 0; m < K; ++m) {
-    const double lm = left[t][p * K + m];
+    const T lm = left[t][p * K + m];
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
   }
-  double top = fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3]));
+  T top = Max(Max(T(0), Max(acc[0], acc[1])), Max(acc[2], acc[3]));
 #pragma unroll
-  for (int o = 1; o < kQ; o <<= 1) top = fmax(top, __shfl_xor(top, o, kQ));
-  const bool live = top > 0.0 && top_w > kNegInf;
-  const double inverse = live ? 1.0 / top : 0.0;
+  for (int o = 1; o < kQ; o <<= 1) top = Max(top, __shfl_xor(top, o, kQ));
+  const bool live = top > T(0) && top_w > kNegInf;
+  const T inverse = live ? T(1) / top : T(0);
   const std::uint64_t out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
-  Quad<double> q;
+  Quad<T> q;
 #pragma unroll
   for (int c = 0; c < 4; ++c) q.v[c] = acc[c] * inverse;
-  StoreQuad(static_cast<double *>(buf.paths) + out_at * K * K + p * K + c0, q);
+  StoreQuad(static_cast<T *>(buf.paths) + out_at * K * K + p * K + c0, q);
   if (c0 == 0)
     buf.path_scale[out_at * K + p] =
-        live ? RowScale(buf, d, op.left, b, p) + buf.node_scale[mid_at] + top_w + log(top) : kNegInf;
+        live ? RowScale(buf, d, op.left, b, p) + buf.node_scale[mid_at] + top_w + log(static_cast<double>(top))
+             : kNegInf;
+}
+
+// Team form of CompressReverse at K = 4, 8 and 16: K*K/4 lanes per (op,
+// item), lane (p, quad q) owning entries 4q..4q+3 of row p. L diag(e), L, R
+// and the ratios Gr are staged in LDS; then lane (p, q) forms row p of S and
+// Gr on its quad, adj_L[p][4q..] and, as lane (m = p, q), adj_R[m][4q..].
+// The arithmetic is CompressReverse's, in the same order and element types
+// (FP64 ratios and sums), so the results are bitwise those of the generic
+// kernel; the middle adjoint sums each row of adj_R sequentially in LDS.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void CompressReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                           std::uint32_t count, LinearBuffers buf) {
+  constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
+  __shared__ double weights[kItems][K];
+  __shared__ T left[kItems][K * K];   // L[p][m] e[m]
+  __shared__ T lraw[kItems][K * K];   // L
+  __shared__ T right[kItems][K * K];  // R
+  __shared__ double gr[kItems][K * K];
+  __shared__ double adjr[kItems][K * K];
+  const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
+  const int p = l / kQ, c0 = (l % kQ) * 4;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  DeviceCompress op{};
+  std::uint32_t b = 0;
+  std::uint64_t mid_at = 0, out_at = 0;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    op = ops.compresses[offset + item / d.B];
+    mid_at = static_cast<std::uint64_t>(op.middle) * d.B + b;
+    out_at = static_cast<std::uint64_t>(op.out - d.E) * d.B + b;
+    if (l < K)
+      weights[t][l] = static_cast<double>(static_cast<const T *>(buf.nodes)[mid_at * K + l]) +
+                      RowScale(buf, d, op.right, b, l);
+  }
+  __syncthreads();
+  double top_w = kNegInf;
+  T e_row = T(0);  // e[p]
+  Quad<T> a{};
+  if (active) {
+    for (int m = 0; m < K; ++m) top_w = fmax(top_w, weights[t][m]);
+    const Quad<T> lq = LoadSlotQuad<T>(buf, d, op.left, b, 4 * l);
+    const Quad<T> rq = LoadSlotQuad<T>(buf, d, op.right, b, 4 * l);
+    a = LoadQuad(static_cast<const T *>(buf.path_adj) + out_at * K * K + 4 * l);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = (4 * l + i) % K;
+      const T e = top_w > kNegInf ? static_cast<T>(exp(weights[t][m] - top_w)) : T(0);
+      left[t][4 * l + i] = lq.v[i] * e;
+      lraw[t][4 * l + i] = lq.v[i];
+      right[t][4 * l + i] = rq.v[i];
+    }
+    e_row = top_w > kNegInf ? static_cast<T>(exp(weights[t][p] - top_w)) : T(0);
+  }
+  __syncthreads();
+  if (active) {
+    T acc[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+      const T lm = left[t][p * K + m];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+      gr[t][p * K + c0 + c] = (a.v[c] != T(0) && acc[c] > T(0))
+                                  ? static_cast<double>(a.v[c]) / static_cast<double>(acc[c])
+                                  : 0.0;
+  }
+  __syncthreads();
+  if (active) {
+    // adj_L[p][m] for m = c0 .. c0 + 3
+    Quad<T> q;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int m = c0 + i;
+      double h = 0.0;
+#pragma unroll
+      for (int c = 0; c < K; ++c) h += gr[t][p * K + c] * static_cast<double>(right[t][m * K + c]);
+      q.v[i] = static_cast<T>(static_cast<double>(left[t][p * K + m]) * h);
+    }
+    StoreAdjointQuad<T>(SlotAdjointTarget<T>(buf, d, op.left, b), static_cast<std::uint64_t>(p) * K + c0, q);
+    // adj_R[m][c] for m = p, c = c0 .. c0 + 3
+    const int m = p;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int pp = 0; pp < K; ++pp) {
+      const double lp = static_cast<double>(lraw[t][pp * K + m]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) r[c] += lp * gr[t][pp * K + c0 + c];
+    }
+    const double em = static_cast<double>(e_row);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double value = static_cast<double>(right[t][m * K + c0 + c]) * em * r[c];
+      q.v[c] = static_cast<T>(value);
+      adjr[t][m * K + c0 + c] = value;
+    }
+    StoreAdjointQuad<T>(SlotAdjointTarget<T>(buf, d, op.right, b), static_cast<std::uint64_t>(m) * K + c0, q);
+  }
+  __syncthreads();
+  if (active && c0 == 0) {
+    double total = 0.0;
+#pragma unroll
+    for (int c = 0; c < K; ++c) total += adjr[t][p * K + c];
+    static_cast<T *>(buf.node_adj)[mid_at * K + p] = static_cast<T>(total);
+  }
 }
 
 // log Z = scale(root) + log sum exp(root), in FP64 (src/inference.cc:778-784).
@@ -506,7 +617,9 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
   Out *out = nodes + tid * d.K;
   const bool rooted = is_root[n] != 0;
   const std::uint32_t o = obs_index[n];
-  const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : obs[static_cast<std::uint64_t>(b) * obs_count + o];
+  // obs is [observation node][batch] (transposed at upload): consecutive
+  // threads read consecutive bytes.
+  const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : obs[static_cast<std::uint64_t>(o) * d.B + b];
   auto value = [&](std::uint32_t s) -> P {
     if (o == 0xFFFFFFFFu) return rooted ? root[s] : P(1);
     const P e = category < categories ? emissions[static_cast<std::uint64_t>(category) * d.K + s] : P(0);
@@ -1330,23 +1443,52 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const s
   Check(hipGetLastError(), "categorical initialisation launch");
 }
 
+// Whether a compression level goes to the team kernels: K = 4, 8, 16 or
+// `max_k` (K*K/4 lanes per item, below 2^31 work-items) and 16-byte aligned
+// slots and adjoint outputs.
+bool TeamCompressionFits(const Launch &l, std::uint32_t ops, const LinearBuffers &buf, std::uint32_t max_k) {
+  const std::uint32_t K = l.dims.K;
+  const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
+  return (K == 4 || K == 8 || K == 16 || K == max_k) &&
+         static_cast<std::uint64_t>(ops) * l.dims.B * K * K / 4 < kMaxLaunchItems && aligned(buf.edges) &&
+         aligned(buf.paths) && aligned(buf.path_adj) && aligned(buf.edge_out) && aligned(buf.node_adj);
+}
+
+template <typename T>
+void LaunchCompressForwardTeam(const Launch &l, std::uint32_t first, std::uint32_t ops, const LinearBuffers &buf) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4, 256);
+  switch (l.dims.K) {
+    case 4: CompressForwardTeam<T, 4><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+    case 8: CompressForwardTeam<T, 8><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+    case 16: CompressForwardTeam<T, 16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+    default: CompressForwardTeam<T, 32><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+  }
+  Check(hipGetLastError(), "compression level launch");
+}
+
+template <typename T>
+void LaunchCompressReverseTeam(const Launch &l, std::uint32_t first, std::uint32_t ops, const LinearBuffers &buf) {
+  const unsigned grid = BlocksFor(static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4, 256);
+  switch (l.dims.K) {
+    case 4: CompressReverseTeam<T, 4><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+    case 8: CompressReverseTeam<T, 8><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+    default: CompressReverseTeam<T, 16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf); break;
+  }
+  Check(hipGetLastError(), "compression level launch");
+}
+
 void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     if (LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
-    // FP64 compressions at K = 8, 16 and 32: the team kernel (16-byte
-    // aligned slots; K*K/4 lanes per item, below 2^31 work-items).
-    const std::uint64_t team_lanes = static_cast<std::uint64_t>(ops) * l.dims.B * l.dims.K * l.dims.K / 4;
-    if (primitive == 3 && l.real == Real::kF64 && (l.dims.K == 8 || l.dims.K == 16 || l.dims.K == 32) &&
-        team_lanes < kMaxLaunchItems && (reinterpret_cast<std::uintptr_t>(buf.edges) & 15u) == 0) {
-      const unsigned grid = BlocksFor(team_lanes, 256);
-      if (l.dims.K == 8)
-        CompressForwardTeam<8><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
-      else if (l.dims.K == 16)
-        CompressForwardTeam<16><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
+    // Compressions at K = 4 and 8, and FP64 ones at K = 16 and 32: the team
+    // kernel (16-byte aligned slots; K*K/4 lanes per item, below 2^31
+    // work-items). FP32 compressions at K = 16 and 32 took the MFMA kernels.
+    if (primitive == 3 && !buf.compress_log_edges && TeamCompressionFits(l, ops, buf, 32)) {
+      if (l.real == Real::kF32)
+        LaunchCompressForwardTeam<float>(l, first, ops, buf);
       else
-        CompressForwardTeam<32><<<grid, 256, 0, StreamOf(l)>>>(l.ops, l.dims, first, ops, buf);
-      Check(hipGetLastError(), "compression level launch");
+        LaunchCompressForwardTeam<double>(l, first, ops, buf);
       return;
     }
     if (buf.edges_f32 && (primitive == 0 || primitive == 3)) {
@@ -1387,6 +1529,15 @@ void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     if (LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
+    // Compressions at K = 4, 8 and 16 (those the MFMA kernels left): the
+    // team kernel.
+    if (primitive == 3 && !buf.compress_log_edges && TeamCompressionFits(l, ops, buf, 16)) {
+      if (l.real == Real::kF32)
+        LaunchCompressReverseTeam<float>(l, first, ops, buf);
+      else
+        LaunchCompressReverseTeam<double>(l, first, ops, buf);
+      return;
+    }
     if (buf.compress_log_edges && primitive == 3) {
       RecordLaunchError(hipErrorInvalidValue, "log edge compression inputs need the K = 32 and 64 MFMA kernels");
       return;
@@ -1412,24 +1563,65 @@ namespace {
 
 // One thread per (e, p, c): the batch-summed adjoint times the rate of change
 // of log P_e[p][c] with respect to t_e.
+// Batch-summed edge adjoint M_e[p][c] times d log P_e[p][c] / d t_e. A block
+// covers 64 consecutive entries (e, p, c); its four wavefronts sum batch
+// items b = w, w + 4, ... of them (each load instruction reads 64
+// consecutive values), and the four partial masses are added in LDS in a
+// fixed order.
 template <typename T>
-__global__ void BranchChainTerms(const T *edge_out, const T *edges, const double *rate, double *terms, Dims d) {
+__global__ __launch_bounds__(256) void BranchChainTerms(const T *edge_out, const T *edges, const double *rate,
+                                                        double *terms, Dims d) {
+  __shared__ double part[4][64];
   const std::uint64_t kk = static_cast<std::uint64_t>(d.K) * d.K;
-  for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(d.E) * kk; tid += GridStride()) {
-    const std::uint64_t e = tid / kk;
-    const std::uint32_t p = static_cast<std::uint32_t>((tid % kk) / d.K), c = static_cast<std::uint32_t>(tid % d.K);
+  const std::uint64_t total = static_cast<std::uint64_t>(d.E) * kk;
+  const std::uint32_t lane = threadIdx.x % 64, slice = threadIdx.x / 64;
+  for (std::uint64_t base = static_cast<std::uint64_t>(blockIdx.x) * 64; base < total;
+       base += static_cast<std::uint64_t>(gridDim.x) * 64) {
+    const std::uint64_t tid = base + lane;
     double mass = 0.0;
-    for (std::uint32_t b = 0; b < d.B; ++b)
-      mass += static_cast<double>(edge_out[static_cast<std::uint64_t>(b) * d.E * kk + tid]);
-    const T *P = edges + e * kk;
-    const double value = static_cast<double>(P[p * d.K + c]);
-    double term = 0.0;
-    if (value > 0.0 && mass != 0.0) {
-      double slope = 0.0;
-      for (std::uint32_t j = 0; j < d.K; ++j) slope += rate[p * d.K + j] * static_cast<double>(P[j * d.K + c]);
-      term = mass * slope / value;
+    if (tid < total) {
+#pragma unroll 8
+      for (std::uint32_t b = slice; b < d.B; b += 4) mass += static_cast<double>(edge_out[b * total + tid]);
     }
-    terms[tid] = term;
+    part[slice][lane] = mass;
+    __syncthreads();
+    if (slice == 0 && tid < total) {
+      mass = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+      const std::uint64_t e = tid / kk;
+      const std::uint32_t p = static_cast<std::uint32_t>((tid % kk) / d.K), c = static_cast<std::uint32_t>(tid % d.K);
+      const T *P = edges + e * kk;
+      const double value = static_cast<double>(P[p * d.K + c]);
+      double term = 0.0;
+      if (value > 0.0 && mass != 0.0) {
+        double slope = 0.0;
+        for (std::uint32_t j = 0; j < d.K; ++j) slope += rate[p * d.K + j] * static_cast<double>(P[j * d.K + c]);
+        term = mass * slope / value;
+      }
+      terms[tid] = term;
+    }
+    __syncthreads();
+  }
+}
+
+// in [rows][cols] bytes -> out [cols][rows] through a 64 x 64 LDS tile; the
+// grid covers the tiles (block i: column tile i % col_tiles, row tile
+// i / col_tiles, stepping by the grid).
+__global__ __launch_bounds__(256) void TransposeBytes(const std::uint8_t *in, std::uint8_t *out, std::uint32_t rows,
+                                                      std::uint32_t cols, std::uint64_t tiles,
+                                                      std::uint32_t col_tiles) {
+  __shared__ std::uint8_t tile[64][65];
+  for (std::uint64_t i = blockIdx.x; i < tiles; i += gridDim.x) {
+    const std::uint64_t r0 = (i / col_tiles) * 64, c0 = (i % col_tiles) * 64;
+    for (std::uint32_t k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+      const std::uint64_t r = r0 + k / 64, c = c0 + k % 64;
+      if (r < rows && c < cols) tile[k / 64][k % 64] = in[r * cols + c];
+    }
+    __syncthreads();
+    for (std::uint32_t k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
+      const std::uint64_t c = c0 + k / 64, r = r0 + k % 64;
+      if (r < rows && c < cols) out[c * rows + r] = tile[k % 64][k / 64];
+    }
+    __syncthreads();
   }
 }
 
@@ -1481,14 +1673,25 @@ void LaunchBranchLengthChain(const Launch &l, const void *edge_out, const void *
                              double *terms, double *grad) {
   const std::uint64_t entries = static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
   if (entries == 0) return;
+  const unsigned blocks = GridStrideBlocks(entries, 64);
   if (l.real == Real::kF32)
-    BranchChainTerms<float><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
-        static_cast<const float *>(edge_out), static_cast<const float *>(edges), rate, terms, l.dims);
+    BranchChainTerms<float><<<blocks, 256, 0, StreamOf(l)>>>(static_cast<const float *>(edge_out),
+                                                             static_cast<const float *>(edges), rate, terms, l.dims);
   else
-    BranchChainTerms<double><<<GridStrideBlocks(entries, kThreads), kThreads, 0, StreamOf(l)>>>(
-        static_cast<const double *>(edge_out), static_cast<const double *>(edges), rate, terms, l.dims);
+    BranchChainTerms<double><<<blocks, 256, 0, StreamOf(l)>>>(static_cast<const double *>(edge_out),
+                                                              static_cast<const double *>(edges), rate, terms, l.dims);
   BranchChainSums<<<BlocksFor(l.dims.E, kThreads), kThreads, 0, StreamOf(l)>>>(terms, grad, l.dims);
   Check(hipGetLastError(), "branch-length chain launch");
+}
+
+void LaunchTransposeObservations(const Launch &l, const std::uint8_t *in, std::uint8_t *out, std::uint32_t rows,
+                                 std::uint32_t cols) {
+  if (rows == 0 || cols == 0) return;
+  const std::uint32_t col_tiles = (cols + 63) / 64;
+  const std::uint64_t tiles = static_cast<std::uint64_t>(col_tiles) * ((rows + 63) / 64);
+  const unsigned grid = static_cast<unsigned>(tiles < (1u << 20) ? tiles : (1u << 20));
+  TransposeBytes<<<grid, 256, 0, StreamOf(l)>>>(in, out, rows, cols, tiles, col_tiles);
+  Check(hipGetLastError(), "observation transpose launch");
 }
 
 void LaunchReduceEdgesOverBatch(const Launch &l, const void *edge_out, void *out) {
