@@ -186,7 +186,12 @@ bool LaunchStageLogEdges(const Launch &l, const void *log_edges, const std::uint
                          void *edges, double *edge_scale);
 void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges,
                              double *edge_scale);
-// Categorical observations -> internal [N][B][K] node factors, the product
+// Observations [rows = batch][cols = observation nodes] (u8) -> [cols][rows],
+// the layout LaunchInitCategorical reads.
+void LaunchTransposeObservations(const Launch &l, const std::uint8_t *in, std::uint8_t *out, std::uint32_t rows,
+                                 std::uint32_t cols);
+// Categorical observations ([observation node][batch] u8) -> internal
+// [N][B][K] node factors, the product
 // root-or-one times emission formed in `product_real` so that it is bitwise
 // equal to the dense construction. `raw` stores the products as they are
 // (element type product_real); otherwise max-shifted logs of element type

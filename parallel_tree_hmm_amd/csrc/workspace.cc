@@ -198,7 +198,8 @@ struct Workspace::Impl {
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
   // inputs
   PinnedBuffer host_nodes, host_edges, host_uniforms, host_observations, host_root, host_emissions;
-  DeviceBuffer input_nodes, uniforms, observations, root_potential, emissions;
+  // observations [B][O] as uploaded, observations_t [O][B] as the kernels read them
+  DeviceBuffer input_nodes, uniforms, observations, observations_t, root_potential, emissions;
   // scaled linear state (Scalar)
   DeviceBuffer edges, edge_scale, nodes, node_scale, paths, path_scale, branches, branch_scale, log_z;
   // adjoint
@@ -231,7 +232,8 @@ struct Workspace::Impl {
     group_member = false;
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
-                            &input_nodes, &uniforms, &observations, &root_potential, &emissions, &edges,
+                            &input_nodes, &uniforms, &observations, &observations_t, &root_potential,
+                            &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
                             &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
                             &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
@@ -1054,10 +1056,14 @@ CategoricalPolicy StageCategorical(Impl &w, const thmm_categorical_model &model,
 }
 
 void UploadCategorical(Impl &w, const thmm_categorical_model &model, const CategoricalPolicy &policy) {
-  if (policy.update_observations && model.observation_count != 0)
+  if (policy.update_observations && model.observation_count != 0) {
     Hip(hipMemcpyAsync(w.observations.ptr, w.host_observations.ptr, model.observation_count, hipMemcpyHostToDevice,
                        w.stream),
         "GPU runtime MemcpyAsync categorical observation upload");
+    LaunchTransposeObservations(w.MakeLaunch(w.real, model.batch, w.stream), w.observations.as<std::uint8_t>(),
+                                w.observations_t.as<std::uint8_t>(), static_cast<std::uint32_t>(model.batch),
+                                static_cast<std::uint32_t>(model.num_observation_nodes));
+  }
   if (policy.update_factors) {
     Hip(hipMemcpyAsync(w.root_potential.ptr, w.host_root.ptr, w.states * w.S(), hipMemcpyHostToDevice, w.stream),
         "GPU runtime MemcpyAsync root-potential upload");
@@ -1089,7 +1095,7 @@ void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double 
   if (src.categorical == nullptr) {
     LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale);
   } else {
-    LaunchInitCategorical(l, w.real, false, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+    LaunchInitCategorical(l, w.real, false, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
                           static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
                           w.root_potential.ptr, w.emissions.ptr, nodes, scale);
@@ -1183,7 +1189,7 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
   if (src.categorical == nullptr) {
     LaunchMaxInit(init, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
   } else {
-    LaunchInitCategorical(l, w.real, true, w.observations.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+    LaunchInitCategorical(l, w.real, true, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
                           static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
                           w.root_potential.ptr, w.emissions.ptr, w.nodes.ptr, nullptr);
@@ -1309,6 +1315,7 @@ void Workspace::ReserveCategorical(const Plan &plan, const void *identity, std::
     w.host_root.Allocate(states * w.S(), "pinned root potential");
     w.host_emissions.Allocate(emission_values * w.S(), "pinned emissions");
     w.observations.Allocate(observation_values, "observations");
+    w.observations_t.Allocate(observation_values, "observations");
     w.root_potential.Allocate(states * w.S(), "root potential");
     w.emissions.Allocate(emission_values * w.S(), "emissions");
     if (flags & THMM_RESERVE_MARGINALS) {
