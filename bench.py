@@ -292,10 +292,10 @@ def op_bytes(phase: int, primitive: int, k: int, s: int) -> int:
     kk = k * k
     if phase == 1:  # forward (4: fused rake level, per rake; 5: partial sums, per FP64 partial read)
         return {0: (2 * k + kk) * s, 1: 3 * k * s, 2: 3 * k * s, 3: (3 * kk + k) * s, 4: (2 * k + kk) * s,
-                5: 8 * k, 6: k * s}[primitive]
+                5: 8 * k, 6: k * s, 9: 0}[primitive]
     if phase == 3:  # reverse
         return {0: (2 * kk + 3 * k) * s, 1: 2 * k * s, 2: 2 * k * s, 3: (5 * kk + 2 * k) * s, 6: k * s,
-                7: k * s, 8: k * s}[primitive]
+                7: k * s, 8: k * s, 9: 0}[primitive]
     return 0
 
 
@@ -314,6 +314,8 @@ KERNEL_NAMES = {
     (3, 6): "ChainReverse",
     (3, 7): "ChainReverseSmall",
     (3, 8): "ChainReversePair",
+    (1, 9): "PersistentForward",
+    (3, 9): "PersistentReverse",
 }
 
 
@@ -355,7 +357,14 @@ def run_gpu(thmm, torch, plan, k, precision, nodes, edges, steps, warmup, tune=T
     stop.record(stream)
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / steps
+    # Per-kernel figures come from the level launches (a persistent pass is
+    # one launch per direction).
+    chosen = ws.tuning()
+    if chosen.get("persistent", 0):
+        ws.set_tuning(persistent=0)
     timings = ws.profile_marginals_device(*args)
+    if chosen.get("persistent", 0):
+        ws.set_tuning(persistent=1)
     torch.cuda.synchronize()
     return ws, ms, timings, log_z, node_out, edge_out, tuning
 
@@ -513,6 +522,41 @@ def roofline_from(timings, k, s, reads=None):
         "kernel_algorithmic_gbs": gbs,
         "kernel_algorithmic_bytes_per_launch": per_launch,
     }
+
+
+# Work items a block of each profiled launch kind covers at batch 1: the
+# team rake kernels hold 16 teams per 256-lane block (K <= 16), the grouped
+# compression kernels 4 groups (one wavefront each).
+ITEMS_PER_BLOCK = {(1, 4): 16, (1, 5): 16, (1, 6): 4, (3, 0): 16, (3, 6): 4, (3, 7): 4, (3, 8): 4, (1, 3): 8, (3, 3): 8}
+
+
+def level_table(timings, k, s, path, title, ops_of=None):
+    """Markdown table of every profiled launch of one marginal pass: kind,
+    op count, microseconds, algorithmic GB/s and blocks; totals the time of
+    launches with fewer than 256 blocks (too few to fill 256 CUs)."""
+    rows, small_ms, total_ms = [], 0.0, 0.0
+    for i, (phase, primitive, count, batch, ms) in enumerate(timings):
+        total_ms += ms
+        if phase not in (1, 3):
+            rows.append(f"| {i} | {'staging' if phase == 0 else 'finish + seed' if phase == 2 else 'output'} | — | — | "
+                        f"{ms * 1e3:.1f} | — | — |")
+            continue
+        nbytes = op_bytes(phase, primitive, k, s) * count * batch
+        ops = ops_of[i] if ops_of else count
+        per = ITEMS_PER_BLOCK.get((phase, primitive))
+        blocks = -(-ops * batch // per) if per else None
+        if blocks is not None and blocks < 256:
+            small_ms += ms
+        rows.append(f"| {i} | {KERNEL_NAMES[(phase, primitive)]} | {ops} | {count} | {ms * 1e3:.1f} | "
+                    f"{nbytes / (ms * 1e-3) / 1e9:.0f} | {blocks if blocks is not None else '—'} |")
+    text = [f"# {title}", "",
+            f"One marginal pass, {len(timings)} launches, {total_ms:.3f} ms in total; launches with fewer than 256 "
+            f"blocks (too few to fill 256 CUs): {small_ms:.3f} ms ({small_ms / total_ms:.1%}).", "",
+            "`ops` counts the level's work units (rakes, partial sums or compression groups), `count` the units the "
+            "algorithmic bytes are counted in (K-vectors for grouped compressions).", "",
+            "| # | kernel | ops | count | us | algorithmic GB/s | blocks |", "|---:|---|---:|---:|---:|---:|---:|"] + rows
+    Path(path).write_text("\n".join(text) + "\n")
+    return small_ms, total_ms
 
 
 # ---------------------------------------------------------------------------
@@ -1059,6 +1103,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--no-half", action="store_true", help="skip the binary16 edge-marginal leg")
     ap.add_argument("--no-map", action="store_true", help="skip the device MAP and sampling legs")
+    ap.add_argument("--persistent", action="store_true",
+                    help="also time the persistent-launch form (switched off in this build: the library refuses it)")
+    ap.add_argument("--level-table", default="", metavar="PATH",
+                    help="write a markdown table of the profiled pass's launches (config 4)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="also run one plain device copy of MIB MiB (the reference point that scales HBM counters)")
     args = ap.parse_args()
@@ -1134,6 +1182,12 @@ def main() -> int:
     )
     value = n / (ms * 1e-3)
     roofline = roofline_from(timings, k, 4)
+    if args.level_table:
+        small, total = level_table(timings, k, 4, args.level_table,
+                                   f"Per-launch times: uneven forest, {len(parts)} trees ({n} nodes), K=16, FP32 "
+                                   "log Z + adjoint (thmm_profile_marginals_device event pairs)",
+                                   ops_of=ws.last_profile_ops)
+        log(f"level table: {small:.3f} of {total:.3f} ms in launches of fewer than 256 blocks")
     log(f"autotuned launch variants: {tuning}")
     log(f"FP32 form: {ms:.3f} ms/step, {value:.4g} nodes/s; dominant kernel {roofline['kernel']} "
         f"{roofline['achieved']} GB/s ({roofline['frac']:.1%} of HBM peak)")
@@ -1153,6 +1207,32 @@ def main() -> int:
     parity["trees_checked"] = len(picks)
     log(f"parity: {parity}")
     log("kernel ms (profiled step): " + json.dumps(roofline["kernel_ms_breakdown"]))
+    persistent = None
+    if args.persistent:
+        # The pass as level launches and as two persistent launches (one per
+        # direction) whose blocks pull level chunks from a device queue, with
+        # the other tuned launch variants; outputs must be bit-equal.
+        kept = ws.tuning()["persistent"]
+        stream = torch.cuda.current_stream()
+        pargs = (nodes.data_ptr(), edges.data_ptr(), 1, log_z.data_ptr(), node_out.data_ptr(), edge_out.data_ptr(),
+                 stream.cuda_stream)
+        legs = {}
+        for mode in (0, 1):
+            ws.set_tuning(persistent=mode)
+            legs[mode] = time_calls(torch, lambda: ws.marginals_device(*pargs), args.steps, args.warmup)
+            ws.synchronize()
+            if mode == 0:
+                ref = (log_z.clone(), node_out.clone(), edge_out.clone())
+        equal = all(bool(torch.equal(a, b)) for a, b in zip(ref, (log_z, node_out, edge_out)))
+        ptimes = ws.profile_marginals_device(*pargs)
+        persistent = {"levels_ms_per_step": round(legs[0], 4), "ms_per_step": round(legs[1], 4),
+                      "value": n / (legs[1] * 1e-3), "speedup_vs_levels": round(legs[0] / legs[1], 4),
+                      "bit_equal_to_levels": equal, "tuned_choice": kept,
+                      "launch_ms": {("forward" if ph == 1 else "reverse"): round(t, 4) for ph, pr, c, b, t in ptimes
+                                    if pr == 9}}
+        log(f"persistent passes: {persistent}")
+        ws.set_tuning(persistent=kept)
+        del ref
     ws.close()
     del eparents, node_view, edge_view, ws
     torch.cuda.empty_cache()
@@ -1244,6 +1324,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "fp64_plain": fp64,
         "half_edge_outputs": half,
+        "persistent": persistent,
         "map_and_sampling": decode,
         "parity": parity,
     }

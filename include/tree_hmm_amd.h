@@ -505,7 +505,10 @@ thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, s
  * compressions read are staged, at K = 32 and 64 in FP32 the compressions
  * read theirs in place too; the caller's log edges must stay unchanged
  * until the backward call) or every edge is staged first (0); both give the
- * same values.
+ * same values; persistent: 0 (every level is its own launch). The value 1
+ * asks for each pass as one persistent launch whose blocks pull level
+ * chunks from a device queue; that mode is switched off in this build (its
+ * launches did not complete on the MI355X) and is refused with logic_error.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
  * forward program, and a new compress_depth also drops a held
@@ -515,6 +518,7 @@ typedef struct thmm_tuning {
   uint32_t rake_block, gather_chunk;
   uint32_t compress_depth;
   uint32_t rake_log_edges;
+  uint32_t persistent;
   double ms;
 } thmm_tuning;
 
@@ -545,10 +549,14 @@ thmm_status thmm_autotune_marginals_device(
  * vectors read and written per batch item: (n + 1) K + n - 1 forward and
  * (2n + 1) K + 2 (n - 1) reverse for a group of n slots), and in phase 3, 7
  * for the part of such a level whose groups have 3 or 4 slots and 8 for the
- * part whose groups have 2; otherwise count is the level's op count. */
+ * part whose groups have 2; otherwise count is the level's op count. ops is
+ * the launch's number of work units (groups, for grouped compressions). */
 typedef struct thmm_launch_timing {
   uint32_t phase, primitive, count, batch;
   double ms;
+  /* Work units of the launch: rakes, partial sums, compressions or
+   * compression groups (equal to count except for grouped compressions). */
+  uint32_t ops;
 } thmm_launch_timing;
 
 /* thmm_marginals_device with a HIP event pair recorded on the launch stream

@@ -112,6 +112,7 @@ class LaunchTiming(ctypes.Structure):
         ("count", c_u32),
         ("batch", c_u32),
         ("ms", ctypes.c_double),
+        ("ops", c_u32),
     ]
 
 
@@ -123,6 +124,7 @@ class Tuning(ctypes.Structure):
         ("gather_chunk", c_u32),
         ("compress_depth", c_u32),
         ("rake_log_edges", c_u32),
+        ("persistent", c_u32),
         ("ms", ctypes.c_double),
     ]
 

@@ -521,11 +521,14 @@ class Workspace:
                 timings, capacity, ctypes.byref(count)
             )
         )
-        return [(t.phase, t.primitive, t.count, t.batch, t.ms) for t in timings[: min(count.value, capacity)]]
+        kept = timings[: min(count.value, capacity)]
+        # work units per launch (compression groups for grouped levels)
+        self.last_profile_ops = [t.ops for t in kept]
+        return [(t.phase, t.primitive, t.count, t.batch, t.ms) for t in kept]
 
     # -- launch tuning -----------------------------------------------------
     _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk",
-                      "compress_depth", "rake_log_edges")
+                      "compress_depth", "rake_log_edges", "persistent")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()

@@ -189,12 +189,13 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
 // m[p] = sum_c t, adj_P = adj_m[p] t / m[p] (ratio in FP64) and
 // adj_v[c] = sum_p adj_P[p][c]. kLogEdges: buf.log_edges may be set (the
 // variant without it holds fewer registers).
+// `tid` is the lane's index in the launch (block * block size + thread).
 template <typename T, int K, bool kLogEdges>
-__global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
-                                                       std::uint32_t count, LinearBuffers buf) {
+__device__ __forceinline__ void RakeReverseTeamBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
+                                                    std::uint32_t count, const LinearBuffers &buf,
+                                                    std::uint32_t tid) {
   using TM = Team<K>;
   constexpr int kT = TM::kLanes, kLpr = TM::kLanesPerRow, kRps = TM::kRowsPerStep, kJ = TM::kSteps;
-  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const std::uint32_t item = tid / kT;
   const int l = static_cast<int>(tid % kT);
   if (item >= count * d.B) return;
@@ -272,6 +273,12 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
   }
 }
 
+template <typename T, int K, bool kLogEdges>
+__global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                       std::uint32_t count, LinearBuffers buf) {
+  RakeReverseTeamBody<T, K, kLogEdges>(ops, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Fused rake + combination + absorption
 // ---------------------------------------------------------------------------
@@ -302,11 +309,10 @@ __device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int
 // registers). FP32 edge slots of an FP64 pass (buf.edges_f32) are widened
 // as they load, which is exact.
 template <typename T, int K, bool kLogEdges>
-__global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::uint32_t offset, std::uint32_t count,
-                                                      LinearBuffers buf) {
+__device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dims &d, std::uint32_t offset,
+                                                   std::uint32_t count, const LinearBuffers &buf, std::uint32_t tid) {
   using TM = Team<K>;
   constexpr int kT = TM::kLanes, kLpr = TM::kLanesPerRow, kRps = TM::kRowsPerStep, kJ = TM::kSteps;
-  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
   const std::uint32_t item = tid / kT;
   const int l = static_cast<int>(tid % kT);
   if (item >= count * d.B) return;
@@ -371,12 +377,18 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
     g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + row] = acc;
 }
 
+template <typename T, int K, bool kLogEdges>
+__global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                      LinearBuffers buf) {
+  RakeGatherTeamBody<T, K, kLogEdges>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Sums a run of partials (K lanes, lane = row) into a partial of the next
 // reduction level or, at the last level, into the parent.
 template <typename T, int K>
-__global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, std::uint32_t offset,
-                                                          std::uint32_t count, LinearBuffers buf) {
-  const std::uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void GatherSegmentsTeamBody(const GatherOps &g, const Dims &d, std::uint32_t offset,
+                                                       std::uint32_t count, const LinearBuffers &buf,
+                                                       std::uint32_t tid) {
   const std::uint32_t item = tid / K;
   const int row = static_cast<int>(tid % K);
   if (item >= count * d.B) return;
@@ -389,6 +401,12 @@ __global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, s
     AbsorbSum<T, K, K>(buf, static_cast<std::uint64_t>(seg.parent) * d.B + b, row, acc, row);
   else
     g.partials[(static_cast<std::uint64_t>(seg.out) * d.B + b) * K + row] = acc;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, std::uint32_t offset,
+                                                          std::uint32_t count, LinearBuffers buf) {
+  GatherSegmentsTeamBody<T, K>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -695,11 +713,12 @@ __device__ inline void ChainMiddle(const LinearBuffers &buf, const Dims &d, cons
   *right_scale_own = RowScale(buf, d, c.slots[j], c.b, lane % 16);
 }
 
+// `block`: the block's index in the launch (kWaves groups per block).
 template <int kMax>
-__global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
-                                                          std::uint32_t count, LinearBuffers buf) {
+__device__ __forceinline__ void ChainForwardBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
+                                                 std::uint32_t count, const LinearBuffers &buf, std::uint32_t block) {
   const int lane = threadIdx.x % 64;
-  const std::uint32_t item = Uniform(blockIdx.x * kWaves + threadIdx.x / 64);
+  const std::uint32_t item = Uniform(block * kWaves + threadIdx.x / 64);
   if (item >= count * d.B) return;
   ChainView c;
   LoadChain(ops, d, offset, item, c);
@@ -736,6 +755,12 @@ __global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d,
   for (int r = 0; r < 4; ++r) q.v[r] = acc[r];
   StoreQuadStream(static_cast<float *>(buf.paths) + out_at * 256 + (lane % 16) * 16 + 4 * (lane / 16), q);
   if (lane < 16) buf.path_scale[out_at * 16 + lane] = scale;
+}
+
+template <int kMax>
+__global__ __launch_bounds__(256) void ChainForwardMfma16(DeviceOps ops, Dims d, std::uint32_t offset,
+                                                          std::uint32_t count, LinearBuffers buf) {
+  ChainForwardBody<kMax>(ops, d, offset, count, buf, blockIdx.x);
 }
 
 // Row-layout quad of an adjoint; edge marginals may be caller memory
@@ -950,9 +975,9 @@ template <int kN, bool kKeepWeights> __device__ inline void ReverseGroup(const G
 template <int kMax>
 __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
                                         std::uint32_t count, const LinearBuffers &buf, int quad_edges,
-                                        float (*tiles)[kMax - 1][16 * kPad]) {
+                                        float (*tiles)[kMax - 1][16 * kPad], std::uint32_t block) {
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const std::uint32_t item = Uniform(blockIdx.x * kWaves + wave);
+  const std::uint32_t item = Uniform(block * kWaves + wave);
   if (item >= count * d.B) return;  // whole wavefront; only wavefront barriers below
   ChainView c;
   LoadChain(ops, d, offset, item, c);
@@ -986,21 +1011,21 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
 __global__ __launch_bounds__(256) void ChainReversePair(DeviceOps ops, Dims d, std::uint32_t offset,
                                                         std::uint32_t count, LinearBuffers buf, int quad_edges) {
   __shared__ float tiles[kWaves][1][16 * kPad];
-  ChainReverseBody<2>(ops, d, offset, count, buf, quad_edges, tiles);
+  ChainReverseBody<2>(ops, d, offset, count, buf, quad_edges, tiles, blockIdx.x);
 }
 
 // Groups of 3 or 4 slots (the kernel handles 2 as well).
 __global__ __launch_bounds__(256) void ChainReverseSmall(DeviceOps ops, Dims d, std::uint32_t offset,
                                                          std::uint32_t count, LinearBuffers buf, int quad_edges) {
   __shared__ float tiles[kWaves][3][16 * kPad];
-  ChainReverseBody<4>(ops, d, offset, count, buf, quad_edges, tiles);
+  ChainReverseBody<4>(ops, d, offset, count, buf, quad_edges, tiles, blockIdx.x);
 }
 
 // Three wavefronts per SIMD (at most 168 registers per lane).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void ChainReverseLarge(
     DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf, int quad_edges) {
   __shared__ float tiles[kWaves][kMaxChain - 1][16 * kPad];
-  ChainReverseBody<kMaxChain>(ops, d, offset, count, buf, quad_edges, tiles);
+  ChainReverseBody<kMaxChain>(ops, d, offset, count, buf, quad_edges, tiles, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1497,6 +1522,120 @@ template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, 
     RakeForwardTeam<T, K><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
 }
 
+// ---------------------------------------------------------------------------
+// Persistent passes (K = 16, FP32, batch 1)
+// ---------------------------------------------------------------------------
+
+__global__ void ClearWords(std::uint32_t *words, std::uint32_t count) {
+  for (std::uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) words[i] = 0;
+}
+
+// Thread 0 takes the next task and waits until its group has completed the
+// task's earlier steps; the block reads the index from LDS.
+//
+// Agent scope spans the device's XCDs, whose L2 caches are separate, so an
+// agent-scope acquire invalidates the XCD's L2 and a release writes it back.
+// The wait therefore polls with relaxed loads (no cache maintenance) and
+// takes one acquire when the wait is over; a task ends with one release.
+// The wait is bounded in wall-clock time (q.wait_ticks of the constant
+// wall clock), after which the launch sets its failure word and stops
+// waiting, so a broken dependency cannot hang the device.
+__device__ inline std::uint32_t TakeTask(const PersistentQueue &q, std::uint32_t *next) {
+  if (threadIdx.x == 0) {
+    const std::uint32_t i = atomicAdd(q.counters, 1u);
+    *next = i;
+    if (i < q.num_tasks) {
+      const DeviceTask t = q.tasks[i];
+      const std::uint32_t *progress = q.counters + 1 + q.steps + t.group;
+      if (__hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t.step) {
+        const std::uint64_t start = wall_clock64();
+        while (__hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < t.step) {
+          if (__hip_atomic_load(q.failure, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+          if (wall_clock64() - start > q.wait_ticks) {
+            atomicOr(q.failure, 1u);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(8);
+        }
+      }
+      // One invalidation (the CU's L1, shared by the block's wavefronts,
+      // and the XCD's L2) before the block reads what other tasks wrote.
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+  __syncthreads();
+  return *next;
+}
+
+// After a task: every wavefront waits for its stores to reach the L2, then
+// one release makes them visible device-wide before the task is counted;
+// the last task of its (group, step) advances the group's progress.
+__device__ inline void FinishTask(const PersistentQueue &q, const DeviceTask &t) {
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const std::uint32_t at = q.need_offset[t.group] + t.step;
+    if (atomicAdd(q.counters + 1 + at, 1u) + 1 == q.need[at])
+      __hip_atomic_store(q.counters + 1 + q.steps + t.group, t.step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The fused forward program (rake chunks, partial sums, compression groups
+// of up to 4 slots), each task run as blocks of the level kernels would.
+template <bool kLogEdges>
+__global__ __launch_bounds__(256) void PersistentForward16(GatherOps g, DeviceOps ops, Dims d, LinearBuffers buf,
+                                                           PersistentQueue q) {
+  __shared__ std::uint32_t next;
+  for (;;) {
+    const std::uint32_t i = TakeTask(q, &next);
+    if (i >= q.num_tasks) return;
+    const DeviceTask t = q.tasks[i];
+    for (std::uint32_t j = 0; j < t.blocks; ++j) {
+      if (t.kind == kTaskGather)
+        RakeGatherTeamBody<float, 16, kLogEdges>(g, d, t.offset, t.count, buf, j * 256 + threadIdx.x);
+      else if (t.kind == kTaskSegments)
+        GatherSegmentsTeamBody<float, 16>(g, d, t.offset, t.count, buf, j * 256 + threadIdx.x);
+      else
+        ChainForwardBody<4>(ops, d, t.offset, t.count, buf, j);
+    }
+    FinishTask(q, t);
+  }
+}
+
+// The reverse program: rake adjoints and compression groups of 2 and of 3
+// or 4 slots.
+template <bool kLogEdges>
+__global__ __launch_bounds__(256) void PersistentReverse16(DeviceOps ops, Dims d, LinearBuffers buf,
+                                                           PersistentQueue q, int quad_edges) {
+  __shared__ std::uint32_t next;
+  __shared__ float tiles[kWaves][3][16 * kPad];
+  for (;;) {
+    const std::uint32_t i = TakeTask(q, &next);
+    if (i >= q.num_tasks) return;
+    const DeviceTask t = q.tasks[i];
+    for (std::uint32_t j = 0; j < t.blocks; ++j) {
+      if (t.kind == kTaskRakeReverse)
+        RakeReverseTeamBody<float, 16, kLogEdges>(ops, d, t.offset, t.count, buf, j * 256 + threadIdx.x);
+      else if (t.kind == kTaskChainReversePair)
+        ChainReverseBody<2>(ops, d, t.offset, t.count, buf, quad_edges,
+                            reinterpret_cast<float (*)[1][16 * kPad]>(&tiles[0][0][0]), j);
+      else
+        ChainReverseBody<4>(ops, d, t.offset, t.count, buf, quad_edges, tiles, j);
+    }
+    FinishTask(q, t);
+  }
+}
+
+template <typename F> std::uint32_t ResidentGrid(F kernel) {
+  int device = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBlock, 0) != hipSuccess)
+    return 0;
+  return static_cast<std::uint32_t>(cus) * static_cast<std::uint32_t>(per_cu);
+}
+
 }  // namespace
 
 bool FusedForwardSupported(std::uint32_t states) {
@@ -1562,6 +1701,44 @@ void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t off
   }
 #undef THMM_SEGMENTS
   RecordLaunchError(hipGetLastError(), "partial absorption launch");
+}
+
+std::uint64_t WallClockTicks(double seconds) {
+  int device = 0, khz = 0;
+  if (hipGetDevice(&device) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || khz <= 0)
+    khz = 100000;  // the 100 MHz clock of gfx9 parts
+  return static_cast<std::uint64_t>(seconds * 1e3 * khz);
+}
+
+std::uint32_t PersistentGrid(bool reverse) {
+  const std::uint32_t a = reverse ? ResidentGrid(PersistentReverse16<true>) : ResidentGrid(PersistentForward16<true>);
+  const std::uint32_t b = reverse ? ResidentGrid(PersistentReverse16<false>) : ResidentGrid(PersistentForward16<false>);
+  return a < b ? a : b;
+}
+
+void LaunchPersistentPass(const Launch &l, bool reverse, const GatherOps &g, const PersistentQueue &q,
+                          const LinearBuffers &buf) {
+  if (q.num_tasks == 0) return;
+  const hipStream_t s = StreamOf(l);
+  const bool log_edges = buf.log_edges != nullptr;
+  const std::uint32_t words = 1 + q.steps + q.groups;
+  ClearWords<<<(words + kBlock - 1) / kBlock < 1024 ? (words + kBlock - 1) / kBlock : 1024, kBlock, 0, s>>>(
+      q.counters, words);
+  const unsigned grid = q.grid == 0 ? 1u : (q.grid < q.num_tasks ? q.grid : q.num_tasks);
+  if (reverse) {
+    const int quad_edges = Aligned16(buf.edge_out);
+    if (log_edges)
+      PersistentReverse16<true><<<grid, kBlock, 0, s>>>(l.ops, l.dims, buf, q, quad_edges);
+    else
+      PersistentReverse16<false><<<grid, kBlock, 0, s>>>(l.ops, l.dims, buf, q, quad_edges);
+  } else {
+    if (log_edges)
+      PersistentForward16<true><<<grid, kBlock, 0, s>>>(g, l.ops, l.dims, buf, q);
+    else
+      PersistentForward16<false><<<grid, kBlock, 0, s>>>(g, l.ops, l.dims, buf, q);
+  }
+  RecordLaunchError(hipGetLastError(), "persistent pass launch");
 }
 
 bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std::uint32_t offset,

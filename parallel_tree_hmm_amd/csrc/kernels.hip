@@ -862,6 +862,11 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
   const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
   double best = kNegInf;  // the strict '>' scan from -inf: a row of -inf keeps its first column
   std::uint32_t arg = c0;
+  // The sequential scan starts at column 0's candidate (src/inference.cc:
+  // 113-126): when that candidate is NaN, no later one compares greater and
+  // the row keeps (NaN, 0). The merge below starts from -inf, so that case is
+  // taken over explicitly after it.
+  bool nan_first = false;
   if (op.slot < d.E && buf.raw_edges != nullptr) {
     const std::uint64_t at = static_cast<std::uint64_t>(op.slot) * K * K + p * K + c0;
     double f[4];
@@ -874,6 +879,11 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
 #pragma unroll
       for (int c = 0; c < 4; ++c) f[c] = q.v[c];
     }
+    // log f + v is NaN when v is NaN, or when log f = -inf (f <= 0 or NaN)
+    // meets v = +inf, or f = +inf meets v = -inf.
+    if (c0 == 0)
+      nan_first = isnan(v.v[0]) || (!(f[0] > 0.0) && v.v[0] == __builtin_huge_val()) ||
+                  (f[0] == __builtin_huge_val() && v.v[0] == kNegInf);
     // Ascending columns: the strict '>' keeps the lower index on a tie.
     for (unsigned mask = ScreenRakeRow<kQ>(f, v); mask != 0; mask &= mask - 1) {
       const int c = __builtin_ctz(mask);
@@ -882,6 +892,7 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
     }
   } else {
     const Quad<double> row = LoadQuad(MaxSlot(buf, d, op.slot, b) + p * K + c0);
+    if (c0 == 0) nan_first = isnan(row.v[0] + v.v[0]);
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
       const double candidate = row.v[c] + v.v[c];
@@ -894,6 +905,7 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
     const std::uint32_t other_arg = __shfl_xor(arg, o, kQ);
     ArgMaxMerge(best, arg, other, other_arg);
   }
+  if (nan_first) best = __builtin_nan(""), arg = 0;
   if (c0 == 0) {
     const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + p;
     buf.branches[at] = best;

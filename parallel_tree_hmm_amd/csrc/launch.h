@@ -158,6 +158,7 @@ struct Tuning {
   std::uint32_t gather_chunk = 16;           // rakes per team in the fused forward: 1 .. 64
   std::uint32_t compress_depth = 2;          // compression-tree levels per grouped compression: 1 (none) .. 3
   std::uint32_t rake_log_edges = 1;          // log-potential calls: rakes read log edges in place (1) or staged (0)
+  std::uint32_t persistent = 0;              // K = 16 FP32 forests: each pass as one persistent launch (1)
 };
 
 struct Launch {
@@ -231,6 +232,44 @@ bool FusedForwardSupported(std::uint32_t states);
 void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf);
 void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t offset, std::uint32_t count,
+                          const LinearBuffers &buf);
+
+// ---- persistent passes (K = 16 FP32, batch 1) ----
+// A pass's levels as one launch: resident blocks pull tasks (a run of ops of
+// one level of one kind) from a queue in level order. The trees of the
+// forest are cut into groups of consecutive trees; a task waits until every
+// earlier level of its own group is done (the ops of one tree depend only on
+// earlier levels of that tree), so groups run ahead of one another instead
+// of meeting at every level boundary.
+constexpr std::uint32_t kTaskGather = 0, kTaskSegments = 1, kTaskChainForward = 2, kTaskRakeReverse = 3,
+                        kTaskChainReversePair = 4, kTaskChainReverseSmall = 5;
+struct DeviceTask {
+  std::uint32_t kind;
+  std::uint32_t step;   // the group's level index among its levels in this pass
+  std::uint32_t group;
+  std::uint32_t offset, count;  // ops of the level's kind
+  std::uint32_t blocks;         // block-sized runs of the ops, processed in turn
+  std::uint32_t pad0, pad1;
+};
+struct PersistentQueue {
+  const DeviceTask *tasks;
+  std::uint32_t num_tasks;
+  std::uint32_t steps, groups;  // (group, step) pairs; groups
+  const std::uint32_t *need;         // [steps] tasks of each (group, step), group-major
+  const std::uint32_t *need_offset;  // [groups] first (group, step) of each group
+  // [1 + steps + groups]: the queue head, finished tasks per (group, step)
+  // and completed steps per group; cleared by the launch.
+  std::uint32_t *counters;
+  // Set (and never cleared by a launch) when a wait exceeds its spin limit.
+  std::uint32_t *failure;
+  std::uint32_t grid;  // blocks launched: at most the blocks the device holds at once
+  std::uint64_t wait_ticks;  // longest dependency wait, in ticks of the device's constant wall clock
+};
+// Ticks of the device's constant wall clock in `seconds`.
+std::uint64_t WallClockTicks(double seconds);
+// Blocks of the pass's kernels the device holds at once (0 if unknown).
+std::uint32_t PersistentGrid(bool reverse);
+void LaunchPersistentPass(const Launch &l, bool reverse, const GatherOps &g, const PersistentQueue &q,
                           const LinearBuffers &buf);
 
 // ---- scaled forward (log Z) ----

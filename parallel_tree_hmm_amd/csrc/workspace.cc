@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <stdexcept>
 #include <string>
@@ -194,6 +195,22 @@ struct Workspace::Impl {
   // This workspace is a group of another one: its state buffers are views.
   bool group_member = false;
 
+  // Task queues of the persistent passes (Tuning::persistent), built with
+  // the fused forward program when the kernels cover it (K = 16 FP32, batch
+  // 1, compression groups of at most 4 slots); num_tasks 0 otherwise.
+  struct PersistentProgram {
+    DeviceBuffer tasks, need, need_offset;
+    std::uint32_t num_tasks = 0, steps = 0, groups = 0, grid = 0;
+    std::uint64_t wait_ticks = 0;
+    void Release() noexcept {
+      tasks.Release(), need.Release(), need_offset.Release();
+      num_tasks = steps = groups = grid = 0;
+      wait_ticks = 0;
+    }
+  };
+  PersistentProgram persistent_forward, persistent_reverse;
+  DeviceBuffer persistent_counters, persistent_failure;
+
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
   // inputs
@@ -230,6 +247,10 @@ struct Workspace::Impl {
     groups.clear();
     arena.Release();
     group_member = false;
+    persistent_forward.Release();
+    persistent_reverse.Release();
+    persistent_counters.Release();
+    persistent_failure.Release();
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
                             &input_nodes, &uniforms, &observations, &observations_t, &root_potential,
@@ -580,6 +601,151 @@ void BuildChains(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &c
 // The fused forward program and, for K = 16 FP32, the grouped compressions
 // it runs; grouping is dropped when the fused program cannot be built (the
 // plan-level forward stores every compression).
+template <typename V> std::vector<V> Download(const DeviceBuffer &src, std::size_t count, const char *what) {
+  std::vector<V> out(count);
+  if (count != 0) Hip(hipMemcpy(out.data(), src.ptr, count * sizeof(V), hipMemcpyDeviceToHost), what);
+  return out;
+}
+
+// Tree of every node (its root's position in plan.roots).
+std::vector<std::uint32_t> TreeOfNodes(const Plan &plan) {
+  std::vector<std::vector<std::uint32_t>> children(plan.num_nodes);
+  for (std::size_t e = 0; e < plan.num_edges; ++e) children[plan.edge_parents[e]].push_back(plan.edge_children[e]);
+  std::vector<std::uint32_t> tree(plan.num_nodes, 0), stack;
+  for (std::size_t r = 0; r < plan.roots.size(); ++r) {
+    stack.push_back(plan.roots[r]);
+    while (!stack.empty()) {
+      const std::uint32_t v = stack.back();
+      stack.pop_back();
+      tree[v] = static_cast<std::uint32_t>(r);
+      for (std::uint32_t c : children[v]) stack.push_back(c);
+    }
+  }
+  return tree;
+}
+
+// The persistent passes are switched off: two runs on the MI355X (a 115 k-node
+// K = 16 forest) did not complete within 120 and 60 seconds, the second with
+// every dependency wait bounded at 2 s of wall clock, so the launches did not
+// finish for a reason the waits do not explain. Their queues are not built
+// and Tuning::persistent = 1 is refused until the cause is shown.
+constexpr bool kPersistentPassesEnabled = false;
+
+// Lays out the task queues of the persistent passes: every level of a pass
+// (in execution order) is cut at tree-group boundaries and into tasks of at
+// most kTaskBlocks blocks; a task's step is the index of its level among the
+// levels in which its group has ops. Groups are consecutive trees of about
+// equal node count, at most kPersistentGroups of them; when a level's ops
+// are not ordered by group, all trees form one group (level barriers).
+void BuildPersistent(Impl &w, std::size_t batch) {
+  w.persistent_forward.Release();
+  w.persistent_reverse.Release();
+  if (!kPersistentPassesEnabled) return;
+  if (w.states != 16 || w.real != Real::kF32 || batch != 1 || w.fused.empty() || w.chain_levels.empty()) return;
+  for (const Impl::FusedStep &step : w.fused)
+    if (step.kind != 0 && step.kind != 1 && step.kind != kChainPrimitive) return;
+  for (const Level &level : w.grouped_levels)
+    if (level.primitive == kChainPrimitive) return;  // groups of more than 4 slots
+  constexpr std::uint32_t kPersistentGroups = 256, kTaskBlocks = 16;
+  const Plan &plan = *w.plan;
+  const std::vector<std::uint32_t> tree = TreeOfNodes(plan);
+  std::vector<std::size_t> tree_nodes(plan.roots.size(), 0);
+  for (std::uint32_t t : tree) ++tree_nodes[t];
+  std::vector<std::uint32_t> group_of_tree(plan.roots.size(), 0);
+  {
+    const std::size_t target = (plan.num_nodes + kPersistentGroups - 1) / kPersistentGroups;
+    std::uint32_t g = 0;
+    std::size_t filled = 0;
+    for (std::size_t t = 0; t < tree_nodes.size(); ++t) {
+      if (filled >= target && g + 1 < kPersistentGroups) ++g, filled = 0;
+      group_of_tree[t] = g;
+      filled += tree_nodes[t];
+    }
+  }
+  Hip(hipStreamSynchronize(w.stream), "persistent program download");
+  const auto chunks = Download<DeviceGatherChunk>(w.gather_chunks, w.gather_chunks.bytes / sizeof(DeviceGatherChunk),
+                                                  "persistent program download");
+  const auto segments = Download<DeviceGatherSegment>(
+      w.gather_segments, w.gather_segments.bytes / sizeof(DeviceGatherSegment), "persistent program download");
+  const auto chains = Download<DeviceChain>(w.chains, w.chains.bytes / sizeof(DeviceChain), "persistent program download");
+  const auto rakes = Download<DeviceRake>(w.rakes, plan.rakes.size(), "persistent program download");
+  const std::uint32_t E = static_cast<std::uint32_t>(plan.num_edges);
+  // One pass: (kind, ops per block, node whose tree owns op i) per level.
+  struct PassLevel {
+    std::uint32_t kind, offset, count, per_block;
+    std::function<std::uint32_t(std::uint32_t)> node;
+  };
+  const auto chain_node = [&](std::uint32_t i) { return plan.compressions[chains[i].out - E].middle; };
+  std::vector<PassLevel> forward, reverse;
+  for (const Impl::FusedStep &step : w.fused) {
+    if (step.kind == 0)
+      forward.push_back({kTaskGather, step.offset, step.count, 16, [&](std::uint32_t i) { return chunks[i].parent; }});
+    else if (step.kind == 1)
+      forward.push_back({kTaskSegments, step.offset, step.count, 16, [&](std::uint32_t i) { return segments[i].parent; }});
+    else
+      forward.push_back({kTaskChainForward, step.offset, step.count, 4, chain_node});
+  }
+  for (std::size_t i = w.grouped_levels.size(); i-- > 0;) {
+    const Level &level = w.grouped_levels[i];
+    if (level.primitive == THMM_RAKE)
+      reverse.push_back({kTaskRakeReverse, level.offset, level.count, 16, [&](std::uint32_t j) { return rakes[j].parent; }});
+    else if (level.primitive == kPairChainPrimitive)
+      reverse.push_back({kTaskChainReversePair, level.offset, level.count, 4, chain_node});
+    else if (level.primitive == kSmallChainPrimitive)
+      reverse.push_back({kTaskChainReverseSmall, level.offset, level.count, 4, chain_node});
+  }
+  const auto build = [&](const std::vector<PassLevel> &levels, Impl::PersistentProgram &out) {
+    std::uint32_t groups = kPersistentGroups;
+    for (const PassLevel &level : levels)
+      for (std::uint32_t i = level.offset + 1; i < level.offset + level.count && groups > 1; ++i)
+        if (group_of_tree[tree[level.node(i)]] < group_of_tree[tree[level.node(i - 1)]]) groups = 1;
+    const auto group_of = [&](std::uint32_t node) { return groups == 1 ? 0u : group_of_tree[tree[node]]; };
+    std::vector<DeviceTask> tasks;
+    std::vector<std::vector<std::uint32_t>> need(kPersistentGroups);
+    for (const PassLevel &level : levels) {
+      const std::uint32_t cap = level.per_block * kTaskBlocks;
+      for (std::uint32_t lo = level.offset; lo < level.offset + level.count;) {
+        const std::uint32_t g = group_of(level.node(lo));
+        std::uint32_t hi = lo + 1;
+        while (hi < level.offset + level.count && group_of(level.node(hi)) == g) ++hi;
+        const std::uint32_t step = static_cast<std::uint32_t>(need[g].size());
+        need[g].push_back(0);
+        for (std::uint32_t o = lo; o < hi; o += cap) {
+          const std::uint32_t n = std::min(cap, hi - o);
+          tasks.push_back({level.kind, step, g, o, n, (n + level.per_block - 1) / level.per_block, 0, 0});
+          ++need[g][step];
+        }
+        lo = hi;
+      }
+    }
+    std::vector<std::uint32_t> flat, offsets;
+    for (const std::vector<std::uint32_t> &n : need) {
+      offsets.push_back(static_cast<std::uint32_t>(flat.size()));
+      flat.insert(flat.end(), n.begin(), n.end());
+    }
+    Upload(out.tasks, tasks, w.stream, "persistent program upload");
+    Upload(out.need, flat, w.stream, "persistent program upload");
+    Upload(out.need_offset, offsets, w.stream, "persistent program upload");
+    out.num_tasks = static_cast<std::uint32_t>(tasks.size());
+    out.steps = static_cast<std::uint32_t>(flat.size());
+    out.groups = kPersistentGroups;
+    Hip(hipStreamSynchronize(w.stream), "persistent program upload");
+  };
+  build(forward, w.persistent_forward);
+  build(reverse, w.persistent_reverse);
+  w.persistent_forward.grid = PersistentGrid(false);
+  w.persistent_reverse.grid = PersistentGrid(true);
+  // A dependency wait far longer than any level (the largest take a few
+  // milliseconds) is reported as a failure rather than waited out.
+  w.persistent_forward.wait_ticks = w.persistent_reverse.wait_ticks = WallClockTicks(2.0);
+  const std::size_t words = 1 + std::max(w.persistent_forward.steps, w.persistent_reverse.steps) + kPersistentGroups;
+  w.persistent_counters.Allocate(words * sizeof(std::uint32_t), "persistent pass counters");
+  if (w.persistent_failure.ptr == nullptr) {
+    w.persistent_failure.Allocate(sizeof(std::uint32_t), "persistent pass status");
+    Hip(hipMemset(w.persistent_failure.ptr, 0, sizeof(std::uint32_t)), "persistent pass status");
+  }
+}
+
 void BuildPrograms(Impl &w, const Plan &plan, const std::vector<DeviceRake> &rakes,
                    const std::vector<DeviceCompress> &compresses, std::size_t batch) {
   w.fused.clear();
@@ -591,6 +757,7 @@ void BuildPrograms(Impl &w, const Plan &plan, const std::vector<DeviceRake> &rak
     w.grouped_levels.clear();
     w.chains.Release();
   }
+  BuildPersistent(w, batch);
 }
 
 void ReserveGroups(Impl &w, const Plan &plan, const void *identity, std::size_t states, Real real, unsigned flags,
@@ -921,9 +1088,9 @@ class Profiler {
       static_cast<void>(hipEventDestroy(e.stop));
     }
   }
-  void Begin(std::uint32_t phase, std::uint32_t primitive, std::uint32_t count) {
+  void Begin(std::uint32_t phase, std::uint32_t primitive, std::uint32_t count, std::uint32_t ops = ~0u) {
     Entry e{};
-    e.info = {phase, primitive, count, batch_, 0.0};
+    e.info = {phase, primitive, count, batch_, 0.0, ops == ~0u ? count : ops};
     Hip(hipEventCreate(&e.start), "GPU runtime EventCreate");
     Hip(hipEventCreate(&e.stop), "GPU runtime EventCreate");
     Hip(hipEventRecord(e.start, stream_), "GPU runtime EventRecord");
@@ -978,7 +1145,8 @@ void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, con
     // absorptions, count = partials), a compression level, or a grouped
     // compression level (kChainPrimitive, count = K-vectors moved per item).
     if (profiler)
-      profiler->Begin(1, step.kind == 3 || step.kind == kChainPrimitive ? step.kind : 4 + step.kind, step.work);
+      profiler->Begin(1, step.kind == 3 || step.kind == kChainPrimitive ? step.kind : 4 + step.kind, step.work,
+                      step.count);
     if (step.kind == 0)
       LaunchGatherChunks(l, g, step.offset, step.count, buf);
     else if (step.kind == 1)
@@ -989,11 +1157,34 @@ void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, con
   }
 }
 
+// Profiled primitive of a persistent pass (count = its tasks).
+constexpr std::uint32_t kPersistentPrimitive = 9;
+
+// Whether this pass runs as one persistent launch: the tuning asks for it,
+// the queues exist (K = 16 FP32, batch 1) and the buffers take the fused
+// program with grouped compressions.
+bool PersistentReady(const Impl &w, const Launch &l, const LinearBuffers &buf) {
+  return w.tuning.persistent != 0 && w.persistent_forward.num_tasks != 0 && l.dims.B == 1 &&
+         UsesFusedForward(w, buf) && buf.compress_log_edges == 0 && !w.grouped_levels.empty() &&
+         buf.branch_adj == nullptr;
+}
+
+PersistentQueue QueueOf(const Impl &w, const Impl::PersistentProgram &p) {
+  return {p.tasks.as<DeviceTask>(), p.num_tasks, p.steps, p.groups, p.need.as<std::uint32_t>(),
+          p.need_offset.as<std::uint32_t>(), w.persistent_counters.as<std::uint32_t>(),
+          w.persistent_failure.as<std::uint32_t>(), p.grid, p.wait_ticks};
+}
+
 void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
-  if (UsesFusedForward(w, buf))
+  if (PersistentReady(w, l, buf)) {
+    if (profiler) profiler->Begin(1, kPersistentPrimitive, w.persistent_forward.num_tasks);
+    LaunchPersistentPass(l, false, w.Gather(), QueueOf(w, w.persistent_forward), buf);
+    if (profiler) profiler->End();
+  } else if (UsesFusedForward(w, buf)) {
     RunFusedSteps(w, w.fused, l, buf, profiler);
-  else
+  } else {
     ForwardLevels(l, w.levels, buf, profiler);
+  }
 }
 
 // The levels whose reverse matches the forward pass SumProductForward ran
@@ -1012,10 +1203,23 @@ void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const Line
       continue;
     const bool grouped = levels[i].primitive == kChainPrimitive || levels[i].primitive == kSmallChainPrimitive ||
                          levels[i].primitive == kPairChainPrimitive;
-    if (profiler) profiler->Begin(3, levels[i].primitive, grouped ? levels[i].reverse_work : levels[i].count);
+    if (profiler)
+      profiler->Begin(3, levels[i].primitive, grouped ? levels[i].reverse_work : levels[i].count, levels[i].count);
     LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
     if (profiler) profiler->End();
   }
+}
+
+// The reverse pass matching SumProductForward's forward pass: one persistent
+// launch, or the reverse program's levels.
+void RunReverse(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
+  if (PersistentReady(w, l, buf)) {
+    if (profiler) profiler->Begin(3, kPersistentPrimitive, w.persistent_reverse.num_tasks);
+    LaunchPersistentPass(l, true, w.Gather(), QueueOf(w, w.persistent_reverse), buf);
+    if (profiler) profiler->End();
+    return;
+  }
+  ReverseLevels(l, ReverseProgram(w, buf), buf, profiler);
 }
 
 // Categorical staging with residency policies (src/gpu_backend_impl.inc:2841-2947).
@@ -1154,7 +1358,7 @@ void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
-  ReverseLevels(l, ReverseProgram(w, buf), buf);
+  RunReverse(w, l, buf);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
   CheckLaunches();
 }
@@ -1718,7 +1922,7 @@ void Workspace::BranchLengthBackward(const double *grad_log_z, const double *rat
   LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
   AdjointsInto(buf, w.last_batch, w.node_out.ptr);
   LaunchSeedAdjoint(l, buf, grad_log_z != nullptr ? w.chain_seed.as<double>() : nullptr);
-  ReverseLevels(l, ReverseProgram(w, buf), buf);
+  RunReverse(w, l, buf);
   LaunchBranchLengthChain(l, w.edge_out.ptr, w.edges.ptr, w.chain_rate.as<double>(), w.chain_terms.as<double>(),
                           w.chain_grad.as<double>());
   CheckLaunches();
@@ -1789,7 +1993,7 @@ void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *ed
   LaunchFinish(l, buf, log_z);
   LaunchSeedAdjoint(l, buf, nullptr);
   if (p) p->End();
-  ReverseLevels(l, ReverseProgram(w, buf), buf, p);
+  RunReverse(w, l, buf, p);
   if (p) p->Begin(4, 0, 0);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
   if (p) p->End();
@@ -1820,6 +2024,19 @@ void EnqueueMarginalsAll(Impl &w, hipStream_t on, const void *nodes, const void 
                      p);
 }
 
+// A persistent pass that stopped waiting on a dependency (its spin limit)
+// leaves its failure word set: reported once, then cleared. Blocks until
+// the device is idle.
+void CheckPersistent(Impl &w) {
+  if (w.persistent_failure.ptr == nullptr) return;
+  Hip(hipDeviceSynchronize(), "tree-HMM accelerator execution");
+  std::uint32_t failed = 0;
+  Hip(hipMemcpy(&failed, w.persistent_failure.ptr, sizeof(failed), hipMemcpyDeviceToHost), "persistent pass status");
+  if (failed == 0) return;
+  Hip(hipMemset(w.persistent_failure.ptr, 0, sizeof(failed)), "persistent pass status");
+  throw std::runtime_error("persistent pass: a dependency wait exceeded its spin limit; its results are invalid");
+}
+
 void ValidateTuning(const Tuning &t) {
   if (t.compress_items_forward < 1 || t.compress_items_forward > 2 || t.compress_items_reverse < 1 ||
       t.compress_items_reverse > 2)
@@ -1829,6 +2046,11 @@ void ValidateTuning(const Tuning &t) {
   if (t.gather_chunk < 1 || t.gather_chunk > 64) throw std::invalid_argument("gather chunk must be in [1, 64]");
   if (t.compress_depth < 1 || t.compress_depth > 3) throw std::invalid_argument("compression depth must be 1, 2 or 3");
   if (t.rake_log_edges > 1) throw std::invalid_argument("rake_log_edges must be 0 or 1");
+  if (t.persistent > 1) throw std::invalid_argument("persistent must be 0 or 1");
+  if (t.persistent != 0 && !kPersistentPassesEnabled)
+    throw std::logic_error(
+        "persistent passes are switched off: on the MI355X their launches did not complete (per-task device-wide "
+        "release and acquire across the XCDs' separate L2 caches); every level runs as its own launch");
 }
 
 // Installs `t`; a new chunk size or compression depth rebuilds the fused
@@ -1921,6 +2143,7 @@ void Workspace::MarginalsDevice(const void *nodes, const void *edges, std::size_
     enqueue(s);
     CheckLaunches();
     *count = p->Report(timings, capacity);
+    CheckPersistent(w);
     return;
   }
   Impl::GraphKey key;
@@ -2063,7 +2286,7 @@ void EnqueueLogZBackward(Impl &w, hipStream_t s, const double *grad, void *grad_
       LaunchInitEdgesFromLogs(l, w.forward_log_edges, w.edges.ptr, w.edge_scale.as<double>());
   }
   LaunchSeedAdjoint(l, buf, grad);
-  ReverseLevels(l, ReverseProgram(w, buf), buf);
+  RunReverse(w, l, buf);
   if (w.last_batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, grad_nodes);
   if (reduce_edges) LaunchReduceEdgesOverBatch(l, w.edge_out.ptr, grad_edges);
 }
@@ -2221,6 +2444,8 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
     if (mfma16) search(&Tuning::compress_items_reverse, {1, 2});
     if (mfma16 && !w.fused.empty() && !w.plan->compressions.empty()) search(&Tuning::compress_depth, {1, 2, 3});
     search(&Tuning::rake_block, {64, 128, 256});
+    if (kPersistentPassesEnabled && w.persistent_forward.num_tasks != 0 && batch == 1)
+      search(&Tuning::persistent, {0, 1});
   } catch (...) {
     static_cast<void>(hipEventDestroy(start));
     static_cast<void>(hipEventDestroy(stop));
@@ -2232,6 +2457,8 @@ double Workspace::AutotuneMarginalsDevice(const void *nodes, const void *edges, 
 }
 
 void Workspace::SetPathBudget(std::size_t bytes) { impl_->path_budget = bytes; }
+
+
 
 void Workspace::PathStorage(std::size_t *groups, std::size_t *bytes) const {
   const Impl &w = *impl_;
@@ -2248,6 +2475,7 @@ void Workspace::Synchronize() {
   if (w.stream == nullptr) return;
   Hip(hipSetDevice(w.device), "GPU runtime SetDevice");
   Hip(hipStreamSynchronize(w.stream), "tree-HMM accelerator execution");
+  CheckPersistent(w);
 }
 
 }  // namespace thmm

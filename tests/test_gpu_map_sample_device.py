@@ -333,3 +333,33 @@ def test_device_calls_replay_in_a_caller_graph(require_gpu, kind):
         assert np.array_equal(states.cpu().numpy(), want[1]), replay
     del graph
     ws.close()
+
+
+@pytest.mark.parametrize("k", [8, 16, 32])
+def test_map_rake_keeps_a_nan_first_column(require_gpu, k):
+    """The reference's argmax (src/inference.cc:113-126) starts at column 0's
+    candidate: when that candidate is NaN (here log 0 + inf: a +inf node
+    potential in state 0 meeting a zero edge column), no later candidate
+    compares greater and every row of the rake keeps choice 0. The team
+    rake kernels must do the same rather than pick the best finite column."""
+    import torch
+
+    plan = thmm.Plan(np.array([-1, 0, 0], np.int64))
+    nodes = torch.ones((1, 3, k), dtype=torch.float32, device="cuda")
+    nodes[0, 1, 0] = float("inf")
+    nodes[0, 1, 5] = 50.0
+    edges = torch.full((2, k, k), 0.5, dtype=torch.float32, device="cuda")
+    edges[:, :, 5] = 2.0  # column 5 dominates every finite row
+    edge1 = int(np.flatnonzero(plan.edge_children == 1)[0])
+    edges[edge1, :, 0] = 0.0
+    ws = thmm.Workspace()
+    ws.reserve(plan, k, 1, precision="f32", flags=_abi.THMM_RESERVE_MAXIMUM | _abi.THMM_RESERVE_DEVICE_RESIDENT)
+    weights = torch.empty(1, dtype=torch.float64, device="cuda")
+    states = torch.full((1, 3), 99, dtype=torch.int32, device="cuda")
+    ws.maximum_device(nodes.data_ptr(), edges.data_ptr(), 1, weights.data_ptr(), states.data_ptr(), 0)
+    torch.cuda.synchronize()
+    ws.close()
+    got = states.cpu().numpy()[0]
+    assert got[1] == 0, f"leaf with a NaN first candidate took state {got[1]}"
+    assert got[2] == 5
+    assert np.isnan(weights.cpu().numpy()[0])
