@@ -325,6 +325,20 @@ thmm_status thmm_categorical_logz_forward(thmm_workspace *workspace,
                                           const thmm_categorical_model *model,
                                           thmm_categorical_update update,
                                           const double **log_z, size_t *count);
+/* The forward half with the batch (sites) summed on the device: sums[r] =
+ * sum over b of log Z_{b,r} (FP64 [roots], added in order of b), the log
+ * likelihood of the whole alignment for each tree; only these roots values
+ * leave the device. Holds the contraction state for
+ * thmm_branch_length_backward like thmm_categorical_logz_forward (a NULL
+ * seed there then gives the gradient of the summed log likelihood). Same
+ * errors as thmm_categorical_logz_forward (domain_error when a site's
+ * partition function is not positive). No reference counterpart: the
+ * reference returns per-item values and leaves the sum to the caller
+ * (src/gpu_backend_impl.inc:2917-2941 writes [B] results). */
+thmm_status thmm_categorical_log_likelihood(thmm_workspace *workspace,
+                                            const thmm_categorical_model *model,
+                                            thmm_categorical_update update,
+                                            const double **sums, size_t *count);
 thmm_status thmm_branch_length_backward(thmm_workspace *workspace,
                                         const double *grad_log_z,
                                         const double *rate_matrix,
@@ -508,7 +522,14 @@ thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, s
  * same values; persistent: 0 (every level is its own launch). The value 1
  * asks for each pass as one persistent launch whose blocks pull level
  * chunks from a device queue; that mode is switched off in this build (its
- * launches did not complete on the MI355X) and is refused with logic_error.
+ * launches did not complete on the MI355X) and is refused with logic_error;
+ * categorical_leaves: categorical partition, marginal and branch-length
+ * calls whose rake levels run on the generic kernels (K other than 8, 16,
+ * 32 and 64, no fused forward program) form the leaf vectors of the leaf
+ * rakes from the observations, emissions and root potential inside the
+ * rake kernels and skip those leaves in the node initialisation (1,
+ * default), or write every node's factor first (0); both give the same
+ * values bit for bit, and a change drops a held forward pass.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
  * forward program, and a new compress_depth also drops a held
@@ -519,6 +540,7 @@ typedef struct thmm_tuning {
   uint32_t compress_depth;
   uint32_t rake_log_edges;
   uint32_t persistent;
+  uint32_t categorical_leaves;
   double ms;
 } thmm_tuning;
 

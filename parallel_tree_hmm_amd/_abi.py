@@ -125,6 +125,7 @@ class Tuning(ctypes.Structure):
         ("compress_depth", c_u32),
         ("rake_log_edges", c_u32),
         ("persistent", c_u32),
+        ("categorical_leaves", c_u32),
         ("ms", ctypes.c_double),
     ]
 
@@ -210,6 +211,10 @@ PROTOTYPES = {
     ),
     "thmm_result_log_partitions_f64": (c_status, [c_void_p, P(c_void_p), P(c_size)]),
     "thmm_categorical_logz_forward": (
+        c_status,
+        [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_size)],
+    ),
+    "thmm_categorical_log_likelihood": (
         c_status,
         [c_void_p, P(CategoricalModel), c_int, P(c_void_p), P(c_size)],
     ),

@@ -528,7 +528,7 @@ class Workspace:
 
     # -- launch tuning -----------------------------------------------------
     _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk",
-                      "compress_depth", "rake_log_edges", "persistent")
+                      "compress_depth", "rake_log_edges", "persistent", "categorical_leaves")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()
@@ -571,6 +571,19 @@ class Workspace:
         update = CategoricalInputUpdate.kAll if update is None else update
         ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
         _check(_lib().thmm_categorical_logz_forward(self._handle, ctypes.byref(m), int(update), ctypes.byref(ptr), ctypes.byref(count)))
+        del keep
+        return _array(ptr, count.value, np.float64).copy()
+
+    def categorical_log_likelihood(self, model: "BatchedCategoricalModelView", update=None) -> np.ndarray:
+        """Forward half with the sites summed on the device: FP64 [roots],
+        sum over the batch of log Z; keeps the contraction state for
+        branch_length_backward (grad_log_z None then differentiates the
+        sum)."""
+        m, keep = _categorical(model, self)
+        update = CategoricalInputUpdate.kAll if update is None else update
+        ptr, count = ctypes.c_void_p(), ctypes.c_size_t()
+        _check(_lib().thmm_categorical_log_likelihood(self._handle, ctypes.byref(m), int(update), ctypes.byref(ptr),
+                                                      ctypes.byref(count)))
         del keep
         return _array(ptr, count.value, np.float64).copy()
 

@@ -72,7 +72,7 @@ thmm::Workspace &W(thmm_workspace *workspace) {
 
 thmm_tuning ToC(const thmm::Tuning &t, double ms) {
   return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, t.compress_depth,
-          t.rake_log_edges, t.persistent, ms};
+          t.rake_log_edges, t.persistent, t.categorical_leaves, ms};
 }
 
 }  // namespace
@@ -265,6 +265,14 @@ thmm_status thmm_categorical_logz_forward(thmm_workspace *workspace, const thmm_
   });
 }
 
+thmm_status thmm_categorical_log_likelihood(thmm_workspace *workspace, const thmm_categorical_model *model,
+                                            thmm_categorical_update update, const double **sums, size_t *count) {
+  return Guard([&] {
+    Require(model && sums && count, "argument is null");
+    W(workspace).CategoricalLogLikelihood(*model, update, sums, count);
+  });
+}
+
 thmm_status thmm_branch_length_backward(thmm_workspace *workspace, const double *grad_log_z,
                                         const double *rate_matrix, const double **grad, size_t *count) {
   return Guard([&] {
@@ -307,6 +315,7 @@ thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning
     t.compress_depth = tuning->compress_depth;
     t.rake_log_edges = tuning->rake_log_edges;
     t.persistent = tuning->persistent;
+    t.categorical_leaves = tuning->categorical_leaves;
     W(workspace).SetTuning(t);
   });
 }

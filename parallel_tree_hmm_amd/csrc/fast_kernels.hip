@@ -1893,13 +1893,17 @@ bool LaunchStageLogEdges(const Launch &l, const void *log_edges, const std::uint
   return true;
 }
 
-bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
-                         std::uint32_t X, std::uint32_t Y, double *scale) {
-  const std::uint32_t K = l.dims.K;
+bool FastTransposeFits(std::uint32_t K, std::uint32_t X, std::uint32_t Y) {
   if (K % 4 != 0 || K > 64 || (K & (K - 1)) != 0) return false;
   // Below 8 states a vector is under 32 bytes: scattered writes only pay when
   // one side of the transposition is trivial.
-  if (K < 8 && X != 1 && Y != 1) return false;
+  return K >= 8 || X == 1 || Y == 1;
+}
+
+bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
+                         std::uint32_t X, std::uint32_t Y, double *scale) {
+  const std::uint32_t K = l.dims.K;
+  if (!FastTransposeFits(K, X, Y)) return false;
   if (!Aligned16(in) || !Aligned16(out)) return false;
   if (static_cast<std::uint64_t>(X) * Y == 0) return true;
   const bool f_in = in_real == Real::kF32, f_out = out_real == Real::kF32;

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <type_traits>
 
 #include "device_common.h"
 #include "launch.h"
@@ -63,27 +64,88 @@ template <int G> __device__ inline Lane LaneOf() {
 // Scaled forward
 // ------------------------------------------------------------------------
 
+// log v for v > 0, else -inf (also for NaN), in the element type.
+__device__ inline float ElementLog(float v) { return v > 0.0f ? logf(v) : NegInf<float>(); }
+
+// Entry s of the max-shifted log leaf vector of categorical node n, item b
+// (lane s of a G-lane group; NegInf for s >= K) and the vector's FP64 scale,
+// formed as InitCategorical forms them: the product in T, its log in FP64
+// (or in FP32 with c.element_logs), the maximum over the group.
+template <typename T, int G>
+__device__ inline T CategoricalLeaf(const CategoricalLeaves &c, const Dims &d, std::uint32_t n, std::uint32_t b,
+                                    std::uint32_t s, double *scale) {
+  const bool rooted = c.is_root[n] != 0;
+  const std::uint32_t o = c.index[n];
+  const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : c.obs[static_cast<std::uint64_t>(o) * d.B + b];
+  const T *root = static_cast<const T *>(c.root);
+  double x = 0.0;
+  if (s < d.K) {
+    T value;
+    if (o == 0xFFFFFFFFu) {
+      value = rooted ? root[s] : T(1);
+    } else {
+      const T e = category < c.categories
+                      ? static_cast<const T *>(c.emissions)[static_cast<std::uint64_t>(category) * d.K + s]
+                      : T(0);
+      value = (rooted ? root[s] : T(1)) * e;
+    }
+    x = static_cast<double>(value);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (c.element_logs) {
+      const float l = s < d.K ? ElementLog(static_cast<float>(x)) : NegInf<float>();
+      const float top = GroupMax<G>(l);
+      const bool live = top > NegInf<float>();
+      *scale = live ? static_cast<double>(top) : kNegInf;
+      return live ? l - top : NegInf<float>();
+    }
+  }
+  const double l = x > 0.0 ? log(x) : kNegInf;
+  const double top = GroupMax<G>(l);
+  *scale = top;
+  return top > kNegInf && x > 0.0 ? static_cast<T>(l - top) : NegInf<T>();
+}
+
+// exp of the leaf vector of a rake: entry c of exp(v) is read from the node
+// state, or for a categorical leaf (`own` = this lane's exp(v[s]), the same
+// flag on every lane of the group) taken from lane c of the group.
+template <typename T, int G>
+__device__ inline T LeafExp(bool categorical, T own, const T *v, std::uint32_t c) {
+  return categorical ? __shfl(own, static_cast<int>(c), G) : Exp(v[c]);
+}
+
 // branch[p] = log sum_c path[p][c] exp(leaf[c]) (src/inference.cc:704-723)
 template <typename T, int G>
 __global__ void RakeForward(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
-  const DeviceRake op = ops.rakes[offset + lane.item / d.B];
+  const std::uint32_t r = offset + static_cast<std::uint32_t>(lane.item / d.B);
+  const DeviceRake op = ops.rakes[r];
   const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
   const T *v = static_cast<const T *>(buf.nodes) + leaf_at * d.K;
-  double message = kNegInf;
-  if (lane.s < d.K) {
-    const T *row = Slot<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(lane.s) * d.K;
-    T acc = T(0);
-    for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
-    if (acc > T(0)) message = RowScale(buf, d, op.slot, b, lane.s) + log(static_cast<double>(acc));
+  const bool categorical = r < buf.cat.rakes;
+  T own = T(0);
+  double leaf_scale = kNegInf;
+  if (categorical) {
+    const T x = CategoricalLeaf<T, G>(buf.cat, d, op.leaf, b, lane.s, &leaf_scale);
+    own = Exp(x);
+  } else {
+    leaf_scale = buf.node_scale[leaf_at];
   }
+  // Lanes s >= K run the loop on row K - 1 (shuffles need the whole group)
+  // and drop the result.
+  const std::uint32_t s = lane.s < d.K ? lane.s : d.K - 1;
+  const T *row = Slot<T>(buf, d, op.slot, b) + static_cast<std::uint64_t>(s) * d.K;
+  T acc = T(0);
+  for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * LeafExp<T, G>(categorical, own, v, c);
+  double message = kNegInf;
+  if (lane.s < d.K && acc > T(0)) message = RowScale(buf, d, op.slot, b, lane.s) + log(static_cast<double>(acc));
   const double top = GroupMax<G>(message);
   const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
   const bool live = top > kNegInf;
   if (lane.s < d.K) static_cast<T *>(buf.branches)[at * d.K + lane.s] = live ? static_cast<T>(message - top) : NegInf<T>();
-  if (lane.s == 0) buf.branch_scale[at] = live ? buf.node_scale[leaf_at] + top : kNegInf;
+  if (lane.s == 0) buf.branch_scale[at] = live ? leaf_scale + top : kNegInf;
 }
 
 // Sum of two shifted log vectors, re-shifted (src/inference.cc:725-747).
@@ -397,24 +459,35 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
-  const DeviceRake op = ops.rakes[offset + lane.item / d.B];
+  const std::uint32_t r = offset + static_cast<std::uint32_t>(lane.item / d.B);
+  const DeviceRake op = ops.rakes[r];
   const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
   const T *v = static_cast<const T *>(buf.nodes) + leaf_at * d.K;
   const T *P = Slot<T>(buf, d, op.slot, b);
+  const bool categorical = r < buf.cat.rakes;
+  T own = T(0);
+  if (categorical) {
+    double unused;
+    own = Exp(CategoricalLeaf<T, G>(buf.cat, d, op.leaf, b, lane.s, &unused));
+  }
+  // Lanes s >= K run the loops on row K - 1 and store nothing.
+  const std::uint32_t s = lane.s < d.K ? lane.s : d.K - 1;
+  const T *row = P + static_cast<std::uint64_t>(s) * d.K;
+  T acc = T(0);
+  for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * LeafExp<T, G>(categorical, own, v, c);
   double g = 0.0;
   if (lane.s < d.K) {
-    const T *row = P + static_cast<std::uint64_t>(lane.s) * d.K;
-    T acc = T(0);
-    for (std::uint32_t c = 0; c < d.K; ++c) acc += row[c] * Exp(v[c]);
     const T a = RakeAdjoint<T>(buf, d, op, b)[lane.s];
     g = (a != T(0) && acc > T(0)) ? static_cast<double>(a) / static_cast<double>(acc) : 0.0;
-    const AdjointTarget<T> out = SlotAdjointTarget<T>(buf, d, op.slot, b);
-    const std::uint64_t at = static_cast<std::uint64_t>(lane.s) * d.K;
-    for (std::uint32_t c = 0; c < d.K; ++c)
-      StoreAdjointValue(out, at + c, static_cast<T>(g * static_cast<double>(row[c] * Exp(v[c]))));
+  }
+  const AdjointTarget<T> out = SlotAdjointTarget<T>(buf, d, op.slot, b);
+  const std::uint64_t at = static_cast<std::uint64_t>(s) * d.K;
+  for (std::uint32_t c = 0; c < d.K; ++c) {
+    const T e = LeafExp<T, G>(categorical, own, v, c);
+    if (lane.s < d.K) StoreAdjointValue(out, at + c, static_cast<T>(g * static_cast<double>(row[c] * e)));
   }
   double column = 0.0;
-  const T e_s = lane.s < d.K ? Exp(v[lane.s]) : T(0);
+  const T e_s = lane.s >= d.K ? T(0) : categorical ? own : Exp(v[lane.s]);
   for (std::uint32_t p = 0; p < d.K; ++p) {
     const double gp = __shfl(g, static_cast<int>(p), G);
     if (lane.s < d.K) column += gp * static_cast<double>(P[static_cast<std::uint64_t>(p) * d.K + lane.s] * e_s);
@@ -609,11 +682,13 @@ __global__ void InitEdgesFromLogs(const T *in, T *edges, double *scale, Dims d) 
 template <typename P, typename Out, bool kLog>
 __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *obs_index, std::uint32_t obs_count,
                                 std::uint32_t categories, const std::uint8_t *is_root, const P *root,
-                                const P *emissions, Out *nodes, double *scale, Dims d) {
+                                const P *emissions, Out *nodes, double *scale, Dims d,
+                                const std::uint8_t *skip, bool element_logs) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (tid >= static_cast<std::uint64_t>(d.N) * d.B) return;
   const std::uint32_t n = static_cast<std::uint32_t>(tid / d.B);
   const std::uint32_t b = static_cast<std::uint32_t>(tid % d.B);
+  if (skip != nullptr && skip[n] != 0) return;
   Out *out = nodes + tid * d.K;
   const bool rooted = is_root[n] != 0;
   const std::uint32_t o = obs_index[n];
@@ -629,6 +704,16 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
     for (std::uint32_t s = 0; s < d.K; ++s) out[s] = static_cast<Out>(value(s));
     return;
   }
+  if constexpr (std::is_same<P, float>::value && std::is_same<Out, float>::value) {
+    if (element_logs) {  // as the team transpose: FP32 logs, shift and maximum
+      float top = NegInf<float>();
+      for (std::uint32_t s = 0; s < d.K; ++s) top = Max(top, ElementLog(value(s)));
+      const bool live = top > NegInf<float>();
+      for (std::uint32_t s = 0; s < d.K; ++s) out[s] = live ? ElementLog(value(s)) - top : NegInf<float>();
+      scale[tid] = live ? static_cast<double>(top) : kNegInf;
+      return;
+    }
+  }
   double top = kNegInf;
   for (std::uint32_t s = 0; s < d.K; ++s) {
     const double x = static_cast<double>(value(s));
@@ -640,6 +725,15 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
     out[s] = live && x > 0.0 ? static_cast<Out>(log(x) - top) : NegInf<Out>();
   }
   scale[tid] = top;
+}
+
+// sums[r] = sum over b of log_z[b * roots + r], added in order of b.
+__global__ void SumOverBatch(const double *log_z, double *sums, Dims d) {
+  const std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= d.roots) return;
+  double sum = 0.0;
+  for (std::uint32_t b = 0; b < d.B; ++b) sum += log_z[static_cast<std::uint64_t>(b) * d.roots + r];
+  sums[r] = sum;
 }
 
 template <typename T>
@@ -1423,7 +1517,8 @@ void LaunchInitEdgesFromLogs(const Launch &l, const void *log_input, void *edges
 void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const std::uint8_t *observations,
                            const std::uint32_t *observation_index_by_node, std::uint32_t observation_count,
                            std::uint32_t categories, const std::uint8_t *is_root, const void *root_potential,
-                           const void *emissions, void *nodes, double *node_scale) {
+                           const void *emissions, void *nodes, double *node_scale, const std::uint8_t *skip,
+                           bool element_logs) {
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(l.dims.N) * l.dims.B, kThreads);
   const auto *obs = observations;
   const auto *index = observation_index_by_node;
@@ -1432,27 +1527,32 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const s
     const auto *em = static_cast<const float *>(emissions);
     if (raw)
       InitCategorical<float, float, false><<<grid, kThreads, 0, StreamOf(l)>>>(
-          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims);
+          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims, skip, element_logs);
     else if (l.real == Real::kF32)
       InitCategorical<float, float, true><<<grid, kThreads, 0, StreamOf(l)>>>(
-          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims);
+          obs, index, observation_count, categories, is_root, root, em, static_cast<float *>(nodes), node_scale, l.dims, skip, element_logs);
     else
       InitCategorical<float, double, true><<<grid, kThreads, 0, StreamOf(l)>>>(
           obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
-          l.dims);
+          l.dims, skip, element_logs);
   } else {
     const auto *root = static_cast<const double *>(root_potential);
     const auto *em = static_cast<const double *>(emissions);
     if (raw)
       InitCategorical<double, double, false><<<grid, kThreads, 0, StreamOf(l)>>>(
           obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
-          l.dims);
+          l.dims, skip, element_logs);
     else
       InitCategorical<double, double, true><<<grid, kThreads, 0, StreamOf(l)>>>(
           obs, index, observation_count, categories, is_root, root, em, static_cast<double *>(nodes), node_scale,
-          l.dims);
+          l.dims, skip, element_logs);
   }
   Check(hipGetLastError(), "categorical initialisation launch");
+}
+
+void LaunchSumOverBatch(const Launch &l, const double *log_z, double *sums) {
+  SumOverBatch<<<BlocksFor(l.dims.roots, kThreads), kThreads, 0, StreamOf(l)>>>(log_z, sums, l.dims);
+  Check(hipGetLastError(), "batch sum launch");
 }
 
 // Whether a compression level goes to the team kernels: K = 4, 8, 16 or
@@ -1492,7 +1592,9 @@ void LaunchCompressReverseTeam(const Launch &l, std::uint32_t first, std::uint32
 void LaunchForwardLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
-    if (LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
+    // Rakes with categorical leaves run on the generic kernels.
+    const bool categorical = primitive == 0 && first < buf.cat.rakes;
+    if (!categorical && LaunchFastLevel(l, false, primitive, first, ops, buf)) return;
     // Compressions at K = 4 and 8, and FP64 ones at K = 16 and 32: the team
     // kernel (16-byte aligned slots; K*K/4 lanes per item, below 2^31
     // work-items). FP32 compressions at K = 16 and 32 took the MFMA kernels.
@@ -1540,7 +1642,8 @@ void LaunchSeedAdjoint(const Launch &l, const LinearBuffers &buf, const double *
 void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset, std::uint32_t count,
                         const LinearBuffers &buf) {
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
-    if (LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
+    const bool categorical = primitive == 0 && first < buf.cat.rakes;
+    if (!categorical && LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
     // Compressions at K = 4, 8 and 16 (those the MFMA kernels left): the
     // team kernel.
     if (primitive == 3 && !buf.compress_log_edges && TeamCompressionFits(l, ops, buf, 16)) {

@@ -85,6 +85,22 @@ struct Dims {
   std::uint32_t roots;  // number of roots
 };
 
+// Observations of a categorical call, for the generic rake kernels to form
+// the leaf vectors of the first `rakes` rake ops (whose leaves are original
+// leaves) themselves instead of reading node state: the factor of node n,
+// item b is (root potential if n is a root, else 1) times the emission row
+// of the observed category (all zero for a category >= categories; 1 for an
+// unobserved node), in the element type. rakes = 0: none.
+struct CategoricalLeaves {
+  const std::uint8_t *obs;        // [observation node][B]
+  const std::uint32_t *index;     // [N] observation index of a node, 0xFFFFFFFF if unobserved
+  const std::uint8_t *is_root;    // [N]
+  const void *root, *emissions;   // [K], [categories][K]
+  std::uint32_t categories;
+  std::uint32_t rakes;
+  std::uint32_t element_logs;     // logs in the element type (see LaunchInitCategorical)
+};
+
 // Scaled contraction state (element type float or double, erased).
 //   nodes      [N][B][K] max-shifted logs     node_scale   [N][B]   FP64
 //   branches   [R][B][K] max-shifted logs     branch_scale [R][B]   FP64
@@ -121,6 +137,9 @@ struct LinearBuffers {
   // FP32 workspace): the K = 8 and 16 kernels widen them on load, which is
   // exact. Only those kernels may run with it set.
   int edges_f32;
+  // Leaf rakes of a categorical call (rake levels with cat.rakes != 0 run
+  // on the generic rake kernels).
+  CategoricalLeaves cat;
   // Compressions read their edge slots from log_edges too (the K = 32 and 64
   // FP32 MFMA kernels; no other compression kernel may run with it set).
   // The forward one writes each row's shift (its maximum, -inf for a row of
@@ -159,6 +178,7 @@ struct Tuning {
   std::uint32_t compress_depth = 2;          // compression-tree levels per grouped compression: 1 (none) .. 3
   std::uint32_t rake_log_edges = 1;          // log-potential calls: rakes read log edges in place (1) or staged (0)
   std::uint32_t persistent = 0;              // K = 16 FP32 forests: each pass as one persistent launch (1)
+  std::uint32_t categorical_leaves = 1;      // categorical calls: leaf rakes form their leaves from observations (1)
 };
 
 struct Launch {
@@ -197,12 +217,24 @@ void LaunchTransposeObservations(const Launch &l, const std::uint8_t *in, std::u
 // equal to the dense construction. `raw` stores the products as they are
 // (element type product_real); otherwise max-shifted logs of element type
 // launch.real plus FP64 scales.
+// Nodes with skip[n] set (null: none) are left unwritten. The logs are
+// taken as the dense initialisation of the same factors takes them: in the
+// element type when `element_logs` (an FP32 workspace whose dense
+// initialisation runs on the team transpose, FastTransposeFits), else in
+// FP64.
 void LaunchInitCategorical(const Launch &l, Real product_real, bool raw,
                            const std::uint8_t *observations,
                            const std::uint32_t *observation_index_by_node,
                            std::uint32_t observation_count, std::uint32_t categories,
                            const std::uint8_t *is_root, const void *root_potential,
-                           const void *emissions, void *nodes, double *node_scale);
+                           const void *emissions, void *nodes, double *node_scale,
+                           const std::uint8_t *skip = nullptr, bool element_logs = false);
+// Whether the initialisation of linear potentials [X][Y][K] (16-byte
+// aligned) runs on the team transpose, which takes logs in the wider of the
+// input and state element types rather than in FP64.
+bool FastTransposeFits(std::uint32_t K, std::uint32_t X, std::uint32_t Y);
+// sums[r] = sum over b of log_z[b * roots + r], in order of b (FP64).
+void LaunchSumOverBatch(const Launch &l, const double *log_z, double *sums);
 
 // ---- fused rake + combination + absorption (sum-product forward) ----
 // The rakes of one round, grouped by the absorption their branch reaches

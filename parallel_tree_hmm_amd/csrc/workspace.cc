@@ -213,6 +213,15 @@ struct Workspace::Impl {
 
   // topology
   DeviceBuffer rakes, combines, absorbs, compresses, roots, is_root, observation_index;
+  // Categorical leaf fusion: the first cat_rakes rakes (their leaves original
+  // leaves) form their leaf vectors from the observations in the rake
+  // kernels of partition and marginal passes; cat_skip [N] marks those
+  // leaves for the node initialisation to skip. cat_rakes = 0: no fusion.
+  DeviceBuffer cat_skip;
+  std::uint32_t cat_rakes = 0;
+  // batch sums of log Z (thmm_categorical_log_likelihood)
+  DeviceBuffer site_sums;
+  PinnedBuffer host_site_sums;
   // inputs
   PinnedBuffer host_nodes, host_edges, host_uniforms, host_observations, host_root, host_emissions;
   // observations [B][O] as uploaded, observations_t [O][B] as the kernels read them
@@ -253,8 +262,8 @@ struct Workspace::Impl {
     persistent_failure.Release();
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
-                            &input_nodes, &uniforms, &observations, &observations_t, &root_potential,
-                            &emissions, &edges,
+                            &cat_skip, &site_sums, &input_nodes, &uniforms, &observations, &observations_t,
+                            &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
                             &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
                             &s_nodes, &s_node_scale, &s_paths, &s_path_scale, &s_branches,
@@ -266,7 +275,7 @@ struct Workspace::Impl {
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
                             &host_edge_out, &host_failure, &host_chain_seed, &host_chain_rate, &host_chain_grad,
-                            &host_expm_factors})
+                            &host_expm_factors, &host_site_sums})
       b->Release();
     for (hipEvent_t &e : events) {
       if (e != nullptr) static_cast<void>(hipEventDestroy(e));
@@ -280,6 +289,7 @@ struct Workspace::Impl {
     flags = 0;
     device = 0;
     categorical = false;
+    cat_rakes = 0;
     observation_nodes.clear();
     resident_observation_batch = 0;
     resident_factors = false;
@@ -1291,19 +1301,44 @@ struct Source {
   const thmm_categorical_model *categorical = nullptr;
 };
 
+// Whether the categorical node state of launch l takes its logs in the
+// element type: exactly when the dense initialisation of the same factors
+// (staged, aligned [B][N][K]) does, so that both give the same state.
+bool ElementLogs(const Impl &w, const Launch &l) {
+  return l.real == Real::kF32 && w.real == Real::kF32 && FastTransposeFits(l.dims.K, l.dims.B, l.dims.N);
+}
+
 // Writes the internal node state (max-shifted logs + scales, element type
 // l.real) of a call from its staged inputs, or for a dense call from
-// `dense_nodes` ([B][N][K], the workspace element type) when given.
+// `dense_nodes` ([B][N][K], the workspace element type) when given. With
+// `fused_leaves` a categorical call leaves the leaves of the workspace's
+// categorical rakes unwritten (their rake kernels form them).
 void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale,
-               const void *dense_nodes = nullptr) {
+               const void *dense_nodes = nullptr, bool fused_leaves = false) {
   if (src.categorical == nullptr) {
     LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale);
   } else {
     LaunchInitCategorical(l, w.real, false, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
                           static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
-                          w.root_potential.ptr, w.emissions.ptr, nodes, scale);
+                          w.root_potential.ptr, w.emissions.ptr, nodes, scale,
+                          fused_leaves && w.cat_rakes != 0 && w.tuning.categorical_leaves != 0
+                              ? w.cat_skip.as<std::uint8_t>()
+                              : nullptr,
+                          ElementLogs(w, l));
   }
+}
+
+// The partition / marginal buffers of a call: for a categorical call on a
+// workspace with leaf fusion, its first cat_rakes rakes read the staged
+// observations, emissions and root potential.
+LinearBuffers CallLinear(const Impl &w, const Launch &l, bool categorical) {
+  LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  if (categorical && w.cat_rakes != 0 && w.tuning.categorical_leaves != 0)
+    buf.cat = {w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
+               w.is_root.as<std::uint8_t>(), w.root_potential.ptr, w.emissions.ptr,
+               static_cast<std::uint32_t>(w.categories), w.cat_rakes, ElementLogs(w, l) ? 1u : 0u};
+  return buf;
 }
 
 void DownloadLogZ(Impl &w, std::size_t batch) {
@@ -1343,8 +1378,8 @@ void AdjointsInto(LinearBuffers &buf, std::size_t batch, void *node_out) {
 // Scaled forward + log Z for a staged call.
 void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
-  const LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true);
+  const LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
   SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
   CheckLaunches();
@@ -1352,8 +1387,8 @@ void RunPartition(Impl &w, std::size_t batch, const Source &src) {
 
 void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>());
-  LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true);
+  LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
   AdjointsInto(buf, batch, w.node_out.ptr);
   SumProductForward(w, l, buf);
   LaunchFinish(l, buf, w.log_z.as<double>());
@@ -1535,6 +1570,22 @@ void Workspace::ReserveCategorical(const Plan &plan, const void *identity, std::
     std::vector<std::uint32_t> index(plan.num_nodes, 0xFFFFFFFFu);
     for (std::size_t i = 0; i < observation_count; ++i) index[observation_nodes[i]] = static_cast<std::uint32_t>(i);
     Upload(w.observation_index, index, w.stream, "categorical topology upload");
+    // Leaf fusion where the levels run on the generic rake kernels (no fused
+    // forward program, K other than 8, 16, 32, 64): the longest prefix of the
+    // rakes whose leaves have no children. Each such leaf is read by its
+    // rake alone, forward and reverse.
+    if (w.fused.empty() && states != 8 && states != 16 && states != 32 && states != 64) {
+      std::vector<std::uint8_t> has_child(plan.num_nodes, 0), skip(plan.num_nodes, 0);
+      for (Index parent : plan.edge_parents) has_child[parent] = 1;
+      std::size_t r = 0;
+      while (r < plan.rakes.size() && !has_child[plan.rakes[r].leaf]) skip[plan.rakes[r++].leaf] = 1;
+      if (r != 0) {
+        Upload(w.cat_skip, skip, w.stream, "categorical topology upload");
+        w.cat_rakes = static_cast<std::uint32_t>(r);
+      }
+    }
+    w.site_sums.Allocate(plan.roots.size() * sizeof(double), "log-likelihood sums");
+    w.host_site_sums.Allocate(plan.roots.size() * sizeof(double), "pinned log-likelihood sums");
     Hip(hipStreamSynchronize(w.stream), "categorical topology upload");
   } catch (...) {
     w.Clear();
@@ -1845,9 +1896,12 @@ void Workspace::MarginalsCategorical(const thmm_categorical_model &model, thmm_c
   FillTimings(w, timer, timings);
 }
 
-void Workspace::CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
-                                       const double **log_z, std::size_t *count) {
-  Impl &w = *impl_;
+namespace {
+
+// The forward half of the branch-length pair; with `summed` the log
+// partitions are summed over the batch on the device and only the [roots]
+// sums are downloaded (into host_site_sums).
+void CategoricalForward(Impl &w, const thmm_categorical_model &model, thmm_categorical_update update, bool summed) {
   RequireHost(w);
   RequireMarginals(w);
   if (!w.categorical) throw std::logic_error("the branch-length pair needs a ReserveCategoricalMarginals workspace");
@@ -1857,15 +1911,47 @@ void Workspace::CategoricalLogZForward(const thmm_categorical_model &model, thmm
   const CategoricalPolicy policy = StageCategorical(w, model, update);
   UploadCategorical(w, model, policy);
   RunPartition(w, model.batch, Source{&model});
-  DownloadLogZ(w, model.batch);
+  if (summed) {
+    const Launch l = w.MakeLaunch(w.real, model.batch, w.stream);
+    LaunchSumOverBatch(l, w.log_z.as<double>(), w.site_sums.as<double>());
+    CheckLaunches();
+    Hip(hipMemcpyAsync(w.host_site_sums.ptr, w.site_sums.ptr, w.Roots() * sizeof(double), hipMemcpyDeviceToHost,
+                       w.stream),
+        "GPU runtime MemcpyAsync output download");
+  } else {
+    DownloadLogZ(w, model.batch);
+  }
   Record(w, 5);
   WaitDownload(w);
   FinishCategorical(w, model, policy);
-  RequireFiniteLogZ(w, model.batch, "the tree HMM has a nonpositive partition function");
+  if (summed) {
+    // A sum is finite exactly when every term is.
+    for (std::size_t r = 0; r < w.Roots(); ++r)
+      if (!std::isfinite(w.host_site_sums.as<double>()[r]))
+        throw std::domain_error("the tree HMM has a nonpositive partition function");
+  } else {
+    RequireFiniteLogZ(w, model.batch, "the tree HMM has a nonpositive partition function");
+  }
   w.last_batch = model.batch;
   w.forward = Impl::Forward::kCategorical;
+}
+
+}  // namespace
+
+void Workspace::CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
+                                       const double **log_z, std::size_t *count) {
+  Impl &w = *impl_;
+  CategoricalForward(w, model, update, false);
   *log_z = w.host_log_z.as<double>();
   *count = model.batch * w.Roots();
+}
+
+void Workspace::CategoricalLogLikelihood(const thmm_categorical_model &model, thmm_categorical_update update,
+                                         const double **sums, std::size_t *count) {
+  Impl &w = *impl_;
+  CategoricalForward(w, model, update, true);
+  *sums = w.host_site_sums.as<double>();
+  *count = w.Roots();
 }
 
 void Workspace::SetBranchLengths(const double *rate_matrix, const double *stationary, const double *branch_lengths,
@@ -1919,7 +2005,7 @@ void Workspace::BranchLengthBackward(const double *grad_log_z, const double *rat
                      w.stream),
       "GPU runtime MemcpyAsync rate-matrix upload");
   const Launch l = w.MakeLaunch(w.real, w.last_batch, w.stream);
-  LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
+  LinearBuffers buf = CallLinear(w, l, true);
   AdjointsInto(buf, w.last_batch, w.node_out.ptr);
   LaunchSeedAdjoint(l, buf, grad_log_z != nullptr ? w.chain_seed.as<double>() : nullptr);
   RunReverse(w, l, buf);
@@ -2047,6 +2133,7 @@ void ValidateTuning(const Tuning &t) {
   if (t.compress_depth < 1 || t.compress_depth > 3) throw std::invalid_argument("compression depth must be 1, 2 or 3");
   if (t.rake_log_edges > 1) throw std::invalid_argument("rake_log_edges must be 0 or 1");
   if (t.persistent > 1) throw std::invalid_argument("persistent must be 0 or 1");
+  if (t.categorical_leaves > 1) throw std::invalid_argument("categorical_leaves must be 0 or 1");
   if (t.persistent != 0 && !kPersistentPassesEnabled)
     throw std::logic_error(
         "persistent passes are switched off: on the MI355X their launches did not complete (per-task device-wide "
@@ -2062,7 +2149,8 @@ void ApplyTuning(Impl &w, const Tuning &t) {
   for (Impl::TreeGroup &g : w.groups) ApplyTuning(*g.w, t);
   const bool rebuild =
       !w.fused.empty() && (t.gather_chunk != w.tuning.gather_chunk || t.compress_depth != w.tuning.compress_depth);
-  if (t.compress_depth != w.tuning.compress_depth) w.forward = Impl::Forward::kNone;
+  if (t.compress_depth != w.tuning.compress_depth || t.categorical_leaves != w.tuning.categorical_leaves)
+    w.forward = Impl::Forward::kNone;
   w.tuning = t;
   for (Impl::GraphCache &g : w.graphs) g.Release();
   if (!rebuild) return;

@@ -71,6 +71,10 @@ class Workspace {
   void LogPartitionsF64(const double **values, std::size_t *count);
   void CategoricalLogZForward(const thmm_categorical_model &model, thmm_categorical_update update,
                               const double **log_z, std::size_t *count);
+  // The forward half with the log partitions summed over the batch on the
+  // device: sums [roots], FP64.
+  void CategoricalLogLikelihood(const thmm_categorical_model &model, thmm_categorical_update update,
+                                const double **sums, std::size_t *count);
   // Writes the categorical workspace's device edge factors as expm(Q t_e)
   // (reversible Q with stationary law pi); needs resident observations and
   // factors, drops a held forward pass.
