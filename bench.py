@@ -1082,6 +1082,14 @@ def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
     worst["node_normalisation_abs"] = float((grad_nodes[0].double().sum(1) - 1.0).abs().max())
     worst["trees_checked"] = 2
     log(f"parity: {worst}")
+    budgeted = None
+    if args.path_budget_frac > 0:
+        budgeted = budgeted_pass(args, thmm, torch, plan, k, ws, log_nodes, log_edges, log_z, grad_nodes, grad_edges,
+                                 ms)
+    ws.close()
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_log(thmm, oracle, parts, offsets, log_nodes[0], log_edges, k, trees=min(trees, 16))
     result = {
         "metric": "nodes/s log Z + adjoint (dense-state forest, log potentials)",
         "value": value,
@@ -1100,10 +1108,82 @@ def run_dense_state_adjoint(args, thmm, oracle, torch) -> int:
         "roofline": {"kernel": "whole step", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_step": int(step_bytes)},
+        "cpu_baseline": cpu,
         "parity": worst,
+        "path_budget": budgeted,
     }
     print(json.dumps(result), flush=True)
     return 0
+
+
+def edge_checksum(torch, grad_edges):
+    """Sum of the int32 bit patterns of every value (as int64), taken in
+    chunks: equal arrays give equal sums."""
+    total = 0
+    flat = grad_edges.view(grad_edges.shape[0], -1)
+    step = max(1, (1 << 27) // flat.shape[1])
+    for i in range(0, flat.shape[0], step):
+        total += int(flat[i:i + step].view(torch.int32).to(torch.int64).sum())
+    return total
+
+
+def budgeted_pass(args, thmm, torch, plan, k, ws, log_nodes, log_edges, log_z, grad_nodes, grad_edges, ms):
+    """The same step with the path storage capped at a fraction of what the
+    unbudgeted reservation holds: trees run in groups sharing one path
+    store and the backward half recomputes each group's paths. Its outputs
+    are compared with the unbudgeted ones (log Z and node gradients bit for
+    bit, edge gradients through a checksum of their bit patterns)."""
+    _, held_full = ws.path_storage()
+    ref_lz, ref_nodes = log_z.clone(), grad_nodes.clone()
+    ref_edges = edge_checksum(torch, grad_edges)
+    ws.close()
+    budget = int(held_full * args.path_budget_frac)
+    wb = thmm.Workspace()
+    wb.set_path_budget(budget)
+    wb.reserve_marginals(plan, k, 1, precision="f32", device_resident=True)
+    groups, held = wb.path_storage()
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        wb.logz_forward_device(log_nodes.data_ptr(), log_edges.data_ptr(), 1, log_z.data_ptr(), stream)
+        wb.logz_backward_device(0, grad_nodes.data_ptr(), grad_edges.data_ptr(), False, stream)
+
+    ms_budget = time_calls(torch, step, args.steps, args.warmup)
+    equal = (bool(torch.equal(log_z, ref_lz)) and bool(torch.equal(grad_nodes, ref_nodes))
+             and edge_checksum(torch, grad_edges) == ref_edges)
+    wb.close()
+    out = {"budget_bytes": budget, "groups": int(groups), "path_bytes_held": int(held),
+           "path_bytes_unbudgeted": int(held_full), "ms_per_step": round(ms_budget, 4),
+           "recompute_cost": round(ms_budget / ms - 1.0, 4), "outputs_equal_unbudgeted": equal}
+    log(f"path budget {budget / 1e9:.2f} GB: {groups} tree groups, {held / 1e9:.2f} GB held "
+        f"(unbudgeted {held_full / 1e9:.2f} GB), {ms_budget:.2f} ms/step, outputs equal: {equal}")
+    return out
+
+
+def cpu_baseline_log(thmm, oracle, parts, offsets, log_nodes, log_edges, k, trees, repeats=5):
+    """The FP64 oracle (log Z + node and edge marginals) on the first `trees`
+    whole trees of a forest given log potentials, one tree per task over
+    CPU_SHARE host threads; one warm-up pass, then the median of `repeats`."""
+    host = host_cpus()
+    threads = max(1, min(CPU_SHARE, host["affinity"]))
+    jobs = []
+    for t in range(trees):
+        lo, hi, elo, ehi = tree_slices(offsets, t)
+        jobs.append((thmm.Plan(parts[t]), np.exp(log_nodes[lo:hi].double().cpu().numpy()),
+                     np.exp(log_edges[elo:ehi].double().cpu().numpy())))
+    total = sum(parts[t].size for t in range(trees))
+
+    def work(job):
+        plan, nn, ee = job
+        return oracle.marginals(plan, k, nn, ee)
+
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        seconds, times = timed_median(lambda: list(pool.map(work, jobs)), repeats)
+    return {"value": total / seconds, "unit": "nodes/s", "cores": threads, "kind": "port",
+            "sample": f"the first {trees} whole trees ({total} nodes), FP64 oracle log Z + node and edge marginals "
+                      f"(the gradients of the log potentials); median of {repeats} passes after one warm-up "
+                      f"({seconds:.2f}s per pass)",
+            "host": host, "pass_seconds": [round(x, 3) for x in times]}
 
 
 def main() -> int:
@@ -1120,6 +1200,9 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch variants")
     ap.add_argument("--no-half", action="store_true", help="skip the binary16 edge-marginal leg")
     ap.add_argument("--no-map", action="store_true", help="skip the device MAP and sampling legs")
+    ap.add_argument("--path-budget-frac", type=float, default=0.25,
+                    help="config 5: also time the step with path storage capped at this fraction of the "
+                         "unbudgeted reservation's (0: skip)")
     ap.add_argument("--persistent", action="store_true",
                     help="also time the persistent-launch form (switched off in this build: the library refuses it)")
     ap.add_argument("--level-table", default="", metavar="PATH",
