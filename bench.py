@@ -417,15 +417,15 @@ def run_map_and_sampling(thmm, torch, plan, k, nodes, edges, args, variate_seed)
     return out
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
-    None when no summary covers it."""
+def measured_traffic(workload, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summaries (FETCH_SIZE + WRITE_SIZE, scaled by a copy of
+    known size), or None when no summary covers it."""
     for path in sorted((ROOT / "profiles").glob("traffic_*.md")):
         for line in path.read_text().splitlines():
             parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return int(parts[2])
+            if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1:3] == [workload, kernel]:
+                return int(parts[3])
     return None
 
 
@@ -485,7 +485,7 @@ class SharedEdgeReads:
         return out
 
 
-def roofline_from(timings, k, s, reads=None):
+def roofline_from(timings, k, s, reads=None, workload=None):
     """The profiled kernel family with the most time, its algorithmic GB/s
     and the breakdown; `reads` (SharedEdgeReads) counts edges shared by a
     batch once per op."""
@@ -514,7 +514,7 @@ def roofline_from(timings, k, s, reads=None):
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(KERNEL_NAMES[key]),
+        "traffic": measured_traffic(workload, KERNEL_NAMES[key]),
         "launches": launches,
         "avg_launch_us": round(ms / launches * 1e3, 3),
         "algorithmic_bytes_per_launch": int(nbytes / launches),
@@ -732,7 +732,7 @@ def run_recursive_fp64(args, thmm, oracle, torch) -> int:
     wse, oute, calle, eargs = marginal_pass(thmm, torch, plan, k, 1, "f64", nodes, edges)
     ms_eager = time_calls(torch, calle, args.steps, args.warmup)
     timings = wse.profile_marginals_device(*eargs)
-    roofline = roofline_from(timings, k, 8)
+    roofline = roofline_from(timings, k, 8, workload="recursive64_k4")
     # Parity: all 64 trees against the FP64 oracle.
     lz = out["log_z"].cpu().numpy()
     nn, ee = nodes[0].cpu().numpy(), edges.cpu().numpy()
@@ -789,7 +789,7 @@ def run_long_chains(args, thmm, oracle, torch) -> int:
     ws, out, call, margs = marginal_pass(thmm, torch, plan, k, batch, "f32", nodes, edges)
     ms = time_calls(torch, call, args.steps, args.warmup)
     timings = ws.profile_marginals_device(*margs)
-    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan))
+    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan), workload="chains16_k8")
     lz = out["log_z"].cpu().numpy()
     # Parity: two whole items against the FP64 oracle; every item's
     # normalisation.
@@ -983,7 +983,7 @@ def run_phylogenetic(args, thmm, oracle, torch) -> int:
     wsd, outd, calld, dargs = marginal_pass(thmm, torch, plan, k, sites, "f32", dense, dedges)
     calld()
     timings = wsd.profile_marginals_device(*dargs)
-    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan))
+    roofline = roofline_from(timings, k, 4, SharedEdgeReads(plan), workload="phylo1024_k4")
     wsd.close()
     log(f"categorical K=4: {ms:.3f} ms/step (per-site log Z {ms_per_site:.3f}, leaves unfused {ms_unfused:.3f}, "
         f"kFactors {ms_factors:.3f}); parity {parity}")
@@ -1281,7 +1281,7 @@ def main() -> int:
         thmm, torch, plan, k, "f32", nodes, edges, args.steps, args.warmup, not args.no_tune
     )
     value = n / (ms * 1e-3)
-    roofline = roofline_from(timings, k, 4)
+    roofline = roofline_from(timings, k, 4, workload="forest4096_k16" if args.scale == 1.0 else None)
     if args.level_table:
         small, total = level_table(timings, k, 4, args.level_table,
                                    f"Per-launch times: uneven forest, {len(parts)} trees ({n} nodes), K=16, FP32 "

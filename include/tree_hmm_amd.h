@@ -326,8 +326,10 @@ thmm_status thmm_categorical_logz_forward(thmm_workspace *workspace,
                                           thmm_categorical_update update,
                                           const double **log_z, size_t *count);
 /* The forward half with the batch (sites) summed on the device: sums[r] =
- * sum over b of log Z_{b,r} (FP64 [roots], added in order of b), the log
- * likelihood of the whole alignment for each tree; only these roots values
+ * sum over b of log Z_{b,r} (FP64 [roots]; 256 partial sums, partial t
+ * adding b = t, t + 256, ... in turn, then added pairwise: t and t + 128,
+ * then t and t + 64, ..., t and t + 1), the log likelihood of the whole
+ * alignment for each tree; only these roots values
  * leave the device. Holds the contraction state for
  * thmm_branch_length_backward like thmm_categorical_logz_forward (a NULL
  * seed there then gives the gradient of the summed log likelihood). Same

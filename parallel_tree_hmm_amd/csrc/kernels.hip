@@ -704,6 +704,11 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
     for (std::uint32_t s = 0; s < d.K; ++s) out[s] = static_cast<Out>(value(s));
     return;
   }
+  if (o == 0xFFFFFFFFu && !rooted) {  // factor 1: every log is 0, as is their maximum
+    for (std::uint32_t s = 0; s < d.K; ++s) out[s] = Out(0);
+    scale[tid] = 0.0;
+    return;
+  }
   if constexpr (std::is_same<P, float>::value && std::is_same<Out, float>::value) {
     if (element_logs) {  // as the team transpose: FP32 logs, shift and maximum
       float top = NegInf<float>();
@@ -727,13 +732,22 @@ __global__ void InitCategorical(const std::uint8_t *obs, const std::uint32_t *ob
   scale[tid] = top;
 }
 
-// sums[r] = sum over b of log_z[b * roots + r], added in order of b.
-__global__ void SumOverBatch(const double *log_z, double *sums, Dims d) {
-  const std::uint64_t r = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= d.roots) return;
+// sums[r] = sum over b of log_z[b * roots + r] in a fixed order: thread t
+// of root r's block adds b = t, t + 256, ... in turn, then the 256 partial
+// sums are added pairwise (t += t + 128, then t += t + 64, ..., 1).
+constexpr unsigned kSumThreads = 256;
+__global__ __launch_bounds__(kSumThreads) void SumOverBatch(const double *log_z, double *sums, Dims d) {
+  __shared__ double part[kSumThreads];
+  const std::uint32_t r = blockIdx.x, t = threadIdx.x;
   double sum = 0.0;
-  for (std::uint32_t b = 0; b < d.B; ++b) sum += log_z[static_cast<std::uint64_t>(b) * d.roots + r];
-  sums[r] = sum;
+  for (std::uint32_t b = t; b < d.B; b += kSumThreads) sum += log_z[static_cast<std::uint64_t>(b) * d.roots + r];
+  part[t] = sum;
+  __syncthreads();
+  for (unsigned s = kSumThreads / 2; s > 0; s >>= 1) {
+    if (t < s) part[t] += part[t + s];
+    __syncthreads();
+  }
+  if (t == 0) sums[r] = part[0];
 }
 
 template <typename T>
@@ -1551,7 +1565,8 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw, const s
 }
 
 void LaunchSumOverBatch(const Launch &l, const double *log_z, double *sums) {
-  SumOverBatch<<<BlocksFor(l.dims.roots, kThreads), kThreads, 0, StreamOf(l)>>>(log_z, sums, l.dims);
+  if (l.dims.roots == 0) return;
+  SumOverBatch<<<l.dims.roots, kSumThreads, 0, StreamOf(l)>>>(log_z, sums, l.dims);
   Check(hipGetLastError(), "batch sum launch");
 }
 

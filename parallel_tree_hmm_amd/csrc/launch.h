@@ -233,7 +233,9 @@ void LaunchInitCategorical(const Launch &l, Real product_real, bool raw,
 // aligned) runs on the team transpose, which takes logs in the wider of the
 // input and state element types rather than in FP64.
 bool FastTransposeFits(std::uint32_t K, std::uint32_t X, std::uint32_t Y);
-// sums[r] = sum over b of log_z[b * roots + r], in order of b (FP64).
+// sums[r] = sum over b of log_z[b * roots + r] (FP64), in a fixed order:
+// 256 partial sums (partial t adds b = t, t + 256, ... in turn), then added
+// pairwise (t and t + 128, then t and t + 64, ..., t and t + 1).
 void LaunchSumOverBatch(const Launch &l, const double *log_z, double *sums);
 
 // ---- fused rake + combination + absorption (sum-product forward) ----

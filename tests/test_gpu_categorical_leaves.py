@@ -14,8 +14,8 @@ workspace and inputs, bit for bit:
 * the dense form on the materialised node factors (the unfused dense
   initialisation) gives the same log partitions bit for bit.
 
-The site sum equals the sequential FP64 sum (in batch order) of the per-item
-log partitions of the same inputs, and its backward with no seed equals the
+The site sum equals the FP64 sum, in the documented fixed order, of the
+per-item log partitions of the same inputs, and its backward with no seed equals the
 pair's backward with no seed after thmm_categorical_logz_forward.
 
 Shapes: K in {2, 3, 4, 5, 7} (the generic rake kernels) and K = 8, 16 (no
@@ -94,6 +94,21 @@ def run_all(ws, model, q, seeds):
     return out
 
 
+def batch_sum(values):
+    """The device's fixed summation order of thmm_categorical_log_likelihood
+    over one root's per-item values (Python floats: IEEE FP64 adds)."""
+    part = [0.0] * 256
+    for t in range(256):
+        for b in range(t, len(values), 256):
+            part[t] += float(values[b])
+    s = 128
+    while s > 0:
+        for t in range(s):
+            part[t] += part[t + s]
+        s //= 2
+    return part[0]
+
+
 def same(a, b):
     a, b = np.asarray(a), np.asarray(b)
     return a.shape == b.shape and a.dtype == b.dtype and a.tobytes() == b.tobytes()
@@ -116,14 +131,11 @@ def test_fused_leaves_equal_the_unfused_form(require_gpu, k, precision, batch):
     plain = run_all(ws, model, c["q"], seeds)
     for name in fused:
         assert same(fused[name], plain[name]), f"K={k} {precision}: {name} differs"
-    # The site sum is the sequential FP64 sum of the per-item log partitions;
-    # its unseeded gradient is the pair's.
+    # The site sum is the fixed-order FP64 sum of the per-item log
+    # partitions; its unseeded gradient is the pair's.
     lz = fused["pair_log_z"].reshape(batch, plan.roots.size)
     for r in range(plan.roots.size):
-        total = 0.0
-        for b in range(batch):
-            total += float(lz[b, r])
-        assert fused["sum"][r] == total, r
+        assert fused["sum"][r] == batch_sum(lz[:, r]), r
     assert same(fused["sum_grad"], fused["pair_grad_ones"])
     # The dense form on the materialised factors (unfused dense
     # initialisation) gives the same log partitions.
@@ -225,9 +237,6 @@ def test_full_categorical_workload_fused_equals_unfused(require_gpu):
         for a, b in zip(results[(1, summed)], results[(0, summed)]):
             assert same(a, b), summed
     lz = results[(1, False)][0]
-    total = 0.0
-    for x in lz:
-        total += float(x)
-    assert results[(1, True)][0][0] == total
+    assert results[(1, True)][0][0] == batch_sum(lz)
     assert np.all(np.isfinite(results[(1, True)][1]))
     ws.close()
