@@ -105,6 +105,18 @@ typedef struct thmm_plan thmm_plan;
 thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes,
                              thmm_plan **out);
 
+/* The same plan as thmm_plan_create (identical arrays), built on HIP device
+ * `device`: per round, leaves are compacted, branches sorted by parent and
+ * unary chains ranked by pointer jumping on the device, and only the ops
+ * come back to the host. `parents` is a host array, or with
+ * parents_on_device != 0 a device array on that device. Same errors as
+ * thmm_plan_create; at most 2^31 - 1 nodes (length_error). No reference
+ * counterpart: the reference plans on the host through btrc::MakePlan
+ * (tests/accelerator_test.h:19-20). */
+thmm_status thmm_plan_create_device(const int64_t *parents, size_t num_nodes,
+                                    int parents_on_device, int device,
+                                    thmm_plan **out);
+
 /* Wraps an externally built plan (e.g. a btrc::Plan flattened by the C++
  * layer). The arrays are copied and checked: every non-root node must be
  * eliminated exactly once and every batch must be write-disjoint. */

@@ -168,6 +168,7 @@ PP = lambda t: ctypes.POINTER(ctypes.POINTER(t))  # noqa: E731
 PROTOTYPES = {
     "thmm_last_error": (ctypes.c_char_p, []),
     "thmm_plan_create": (c_status, [P(ctypes.c_int64), c_size, P(c_void_p)]),
+    "thmm_plan_create_device": (c_status, [c_void_p, c_size, c_int, c_int, P(c_void_p)]),
     "thmm_plan_create_from_arrays": (c_status, [P(PlanArrays), P(c_void_p)]),
     "thmm_plan_get_arrays": (c_status, [c_void_p, P(PlanArrays)]),
     "thmm_plan_destroy": (None, [c_void_p]),

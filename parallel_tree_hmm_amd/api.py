@@ -109,6 +109,26 @@ class Plan:
         self._load_arrays()
 
     @classmethod
+    def on_device(cls, parents, device: int = 0) -> "Plan":
+        """The same plan, built on HIP device `device` (thmm_plan_create_device).
+        `parents` is array-like (host), or a torch int64 tensor on that
+        device (read in place)."""
+        self = cls.__new__(cls)
+        handle = ctypes.c_void_p()
+        if hasattr(parents, "is_cuda") and parents.is_cuda:
+            import torch
+
+            if parents.dtype != torch.int64 or not parents.is_contiguous():
+                raise InvalidArgument("device parents must be a contiguous int64 tensor")
+            _check(_lib().thmm_plan_create_device(parents.data_ptr(), parents.numel(), 1, device, ctypes.byref(handle)))
+        else:
+            host = np.ascontiguousarray(np.asarray(parents, dtype=np.int64))
+            _check(_lib().thmm_plan_create_device(host.ctypes.data, host.size, 0, device, ctypes.byref(handle)))
+        self._handle = handle
+        self._load_arrays()
+        return self
+
+    @classmethod
     def from_arrays(cls, arrays: "_abi.PlanArrays") -> "Plan":
         self = cls.__new__(cls)
         handle = ctypes.c_void_p()

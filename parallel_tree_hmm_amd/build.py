@@ -42,7 +42,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", f"-I{INCLUDE}", f"-I{CSRC}"]
 
-SOURCES = ["plan.cc", "expm.cc", "workspace.cc", "capi.cc", "kernels.hip", "fast_kernels.hip", "static_scratch.hip"]
+SOURCES = ["plan.cc", "expm.cc", "workspace.cc", "capi.cc", "kernels.hip", "fast_kernels.hip", "static_scratch.hip",
+           "device_plan.hip"]
 
 
 def _run(cmd: list[str]) -> None:

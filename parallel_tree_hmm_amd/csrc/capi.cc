@@ -91,6 +91,17 @@ thmm_status thmm_plan_create(const int64_t *parents, size_t num_nodes, thmm_plan
   });
 }
 
+thmm_status thmm_plan_create_device(const int64_t *parents, size_t num_nodes, int parents_on_device, int device,
+                                    thmm_plan **out) {
+  return Guard([&] {
+    Require(out != nullptr, "output pointer is null");
+    Require(parents != nullptr || num_nodes == 0, "parents are null");
+    auto *plan = new thmm_plan{thmm::MakePlanOnDevice(parents, num_nodes, parents_on_device != 0, device), {}};
+    plan->renamed = thmm::RenameSlots(plan->plan);
+    *out = plan;
+  });
+}
+
 thmm_status thmm_plan_create_from_arrays(const thmm_plan_arrays *arrays, thmm_plan **out) {
   return Guard([&] {
     Require(out != nullptr && arrays != nullptr, "argument is null");

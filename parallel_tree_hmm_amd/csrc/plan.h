@@ -72,6 +72,11 @@ RenamedOps RenameSlots(const Plan &plan);
 // out-of-range parents or cycles and length_error beyond 2^32 - 2 nodes.
 Plan MakePlan(const std::int64_t *parents, std::size_t num_nodes);
 
+// The same plan as MakePlan, built on device `device` (device_plan.hip).
+// `parents` is a host array, or with parents_on_device a device array on
+// that device. Same errors as MakePlan; at most 2^31 - 1 nodes.
+Plan MakePlanOnDevice(const std::int64_t *parents, std::size_t num_nodes, bool parents_on_device, int device);
+
 // Copies an external plan and verifies it (see Validate).
 Plan PlanFromArrays(const thmm_plan_arrays &arrays);
 
