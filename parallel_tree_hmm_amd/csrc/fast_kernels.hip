@@ -989,17 +989,20 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
     ReverseGroup<2, true>(g, adj);
     return;
   }
+  // Groups of more than 2 slots gather their middle weights again in
+  // Reverse (the same shuffles of e_own as in Build), which keeps the
+  // kernel of groups of 3 and 4 at 96 registers: five wavefronts per SIMD.
   switch (c.n) {
-    case 2: ReverseGroup<2, (kMax <= 4)>(g, adj); break;
-    case 3: ReverseGroup<3, (kMax <= 4)>(g, adj); break;
-    case 4: ReverseGroup<4, (kMax <= 4)>(g, adj); break;
+    case 2: ReverseGroup<2, false>(g, adj); break;
+    case 3: ReverseGroup<3, false>(g, adj); break;
+    case 4: ReverseGroup<4, false>(g, adj); break;
     default:
       if constexpr (kMax > 4) {
         switch (c.n) {
-          case 5: ReverseGroup<5, (kMax <= 4)>(g, adj); break;
-          case 6: ReverseGroup<6, (kMax <= 4)>(g, adj); break;
-          case 7: ReverseGroup<7, (kMax <= 4)>(g, adj); break;
-          case 8: ReverseGroup<8, (kMax <= 4)>(g, adj); break;
+          case 5: ReverseGroup<5, false>(g, adj); break;
+          case 6: ReverseGroup<6, false>(g, adj); break;
+          case 7: ReverseGroup<7, false>(g, adj); break;
+          case 8: ReverseGroup<8, false>(g, adj); break;
           default: break;
         }
       }
@@ -1014,8 +1017,9 @@ __global__ __launch_bounds__(256) void ChainReversePair(DeviceOps ops, Dims d, s
   ChainReverseBody<2>(ops, d, offset, count, buf, quad_edges, tiles, blockIdx.x);
 }
 
-// Groups of 3 or 4 slots (the kernel handles 2 as well).
-__global__ __launch_bounds__(256) void ChainReverseSmall(DeviceOps ops, Dims d, std::uint32_t offset,
+// Groups of 3 or 4 slots (the kernel handles 2 as well); five wavefronts
+// per SIMD (at most 102 registers per lane).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void ChainReverseSmall(DeviceOps ops, Dims d, std::uint32_t offset,
                                                          std::uint32_t count, LinearBuffers buf, int quad_edges) {
   __shared__ float tiles[kWaves][3][16 * kPad];
   ChainReverseBody<4>(ops, d, offset, count, buf, quad_edges, tiles, blockIdx.x);
