@@ -919,15 +919,15 @@ def run_phylogenetic(args, thmm, oracle, torch) -> int:
 
     # Warm-up of every variant first, then each timed over the same K steps.
     for leaves_flag in (0, 1):
-        ws.set_tuning(categorical_leaves=leaves_flag)
+        ws.set_tuning(input_leaves=leaves_flag)
         for i in range(args.warmup):
             step(i)
             step(i, summed=False)
     grad, ms = timed()
     _, ms_per_site = timed(summed=False)
-    ws.set_tuning(categorical_leaves=0)  # every node factor written first
+    ws.set_tuning(input_leaves=0)  # every node factor written first
     _, ms_unfused = timed()
-    ws.set_tuning(categorical_leaves=1)
+    ws.set_tuning(input_leaves=1)
     start = time.perf_counter()
     for i in range(args.steps):
         ws.categorical_logz_forward(model, thmm.CategoricalInputUpdate.kFactors)

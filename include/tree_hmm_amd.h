@@ -537,13 +537,16 @@ thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, s
  * asks for each pass as one persistent launch whose blocks pull level
  * chunks from a device queue; that mode is switched off in this build (its
  * launches did not complete on the MI355X) and is refused with logic_error;
- * categorical_leaves: categorical partition, marginal and branch-length
- * calls whose rake levels run on the generic kernels (K other than 8, 16,
- * 32 and 64, no fused forward program) form the leaf vectors of the leaf
- * rakes from the observations, emissions and root potential inside the
- * rake kernels and skip those leaves in the node initialisation (1,
- * default), or write every node's factor first (0); both give the same
- * values bit for bit, and a change drops a held forward pass.
+ * input_leaves: the rakes of original leaves form their leaf vectors from
+ * the call's inputs inside the rake kernels, and the node initialisation
+ * skips those leaves (1, default), or every node's state is written first
+ * (0). It covers categorical partition, marginal and branch-length calls
+ * whose rake levels run on the generic kernels (K other than 8, 16, 32 and
+ * 64, no fused forward program: the observations, emissions and root
+ * potential), and dense partition and marginal calls on linear potentials
+ * with the fused forward program (K = 8, 16, 32, 64; 16-byte aligned
+ * buffers: round 0's rakes read the caller's potentials). Both settings
+ * give the same values bit for bit; a change drops a held forward pass.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused
  * forward program, and a new compress_depth also drops a held
@@ -554,7 +557,7 @@ typedef struct thmm_tuning {
   uint32_t compress_depth;
   uint32_t rake_log_edges;
   uint32_t persistent;
-  uint32_t categorical_leaves;
+  uint32_t input_leaves;
   double ms;
 } thmm_tuning;
 

@@ -125,7 +125,7 @@ class Tuning(ctypes.Structure):
         ("compress_depth", c_u32),
         ("rake_log_edges", c_u32),
         ("persistent", c_u32),
-        ("categorical_leaves", c_u32),
+        ("input_leaves", c_u32),
         ("ms", ctypes.c_double),
     ]
 

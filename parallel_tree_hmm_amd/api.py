@@ -548,7 +548,7 @@ class Workspace:
 
     # -- launch tuning -----------------------------------------------------
     _TUNING_FIELDS = ("compress_items_forward", "compress_items_reverse", "rake_block", "gather_chunk",
-                      "compress_depth", "rake_log_edges", "persistent", "categorical_leaves")
+                      "compress_depth", "rake_log_edges", "persistent", "input_leaves")
 
     def tuning(self) -> dict:
         t = _abi.Tuning()

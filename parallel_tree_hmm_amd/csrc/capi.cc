@@ -72,7 +72,7 @@ thmm::Workspace &W(thmm_workspace *workspace) {
 
 thmm_tuning ToC(const thmm::Tuning &t, double ms) {
   return {t.compress_items_forward, t.compress_items_reverse, t.rake_block, t.gather_chunk, t.compress_depth,
-          t.rake_log_edges, t.persistent, t.categorical_leaves, ms};
+          t.rake_log_edges, t.persistent, t.input_leaves, ms};
 }
 
 }  // namespace
@@ -326,7 +326,7 @@ thmm_status thmm_set_tuning(thmm_workspace *workspace, const thmm_tuning *tuning
     t.compress_depth = tuning->compress_depth;
     t.rake_log_edges = tuning->rake_log_edges;
     t.persistent = tuning->persistent;
-    t.categorical_leaves = tuning->categorical_leaves;
+    t.input_leaves = tuning->input_leaves;
     W(workspace).SetTuning(t);
   });
 }

@@ -25,7 +25,7 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
 // max-shifted exponentials, the latter three writing FP64 scales); returns
 // false when (K, types, shape) are left to the generic kernels.
 bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
-                         std::uint32_t X, std::uint32_t Y, double *scale);
+                         std::uint32_t X, std::uint32_t Y, double *scale, const std::uint8_t *skip = nullptr);
 
 namespace dev {
 

@@ -121,6 +121,11 @@ static_assert(Team<8>::kSteps <= Team<8>::kLanesPerRow && Team<16>::kSteps <= Te
                   Team<32>::kSteps <= Team<32>::kLanesPerRow && Team<64>::kSteps <= Team<64>::kLanesPerRow,
               "row ownership");
 
+// log v for v > 0, else -inf (NaN included), in the element type: the
+// node initialisation's log of a linear potential.
+template <typename T> __device__ inline T LogOf(T v) { return v > T(0) ? log(v) : NegInf<T>(); }
+template <> __device__ inline float LogOf(float v) { return v > 0.0f ? logf(v) : NegInf<float>(); }
+
 // exp of the team's quad of a K-vector: the lanes sharing a quad (same
 // l % kLpr) each exponentiate one element and exchange the results, so every
 // element is exponentiated once per team.
@@ -130,6 +135,34 @@ __device__ inline void TeamExp(const Quad<T> &v, int l, T (&e)[4]) {
   const T mine = Exp(v.v[(l / kLpr) % 4]);
 #pragma unroll
   for (int i = 0; i < 4; ++i) e[i] = __shfl(mine, l % kLpr + kLpr * i, kT);
+}
+
+// A rake's leaf vector: the node state's max-shifted logs, or (kRaw) the
+// caller's linear potentials [B][N][K] (buf.raw_leaves).
+template <typename T, int K, bool kRaw>
+__device__ inline const T *LeafVector(const LinearBuffers &buf, const Dims &d, std::uint32_t leaf, std::uint32_t b) {
+  if constexpr (kRaw)
+    return static_cast<const T *>(buf.raw_leaves) + (static_cast<std::uint64_t>(b) * d.N + leaf) * K;
+  return static_cast<const T *>(buf.nodes) + (static_cast<std::uint64_t>(leaf) * d.B + b) * K;
+}
+
+// TeamExp of a leaf vector given as linear potentials (x: the lane's quad),
+// formed as the node initialisation (TransposeTeam, kMode 1, same element
+// type) forms the vector: w = log x in the element type, top = the vector's
+// maximum, v = w - top (-inf without support); returns the FP64 scale (top,
+// or -inf). Each lane logs the one element of its quad that it
+// exponentiates, and the team's maximum covers the whole vector.
+template <typename T, int kT, int kLpr>
+__device__ inline double RawLeafExp(const Quad<T> &x, int l, T (&e)[4]) {
+  const T w = LogOf(x.v[(l / kLpr) % 4]);
+  T top = w;
+#pragma unroll
+  for (int o = kT / 2; o > 0; o >>= 1) top = Max(top, __shfl_xor(top, o, kT));
+  const bool live = top > NegInf<T>();
+  const T mine = Exp(live ? static_cast<T>(w - top) : NegInf<T>());
+#pragma unroll
+  for (int i = 0; i < 4; ++i) e[i] = __shfl(mine, l % kLpr + kLpr * i, kT);
+  return live ? static_cast<double>(top) : kNegInf;
 }
 
 // log of a positive value: FP32 mantissa logarithm plus the exact exponent
@@ -190,7 +223,8 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
 // adj_v[c] = sum_p adj_P[p][c]. kLogEdges: buf.log_edges may be set (the
 // variant without it holds fewer registers).
 // `tid` is the lane's index in the launch (block * block size + thread).
-template <typename T, int K, bool kLogEdges>
+// kRaw: the leaves are original leaves read from buf.raw_leaves.
+template <typename T, int K, bool kLogEdges, bool kRaw = false>
 __device__ __forceinline__ void RakeReverseTeamBody(const DeviceOps &ops, const Dims &d, std::uint32_t offset,
                                                     std::uint32_t count, const LinearBuffers &buf,
                                                     std::uint32_t tid) {
@@ -203,9 +237,12 @@ __device__ __forceinline__ void RakeReverseTeamBody(const DeviceOps &ops, const 
   const DeviceRake op = ops.rakes[offset + item / d.B];
   const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
   const int c0 = (l % kLpr) * 4;
-  const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
+  const Quad<T> vq = LoadQuad(LeafVector<T, K, kRaw>(buf, d, op.leaf, b) + c0);
   T e[4];
-  TeamExp<T, kT, kLpr>(vq, l, e);
+  if constexpr (kRaw)
+    RawLeafExp<T, kT, kLpr>(vq, l, e);
+  else
+    TeamExp<T, kT, kLpr>(vq, l, e);
   // Log edges (see RakeGatherTeam): exp(x - shift) with the row shifts the
   // forward pass stored, the values the forward pass used.
   const bool log_edge = kLogEdges && buf.log_edges != nullptr && op.slot < d.E;
@@ -273,10 +310,10 @@ __device__ __forceinline__ void RakeReverseTeamBody(const DeviceOps &ops, const 
   }
 }
 
-template <typename T, int K, bool kLogEdges>
+template <typename T, int K, bool kLogEdges, bool kRaw = false>
 __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
                                                        std::uint32_t count, LinearBuffers buf) {
-  RakeReverseTeamBody<T, K, kLogEdges>(ops, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
+  RakeReverseTeamBody<T, K, kLogEdges, kRaw>(ops, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,8 +344,9 @@ __device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int
 
 // kLogEdges: buf.log_edges may be set (the variant without it holds fewer
 // registers). FP32 edge slots of an FP64 pass (buf.edges_f32) are widened
-// as they load, which is exact.
-template <typename T, int K, bool kLogEdges>
+// as they load, which is exact. kRaw: the leaves are original leaves read
+// from buf.raw_leaves.
+template <typename T, int K, bool kLogEdges, bool kRaw = false>
 __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dims &d, std::uint32_t offset,
                                                    std::uint32_t count, const LinearBuffers &buf, std::uint32_t tid) {
   using TM = Team<K>;
@@ -330,7 +368,7 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
     DeviceRake next = op;
     if (i + 1 < chunk.count) next = g.rakes[chunk.first + i + 1];
     const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
-    const Quad<T> vq = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K + c0);
+    const Quad<T> vq = LoadQuad(LeafVector<T, K, kRaw>(buf, d, op.leaf, b) + c0);
     const bool log_edge = kLogEdges && buf.log_edges != nullptr && op.slot < d.E;  // the same for the whole team
     const T *P = log_edge ? static_cast<const T *>(buf.log_edges) + static_cast<std::uint64_t>(op.slot) * K * K
                           : Slot<T>(buf, d, op.slot, b);
@@ -349,7 +387,8 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
 #pragma unroll
       for (int j = 0; j < kJ; ++j) pq[j] = LoadQuadStream(P + static_cast<std::uint32_t>(j * kT + l) * 4);
     }
-    const double leaf_scale = buf.node_scale[leaf_at];
+    double leaf_scale = 0.0;
+    if constexpr (!kRaw) leaf_scale = buf.node_scale[leaf_at];
     double row_scale;
     if (log_edge) {
       row_scale = ExpLogRows<T, kLpr, kJ>(pq, owned);
@@ -359,7 +398,10 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
       row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
     }
     T e[4];
-    TeamExp<T, kT, kLpr>(vq, l, e);
+    if constexpr (kRaw)
+      leaf_scale = RawLeafExp<T, kT, kLpr>(vq, l, e);
+    else
+      TeamExp<T, kT, kLpr>(vq, l, e);
     T mine = T(0);
 #pragma unroll
     for (int j = 0; j < kJ; ++j) {
@@ -377,10 +419,10 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
     g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + row] = acc;
 }
 
-template <typename T, int K, bool kLogEdges>
+template <typename T, int K, bool kLogEdges, bool kRaw = false>
 __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::uint32_t offset, std::uint32_t count,
                                                       LinearBuffers buf) {
-  RakeGatherTeamBody<T, K, kLogEdges>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
+  RakeGatherTeamBody<T, K, kLogEdges, kRaw>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Sums a run of partials (K lanes, lane = row) into a partial of the next
@@ -1462,19 +1504,17 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 // kMode 2 log potentials, both then shifted by the vector's maximum, which
 // goes to `scale` in FP64 (-inf for a vector without support); kMode 3 turns
 // log potentials into exp(v - max) with the same FP64 scale (edge rows).
-template <typename T> __device__ inline T LogOf(T v) { return v > T(0) ? log(v) : NegInf<T>(); }
-template <> __device__ inline float LogOf(float v) { return v > 0.0f ? logf(v) : NegInf<float>(); }
-
 template <typename In, typename Out, int kMode, int K>
 __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
-                                                     double *scale) {
+                                                     double *scale, const std::uint8_t *skip) {
   constexpr int kLanes = K / 4;
   // Grid-stride: the stride is a whole number of teams, so a team's lanes
-  // stay together.
+  // stay together (and skip together: skip is indexed by y).
   for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(X) * Y * kLanes; tid += GridStride()) {
     const std::uint64_t v = tid / kLanes;
     const int l = static_cast<int>(tid % kLanes);
     const std::uint64_t x = v / Y, y = v % Y;
+    if (skip != nullptr && skip[y] != 0) continue;
     const std::uint64_t to = y * X + x;
     const Quad<In> q = LoadQuad(in + v * K + l * 4);
     // Logs are taken in the wider of the two element types.
@@ -1518,7 +1558,9 @@ template <typename T, int K> void LaunchRakeTeam(const Launch &l, bool reverse, 
                                                  const LinearBuffers &buf) {
   const unsigned block = l.tuning.rake_block;
   const unsigned grid = BlocksOf(static_cast<std::uint64_t>(count) * l.dims.B * Team<K>::kLanes, block);
-  if (reverse && buf.log_edges != nullptr)
+  if (reverse && buf.raw_leaves != nullptr)
+    RakeReverseTeam<T, K, false, true><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+  else if (reverse && buf.log_edges != nullptr)
     RakeReverseTeam<T, K, true><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
   else if (reverse)
     RakeReverseTeam<T, K, false><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
@@ -1660,7 +1702,9 @@ void LaunchGatherChunks(const Launch &l, const GatherOps &g, std::uint32_t offse
   const hipStream_t s = StreamOf(l);
 #define THMM_GATHER(T, KK)                                                                  \
   do {                                                                                      \
-    if (buf.log_edges != nullptr)                                                           \
+    if (buf.raw_leaves != nullptr)                                                          \
+      RakeGatherTeam<T, KK, false, true><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf); \
+    else if (buf.log_edges != nullptr)                                                      \
       RakeGatherTeam<T, KK, true><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf);    \
     else                                                                                    \
       RakeGatherTeam<T, KK, false><<<grid, block, 0, s>>>(g, l.dims, offset, count, buf);   \
@@ -1823,17 +1867,18 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
 namespace {
 
 template <typename In, typename Out, int kMode>
-bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
+bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale,
+                      const std::uint8_t *skip = nullptr) {
   const std::uint64_t threads = static_cast<std::uint64_t>(X) * Y * (l.dims.K / 4);
   const unsigned grid = GridStrideBlocks(threads, kBlock);
   const auto *i = static_cast<const In *>(in);
   auto *o = static_cast<Out *>(out);
   switch (l.dims.K) {
-    case 4: TransposeTeam<In, Out, kMode, 4><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
-    case 8: TransposeTeam<In, Out, kMode, 8><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
-    case 16: TransposeTeam<In, Out, kMode, 16><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
-    case 32: TransposeTeam<In, Out, kMode, 32><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
-    case 64: TransposeTeam<In, Out, kMode, 64><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale); break;
+    case 4: TransposeTeam<In, Out, kMode, 4><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
+    case 8: TransposeTeam<In, Out, kMode, 8><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
+    case 16: TransposeTeam<In, Out, kMode, 16><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
+    case 32: TransposeTeam<In, Out, kMode, 32><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
+    case 64: TransposeTeam<In, Out, kMode, 64><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
     default: return false;
   }
   RecordLaunchError(hipGetLastError(), "vector transpose launch");
@@ -1905,22 +1950,23 @@ bool FastTransposeFits(std::uint32_t K, std::uint32_t X, std::uint32_t Y) {
 }
 
 bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
-                         std::uint32_t X, std::uint32_t Y, double *scale) {
+                         std::uint32_t X, std::uint32_t Y, double *scale, const std::uint8_t *skip) {
   const std::uint32_t K = l.dims.K;
   if (!FastTransposeFits(K, X, Y)) return false;
   if (!Aligned16(in) || !Aligned16(out)) return false;
   if (static_cast<std::uint64_t>(X) * Y == 0) return true;
   const bool f_in = in_real == Real::kF32, f_out = out_real == Real::kF32;
+  if (skip != nullptr && mode != 1) return false;
   switch (mode) {
     case 0:
       if (f_in != f_out) return false;
       return f_in ? TransposeTeamFor<float, float, 0>(l, in, out, X, Y, scale)
                   : TransposeTeamFor<double, double, 0>(l, in, out, X, Y, scale);
     case 1:
-      if (f_in && f_out) return TransposeTeamFor<float, float, 1>(l, in, out, X, Y, scale);
-      if (f_in) return TransposeTeamFor<float, double, 1>(l, in, out, X, Y, scale);
-      if (f_out) return TransposeTeamFor<double, float, 1>(l, in, out, X, Y, scale);
-      return TransposeTeamFor<double, double, 1>(l, in, out, X, Y, scale);
+      if (f_in && f_out) return TransposeTeamFor<float, float, 1>(l, in, out, X, Y, scale, skip);
+      if (f_in) return TransposeTeamFor<float, double, 1>(l, in, out, X, Y, scale, skip);
+      if (f_out) return TransposeTeamFor<double, float, 1>(l, in, out, X, Y, scale, skip);
+      return TransposeTeamFor<double, double, 1>(l, in, out, X, Y, scale, skip);
     case 2:
       if (f_in != f_out) return false;
       return f_in ? TransposeTeamFor<float, float, 2>(l, in, out, X, Y, scale)

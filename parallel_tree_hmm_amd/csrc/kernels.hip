@@ -1457,10 +1457,15 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
   }
 
 template <typename In, typename Out, int kMode>
-void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale) {
+void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale,
+                     const std::uint8_t *skip = nullptr) {
   constexpr Real in_real = sizeof(In) == 4 ? Real::kF32 : Real::kF64;
   constexpr Real out_real = sizeof(Out) == 4 ? Real::kF32 : Real::kF64;
-  if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale)) return;
+  if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale, skip)) return;
+  if (skip != nullptr) {
+    RecordLaunchError(hipErrorInvalidValue, "skipped nodes need the team transpose");
+    return;
+  }
   const std::uint32_t ty = 64 / l.dims.K;
   const std::uint32_t x_tiles = (X + 63) / 64;
   const std::uint64_t tiles = static_cast<std::uint64_t>(x_tiles) * ((Y + ty - 1) / ty);
@@ -1485,18 +1490,19 @@ const char *LastLaunchError() {
   return reported.c_str();
 }
 
-void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes, double *node_scale) {
+void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes, double *node_scale,
+                     const std::uint8_t *skip) {
   const Dims &d = l.dims;
   if (l.real == Real::kF32) {
     if (in_real == Real::kF32)
-      LaunchTranspose<float, float, 1>(l, input, nodes, d.B, d.N, node_scale);
+      LaunchTranspose<float, float, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
     else
-      LaunchTranspose<double, float, 1>(l, input, nodes, d.B, d.N, node_scale);
+      LaunchTranspose<double, float, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
   } else {
     if (in_real == Real::kF32)
-      LaunchTranspose<float, double, 1>(l, input, nodes, d.B, d.N, node_scale);
+      LaunchTranspose<float, double, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
     else
-      LaunchTranspose<double, double, 1>(l, input, nodes, d.B, d.N, node_scale);
+      LaunchTranspose<double, double, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
   }
   Check(hipGetLastError(), "node initialisation launch");
 }
@@ -1659,6 +1665,10 @@ void LaunchReverseLevel(const Launch &l, std::uint32_t primitive, std::uint32_t 
   InOpChunks(offset, count, l.dims.B, [&](std::uint32_t first, std::uint32_t ops) {
     const bool categorical = primitive == 0 && first < buf.cat.rakes;
     if (!categorical && LaunchFastLevel(l, true, primitive, first, ops, buf)) return;
+    if (buf.raw_leaves != nullptr) {
+      RecordLaunchError(hipErrorInvalidValue, "raw leaf vectors need the team rake kernels");
+      return;
+    }
     // Compressions at K = 4, 8 and 16 (those the MFMA kernels left): the
     // team kernel.
     if (primitive == 3 && !buf.compress_log_edges && TeamCompressionFits(l, ops, buf, 16)) {

@@ -140,6 +140,12 @@ struct LinearBuffers {
   // Leaf rakes of a categorical call (rake levels with cat.rakes != 0 run
   // on the generic rake kernels).
   CategoricalLeaves cat;
+  // The caller's linear node potentials [B][N][K] (element type): set on the
+  // launches of round 0's rakes (every leaf an original leaf) when the node
+  // initialisation skipped those leaves; the team rake kernels then form the
+  // leaf vectors as the initialisation would (logs in the element type,
+  // shifted by the vector's maximum, FP64 scale).
+  const void *raw_leaves;
   // Compressions read their edge slots from log_edges too (the K = 32 and 64
   // FP32 MFMA kernels; no other compression kernel may run with it set).
   // The forward one writes each row's shift (its maximum, -inf for a row of
@@ -178,7 +184,7 @@ struct Tuning {
   std::uint32_t compress_depth = 2;          // compression-tree levels per grouped compression: 1 (none) .. 3
   std::uint32_t rake_log_edges = 1;          // log-potential calls: rakes read log edges in place (1) or staged (0)
   std::uint32_t persistent = 0;              // K = 16 FP32 forests: each pass as one persistent launch (1)
-  std::uint32_t categorical_leaves = 1;      // categorical calls: leaf rakes form their leaves from observations (1)
+  std::uint32_t input_leaves = 1;           // leaf rakes form their leaf vectors from the call's inputs (1)
 };
 
 struct Launch {
@@ -193,8 +199,10 @@ struct Launch {
 // Linear potentials, public [B][N][K] -> internal [N][B][K] max-shifted
 // logs + FP64 scales. `in_real` is the element type of `input`; `nodes`
 // has element type launch.real.
+// Nodes with skip[n] set (null: none) are left unwritten; skipping needs the
+// team transpose (FastTransposeFits and 16-byte aligned buffers).
 void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes,
-                     double *node_scale);
+                     double *node_scale, const std::uint8_t *skip = nullptr);
 // Log potentials [B][N][K] -> max-shifted logs + FP64 scales.
 void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes,
                              double *node_scale);

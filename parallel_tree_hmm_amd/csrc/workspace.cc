@@ -137,6 +137,12 @@ struct Workspace::Impl {
   // The same program with every compression level plain (kind 3): the FP64
   // sampling forward, which has no grouped compressions.
   std::vector<FusedStep> fused_plain;
+  // Raw leaves: round 0's rakes (the first round0_rakes of the plan, every
+  // leaf an original leaf) read the caller's linear potentials and the node
+  // initialisation skips their leaves (leaf_skip [N]); 0: not available
+  // (no fused forward program, or round 0 does not start with rakes).
+  std::uint32_t round0_rakes = 0;
+  DeviceBuffer leaf_skip;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
   // Grouped compressions (K = 16 FP32 with the fused forward): the level of
   // each plan batch that holds compressions (count 0: none rooted there),
@@ -262,7 +268,8 @@ struct Workspace::Impl {
     persistent_failure.Release();
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
-                            &cat_skip, &site_sums, &input_nodes, &uniforms, &observations, &observations_t,
+                            &cat_skip, &site_sums, &leaf_skip, &input_nodes, &uniforms, &observations,
+                            &observations_t,
                             &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
                             &log_z, &node_adj, &branch_adj, &path_adj, &node_out, &edge_out, &s_edges,
@@ -290,6 +297,7 @@ struct Workspace::Impl {
     device = 0;
     categorical = false;
     cat_rakes = 0;
+    round0_rakes = 0;
     observation_nodes.clear();
     resident_observation_batch = 0;
     resident_factors = false;
@@ -839,6 +847,23 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   }
   if (w.rake_adjoint_from_parent && FusedForwardSupported(static_cast<std::uint32_t>(states)))
     BuildPrograms(w, plan, rakes, compresses, batch);
+  // Raw leaves where round 0's rakes run as the first gather step and as the
+  // first level of the reverse program (checked again per call).
+  if (!w.fused.empty() && w.fused.front().kind == 0 && !plan.batches.empty() &&
+      plan.batches.front().primitive == THMM_RAKE && plan.batches.front().offset == 0) {
+    const thmm_primitive_batch &b0 = plan.batches.front();
+    std::vector<std::uint8_t> has_child(plan.num_nodes, 0), skip(plan.num_nodes, 0);
+    for (Index parent : plan.edge_parents) has_child[parent] = 1;
+    bool leaves = w.fused.front().work == b0.count;
+    for (std::uint32_t r = 0; leaves && r < b0.count; ++r) {
+      leaves = !has_child[plan.rakes[r].leaf];
+      skip[plan.rakes[r].leaf] = 1;
+    }
+    if (leaves) {
+      Upload(w.leaf_skip, skip, w.stream, "topology upload");
+      w.round0_rakes = b0.count;
+    }
+  }
 
   const std::size_t K = states, KK = K * K, N = plan.num_nodes, R = plan.num_branches, C = plan.num_compressions();
   const std::size_t S = RealSize(real);
@@ -1147,9 +1172,13 @@ bool UsesFusedForward(const Impl &w, const LinearBuffers &buf) {
 
 // The sum-product forward pass: the fused program when the workspace has
 // one, else the plan's levels.
+// `raw_leaves`: the caller's linear node potentials for round 0's gather
+// (the first step), whose leaves the node initialisation skipped.
 void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, const Launch &l,
-                   const LinearBuffers &buf, Profiler *profiler = nullptr) {
+                   const LinearBuffers &buf, Profiler *profiler = nullptr, const void *raw_leaves = nullptr) {
   const GatherOps g = w.Gather();
+  LinearBuffers raw = buf;
+  raw.raw_leaves = raw_leaves;
   for (const Impl::FusedStep &step : steps) {
     // Profiled as primitive 4 (fused rake chunks, count = rakes), 5 (partial
     // absorptions, count = partials), a compression level, or a grouped
@@ -1158,7 +1187,7 @@ void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, con
       profiler->Begin(1, step.kind == 3 || step.kind == kChainPrimitive ? step.kind : 4 + step.kind, step.work,
                       step.count);
     if (step.kind == 0)
-      LaunchGatherChunks(l, g, step.offset, step.count, buf);
+      LaunchGatherChunks(l, g, step.offset, step.count, &step == &steps.front() ? raw : buf);
     else if (step.kind == 1)
       LaunchGatherSegments(l, g, step.offset, step.count, buf);
     else
@@ -1185,13 +1214,15 @@ PersistentQueue QueueOf(const Impl &w, const Impl::PersistentProgram &p) {
           w.persistent_failure.as<std::uint32_t>(), p.grid, p.wait_ticks};
 }
 
-void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
+// `raw_leaves`: see RawLeaves (only with the fused program).
+void SumProductForward(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr,
+                       const void *raw_leaves = nullptr) {
   if (PersistentReady(w, l, buf)) {
     if (profiler) profiler->Begin(1, kPersistentPrimitive, w.persistent_forward.num_tasks);
     LaunchPersistentPass(l, false, w.Gather(), QueueOf(w, w.persistent_forward), buf);
     if (profiler) profiler->End();
   } else if (UsesFusedForward(w, buf)) {
-    RunFusedSteps(w, w.fused, l, buf, profiler);
+    RunFusedSteps(w, w.fused, l, buf, profiler, raw_leaves);
   } else {
     ForwardLevels(l, w.levels, buf, profiler);
   }
@@ -1205,8 +1236,11 @@ const std::vector<Level> &ReverseProgram(const Impl &w, const LinearBuffers &buf
 
 // Without branch adjoint storage (buf.branch_adj null) rakes take their
 // adjoint from the parent node and the pure-copy levels are skipped.
+// `raw_leaves` applies to the first level (round 0's rakes).
 void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const LinearBuffers &buf,
-                   Profiler *profiler = nullptr) {
+                   Profiler *profiler = nullptr, const void *raw_leaves = nullptr) {
+  LinearBuffers raw = buf;
+  raw.raw_leaves = raw_leaves;
   for (std::size_t i = levels.size(); i-- > 0;) {
     if (buf.branch_adj == nullptr &&
         (levels[i].primitive == THMM_BRANCH_COMBINATION || levels[i].primitive == THMM_BRANCH_ABSORPTION))
@@ -1215,21 +1249,43 @@ void ReverseLevels(const Launch &l, const std::vector<Level> &levels, const Line
                          levels[i].primitive == kPairChainPrimitive;
     if (profiler)
       profiler->Begin(3, levels[i].primitive, grouped ? levels[i].reverse_work : levels[i].count, levels[i].count);
-    LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, buf);
+    LaunchReverseLevel(l, levels[i].primitive, levels[i].offset, levels[i].count, i == 0 ? raw : buf);
     if (profiler) profiler->End();
   }
 }
 
 // The reverse pass matching SumProductForward's forward pass: one persistent
 // launch, or the reverse program's levels.
-void RunReverse(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr) {
+void RunReverse(const Impl &w, const Launch &l, const LinearBuffers &buf, Profiler *profiler = nullptr,
+                const void *raw_leaves = nullptr) {
   if (PersistentReady(w, l, buf)) {
     if (profiler) profiler->Begin(3, kPersistentPrimitive, w.persistent_reverse.num_tasks);
     LaunchPersistentPass(l, true, w.Gather(), QueueOf(w, w.persistent_reverse), buf);
     if (profiler) profiler->End();
     return;
   }
-  ReverseLevels(l, ReverseProgram(w, buf), buf, profiler);
+  ReverseLevels(l, ReverseProgram(w, buf), buf, profiler, raw_leaves);
+}
+
+// Whether round 0's rakes of a dense call can read the caller's linear node
+// potentials `nodes` ([B][N][K], workspace element type) while the node
+// initialisation skips their leaves: the workspace has the raw-leaf layout,
+// the pass runs the fused program (its first gather step holds round 0's
+// rakes) and, for a reverse pass too, the reverse program's first level is
+// round 0's rakes on the team kernels (16-byte aligned buffers); the
+// initialisation must run on the team transpose.
+bool RawLeaves(const Impl &w, const Launch &l, const void *nodes, const LinearBuffers &buf, bool reverse) {
+  const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
+  if (w.tuning.input_leaves == 0 || w.round0_rakes == 0 || !UsesFusedForward(w, buf) || PersistentReady(w, l, buf) ||
+      buf.log_edges != nullptr || buf.edges_f32 != 0)
+    return false;
+  if (nodes == nullptr || !aligned(nodes) || !aligned(w.nodes.ptr) || !FastTransposeFits(l.dims.K, l.dims.B, l.dims.N))
+    return false;
+  if (!reverse) return true;
+  const std::vector<Level> &program = ReverseProgram(w, buf);
+  return !program.empty() && program.front().primitive == THMM_RAKE && program.front().offset == 0 &&
+         program.front().count == w.round0_rakes && aligned(buf.edges) && aligned(buf.edge_out) &&
+         aligned(buf.node_adj);
 }
 
 // Categorical staging with residency policies (src/gpu_backend_impl.inc:2841-2947).
@@ -1314,15 +1370,15 @@ bool ElementLogs(const Impl &w, const Launch &l) {
 // `fused_leaves` a categorical call leaves the leaves of the workspace's
 // categorical rakes unwritten (their rake kernels form them).
 void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale,
-               const void *dense_nodes = nullptr, bool fused_leaves = false) {
+               const void *dense_nodes = nullptr, bool fused_leaves = false, const std::uint8_t *dense_skip = nullptr) {
   if (src.categorical == nullptr) {
-    LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale);
+    LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale, dense_skip);
   } else {
     LaunchInitCategorical(l, w.real, false, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
                           static_cast<std::uint32_t>(src.categorical->categories), w.is_root.as<std::uint8_t>(),
                           w.root_potential.ptr, w.emissions.ptr, nodes, scale,
-                          fused_leaves && w.cat_rakes != 0 && w.tuning.categorical_leaves != 0
+                          fused_leaves && w.cat_rakes != 0 && w.tuning.input_leaves != 0
                               ? w.cat_skip.as<std::uint8_t>()
                               : nullptr,
                           ElementLogs(w, l));
@@ -1334,7 +1390,7 @@ void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double 
 // observations, emissions and root potential.
 LinearBuffers CallLinear(const Impl &w, const Launch &l, bool categorical) {
   LinearBuffers buf = w.Linear(w.edges.ptr, nullptr);
-  if (categorical && w.cat_rakes != 0 && w.tuning.categorical_leaves != 0)
+  if (categorical && w.cat_rakes != 0 && w.tuning.input_leaves != 0)
     buf.cat = {w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                w.is_root.as<std::uint8_t>(), w.root_potential.ptr, w.emissions.ptr,
                static_cast<std::uint32_t>(w.categories), w.cat_rakes, ElementLogs(w, l) ? 1u : 0u};
@@ -1376,24 +1432,35 @@ void AdjointsInto(LinearBuffers &buf, std::size_t batch, void *node_out) {
 }
 
 // Scaled forward + log Z for a staged call.
+// Round 0's rakes of a dense staged call read the staged linear potentials
+// when RawLeaves allows it (the initialisation then skips their leaves).
+const void *StagedRawLeaves(const Impl &w, const Launch &l, const Source &src, const LinearBuffers &buf,
+                            bool reverse) {
+  return src.categorical == nullptr && RawLeaves(w, l, w.input_nodes.ptr, buf, reverse) ? w.input_nodes.ptr : nullptr;
+}
+
 void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true);
   const LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
-  SumProductForward(w, l, buf);
+  const void *raw = StagedRawLeaves(w, l, src, buf, false);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true,
+            raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+  SumProductForward(w, l, buf, nullptr, raw);
   LaunchFinish(l, buf, w.log_z.as<double>());
   CheckLaunches();
 }
 
 void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
-  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true);
   LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
   AdjointsInto(buf, batch, w.node_out.ptr);
-  SumProductForward(w, l, buf);
+  const void *raw = StagedRawLeaves(w, l, src, buf, true);
+  InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true,
+            raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+  SumProductForward(w, l, buf, nullptr, raw);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
-  RunReverse(w, l, buf);
+  RunReverse(w, l, buf, nullptr, raw);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, w.node_out.ptr);
   CheckLaunches();
 }
@@ -2068,18 +2135,20 @@ void RunOrReplay(Impl &w, int kind, const Impl::GraphKey &key, hipStream_t s, co
 void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *edges, std::size_t batch,
                       double *log_z, void *node_out, void *edge_out, Profiler *p) {
   const Launch l = w.MakeLaunch(w.real, batch, on);
-  if (p) p->Begin(0, 0, 0);
-  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>());
-  if (p) p->End();
   LinearBuffers buf = w.Linear(edges, nullptr);
   buf.edge_out = edge_out;
   AdjointsInto(buf, batch, node_out);
-  SumProductForward(w, l, buf, p);
+  const void *raw = RawLeaves(w, l, nodes, buf, true) ? nodes : nullptr;
+  if (p) p->Begin(0, 0, 0);
+  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>(),
+                  raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+  if (p) p->End();
+  SumProductForward(w, l, buf, p, raw);
   if (p) p->Begin(2, 0, 0);
   LaunchFinish(l, buf, log_z);
   LaunchSeedAdjoint(l, buf, nullptr);
   if (p) p->End();
-  RunReverse(w, l, buf, p);
+  RunReverse(w, l, buf, p, raw);
   if (p) p->Begin(4, 0, 0);
   if (batch != 1) LaunchNodeAdjointOut(l, w.node_adj.ptr, node_out);
   if (p) p->End();
@@ -2133,7 +2202,7 @@ void ValidateTuning(const Tuning &t) {
   if (t.compress_depth < 1 || t.compress_depth > 3) throw std::invalid_argument("compression depth must be 1, 2 or 3");
   if (t.rake_log_edges > 1) throw std::invalid_argument("rake_log_edges must be 0 or 1");
   if (t.persistent > 1) throw std::invalid_argument("persistent must be 0 or 1");
-  if (t.categorical_leaves > 1) throw std::invalid_argument("categorical_leaves must be 0 or 1");
+  if (t.input_leaves > 1) throw std::invalid_argument("input_leaves must be 0 or 1");
   if (t.persistent != 0 && !kPersistentPassesEnabled)
     throw std::logic_error(
         "persistent passes are switched off: on the MI355X their launches did not complete (per-task device-wide "
@@ -2149,7 +2218,7 @@ void ApplyTuning(Impl &w, const Tuning &t) {
   for (Impl::TreeGroup &g : w.groups) ApplyTuning(*g.w, t);
   const bool rebuild =
       !w.fused.empty() && (t.gather_chunk != w.tuning.gather_chunk || t.compress_depth != w.tuning.compress_depth);
-  if (t.compress_depth != w.tuning.compress_depth || t.categorical_leaves != w.tuning.categorical_leaves)
+  if (t.compress_depth != w.tuning.compress_depth || t.input_leaves != w.tuning.input_leaves)
     w.forward = Impl::Forward::kNone;
   w.tuning = t;
   for (Impl::GraphCache &g : w.graphs) g.Release();

@@ -1,4 +1,4 @@
-"""Categorical leaf fusion (thmm_tuning.categorical_leaves) and the
+"""Categorical leaf fusion (thmm_tuning.input_leaves) and the
 site-summed log likelihood (thmm_categorical_log_likelihood).
 
 With leaf fusion on (the default) the rake kernels of a categorical
@@ -6,7 +6,7 @@ partition, marginal or branch-length call form the leaf vectors of the leaf
 rakes from the observations, emissions and root potential, and the node
 initialisation skips those leaves (the reference writes every node's factor
 first: src/gpu_backend_impl.inc:2917-2941, :326-382). Every case compares
-the fused form with the unfused one (categorical_leaves = 0) on the same
+the fused form with the unfused one (input_leaves = 0) on the same
 workspace and inputs, bit for bit:
 
 * log partitions (Scalar and FP64), node and edge marginals;
@@ -125,9 +125,9 @@ def test_fused_leaves_equal_the_unfused_form(require_gpu, k, precision, batch):
     ws.reserve_categorical_marginals(plan, k, batch, c["categories"], c["observed"], precision=precision)
     model = model_of(plan, k, c, dtype, batch)
     seeds = np.random.default_rng(k).normal(size=batch * plan.roots.size)
-    assert ws.tuning()["categorical_leaves"] == 1
+    assert ws.tuning()["input_leaves"] == 1
     fused = run_all(ws, model, c["q"], seeds)
-    ws.set_tuning(categorical_leaves=0)
+    ws.set_tuning(input_leaves=0)
     plain = run_all(ws, model, c["q"], seeds)
     for name in fused:
         assert same(fused[name], plain[name]), f"K={k} {precision}: {name} differs"
@@ -168,7 +168,7 @@ def test_zero_factor_leaves(require_gpu, precision):
     ws.reserve_categorical(plan, k, batch, c["categories"], c["observed"], precision=precision)
     model = model_of(plan, k, c, dtype, batch, obs=obs)
     fused = thmm.log_partition_function_prepared(model, ws).values.copy()
-    ws.set_tuning(categorical_leaves=0)
+    ws.set_tuning(input_leaves=0)
     plain = thmm.log_partition_function_prepared(model, ws).values.copy()
     assert same(fused, plain)
     per_root = fused.reshape(batch, plan.roots.size)
@@ -193,7 +193,7 @@ def test_log_likelihood_errors(require_gpu):
     with pytest.raises(thmm.LogicError):
         ws.branch_length_backward(c["q"], None)  # the failed forward holds no state
     ws.categorical_log_likelihood(model)
-    ws.set_tuning(categorical_leaves=0)  # drops the held forward pass
+    ws.set_tuning(input_leaves=0)  # drops the held forward pass
     with pytest.raises(thmm.LogicError):
         ws.branch_length_backward(c["q"], None)
     ws.close()
@@ -221,7 +221,7 @@ def test_full_categorical_workload_fused_equals_unfused(require_gpu):
 
     results, times = {}, {}
     for leaves_flag in (1, 0):
-        ws.set_tuning(categorical_leaves=leaves_flag)
+        ws.set_tuning(input_leaves=leaves_flag)
         for summed in (False, True):
             out = step(summed)
             step(summed)

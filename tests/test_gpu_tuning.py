@@ -41,7 +41,7 @@ def test_variants_agree(require_gpu):
     plan, nodes, edges, ws = setup(torch)
     assert ws.tuning() == {"compress_items_forward": 2, "compress_items_reverse": 2, "rake_block": 256,
                            "gather_chunk": 16, "compress_depth": 2, "rake_log_edges": 1, "persistent": 0,
-                           "categorical_leaves": 1}
+                           "input_leaves": 1}
     base = run(torch, ws, plan, nodes, edges)
     # Launch shapes only: bit-exact. The binary compression kernels'
     # variants are seen with grouping off (depth 1).
@@ -64,7 +64,7 @@ def test_variants_agree(require_gpu):
         assert float((em - base[2]).abs().max()) <= 1e-6, fields
     for bad in ({"compress_items_forward": 3}, {"rake_block": 96}, {"gather_chunk": 0}, {"gather_chunk": 65},
                 {"compress_depth": 0}, {"compress_depth": 4}, {"rake_log_edges": 2}, {"persistent": 2},
-                {"categorical_leaves": 2}):
+                {"input_leaves": 2}):
         with pytest.raises(thmm.InvalidArgument):
             ws.set_tuning(**bad)
     # persistent passes are switched off in this build
