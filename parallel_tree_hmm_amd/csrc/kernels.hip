@@ -773,13 +773,15 @@ __global__ void Convert(const In *in, Out *out, std::uint64_t count) {
 // Logs of public [B][N][K] node potentials into internal [N][B][K], and of
 // the edges (src/inference.cc:95-98).
 template <typename In>
-__global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d) {
+__global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
+                        const std::uint8_t *skip) {
   const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
   const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
   for (std::uint64_t tid = GridThread(); tid < node_values + edge_values; tid += GridStride()) {
     if (tid < node_values) {
       const std::uint64_t s = tid % d.K, nb = tid / d.K;
       const std::uint64_t b = nb % d.B, n = nb / d.B;
+      if (skip != nullptr && skip[n] != 0) continue;
       const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
       nodes[tid] = x > 0.0 ? log(x) : kNegInf;
     } else {
@@ -787,6 +789,39 @@ __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, d
       const double x = static_cast<double>(edges_in[i]);
       edges[i] = x > 0.0 ? log(x) : kNegInf;
     }
+  }
+}
+
+// MaxInit four values at a time (K a multiple of 4, 16-byte aligned arrays):
+// quad loads of the inputs, 32-byte stores of their FP64 logs, the same
+// x > 0 ? log x : -inf per value.
+template <typename In>
+__global__ void MaxInitQuad(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
+                            const std::uint8_t *skip) {
+  const std::uint64_t node_quads = static_cast<std::uint64_t>(d.N) * d.B * d.K / 4;
+  const std::uint64_t edge_quads = static_cast<std::uint64_t>(d.E) * d.K * d.K / 4;
+  const std::uint32_t kq = d.K / 4;
+  for (std::uint64_t tid = GridThread(); tid < node_quads + edge_quads; tid += GridStride()) {
+    Quad<In> x;
+    double *out;
+    if (tid < node_quads) {
+      const std::uint64_t q = tid % kq, nb = tid / kq;
+      const std::uint64_t b = nb % d.B, n = nb / d.B;
+      if (skip != nullptr && skip[n] != 0) continue;
+      x = LoadQuad(nodes_in + (b * d.N + n) * d.K + 4 * q);
+      out = nodes + 4 * tid;
+    } else {
+      const std::uint64_t i = tid - node_quads;
+      x = LoadQuad(edges_in + 4 * i);
+      out = edges + 4 * i;
+    }
+    Quad<double> y;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const double v = static_cast<double>(x.v[r]);
+      y.v[r] = v > 0.0 ? log(v) : kNegInf;
+    }
+    StoreQuad(out, y);
   }
 }
 
@@ -1864,16 +1899,32 @@ void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t cou
 }
 
 void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in, double *nodes,
-                   double *edges) {
+                   double *edges, const std::uint8_t *skip) {
   const std::uint64_t total = static_cast<std::uint64_t>(l.dims.N) * l.dims.B * l.dims.K +
                               static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
+  if (l.dims.K % 4 == 0 && aligned(nodes_in) && aligned(nodes) && (l.dims.E == 0 || (aligned(edges_in) && aligned(edges)))) {
+    const unsigned grid = GridStrideBlocks(total / 4, kThreads);
+    if (in_real == Real::kF32)
+      MaxInitQuad<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
+                                                            static_cast<const float *>(edges_in), nodes, edges, l.dims,
+                                                            skip);
+    else
+      MaxInitQuad<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(nodes_in),
+                                                             static_cast<const double *>(edges_in), nodes, edges,
+                                                             l.dims, skip);
+    Check(hipGetLastError(), "max-product initialisation launch");
+    return;
+  }
   const unsigned grid = GridStrideBlocks(total, kThreads);
   if (in_real == Real::kF32)
     MaxInit<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
-                                                      static_cast<const float *>(edges_in), nodes, edges, l.dims);
+                                                      static_cast<const float *>(edges_in), nodes, edges, l.dims,
+                                                      skip);
   else
     MaxInit<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(nodes_in),
-                                                       static_cast<const double *>(edges_in), nodes, edges, l.dims);
+                                                       static_cast<const double *>(edges_in), nodes, edges, l.dims,
+                                                       skip);
   Check(hipGetLastError(), "max-product initialisation launch");
 }
 

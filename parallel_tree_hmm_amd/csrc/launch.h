@@ -346,8 +346,10 @@ void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t c
 void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t count);
 
 // ---- max-product (FP64) ----
+// Nodes with skip[n] set (null: none) are left unwritten.
 void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in,
-                   double *nodes, double *edges);
+                   double *nodes, double *edges,
+                   const std::uint8_t *skip = nullptr);
 void LaunchMaxLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
                     std::uint32_t count, const MaxBuffers &buf);
 void LaunchMaxFinish(const Launch &l, const MaxBuffers &buf, double *log_weight,

@@ -138,10 +138,12 @@ struct Workspace::Impl {
   // sampling forward, which has no grouped compressions.
   std::vector<FusedStep> fused_plain;
   // Raw leaves: round 0's rakes (the first round0_rakes of the plan, every
-  // leaf an original leaf) read the caller's linear potentials and the node
+  // leaf an original leaf) read the caller's potentials and the node
   // initialisation skips their leaves (leaf_skip [N]); 0: not available
-  // (no fused forward program, or round 0 does not start with rakes).
+  // (round 0 does not start with rakes). round0_gather: the fused forward
+  // program's first step holds exactly those rakes.
   std::uint32_t round0_rakes = 0;
+  bool round0_gather = false;
   DeviceBuffer leaf_skip;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
   // Grouped compressions (K = 16 FP32 with the fused forward): the level of
@@ -298,6 +300,7 @@ struct Workspace::Impl {
     categorical = false;
     cat_rakes = 0;
     round0_rakes = 0;
+    round0_gather = false;
     observation_nodes.clear();
     resident_observation_batch = 0;
     resident_factors = false;
@@ -847,14 +850,13 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
   }
   if (w.rake_adjoint_from_parent && FusedForwardSupported(static_cast<std::uint32_t>(states)))
     BuildPrograms(w, plan, rakes, compresses, batch);
-  // Raw leaves where round 0's rakes run as the first gather step and as the
-  // first level of the reverse program (checked again per call).
-  if (!w.fused.empty() && w.fused.front().kind == 0 && !plan.batches.empty() &&
-      plan.batches.front().primitive == THMM_RAKE && plan.batches.front().offset == 0) {
+  // Raw leaves where round 0 starts with a level of rakes of original leaves
+  // (the per-call conditions are checked again per call).
+  if (!plan.batches.empty() && plan.batches.front().primitive == THMM_RAKE && plan.batches.front().offset == 0) {
     const thmm_primitive_batch &b0 = plan.batches.front();
     std::vector<std::uint8_t> has_child(plan.num_nodes, 0), skip(plan.num_nodes, 0);
     for (Index parent : plan.edge_parents) has_child[parent] = 1;
-    bool leaves = w.fused.front().work == b0.count;
+    bool leaves = true;
     for (std::uint32_t r = 0; leaves && r < b0.count; ++r) {
       leaves = !has_child[plan.rakes[r].leaf];
       skip[plan.rakes[r].leaf] = 1;
@@ -862,6 +864,7 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
     if (leaves) {
       Upload(w.leaf_skip, skip, w.stream, "topology upload");
       w.round0_rakes = b0.count;
+      w.round0_gather = !w.fused.empty() && w.fused.front().kind == 0 && w.fused.front().work == b0.count;
     }
   }
 
@@ -1276,7 +1279,7 @@ void RunReverse(const Impl &w, const Launch &l, const LinearBuffers &buf, Profil
 // initialisation must run on the team transpose.
 bool RawLeaves(const Impl &w, const Launch &l, const void *nodes, const LinearBuffers &buf, bool reverse) {
   const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
-  if (w.tuning.input_leaves == 0 || w.round0_rakes == 0 || !UsesFusedForward(w, buf) || PersistentReady(w, l, buf) ||
+  if (w.tuning.input_leaves == 0 || !w.round0_gather || !UsesFusedForward(w, buf) || PersistentReady(w, l, buf) ||
       buf.log_edges != nullptr || buf.edges_f32 != 0)
     return false;
   if (nodes == nullptr || !aligned(nodes) || !aligned(w.nodes.ptr) || !FastTransposeFits(l.dims.K, l.dims.B, l.dims.N))
@@ -1491,7 +1494,9 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
   Launch init = l;
   if (edges_on_load) init.dims.E = 0;
   // Logs of the raw staged factors (bit-exact with the oracle's log of the
-  // same values), internal layout.
+  // same values), internal layout. The leaves are logged here too: a rake
+  // kernel logging its leaf on load would repeat each log across the K
+  // rows of its K*K/4 lanes.
   if (src.categorical == nullptr) {
     LaunchMaxInit(init, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
   } else {

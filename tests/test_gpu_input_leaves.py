@@ -9,7 +9,9 @@ host-staged marginal and partition calls. Forests mix
 stars (many leaves of one parent), paths (no leaves but the last), recursive
 trees and single nodes; zero potentials, and leaves without support (device
 calls); batch 1 and 3; FP32 and FP64. A node array off 16-byte alignment
-takes the initialise-first path."""
+takes the initialise-first path. MAP's log transform of the node potentials
+(quad kernel for aligned inputs, one-value kernel otherwise) gives the same
+states and weights either way."""
 
 from __future__ import annotations
 
@@ -128,3 +130,43 @@ def test_unaligned_inputs_fall_back(require_gpu):
     assert np.allclose(outs[0][0], outs[1][0], rtol=1e-6, atol=1e-6)
     assert np.max(np.abs(outs[0][1] - outs[1][1])) <= 1e-6
     assert np.max(np.abs(outs[0][2] - outs[1][2])) <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+@pytest.mark.parametrize("k", [8, 16, 32])
+@pytest.mark.parametrize("batch", [1, 3])
+def test_map_log_transform_aligned_and_unaligned(require_gpu, k, precision, batch):
+    """MAP's log transform of the node potentials: 16-byte aligned inputs take
+    the quad kernel, others the one-value kernel; both log each value in
+    FP64 the same way, so states and log weights are equal bit for bit, and
+    the states equal the oracle's. (MAP keeps the separate log transform for
+    every node: a rake kernel logging its leaf on load would repeat each log
+    across the K rows of its lanes.)"""
+    import torch
+
+    import oracle
+
+    plan, nodes, edges = case(k, precision, batch, 7 * k + batch)
+    n = plan.num_nodes
+    dtype = torch.float32 if precision == "f32" else torch.float64
+    aligned = torch.from_numpy(nodes).cuda()
+    storage = torch.empty(nodes.size + 1, dtype=dtype, device="cuda")
+    shifted = storage[1:]
+    shifted.copy_(aligned.reshape(-1))
+    de = torch.from_numpy(edges).cuda()
+    ws = thmm.Workspace()
+    ws.reserve(plan, k, batch, precision=precision,
+               flags=thmm._abi.THMM_RESERVE_MAXIMUM | thmm._abi.THMM_RESERVE_DEVICE_RESIDENT)
+    results = []
+    for dn in (aligned, shifted):
+        lw = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+        states = torch.empty(batch * n, dtype=torch.int32, device="cuda")
+        ws.maximum_device(dn.data_ptr(), de.data_ptr(), batch, lw.data_ptr(), states.data_ptr(),
+                          torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        results.append((lw.cpu().numpy(), states.cpu().numpy()))
+    ws.close()
+    for a, b in zip(results[0], results[1]):
+        assert same(a, b), f"K={k} {precision} batch {batch}: MAP differs"
+    lw0, expected = oracle.maximum(plan, k, nodes[0].astype(np.float64), edges.astype(np.float64))
+    assert np.array_equal(results[0][1].reshape(batch, n)[0].astype(np.uint32), expected)
