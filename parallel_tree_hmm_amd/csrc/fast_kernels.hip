@@ -821,7 +821,7 @@ __device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, 
 
 // Reverse pass of one group over a balanced bracketing of its product
 // (matrix products associate): span [lo, hi) splits at mid = (lo + hi) / 2,
-// whose middle node is middles[mid - 1]. Build() forms every span's
+// whose middle node is middles[mid - 1]. Load() then Combine() form every span's
 // row-normalised product bottom-up in row and column layout (the operand
 // layouts of the adjoint step), so the spans of one depth are independent
 // and the critical path is log2(n) adjoint steps. Reverse() runs the binary
@@ -830,7 +830,7 @@ __device__ inline void StoreAdjointRow(const LinearBuffers &buf, const Dims &d, 
 // row), top-down, each split with its own LDS tile for the Gt transposition.
 struct Part {
   float row[4], col[4];  // normalised product: row layout, column layout
-  double scale;          // FP64 log scale of row l%16 (Build only)
+  double scale;          // FP64 log scale of row l%16 (Load and Combine only)
   float top;             // row l%16 of the product is top * row (1 for a stored slot)
   float e_own;           // split's middle weight e[m], m = l%16
   float e[4];            // split's middle weights e[m], m = 4(l/16) + r (kept when kKeepWeights)
@@ -845,19 +845,20 @@ struct GroupState {
   bool quad_edges;
 };
 
-// kKeepWeights: splits keep their gathered middle weights from Build for
+// kKeepWeights: splits keep their gathered middle weights from Combine for
 // Reverse (four registers per split) instead of gathering them again.
 template <int kLo, int kHi, bool kKeepWeights, bool kLeaf = (kHi - kLo == 1)> struct Span;
 
 template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeights, true> {
   Part part;
-  __device__ inline void Build(const GroupState &g) {
+  __device__ inline void Load(const GroupState &g) {
     const float *x = Slot<float>(g.buf, g.d, g.c.slots[kLo], g.c.b);
     LoadRows<true>(x, g.lane, part.row);
     LoadColumns<false>(x, g.lane, part.col);
     part.scale = RowScale(g.buf, g.d, g.c.slots[kLo], g.c.b, g.lane % 16);
     part.top = 1.0f;
   }
+  __device__ inline void Combine(const GroupState &) {}
   __device__ inline void Reverse(const GroupState &g, const float (&adj)[4]) const {
     Quad<float> q;
 #pragma unroll
@@ -871,13 +872,24 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
   Span<kLo, kMid, kKeepWeights> left;
   Span<kMid, kHi, kKeepWeights> right;
   Part part;
+  float mid_own;      // middle node's value at m = l%16 (Load to Combine)
+  double mid_scale;   // its FP64 log scale
 
-  __device__ inline void Build(const GroupState &g) {
-    left.Build(g);
-    right.Build(g);
-    const int lane = g.lane;
+  // Every global load of the group (slots, row scales, middle nodes) is
+  // issued before the first product: Combine's LDS transpositions order
+  // memory, so loads left after one would wait for it.
+  __device__ inline void Load(const GroupState &g) {
+    left.Load(g);
+    right.Load(g);
     const std::uint64_t mid_at = static_cast<std::uint64_t>(g.c.middles[kMid - 1]) * g.d.B + g.c.b;
-    const float mid_own = static_cast<const float *>(g.buf.nodes)[mid_at * 16 + lane % 16];
+    mid_own = static_cast<const float *>(g.buf.nodes)[mid_at * 16 + g.lane % 16];
+    mid_scale = g.buf.node_scale[mid_at];
+  }
+
+  __device__ inline void Combine(const GroupState &g) {
+    left.Combine(g);
+    right.Combine(g);
+    const int lane = g.lane;
     const double w_top = StepWeights16(mid_own, right.part.scale, lane, part.e, &part.e_own);
     float scaled[4];
 #pragma unroll
@@ -899,7 +911,7 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
     for (int r = 0; r < 4; ++r) part.col[r] = tile[(4 * (lane / 16) + r) * kPad + lane % 16];
-    part.scale = live ? left.part.scale + g.buf.node_scale[mid_at] + w_top + LogPositive(top) : kNegInf;
+    part.scale = live ? left.part.scale + mid_scale + w_top + LogPositive(top) : kNegInf;
   }
 
   // adj: marginal-form adjoint of this span's product, row layout. The
@@ -1008,7 +1020,8 @@ template <int kLo, int kHi, bool kKeepWeights> struct Span<kLo, kHi, kKeepWeight
 
 template <int kN, bool kKeepWeights> __device__ inline void ReverseGroup(const GroupState &g, const float (&adj)[4]) {
   Span<0, kN, kKeepWeights> root;
-  root.Build(g);
+  root.Load(g);
+  root.Combine(g);
   root.Reverse(g, adj);
 }
 
@@ -1032,7 +1045,7 @@ __device__ inline void ChainReverseBody(const DeviceOps &ops, const Dims &d, std
     return;
   }
   // Groups of more than 2 slots gather their middle weights again in
-  // Reverse (the same shuffles of e_own as in Build), which keeps the
+  // Reverse (the same shuffles of e_own as in Combine), which keeps the
   // kernel of groups of 3 and 4 at 96 registers: five wavefronts per SIMD.
   switch (c.n) {
     case 2: ReverseGroup<2, false>(g, adj); break;
