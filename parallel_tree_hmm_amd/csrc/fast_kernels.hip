@@ -121,6 +121,18 @@ static_assert(Team<8>::kSteps <= Team<8>::kLanesPerRow && Team<16>::kSteps <= Te
                   Team<32>::kSteps <= Team<32>::kLanesPerRow && Team<64>::kSteps <= Team<64>::kLanesPerRow,
               "row ownership");
 
+// A team's rows are owned one per lane: row j kRps + l / kLpr on the lane
+// with l % kLpr == j % kLpr. RowToLane moves row r's value to lane r (r <
+// K; lanes from K on get some row's value) so that a store of the team's
+// rows runs in lane order: a store whose lanes are scattered over a line
+// costs one partial-line write to HBM per lane.
+template <int K, typename V> __device__ inline V RowToLane(V value, int l) {
+  using TM = Team<K>;
+  constexpr int kLpr = TM::kLanesPerRow, kRps = TM::kRowsPerStep;
+  const int r = l % K;
+  return __shfl(value, (r % kRps) * kLpr + r / kRps, TM::kLanes);
+}
+
 // log v for v > 0, else -inf (NaN included), in the element type: the
 // node initialisation's log of a linear potential.
 template <typename T> __device__ inline T LogOf(T v) { return v > T(0) ? log(v) : NegInf<T>(); }
@@ -193,7 +205,7 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
   TeamExp<T, kT, kLpr>(vq, l, e);
   // After the row reduction all kLpr lanes of a row hold its sum; the lane
   // with l % kLpr == j % kLpr takes the logarithm, so each row's log is
-  // formed once and the team's rows are stored by one contiguous store.
+  // formed once; RowToLane puts the rows in lane order for the store.
   T sums[kJ];
 #pragma unroll
   for (int j = 0; j < kJ; ++j) {
@@ -214,7 +226,8 @@ __global__ __launch_bounds__(256) void RakeForwardTeam(DeviceOps ops, Dims d, st
   for (int o = kT / 2; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kT));
   const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
   const bool live = top > kNegInf;
-  if (row >= 0) static_cast<T *>(buf.branches)[at * K + row] = live ? static_cast<T>(message - top) : NegInf<T>();
+  const double ordered = RowToLane<K>(message, l);
+  if (l < K) static_cast<T *>(buf.branches)[at * K + l] = live ? static_cast<T>(ordered - top) : NegInf<T>();
   if (l == 0) buf.branch_scale[at] = live ? buf.node_scale[leaf_at] + top : kNegInf;
 }
 
@@ -392,8 +405,9 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
     double row_scale;
     if (log_edge) {
       row_scale = ExpLogRows<T, kLpr, kJ>(pq, owned);
-      if (row >= 0)
-        buf.log_edge_shift[static_cast<std::uint64_t>(op.slot) * K + row] = row_scale > kNegInf ? row_scale : 0.0;
+      const double ordered = RowToLane<K>(row_scale, l);
+      if (l < K)
+        buf.log_edge_shift[static_cast<std::uint64_t>(op.slot) * K + l] = ordered > kNegInf ? ordered : 0.0;
     } else {
       row_scale = row >= 0 ? RowScale(buf, d, op.slot, b, row) : 0.0;
     }
@@ -413,10 +427,13 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
     acc += (row >= 0 && mine > T(0)) ? leaf_scale + row_scale + LogPositive(mine) : kNegInf;
     op = next;
   }
+  // Rows in lane order for the stores below.
+  const double ordered = RowToLane<K>(acc, l);
+  const int lane_row = l < K ? l : -1;
   if (chunk.partial == kDirectChunk)
-    AbsorbSum<T, K, kT>(buf, static_cast<std::uint64_t>(chunk.parent) * d.B + b, row, acc, l);
-  else if (row >= 0)
-    g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + row] = acc;
+    AbsorbSum<T, K, kT>(buf, static_cast<std::uint64_t>(chunk.parent) * d.B + b, lane_row, ordered, l);
+  else if (l < K)
+    g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + l] = ordered;
 }
 
 template <typename T, int K, bool kLogEdges, bool kRaw = false>

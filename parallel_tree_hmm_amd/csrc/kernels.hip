@@ -936,9 +936,10 @@ __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double othe
 }
 
 // Linear-domain screen of one rake row of raw edge factors f against the
-// leaf's log vector v (lanes of kQ hold the row's K columns, 4 each):
-// returns the mask of this lane's columns c whose log-domain value
-// log f_c + v_c may be the row's largest, so that only those are logged.
+// leaf's log vector v (kQ consecutive lanes hold the row's K columns, 4
+// each; ScreenLeaf then ScreenRow): the mask of this lane's columns c whose
+// log-domain value log f_c + v_c may be the row's largest, so that only
+// those are logged.
 //
 // l_c = f_c 2^y_c, y_c = (v_c - vmax) log2 e, is formed in FP32: y_c from
 // the FP64 difference rounded to FP32 (absolute error below 2^-15 where
@@ -952,30 +953,51 @@ __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double othe
 // |v| < 2^30) cannot close that gap: it neither wins nor ties. Rows outside
 // these conditions log every column: vmax not finite or |vmax| >= 2^30,
 // a factor above 2^126 or not a number, or lmax < 2^-100.
+
+// The leaf-side half of the screen, shared by every row of a rake: vmax
+// over the row's kQ lanes (every lane of a row holds four of the K
+// columns), and for each of the lane's columns n_c = floor(y_c) and
+// 2^(y_c - n_c) where y_c >= -300 (ok_c), else nothing.
+struct ScreenColumns {
+  bool bounded;  // vmax finite and |vmax| < 2^30
+  bool ok[4];
+  float scale[4];
+  int shift[4];
+};
+
 template <int kQ>
-__device__ inline unsigned ScreenRakeRow(const double (&f)[4], const Quad<double> &v) {
+__device__ inline ScreenColumns ScreenLeaf(const Quad<double> &v) {
   double vmax = fmax(fmax(v.v[0], v.v[1]), fmax(v.v[2], v.v[3]));
-  // 0 <= f <= 2^126 for every column, NaN included (a failed comparison).
-  float wild = 0.0f;
 #pragma unroll
-  for (int c = 0; c < 4; ++c) wild = (f[c] >= 0.0 && f[c] <= 0x1p126) ? wild : 1.0f;
-#pragma unroll
-  for (int o = 1; o < kQ; o <<= 1) {
-    vmax = fmax(vmax, __shfl_xor(vmax, o, kQ));
-    wild = fmaxf(wild, __shfl_xor(wild, o, kQ));
-  }
-  const bool bounded = vmax > kNegInf && fabs(vmax) < 0x1p30 && wild == 0.0f;
-  float lin[4];
+  for (int o = 1; o < kQ; o <<= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, kQ));
+  ScreenColumns s;
+  s.bounded = vmax > kNegInf && fabs(vmax) < 0x1p30;
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const float y = static_cast<float>(v.v[c] - vmax) * 1.44269504f;
-    float value = 0.0f;
-    if (bounded && y > -300.0f) {  // false for -inf and NaN
-      const float n = floorf(y);
-      value = ldexpf(static_cast<float>(f[c]) * __builtin_amdgcn_exp2f(y - n), static_cast<int>(n));
-    }
-    lin[c] = value;
+    s.ok[c] = s.bounded && y > -300.0f;  // false for -inf and NaN
+    const float n = s.ok[c] ? floorf(y) : 0.0f;
+    s.scale[c] = s.ok[c] ? __builtin_amdgcn_exp2f(y - n) : 0.0f;
+    s.shift[c] = static_cast<int>(n);
   }
+  return s;
+}
+
+// The row half: l_c = ldexp(f_c 2^(y_c - n_c), n_c) for the lane's columns
+// and the mask of those that may win (above).
+template <int kQ, typename F>
+__device__ inline unsigned ScreenRow(const ScreenColumns &s, const F (&f)[4]) {
+  // 0 <= f <= 2^126 for every column, NaN included (a failed comparison).
+  float wild = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) wild = (f[c] >= F(0) && f[c] <= F(0x1p126)) ? wild : 1.0f;
+#pragma unroll
+  for (int o = 1; o < kQ; o <<= 1) wild = fmaxf(wild, __shfl_xor(wild, o, kQ));
+  const bool bounded = s.bounded && wild == 0.0f;
+  float lin[4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+    lin[c] = bounded && s.ok[c] ? ldexpf(static_cast<float>(f[c]) * s.scale[c], s.shift[c]) : 0.0f;
   float lmax = fmaxf(fmaxf(lin[0], lin[1]), fmaxf(lin[2], lin[3]));
 #pragma unroll
   for (int o = 1; o < kQ; o <<= 1) lmax = fmaxf(lmax, __shfl_xor(lmax, o, kQ));
@@ -989,7 +1011,7 @@ __device__ inline unsigned ScreenRakeRow(const double (&f)[4], const Quad<double
 }
 
 // One rake row per kQ = K/4 consecutive lanes; K rows per (op, item).
-// Raw edge factors are screened (ScreenRakeRow) before they are logged;
+// Raw edge factors are screened (ScreenLeaf, ScreenRow) before they are logged;
 // the values compared are the same FP64 sums as without the screen.
 template <int K>
 __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
@@ -1028,7 +1050,7 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
       nan_first = isnan(v.v[0]) || (!(f[0] > 0.0) && v.v[0] == __builtin_huge_val()) ||
                   (f[0] == __builtin_huge_val() && v.v[0] == kNegInf);
     // Ascending columns: the strict '>' keeps the lower index on a tie.
-    for (unsigned mask = ScreenRakeRow<kQ>(f, v); mask != 0; mask &= mask - 1) {
+    for (unsigned mask = ScreenRow<kQ>(ScreenLeaf<kQ>(v), f); mask != 0; mask &= mask - 1) {
       const int c = __builtin_ctz(mask);
       const double candidate = LogFactor(f[c]) + v.v[c];
       if (candidate > best) best = candidate, arg = c0 + c;
@@ -1049,10 +1071,132 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
     ArgMaxMerge(best, arg, other, other_arg);
   }
   if (nan_first) best = __builtin_nan(""), arg = 0;
+  // The choices of rows p .. p + 3 (p % 4 == 0) go out as one 4-byte store
+  // (lanes c0 == 0 of the next three rows are kQ, 2 kQ, 3 kQ lanes on).
+  std::uint32_t packed = arg;
+#pragma unroll
+  for (int i = 1; i < 4; ++i)
+    packed |= static_cast<std::uint32_t>(__shfl(arg, (l + i * kQ) % kLanes, kLanes)) << (8 * i);
   if (c0 == 0) {
     const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + p;
     buf.branches[at] = best;
-    buf.rake_choice[at] = static_cast<std::uint8_t>(arg);
+    if (p % 4 == 0) *reinterpret_cast<std::uint32_t *>(buf.rake_choice + at) = packed;
+  }
+}
+
+// MaxRakeTeam over row steps, K = 16 and 32: kT lanes per (op, item)
+// (16 at K = 16, 64 at K = 32) take kRps = 4 kT / K rows per step over
+// kJ = K / kRps steps, each row on kQ = K / 4 consecutive lanes as above.
+// A lane keeps the same four columns over its rows, so the leaf quad is
+// loaded once and the screen's leaf half is formed once per lane; each row
+// is screened, logged and merged exactly as in MaxRakeTeam (the same values
+// and choices). After the merge every lane of a row holds its result; the
+// first K lanes store the rows in order.
+template <int K>
+__global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                    MaxBuffers buf) {
+  constexpr int kQ = K / 4, kT = K <= 16 ? 16 : 64, kRps = 4 * kT / K, kJ = K / kRps;
+  static_assert(kJ <= kQ && kRps * kQ == kT, "row steps");
+  const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const std::uint64_t item = tid / kT;
+  const int l = static_cast<int>(tid % kT);
+  if (item >= static_cast<std::uint64_t>(count) * d.B) return;  // whole teams
+  const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
+  const DeviceRake op = ops.rakes[offset + item / d.B];
+  const int p0 = l / kQ, c0 = (l % kQ) * 4;
+  const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
+  double best[kJ];
+  std::uint32_t arg[kJ];
+  bool nan_first[kJ];
+#pragma unroll
+  for (int j = 0; j < kJ; ++j) best[j] = kNegInf, arg[j] = c0, nan_first[j] = false;
+  if (op.slot < d.E && buf.raw_edges != nullptr) {
+    const std::uint64_t at = static_cast<std::uint64_t>(op.slot) * K * K + p0 * K + c0;
+    const ScreenColumns screen = ScreenLeaf<kQ>(v);
+    if (buf.raw_edges_f32) {
+      Quad<float> q[kJ];
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) q[j] = LoadQuad(static_cast<const float *>(buf.raw_edges) + at + j * kRps * K);
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) {
+        // A four-value copy per row: indexed by column below, it stays in
+        // registers (an index into q would place q in scratch memory).
+        float f[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) f[c] = q[j].v[c];
+        if (c0 == 0)
+          nan_first[j] = isnan(v.v[0]) || (!(f[0] > 0.0f) && v.v[0] == __builtin_huge_val()) ||
+                         (f[0] == __builtin_huge_valf() && v.v[0] == kNegInf);
+        for (unsigned mask = ScreenRow<kQ>(screen, f); mask != 0; mask &= mask - 1) {
+          const int c = __builtin_ctz(mask);
+          const double candidate = LogFactor(static_cast<double>(f[c])) + v.v[c];
+          if (candidate > best[j]) best[j] = candidate, arg[j] = c0 + c;
+        }
+      }
+    } else {
+      Quad<double> q[kJ];
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) q[j] = LoadQuad(static_cast<const double *>(buf.raw_edges) + at + j * kRps * K);
+#pragma unroll
+      for (int j = 0; j < kJ; ++j) {
+        double f[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) f[c] = q[j].v[c];
+        if (c0 == 0)
+          nan_first[j] = isnan(v.v[0]) || (!(f[0] > 0.0) && v.v[0] == __builtin_huge_val()) ||
+                         (f[0] == __builtin_huge_val() && v.v[0] == kNegInf);
+        for (unsigned mask = ScreenRow<kQ>(screen, f); mask != 0; mask &= mask - 1) {
+          const int c = __builtin_ctz(mask);
+          const double candidate = LogFactor(f[c]) + v.v[c];
+          if (candidate > best[j]) best[j] = candidate, arg[j] = c0 + c;
+        }
+      }
+    }
+  } else {
+    const double *slot = MaxSlot(buf, d, op.slot, b) + p0 * K + c0;
+    Quad<double> row[kJ];
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) row[j] = LoadQuad(slot + j * kRps * K);
+#pragma unroll
+    for (int j = 0; j < kJ; ++j) {
+      if (c0 == 0) nan_first[j] = isnan(row[j].v[0] + v.v[0]);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double candidate = row[j].v[c] + v.v[c];
+        if (candidate > best[j]) best[j] = candidate, arg[j] = c0 + c;
+      }
+    }
+  }
+  double out = 0.0;
+  std::uint32_t out_arg = 0;
+  const int q = l % kQ;
+#pragma unroll
+  for (int j = 0; j < kJ; ++j) {
+    int nan_row = nan_first[j] ? 1 : 0;
+#pragma unroll
+    for (int o = 1; o < kQ; o <<= 1) {
+      const double other = __shfl_xor(best[j], o, kQ);
+      const std::uint32_t other_arg = __shfl_xor(arg[j], o, kQ);
+      ArgMaxMerge(best[j], arg[j], other, other_arg);
+      nan_row |= __shfl_xor(nan_row, o, kQ);
+    }
+    if (nan_row) best[j] = __builtin_nan(""), arg[j] = 0;
+    if (j == q) out = best[j], out_arg = arg[j];
+  }
+  // Lane (p0, q) holds row q kRps + p0. Lane r < K takes row r from lane
+  // (r % kRps) kQ + r / kRps so that the stores run in lane order: scattered
+  // lanes would each cost a partial-line write to HBM. The choices of rows
+  // r .. r + 3 (r % 4 == 0) go out as one 4-byte store.
+  const int source = (l % kRps) * kQ + (l / kRps) % kJ;
+  const double row_best = __shfl(out, source, kT);
+  const std::uint32_t row_arg = __shfl(out_arg, source, kT);
+  std::uint32_t packed = row_arg;
+#pragma unroll
+  for (int i = 1; i < 4; ++i) packed |= static_cast<std::uint32_t>(__shfl(row_arg, (l + i) % kT, kT)) << (8 * i);
+  if (l < K) {
+    const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + l;
+    buf.branches[at] = row_best;
+    if (l % 4 == 0) *reinterpret_cast<std::uint32_t *>(buf.rake_choice + at) = packed;
   }
 }
 
@@ -1450,9 +1594,11 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
       if (l.dims.K == 8)
         MaxRakeTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
       else if (l.dims.K == 16)
-        MaxRakeTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+        MaxRakeSteps<16><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * 16, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
       else
-        MaxRakeTeam<32><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+        MaxRakeSteps<32><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * 64, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
       Check(hipGetLastError(), "max-product level launch");
       return;
     }

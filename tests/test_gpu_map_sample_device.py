@@ -226,7 +226,7 @@ def test_on_load_edge_transforms_equal_the_whole_array_passes(require_gpu, k):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("k,precision", [(8, "f32"), (16, "f32"), (16, "f64")])
+@pytest.mark.parametrize("k,precision", [(8, "f32"), (16, "f32"), (16, "f64"), (32, "f32"), (32, "f64")])
 def test_screened_rake_rows_match_the_logged_rows(require_gpu, k, precision):
     """The MAP rake kernel screens raw edge rows in the linear domain and
     logs only the columns that can win. On inputs built to stress the screen
