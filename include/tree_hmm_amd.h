@@ -526,7 +526,11 @@ thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, s
  * (64, 128 or 256); gather_chunk: rakes a team folds into one parent per
  * step of the fused forward pass (1..64); compress_depth: compression-tree
  * levels evaluated as one grouped product of up to 2^depth slots on the
- * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2);
+ * K = 16 FP32 path (1 = every compression stored, 2 or 3; default 2); in
+ * MAP calls at K = 8, 16 and 32, depth 2 or 3 evaluates each compression
+ * together with the input compressions absorbed into it (paths kept in
+ * registers, the same values and choices) and 1 runs every compression
+ * level on its own;
  * rake_log_edges: in thmm_logz_forward_device / _backward_device with
  * K = 8, 16, 32 or 64, the rake kernels read the log edge potentials in
  * place (1, default; at K = 8 and 16 and in FP64 only the edges

@@ -1252,6 +1252,106 @@ __global__ __launch_bounds__(256) void MaxCompressTeam(DeviceOps ops, Dims d, st
   *reinterpret_cast<std::uint32_t *>(buf.compress_choice + out_at * K * K + p * K + c0) = packed;
 }
 
+// One lane's quad of a max-product compression from staged L + mid and R
+// (the scan of MaxCompressTeam: row l / (K/4), columns 4 (l % (K/4)) .. +3,
+// candidates (L + mid) + R over ascending m with strict '>'); the quad is
+// elements 4l .. 4l+3 of the output path, the quad lane l loads of it as an
+// operand. Stores the lane's four choices into `choices`, the compression's
+// K x K choice array of this item.
+template <int K>
+__device__ inline Quad<double> MaxScanQuad(const double *left_mid, const double *right, int l,
+                                           std::uint8_t *choices) {
+  constexpr int kQ = K / 4;
+  const int p = l / kQ, c0 = (l % kQ) * 4;
+  Quad<double> best;
+  std::uint32_t arg[4] = {0, 0, 0, 0};
+  const double lm0 = left_mid[p * K];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) best.v[c] = lm0 + right[c0 + c];
+#pragma unroll
+  for (int m = 1; m < K; ++m) {
+    const double lm = left_mid[p * K + m];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const double candidate = lm + right[m * K + c0 + c];
+      if (candidate > best.v[c]) best.v[c] = candidate, arg[c] = m;
+    }
+  }
+  *reinterpret_cast<std::uint32_t *>(choices + 4 * l) = arg[0] | (arg[1] << 8) | (arg[2] << 16) | (arg[3] << 24);
+  return best;
+}
+
+// Max-product compression pieces (DeviceMaxPiece), K*K/4 lanes per (piece,
+// item) as in MaxCompressTeam: the absorbed input compressions are evaluated
+// first, both at once, their paths kept in the lanes' registers (lane l's
+// output quad is the quad it stages of an operand), then the root. Every
+// compression is the scan of MaxCompressTeam on the same FP64 values, so
+// paths and choices are bit-identical to the level-by-level form; only the
+// absorbed paths are never written or read back. The absorbed compressions'
+// middles are final when the root's level runs (a compression's middle node
+// leaves the tree), and their inputs are slots written at earlier levels.
+template <int K>
+__global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                    MaxBuffers buf) {
+  constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
+  // [0], [1]: L + mid and R of the left input compression; [2], [3]: those
+  // of the right one; the root restages into [0] and [1].
+  __shared__ double stage[kItems][4][K * K];
+  const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
+  const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  DeviceCompress root{};
+  std::uint32_t b = 0, flags = 0;
+  Quad<double> side[2], root_mid;
+  if (active) {
+    b = static_cast<std::uint32_t>(item % d.B);
+    const DeviceMaxPiece piece = ops.max_pieces[offset + item / d.B];
+    flags = piece.flags;
+    root = ops.compresses[piece.root];
+    root_mid = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(root.middle) * d.B + b) * K + (4 * l) % K);
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const std::uint32_t slot = s == 0 ? root.left : root.right;
+      if (flags & (1u << s)) {
+        const DeviceCompress op = ops.compresses[slot - d.E];
+        const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l), rq = MaxQuad(buf, d, op.right, b, 4 * l);
+        const Quad<double> mq =
+            LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K + (4 * l) % K);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          stage[t][2 * s][4 * l + i] = lq.v[i] + mq.v[i];
+          stage[t][2 * s + 1][4 * l + i] = rq.v[i];
+        }
+      } else {
+        side[s] = MaxQuad(buf, d, slot, b, 4 * l);
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      if (!(flags & (1u << s))) continue;
+      const std::uint32_t out = (s == 0 ? root.left : root.right) - d.E;
+      side[s] = MaxScanQuad<K>(stage[t][2 * s], stage[t][2 * s + 1], l,
+                               buf.compress_choice + (static_cast<std::uint64_t>(out) * d.B + b) * K * K);
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      stage[t][0][4 * l + i] = side[0].v[i] + root_mid.v[i];
+      stage[t][1][4 * l + i] = side[1].v[i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  const std::uint64_t out_at = static_cast<std::uint64_t>(root.out - d.E) * d.B + b;
+  const Quad<double> q = MaxScanQuad<K>(stage[t][0], stage[t][1], l, buf.compress_choice + out_at * K * K);
+  StoreQuad(buf.paths + out_at * K * K + 4 * l, q);
+}
+
 // src/inference.cc:641-649
 __global__ void MaxFinish(DeviceOps ops, Dims d, MaxBuffers buf, double *log_weight, std::uint32_t *assignment) {
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1602,6 +1702,16 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
       Check(hipGetLastError(), "max-product level launch");
       return;
     }
+    if (primitive == kMaxPiecePrimitive) {
+      if (l.dims.K == 8)
+        MaxPieceTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else if (l.dims.K == 16)
+        MaxPieceTeam<16><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      else
+        MaxPieceTeam<32><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      Check(hipGetLastError(), "max-product level launch");
+      return;
+    }
     if (primitive == 3) {
       if (l.dims.K == 8)
         MaxCompressTeam<8><<<BlocksFor(lanes, 256), 256, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
@@ -1612,6 +1722,11 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
       Check(hipGetLastError(), "max-product level launch");
       return;
     }
+  }
+  // Pieces are built only where the team kernel takes every piece level.
+  if (primitive == kMaxPiecePrimitive) {
+    Check(hipErrorInvalidValue, "max-product pieces outside the team kernels");
+    return;
   }
   const unsigned block = GroupBlock(G);
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);

@@ -47,6 +47,17 @@ constexpr std::uint32_t kChainPrimitive = 6;
 constexpr std::uint32_t kSmallChainPrimitive = 7;
 constexpr std::uint32_t kPairChainPrimitive = 8;
 
+// Max-product compression pieces: a root compression and those of its two
+// input slots that are compressions absorbed into it (flag bit 0: the left
+// input, the compression left - E; bit 1: the right). An absorbed
+// compression's path lives in registers only; its choices are stored as
+// usual, so the MAP backtrack is unchanged. Level primitive 9 runs a range
+// of pieces.
+struct DeviceMaxPiece {
+  std::uint32_t root, flags;
+};
+constexpr std::uint32_t kMaxPiecePrimitive = 9;
+
 // A dispatch's grid size is a 32-bit count of work-items. Kernels that step
 // through their arrays by the grid's size take at most 2^20 blocks;
 // level launches split their op range so that each dispatch stays below
@@ -75,6 +86,7 @@ struct DeviceOps {
   const DeviceCompress *compresses;
   const std::uint32_t *roots;
   const DeviceChain *chains;
+  const DeviceMaxPiece *max_pieces;
 };
 
 struct Dims {
