@@ -33,6 +33,7 @@
 
 #include "device_common.h"
 #include "launch.h"
+#include "log_f32.h"
 
 namespace thmm {
 namespace {
@@ -770,11 +771,32 @@ __global__ void Convert(const In *in, Out *out, std::uint64_t count) {
 // Max-product (FP64 log domain)
 // ------------------------------------------------------------------------
 
+// Log factors (src/inference.cc:95-98: the log of a positive potential,
+// else -inf): FP64 potentials through the device library's log, FP32 ones
+// through LogOfFloat (log_f32.h) with its reduction table staged in LDS.
+// Every MAP kernel logs an input through these, so a value's log has the
+// same bits whichever kernel takes it.
+__device__ const LogF32Entry kLogTableDevice[128] = THMM_LOG_F32_TABLE;
+
+// Copies the reduction table into the block's LDS; every thread of the
+// block calls it (before any early return).
+__device__ inline void StageLogTable(LogF32Entry *table) {
+  for (unsigned i = threadIdx.x; i < 128; i += blockDim.x) table[i] = kLogTableDevice[i];
+  __syncthreads();
+}
+
+__device__ inline double LogFactor(double x, const LogF32Entry *) { return x > 0.0 ? log(x) : kNegInf; }
+__device__ inline double LogFactor(float x, const LogF32Entry *table) {
+  return x > 0.0f ? LogOfFloat(x, table) : kNegInf;
+}
+
 // Logs of public [B][N][K] node potentials into internal [N][B][K], and of
 // the edges (src/inference.cc:95-98).
 template <typename In>
 __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
                         const std::uint8_t *skip) {
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
   const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
   for (std::uint64_t tid = GridThread(); tid < node_values + edge_values; tid += GridStride()) {
@@ -782,12 +804,10 @@ __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, d
       const std::uint64_t s = tid % d.K, nb = tid / d.K;
       const std::uint64_t b = nb % d.B, n = nb / d.B;
       if (skip != nullptr && skip[n] != 0) continue;
-      const double x = static_cast<double>(nodes_in[(b * d.N + n) * d.K + s]);
-      nodes[tid] = x > 0.0 ? log(x) : kNegInf;
+      nodes[tid] = LogFactor(nodes_in[(b * d.N + n) * d.K + s], table);
     } else {
       const std::uint64_t i = tid - node_values;
-      const double x = static_cast<double>(edges_in[i]);
-      edges[i] = x > 0.0 ? log(x) : kNegInf;
+      edges[i] = LogFactor(edges_in[i], table);
     }
   }
 }
@@ -798,6 +818,8 @@ __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, d
 template <typename In>
 __global__ void MaxInitQuad(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
                             const std::uint8_t *skip) {
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const std::uint64_t node_quads = static_cast<std::uint64_t>(d.N) * d.B * d.K / 4;
   const std::uint64_t edge_quads = static_cast<std::uint64_t>(d.E) * d.K * d.K / 4;
   const std::uint32_t kq = d.K / 4;
@@ -817,10 +839,7 @@ __global__ void MaxInitQuad(const In *nodes_in, const In *edges_in, double *node
     }
     Quad<double> y;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const double v = static_cast<double>(x.v[r]);
-      y.v[r] = v > 0.0 ? log(v) : kNegInf;
-    }
+    for (int r = 0; r < 4; ++r) y.v[r] = LogFactor(x.v[r], table);
     StoreQuad(out, y);
   }
 }
@@ -906,23 +925,22 @@ __global__ void MaxCompress(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
 // lane owns a quad of four consecutive columns of one row, loads and stores
 // move whole quads, and choices are stored four at a time.
 
-__device__ inline double LogFactor(double x) { return x > 0.0 ? log(x) : kNegInf; }
 
 // Four consecutive log factors of a slot from element `at` (a multiple of
 // 4): edge slots from the raw potentials when given, logged here.
 __device__ inline Quad<double> MaxQuad(const MaxBuffers &buf, const Dims &d, std::uint32_t slot, std::uint32_t b,
-                                       std::uint64_t at) {
+                                       std::uint64_t at, const LogF32Entry *table) {
   if (slot < d.E && buf.raw_edges != nullptr) {
     const std::uint64_t i = static_cast<std::uint64_t>(slot) * d.K * d.K + at;
     Quad<double> q;
     if (buf.raw_edges_f32) {
       const Quad<float> f = LoadQuad(static_cast<const float *>(buf.raw_edges) + i);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(static_cast<double>(f.v[r]));
+      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(f.v[r], table);
     } else {
       const Quad<double> f = LoadQuad(static_cast<const double *>(buf.raw_edges) + i);
 #pragma unroll
-      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(f.v[r]);
+      for (int r = 0; r < 4; ++r) q.v[r] = LogFactor(f.v[r], table);
     }
     return q;
   }
@@ -1017,6 +1035,8 @@ template <int K>
 __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
                                                    MaxBuffers buf) {
   constexpr int kQ = K / 4, kLanes = K * kQ;
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::uint64_t item = tid / kLanes;
   const int l = static_cast<int>(tid % kLanes);
@@ -1052,7 +1072,10 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
     // Ascending columns: the strict '>' keeps the lower index on a tie.
     for (unsigned mask = ScreenRow<kQ>(ScreenLeaf<kQ>(v), f); mask != 0; mask &= mask - 1) {
       const int c = __builtin_ctz(mask);
-      const double candidate = LogFactor(f[c]) + v.v[c];
+      // An FP32 factor is logged as the float it is (an exact round trip).
+      const double log_f =
+          buf.raw_edges_f32 ? LogFactor(static_cast<float>(f[c]), table) : LogFactor(f[c], table);
+      const double candidate = log_f + v.v[c];
       if (candidate > best) best = candidate, arg = c0 + c;
     }
   } else {
@@ -1097,6 +1120,8 @@ __global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::
                                                     MaxBuffers buf) {
   constexpr int kQ = K / 4, kT = K <= 16 ? 16 : 64, kRps = 4 * kT / K, kJ = K / kRps;
   static_assert(kJ <= kQ && kRps * kQ == kT, "row steps");
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const std::uint64_t tid = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const std::uint64_t item = tid / kT;
   const int l = static_cast<int>(tid % kT);
@@ -1129,7 +1154,7 @@ __global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::
                          (f[0] == __builtin_huge_valf() && v.v[0] == kNegInf);
         for (unsigned mask = ScreenRow<kQ>(screen, f); mask != 0; mask &= mask - 1) {
           const int c = __builtin_ctz(mask);
-          const double candidate = LogFactor(static_cast<double>(f[c])) + v.v[c];
+          const double candidate = LogFactor(f[c], table) + v.v[c];
           if (candidate > best[j]) best[j] = candidate, arg[j] = c0 + c;
         }
       }
@@ -1147,7 +1172,7 @@ __global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::
                          (f[0] == __builtin_huge_val() && v.v[0] == kNegInf);
         for (unsigned mask = ScreenRow<kQ>(screen, f); mask != 0; mask &= mask - 1) {
           const int c = __builtin_ctz(mask);
-          const double candidate = LogFactor(f[c]) + v.v[c];
+          const double candidate = LogFactor(f[c], table) + v.v[c];
           if (candidate > best[j]) best[j] = candidate, arg[j] = c0 + c;
         }
       }
@@ -1209,6 +1234,8 @@ __global__ __launch_bounds__(256) void MaxCompressTeam(DeviceOps ops, Dims d, st
   constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
   __shared__ double left_mid[kItems][K * K];  // L[p][m] + mid[m]
   __shared__ double right[kItems][K * K];
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
   const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
   const bool active = item < static_cast<std::uint64_t>(count) * d.B;
@@ -1218,7 +1245,7 @@ __global__ __launch_bounds__(256) void MaxCompressTeam(DeviceOps ops, Dims d, st
     b = static_cast<std::uint32_t>(item % d.B);
     op = ops.compresses[offset + item / d.B];
     const double *mid = buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K;
-    const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l), rq = MaxQuad(buf, d, op.right, b, 4 * l);
+    const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l, table), rq = MaxQuad(buf, d, op.right, b, 4 * l, table);
     const Quad<double> mq = LoadQuad(mid + (4 * l) % K);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -1297,6 +1324,8 @@ __global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::
   // [0], [1]: L + mid and R of the left input compression; [2], [3]: those
   // of the right one; the root restages into [0] and [1].
   __shared__ double stage[kItems][4][K * K];
+  __shared__ LogF32Entry table[128];
+  StageLogTable(table);
   const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
   const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
   const bool active = item < static_cast<std::uint64_t>(count) * d.B;
@@ -1314,7 +1343,8 @@ __global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::
       const std::uint32_t slot = s == 0 ? root.left : root.right;
       if (flags & (1u << s)) {
         const DeviceCompress op = ops.compresses[slot - d.E];
-        const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l), rq = MaxQuad(buf, d, op.right, b, 4 * l);
+        const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l, table),
+                           rq = MaxQuad(buf, d, op.right, b, 4 * l, table);
         const Quad<double> mq =
             LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K + (4 * l) % K);
 #pragma unroll
@@ -1323,7 +1353,7 @@ __global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::
           stage[t][2 * s + 1][4 * l + i] = rq.v[i];
         }
       } else {
-        side[s] = MaxQuad(buf, d, slot, b, 4 * l);
+        side[s] = MaxQuad(buf, d, slot, b, 4 * l, table);
       }
     }
   }
