@@ -1310,7 +1310,7 @@ __device__ inline Quad<double> MaxScanQuad(const double *left_mid, const double 
 
 // Max-product compression pieces (DeviceMaxPiece), K*K/4 lanes per (piece,
 // item) as in MaxCompressTeam: the absorbed input compressions are evaluated
-// first, both at once, their paths kept in the lanes' registers (lane l's
+// first, one after the other, their paths kept in the lanes' registers (lane l's
 // output quad is the quad it stages of an operand), then the root. Every
 // compression is the scan of MaxCompressTeam on the same FP64 values, so
 // paths and choices are bit-identical to the level-by-level form; only the
@@ -1321,9 +1321,10 @@ template <int K>
 __global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
                                                     MaxBuffers buf) {
   constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
-  // [0], [1]: L + mid and R of the left input compression; [2], [3]: those
-  // of the right one; the root restages into [0] and [1].
-  __shared__ double stage[kItems][4][K * K];
+  // L + mid and R of one compression at a time: the absorbed inputs in
+  // turn, then the root (half the LDS of staging both inputs at once, so
+  // more wavefronts fit per SIMD).
+  __shared__ double stage[kItems][2][K * K];
   __shared__ LogF32Entry table[128];
   StageLogTable(table);
   const int t = threadIdx.x / kLanes, l = threadIdx.x % kLanes;
@@ -1337,38 +1338,36 @@ __global__ __launch_bounds__(256) void MaxPieceTeam(DeviceOps ops, Dims d, std::
     const DeviceMaxPiece piece = ops.max_pieces[offset + item / d.B];
     flags = piece.flags;
     root = ops.compresses[piece.root];
-    root_mid = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(root.middle) * d.B + b) * K + (4 * l) % K);
+  }
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const std::uint32_t slot = s == 0 ? root.left : root.right;
-      if (flags & (1u << s)) {
-        const DeviceCompress op = ops.compresses[slot - d.E];
-        const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l, table),
-                           rq = MaxQuad(buf, d, op.right, b, 4 * l, table);
-        const Quad<double> mq =
-            LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K + (4 * l) % K);
+  for (int s = 0; s < 2; ++s) {
+    const bool absorbed = active && (flags & (1u << s)) != 0;
+    const std::uint32_t slot = s == 0 ? root.left : root.right;
+    if (absorbed) {
+      const DeviceCompress op = ops.compresses[slot - d.E];
+      const Quad<double> lq = MaxQuad(buf, d, op.left, b, 4 * l, table),
+                         rq = MaxQuad(buf, d, op.right, b, 4 * l, table);
+      const Quad<double> mq =
+          LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.middle) * d.B + b) * K + (4 * l) % K);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          stage[t][2 * s][4 * l + i] = lq.v[i] + mq.v[i];
-          stage[t][2 * s + 1][4 * l + i] = rq.v[i];
-        }
-      } else {
-        side[s] = MaxQuad(buf, d, slot, b, 4 * l, table);
+      for (int i = 0; i < 4; ++i) {
+        stage[t][0][4 * l + i] = lq.v[i] + mq.v[i];
+        stage[t][1][4 * l + i] = rq.v[i];
       }
     }
+    __syncthreads();
+    if (absorbed)
+      side[s] = MaxScanQuad<K>(stage[t][0], stage[t][1], l,
+                               buf.compress_choice + (static_cast<std::uint64_t>(slot - d.E) * d.B + b) * K * K);
+    __syncthreads();
   }
-  __syncthreads();
   if (active) {
+    // The root's slot operands and middle are loaded here, after the
+    // absorbed inputs, which keeps the kernel at 72 registers.
+    root_mid = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(root.middle) * d.B + b) * K + (4 * l) % K);
 #pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      if (!(flags & (1u << s))) continue;
-      const std::uint32_t out = (s == 0 ? root.left : root.right) - d.E;
-      side[s] = MaxScanQuad<K>(stage[t][2 * s], stage[t][2 * s + 1], l,
-                               buf.compress_choice + (static_cast<std::uint64_t>(out) * d.B + b) * K * K);
-    }
-  }
-  __syncthreads();
-  if (active) {
+    for (int s = 0; s < 2; ++s)
+      if (!(flags & (1u << s))) side[s] = MaxQuad(buf, d, s == 0 ? root.left : root.right, b, 4 * l, table);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       stage[t][0][4 * l + i] = side[0].v[i] + root_mid.v[i];
