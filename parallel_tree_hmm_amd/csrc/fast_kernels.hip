@@ -341,12 +341,34 @@ __global__ __launch_bounds__(256) void RakeReverseTeam(DeviceOps ops, Dims d, st
 // either folds the sum into the parent or leaves it as a partial.
 
 // Adds the FP64 absolute log vector `acc` (rows owned by lanes with row >= 0)
-// into node vector `at` and re-shifts it by its maximum.
-template <typename T, int K, int kT>
-__device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int row, double acc, int lane) {
+// into node vector `at` (node `parent`, item `b`) and re-shifts it by its
+// maximum. kRaw: the parent absorbs for the first time in round 0 and the
+// node initialisation skipped it; its vector is formed here from the caller's
+// linear potentials [B][N][K] (buf.raw_leaves) as TransposeTeam (kMode 1,
+// same element type) forms it: w = log x, top = the maximum over the vector
+// (exact, so the reduction order does not matter), w - top in the element
+// type and top as the FP64 scale.
+template <typename T, int K, int kT, bool kRaw = false>
+__device__ inline void AbsorbSum(const LinearBuffers &buf, const Dims &d, std::uint32_t parent, std::uint32_t b,
+                                 int row, double acc, int lane) {
+  const std::uint64_t at = static_cast<std::uint64_t>(parent) * d.B + b;
   T *x = static_cast<T *>(buf.nodes) + at * K;
-  const double scale = buf.node_scale[at];
-  const double y = row >= 0 ? static_cast<double>(x[row]) + scale + acc : kNegInf;
+  T value;
+  double scale;
+  if constexpr (kRaw) {
+    const T *raw = static_cast<const T *>(buf.raw_leaves) + (static_cast<std::uint64_t>(b) * d.N + parent) * K;
+    const T w = row >= 0 ? LogOf(raw[row]) : NegInf<T>();
+    T vector_top = w;
+#pragma unroll
+    for (int o = kT / 2; o > 0; o >>= 1) vector_top = Max(vector_top, __shfl_xor(vector_top, o, kT));
+    const bool support = vector_top > NegInf<T>();
+    value = support ? static_cast<T>(w - vector_top) : NegInf<T>();
+    scale = support ? static_cast<double>(vector_top) : kNegInf;
+  } else {
+    value = row >= 0 ? x[row] : T(0);
+    scale = buf.node_scale[at];
+  }
+  const double y = row >= 0 ? static_cast<double>(value) + scale + acc : kNegInf;
   double top = y;
 #pragma unroll
   for (int o = kT / 2; o > 0; o >>= 1) top = fmax(top, __shfl_xor(top, o, kT));
@@ -357,8 +379,9 @@ __device__ inline void AbsorbSum(const LinearBuffers &buf, std::uint64_t at, int
 
 // kLogEdges: buf.log_edges may be set (the variant without it holds fewer
 // registers). FP32 edge slots of an FP64 pass (buf.edges_f32) are widened
-// as they load, which is exact. kRaw: the leaves are original leaves read
-// from buf.raw_leaves.
+// as they load, which is exact. kRaw: round 0's chunks; the leaves are
+// original leaves and the direct chunks' parents absorb for the first time,
+// both read from buf.raw_leaves.
 template <typename T, int K, bool kLogEdges, bool kRaw = false>
 __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dims &d, std::uint32_t offset,
                                                    std::uint32_t count, const LinearBuffers &buf, std::uint32_t tid) {
@@ -431,7 +454,7 @@ __device__ __forceinline__ void RakeGatherTeamBody(const GatherOps &g, const Dim
   const double ordered = RowToLane<K>(acc, l);
   const int lane_row = l < K ? l : -1;
   if (chunk.partial == kDirectChunk)
-    AbsorbSum<T, K, kT>(buf, static_cast<std::uint64_t>(chunk.parent) * d.B + b, lane_row, ordered, l);
+    AbsorbSum<T, K, kT, kRaw>(buf, d, chunk.parent, b, lane_row, ordered, l);
   else if (l < K)
     g.partials[(static_cast<std::uint64_t>(chunk.partial) * d.B + b) * K + l] = ordered;
 }
@@ -443,8 +466,9 @@ __global__ __launch_bounds__(256) void RakeGatherTeam(GatherOps g, Dims d, std::
 }
 
 // Sums a run of partials (K lanes, lane = row) into a partial of the next
-// reduction level or, at the last level, into the parent.
-template <typename T, int K>
+// reduction level or, at the last level, into the parent. kRaw: round 0's
+// reduction levels, whose parents the node initialisation skipped.
+template <typename T, int K, bool kRaw = false>
 __device__ __forceinline__ void GatherSegmentsTeamBody(const GatherOps &g, const Dims &d, std::uint32_t offset,
                                                        std::uint32_t count, const LinearBuffers &buf,
                                                        std::uint32_t tid) {
@@ -457,15 +481,15 @@ __device__ __forceinline__ void GatherSegmentsTeamBody(const GatherOps &g, const
   for (std::uint32_t i = 0; i < seg.count; ++i)
     acc += g.partials[(static_cast<std::uint64_t>(seg.first_partial + i) * d.B + b) * K + row];
   if (seg.out == kDirectChunk)
-    AbsorbSum<T, K, K>(buf, static_cast<std::uint64_t>(seg.parent) * d.B + b, row, acc, row);
+    AbsorbSum<T, K, K, kRaw>(buf, d, seg.parent, b, row, acc, row);
   else
     g.partials[(static_cast<std::uint64_t>(seg.out) * d.B + b) * K + row] = acc;
 }
 
-template <typename T, int K>
+template <typename T, int K, bool kRaw = false>
 __global__ __launch_bounds__(256) void GatherSegmentsTeam(GatherOps g, Dims d, std::uint32_t offset,
                                                           std::uint32_t count, LinearBuffers buf) {
-  GatherSegmentsTeamBody<T, K>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
+  GatherSegmentsTeamBody<T, K, kRaw>(g, d, offset, count, buf, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1765,7 +1789,13 @@ void LaunchGatherSegments(const Launch &l, const GatherOps &g, std::uint32_t off
   const std::uint32_t K = l.dims.K;
   const unsigned grid = Blocks(static_cast<std::uint64_t>(count) * l.dims.B * K);
   const hipStream_t s = StreamOf(l);
-#define THMM_SEGMENTS(T, KK) GatherSegmentsTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf)
+#define THMM_SEGMENTS(T, KK)                                                                  \
+  do {                                                                                        \
+    if (buf.raw_leaves != nullptr)                                                            \
+      GatherSegmentsTeam<T, KK, true><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf); \
+    else                                                                                      \
+      GatherSegmentsTeam<T, KK><<<grid, kBlock, 0, s>>>(g, l.dims, offset, count, buf);       \
+  } while (0)
   if (l.real == Real::kF32) {
     if (K == 8) THMM_SEGMENTS(float, 8);
     if (K == 16) THMM_SEGMENTS(float, 16);

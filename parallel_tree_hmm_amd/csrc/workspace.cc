@@ -920,6 +920,16 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
       leaves = !has_child[plan.rakes[r].leaf];
       skip[plan.rakes[r].leaf] = 1;
     }
+    // Round 0's absorbing parents are skipped too: with raw leaves the
+    // fused program's round 0 (its gather and partial-sum levels) forms
+    // their vectors from the caller's potentials when it first absorbs into
+    // them, and nothing reads them before.
+    for (std::size_t i = 0; leaves && i < plan.batches.size(); ++i) {
+      const thmm_primitive_batch &b = plan.batches[i];
+      if (i > 0 && b.primitive < plan.batches[i - 1].primitive) break;  // round 1 begins
+      if (b.primitive == THMM_BRANCH_ABSORPTION)
+        for (std::uint32_t j = b.offset; j < b.offset + b.count; ++j) skip[plan.absorptions[j].parent] = 1;
+    }
     if (leaves) {
       Upload(w.leaf_skip, skip, w.stream, "topology upload");
       w.round0_rakes = b0.count;
@@ -1235,13 +1245,17 @@ bool UsesFusedForward(const Impl &w, const LinearBuffers &buf) {
 // The sum-product forward pass: the fused program when the workspace has
 // one, else the plan's levels.
 // `raw_leaves`: the caller's linear node potentials for round 0's gather
-// (the first step), whose leaves the node initialisation skipped.
+// (the first step) and its partial-sum levels (the steps of kind 1 that
+// follow it), whose leaves and absorbing parents the node initialisation
+// skipped.
 void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, const Launch &l,
                    const LinearBuffers &buf, Profiler *profiler = nullptr, const void *raw_leaves = nullptr) {
   const GatherOps g = w.Gather();
   LinearBuffers raw = buf;
   raw.raw_leaves = raw_leaves;
+  bool round0 = true;
   for (const Impl::FusedStep &step : steps) {
+    round0 = round0 && (&step == &steps.front() || step.kind == 1);
     // Profiled as primitive 4 (fused rake chunks, count = rakes), 5 (partial
     // absorptions, count = partials), a compression level, or a grouped
     // compression level (kChainPrimitive, count = K-vectors moved per item).
@@ -1249,9 +1263,9 @@ void RunFusedSteps(const Impl &w, const std::vector<Impl::FusedStep> &steps, con
       profiler->Begin(1, step.kind == 3 || step.kind == kChainPrimitive ? step.kind : 4 + step.kind, step.work,
                       step.count);
     if (step.kind == 0)
-      LaunchGatherChunks(l, g, step.offset, step.count, &step == &steps.front() ? raw : buf);
+      LaunchGatherChunks(l, g, step.offset, step.count, round0 ? raw : buf);
     else if (step.kind == 1)
-      LaunchGatherSegments(l, g, step.offset, step.count, buf);
+      LaunchGatherSegments(l, g, step.offset, step.count, round0 ? raw : buf);
     else
       LaunchForwardLevel(l, step.kind, step.offset, step.count, buf);
     if (profiler) profiler->End();

@@ -1,14 +1,18 @@
 """Dense leaf rakes reading the caller's linear potentials (thmm_tuning
 input_leaves = 1, the default): round 0's rakes on the fused forward program
 (K = 8, 16, 32, 64) form their leaf vectors from the caller's node
-potentials and the node initialisation skips those leaves. Every case
+potentials, round 0's absorptions (in the gather's direct chunks and in its
+partial-sum levels) form their parents' vectors from them too, and the node
+initialisation skips those leaves and parents. Every case
 compares the default with input_leaves = 0 (every node initialised first)
 on the same workspace and inputs, bit for bit: thmm_marginals_device (log Z,
 node and edge marginals, eager and captured-graph replay) and the
 host-staged marginal and partition calls. Forests mix
 stars (many leaves of one parent), paths (no leaves but the last), recursive
 trees and single nodes; zero potentials, and leaves without support (device
-calls); batch 1 and 3; FP32 and FP64. A node array off 16-byte alignment
+calls) and a round-0 parent without support; a star of 600 leaves whose
+partials take two reduction levels; batch 1 and 3; FP32 and FP64. A node
+array off 16-byte alignment
 takes the initialise-first path. MAP's log transform of the node potentials
 (quad kernel for aligned inputs, one-value kernel otherwise) gives the same
 states and weights either way."""
@@ -27,7 +31,7 @@ pytestmark = pytest.mark.gpu
 def case(k, precision, batch, seed):
     rng = np.random.default_rng(seed)
     parents = forest([random_recursive(300, rng), star(70), path(40), np.array([-1]), random_recursive(90, rng),
-                      np.array([-1, 0])])
+                      np.array([-1, 0]), star(600)])
     plan = thmm.Plan(parents)
     dtype = np.float32 if precision == "f32" else np.float64
     nodes = np.exp(rng.normal(size=(batch, plan.num_nodes, k)) * 2.0)
@@ -67,6 +71,7 @@ def test_input_leaves_equal_initialised_leaves(require_gpu, k, precision, batch)
     dead = nodes.copy()
     leaves = np.setdiff1d(plan.edge_children, plan.edge_parents)
     dead[0, leaves[:3], :] = 0.0
+    dead[0, 300, :] = 0.0  # the centre of star(70), absorbed in round 0
     results = {}
     for flag in (1, 0):
         out = {}
@@ -126,10 +131,11 @@ def test_unaligned_inputs_fall_back(require_gpu):
         outs.append((lz.cpu().numpy(), no.cpu().numpy(), eo.cpu().numpy()))
     ws.close()
     # The unaligned initialisation takes FP64 logs where the team transpose
-    # takes FP32 ones, so the two agree to FP32 rounding, not bit for bit.
+    # takes FP32 ones, so the two agree to FP32 rounding, not bit for bit
+    # (the centre of the 600-leaf star sums 600 such messages).
     assert np.allclose(outs[0][0], outs[1][0], rtol=1e-6, atol=1e-6)
-    assert np.max(np.abs(outs[0][1] - outs[1][1])) <= 1e-6
-    assert np.max(np.abs(outs[0][2] - outs[1][2])) <= 1e-6
+    assert np.max(np.abs(outs[0][1] - outs[1][1])) <= 4e-6
+    assert np.max(np.abs(outs[0][2] - outs[1][2])) <= 4e-6
 
 
 @pytest.mark.parametrize("precision", ["f32", "f64"])
