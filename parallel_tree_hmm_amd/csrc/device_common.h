@@ -23,9 +23,10 @@ bool LaunchFastLevel(const Launch &l, bool reverse, std::uint32_t primitive, std
 // Transposes K-vectors [X][Y][K] -> [Y][X][K] on the coalesced team kernel
 // (mode 0 copy, 1 linear -> shifted logs, 2 logs -> shifted logs, 3 logs ->
 // max-shifted exponentials, the latter three writing FP64 scales); returns
-// false when (K, types, shape) are left to the generic kernels.
+// false when (K, types, shape) are left to the generic kernels. With `only`
+// (mode 1) just the vectors y listed there are moved, for every x.
 bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
-                         std::uint32_t X, std::uint32_t Y, double *scale, const std::uint8_t *skip = nullptr);
+                         std::uint32_t X, std::uint32_t Y, double *scale, const NodeList *only = nullptr);
 
 namespace dev {
 

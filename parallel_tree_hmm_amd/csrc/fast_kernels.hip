@@ -1558,43 +1558,62 @@ __global__ __launch_bounds__(64 * kWideWaves) void CompressReverseMfmaWide(Devic
 // kMode 2 log potentials, both then shifted by the vector's maximum, which
 // goes to `scale` in FP64 (-inf for a vector without support); kMode 3 turns
 // log potentials into exp(v - max) with the same FP64 scale (edge rows).
+// Lane l of a team moves input vector v to output vector `to`.
+template <typename In, typename Out, int kMode, int K>
+__device__ inline void TransposeVector(const In *in, Out *out, std::uint64_t v, std::uint64_t to, int l,
+                                       double *scale) {
+  constexpr int kLanes = K / 4;
+  const Quad<In> q = LoadQuad(in + v * K + l * 4);
+  // Logs are taken in the wider of the two element types.
+  using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
+  Quad<Out> o;
+  if constexpr (kMode == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o.v[i] = static_cast<Out>(q.v[i]);
+  } else {
+    W w[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = kMode == 1 ? LogOf(static_cast<W>(q.v[i])) : static_cast<W>(q.v[i]);
+    W top = Max(Max(w[0], w[1]), Max(w[2], w[3]));
+#pragma unroll
+    for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
+    const bool live = top > NegInf<W>();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (kMode == 3)
+        o.v[i] = live ? ExpNonPositive(static_cast<Out>(w[i] - top)) : Out(0);
+      else
+        o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
+    }
+    if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
+  }
+  StoreQuad(out + to * K + l * 4, o);
+}
+
 template <typename In, typename Out, int kMode, int K>
 __global__ __launch_bounds__(256) void TransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
-                                                     double *scale, const std::uint8_t *skip) {
+                                                     double *scale) {
   constexpr int kLanes = K / 4;
   // Grid-stride: the stride is a whole number of teams, so a team's lanes
-  // stay together (and skip together: skip is indexed by y).
+  // stay together.
   for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(X) * Y * kLanes; tid += GridStride()) {
     const std::uint64_t v = tid / kLanes;
-    const int l = static_cast<int>(tid % kLanes);
     const std::uint64_t x = v / Y, y = v % Y;
-    if (skip != nullptr && skip[y] != 0) continue;
-    const std::uint64_t to = y * X + x;
-    const Quad<In> q = LoadQuad(in + v * K + l * 4);
-    // Logs are taken in the wider of the two element types.
-    using W = typename std::conditional<sizeof(In) >= sizeof(Out), In, Out>::type;
-    Quad<Out> o;
-    if constexpr (kMode == 0) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) o.v[i] = static_cast<Out>(q.v[i]);
-    } else {
-      W w[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) w[i] = kMode == 1 ? LogOf(static_cast<W>(q.v[i])) : static_cast<W>(q.v[i]);
-      W top = Max(Max(w[0], w[1]), Max(w[2], w[3]));
-#pragma unroll
-      for (int s = kLanes / 2; s > 0; s >>= 1) top = Max(top, __shfl_xor(top, s, kLanes));
-      const bool live = top > NegInf<W>();
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if constexpr (kMode == 3)
-          o.v[i] = live ? ExpNonPositive(static_cast<Out>(w[i] - top)) : Out(0);
-        else
-          o.v[i] = live ? static_cast<Out>(w[i] - top) : NegInf<Out>();
-      }
-      if (l == 0) scale[to] = live ? static_cast<double>(top) : kNegInf;
-    }
-    StoreQuad(out + to * K + l * 4, o);
+    TransposeVector<In, Out, kMode, K>(in, out, v, y * X + x, static_cast<int>(tid % kLanes), scale);
+  }
+}
+
+// The same for the vectors y = ids[i] (i < count) only, for every x.
+template <typename In, typename Out, int kMode, int K>
+__global__ __launch_bounds__(256) void ListedTransposeTeam(const In *in, Out *out, std::uint32_t X, std::uint32_t Y,
+                                                           const std::uint32_t *ids, std::uint32_t count,
+                                                           double *scale) {
+  constexpr int kLanes = K / 4;
+  for (std::uint64_t tid = GridThread(); tid < static_cast<std::uint64_t>(count) * X * kLanes; tid += GridStride()) {
+    const std::uint64_t team = tid / kLanes;
+    const std::uint64_t i = team / X, x = team % X;
+    const std::uint64_t y = ids[i];
+    TransposeVector<In, Out, kMode, K>(in, out, x * Y + y, y * X + x, static_cast<int>(tid % kLanes), scale);
   }
 }
 
@@ -1928,19 +1947,30 @@ namespace {
 
 template <typename In, typename Out, int kMode>
 bool TransposeTeamFor(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale,
-                      const std::uint8_t *skip = nullptr) {
-  const std::uint64_t threads = static_cast<std::uint64_t>(X) * Y * (l.dims.K / 4);
-  const unsigned grid = GridStrideBlocks(threads, kBlock);
+                      const NodeList *only = nullptr) {
+  const std::uint64_t vectors = only != nullptr ? static_cast<std::uint64_t>(only->count) * X
+                                                : static_cast<std::uint64_t>(X) * Y;
+  if (vectors == 0) return true;
+  const unsigned grid = GridStrideBlocks(vectors * (l.dims.K / 4), kBlock);
   const auto *i = static_cast<const In *>(in);
   auto *o = static_cast<Out *>(out);
+  const hipStream_t s = StreamOf(l);
+#define THMM_TRANSPOSE(KK)                                                                                        \
+  do {                                                                                                            \
+    if (only != nullptr)                                                                                          \
+      ListedTransposeTeam<In, Out, kMode, KK><<<grid, kBlock, 0, s>>>(i, o, X, Y, only->ids, only->count, scale); \
+    else                                                                                                          \
+      TransposeTeam<In, Out, kMode, KK><<<grid, kBlock, 0, s>>>(i, o, X, Y, scale);                               \
+  } while (0)
   switch (l.dims.K) {
-    case 4: TransposeTeam<In, Out, kMode, 4><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
-    case 8: TransposeTeam<In, Out, kMode, 8><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
-    case 16: TransposeTeam<In, Out, kMode, 16><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
-    case 32: TransposeTeam<In, Out, kMode, 32><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
-    case 64: TransposeTeam<In, Out, kMode, 64><<<grid, kBlock, 0, StreamOf(l)>>>(i, o, X, Y, scale, skip); break;
+    case 4: THMM_TRANSPOSE(4); break;
+    case 8: THMM_TRANSPOSE(8); break;
+    case 16: THMM_TRANSPOSE(16); break;
+    case 32: THMM_TRANSPOSE(32); break;
+    case 64: THMM_TRANSPOSE(64); break;
     default: return false;
   }
+#undef THMM_TRANSPOSE
   RecordLaunchError(hipGetLastError(), "vector transpose launch");
   return true;
 }
@@ -2010,23 +2040,23 @@ bool FastTransposeFits(std::uint32_t K, std::uint32_t X, std::uint32_t Y) {
 }
 
 bool LaunchFastTranspose(const Launch &l, int mode, Real in_real, Real out_real, const void *in, void *out,
-                         std::uint32_t X, std::uint32_t Y, double *scale, const std::uint8_t *skip) {
+                         std::uint32_t X, std::uint32_t Y, double *scale, const NodeList *only) {
   const std::uint32_t K = l.dims.K;
   if (!FastTransposeFits(K, X, Y)) return false;
   if (!Aligned16(in) || !Aligned16(out)) return false;
   if (static_cast<std::uint64_t>(X) * Y == 0) return true;
   const bool f_in = in_real == Real::kF32, f_out = out_real == Real::kF32;
-  if (skip != nullptr && mode != 1) return false;
+  if (only != nullptr && mode != 1) return false;
   switch (mode) {
     case 0:
       if (f_in != f_out) return false;
       return f_in ? TransposeTeamFor<float, float, 0>(l, in, out, X, Y, scale)
                   : TransposeTeamFor<double, double, 0>(l, in, out, X, Y, scale);
     case 1:
-      if (f_in && f_out) return TransposeTeamFor<float, float, 1>(l, in, out, X, Y, scale, skip);
-      if (f_in) return TransposeTeamFor<float, double, 1>(l, in, out, X, Y, scale, skip);
-      if (f_out) return TransposeTeamFor<double, float, 1>(l, in, out, X, Y, scale, skip);
-      return TransposeTeamFor<double, double, 1>(l, in, out, X, Y, scale, skip);
+      if (f_in && f_out) return TransposeTeamFor<float, float, 1>(l, in, out, X, Y, scale, only);
+      if (f_in) return TransposeTeamFor<float, double, 1>(l, in, out, X, Y, scale, only);
+      if (f_out) return TransposeTeamFor<double, float, 1>(l, in, out, X, Y, scale, only);
+      return TransposeTeamFor<double, double, 1>(l, in, out, X, Y, scale, only);
     case 2:
       if (f_in != f_out) return false;
       return f_in ? TransposeTeamFor<float, float, 2>(l, in, out, X, Y, scale)

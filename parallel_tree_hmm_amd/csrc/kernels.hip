@@ -1783,12 +1783,12 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
 
 template <typename In, typename Out, int kMode>
 void LaunchTranspose(const Launch &l, const void *in, void *out, std::uint32_t X, std::uint32_t Y, double *scale,
-                     const std::uint8_t *skip = nullptr) {
+                     const NodeList *only = nullptr) {
   constexpr Real in_real = sizeof(In) == 4 ? Real::kF32 : Real::kF64;
   constexpr Real out_real = sizeof(Out) == 4 ? Real::kF32 : Real::kF64;
-  if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale, skip)) return;
-  if (skip != nullptr) {
-    RecordLaunchError(hipErrorInvalidValue, "skipped nodes need the team transpose");
+  if (LaunchFastTranspose(l, kMode, in_real, out_real, in, out, X, Y, scale, only)) return;
+  if (only != nullptr) {
+    RecordLaunchError(hipErrorInvalidValue, "a partial node initialisation needs the team transpose");
     return;
   }
   const std::uint32_t ty = 64 / l.dims.K;
@@ -1816,18 +1816,18 @@ const char *LastLaunchError() {
 }
 
 void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes, double *node_scale,
-                     const std::uint8_t *skip) {
+                     const NodeList *only) {
   const Dims &d = l.dims;
   if (l.real == Real::kF32) {
     if (in_real == Real::kF32)
-      LaunchTranspose<float, float, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
+      LaunchTranspose<float, float, 1>(l, input, nodes, d.B, d.N, node_scale, only);
     else
-      LaunchTranspose<double, float, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
+      LaunchTranspose<double, float, 1>(l, input, nodes, d.B, d.N, node_scale, only);
   } else {
     if (in_real == Real::kF32)
-      LaunchTranspose<float, double, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
+      LaunchTranspose<float, double, 1>(l, input, nodes, d.B, d.N, node_scale, only);
     else
-      LaunchTranspose<double, double, 1>(l, input, nodes, d.B, d.N, node_scale, skip);
+      LaunchTranspose<double, double, 1>(l, input, nodes, d.B, d.N, node_scale, only);
   }
   Check(hipGetLastError(), "node initialisation launch");
 }

@@ -211,10 +211,15 @@ struct Launch {
 // Linear potentials, public [B][N][K] -> internal [N][B][K] max-shifted
 // logs + FP64 scales. `in_real` is the element type of `input`; `nodes`
 // has element type launch.real.
-// Nodes with skip[n] set (null: none) are left unwritten; skipping needs the
-// team transpose (FastTransposeFits and 16-byte aligned buffers).
+// With `only` (null: every node) just the listed nodes are written, for
+// every item; a list needs the team transpose (FastTransposeFits and 16-byte
+// aligned buffers).
+struct NodeList {
+  const std::uint32_t *ids;
+  std::uint32_t count;
+};
 void LaunchInitNodes(const Launch &l, Real in_real, const void *input, void *nodes,
-                     double *node_scale, const std::uint8_t *skip = nullptr);
+                     double *node_scale, const NodeList *only = nullptr);
 // Log potentials [B][N][K] -> max-shifted logs + FP64 scales.
 void LaunchInitNodesFromLogs(const Launch &l, const void *log_input, void *nodes,
                              double *node_scale);

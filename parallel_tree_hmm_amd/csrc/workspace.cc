@@ -138,13 +138,15 @@ struct Workspace::Impl {
   // sampling forward, which has no grouped compressions.
   std::vector<FusedStep> fused_plain;
   // Raw leaves: round 0's rakes (the first round0_rakes of the plan, every
-  // leaf an original leaf) read the caller's potentials and the node
-  // initialisation skips their leaves (leaf_skip [N]); 0: not available
-  // (round 0 does not start with rakes). round0_gather: the fused forward
-  // program's first step holds exactly those rakes.
+  // leaf an original leaf) read the caller's potentials, as do round 0's
+  // absorptions for their parents, and the node initialisation writes only
+  // the other nodes (init_nodes, init_count of them, ascending); 0: not
+  // available (round 0 does not start with rakes). round0_gather: the fused
+  // forward program's first step holds exactly those rakes.
   std::uint32_t round0_rakes = 0;
   bool round0_gather = false;
-  DeviceBuffer leaf_skip;
+  DeviceBuffer init_nodes;
+  std::uint32_t init_count = 0;
   DeviceBuffer gather_rakes, gather_chunks, gather_segments, gather_partials;
   // Grouped compressions (K = 16 FP32 with the fused forward): the level of
   // each plan batch that holds compressions (count 0: none rooted there),
@@ -275,7 +277,7 @@ struct Workspace::Impl {
     persistent_failure.Release();
     tuning = Tuning{};
     for (DeviceBuffer *b : {&rakes, &combines, &absorbs, &compresses, &roots, &is_root, &observation_index,
-                            &cat_skip, &site_sums, &leaf_skip, &input_nodes, &uniforms, &observations,
+                            &cat_skip, &site_sums, &init_nodes, &input_nodes, &uniforms, &observations,
                             &observations_t,
                             &root_potential, &emissions, &edges,
                             &edge_scale, &nodes, &node_scale, &paths, &path_scale, &branches, &branch_scale,
@@ -306,6 +308,7 @@ struct Workspace::Impl {
     cat_rakes = 0;
     round0_rakes = 0;
     round0_gather = false;
+    init_count = 0;
     observation_nodes.clear();
     resident_observation_batch = 0;
     resident_factors = false;
@@ -334,6 +337,8 @@ struct Workspace::Impl {
   std::size_t KK() const { return states * states; }
   std::size_t S() const { return RealSize(real); }
 
+  // The nodes a raw-leaf call initialises.
+  NodeList InitList() const { return {init_nodes.as<std::uint32_t>(), init_count}; }
   GatherOps Gather() const {
     return {gather_rakes.as<DeviceRake>(), gather_chunks.as<DeviceGatherChunk>(),
             gather_segments.as<DeviceGatherSegment>(), gather_partials.as<double>()};
@@ -931,7 +936,11 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
         for (std::uint32_t j = b.offset; j < b.offset + b.count; ++j) skip[plan.absorptions[j].parent] = 1;
     }
     if (leaves) {
-      Upload(w.leaf_skip, skip, w.stream, "topology upload");
+      std::vector<std::uint32_t> init;
+      for (std::size_t n = 0; n < plan.num_nodes; ++n)
+        if (skip[n] == 0) init.push_back(static_cast<std::uint32_t>(n));
+      Upload(w.init_nodes, init, w.stream, "topology upload");
+      w.init_count = static_cast<std::uint32_t>(init.size());
       w.round0_rakes = b0.count;
       w.round0_gather = !w.fused.empty() && w.fused.front().kind == 0 && w.fused.front().work == b0.count;
     }
@@ -1446,9 +1455,9 @@ bool ElementLogs(const Impl &w, const Launch &l) {
 // `fused_leaves` a categorical call leaves the leaves of the workspace's
 // categorical rakes unwritten (their rake kernels form them).
 void InitState(Impl &w, const Source &src, const Launch &l, void *nodes, double *scale,
-               const void *dense_nodes = nullptr, bool fused_leaves = false, const std::uint8_t *dense_skip = nullptr) {
+               const void *dense_nodes = nullptr, bool fused_leaves = false, const NodeList *dense_only = nullptr) {
   if (src.categorical == nullptr) {
-    LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale, dense_skip);
+    LaunchInitNodes(l, w.real, dense_nodes != nullptr ? dense_nodes : w.input_nodes.ptr, nodes, scale, dense_only);
   } else {
     LaunchInitCategorical(l, w.real, false, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
@@ -1519,8 +1528,9 @@ void RunPartition(Impl &w, std::size_t batch, const Source &src) {
   const Launch l = w.MakeLaunch(w.real, batch, w.stream);
   const LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
   const void *raw = StagedRawLeaves(w, l, src, buf, false);
+  const NodeList init = w.InitList();
   InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true,
-            raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+            raw != nullptr ? &init : nullptr);
   SumProductForward(w, l, buf, nullptr, raw);
   LaunchFinish(l, buf, w.log_z.as<double>());
   CheckLaunches();
@@ -1531,8 +1541,9 @@ void RunMarginals(Impl &w, std::size_t batch, const Source &src) {
   LinearBuffers buf = CallLinear(w, l, src.categorical != nullptr);
   AdjointsInto(buf, batch, w.node_out.ptr);
   const void *raw = StagedRawLeaves(w, l, src, buf, true);
+  const NodeList init = w.InitList();
   InitState(w, src, l, w.nodes.ptr, w.node_scale.as<double>(), nullptr, true,
-            raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+            raw != nullptr ? &init : nullptr);
   SumProductForward(w, l, buf, nullptr, raw);
   LaunchFinish(l, buf, w.log_z.as<double>());
   LaunchSeedAdjoint(l, buf, nullptr);
@@ -2220,8 +2231,8 @@ void EnqueueMarginals(Impl &w, hipStream_t on, const void *nodes, const void *ed
   AdjointsInto(buf, batch, node_out);
   const void *raw = RawLeaves(w, l, nodes, buf, true) ? nodes : nullptr;
   if (p) p->Begin(0, 0, 0);
-  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>(),
-                  raw != nullptr ? w.leaf_skip.as<std::uint8_t>() : nullptr);
+  const NodeList init = w.InitList();
+  LaunchInitNodes(l, w.real, nodes, w.nodes.ptr, w.node_scale.as<double>(), raw != nullptr ? &init : nullptr);
   if (p) p->End();
   SumProductForward(w, l, buf, p, raw);
   if (p) p->Begin(2, 0, 0);
