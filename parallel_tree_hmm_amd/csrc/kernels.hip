@@ -791,20 +791,21 @@ __device__ inline double LogFactor(float x, const LogF32Entry *table) {
 }
 
 // Logs of public [B][N][K] node potentials into internal [N][B][K], and of
-// the edges (src/inference.cc:95-98).
+// the edges (src/inference.cc:95-98). With `ids`, only nodes ids[0 ..
+// listed) are written.
 template <typename In>
 __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
-                        const std::uint8_t *skip) {
+                        const std::uint32_t *ids, std::uint32_t listed) {
   __shared__ LogF32Entry table[128];
   StageLogTable(table);
-  const std::uint64_t node_values = static_cast<std::uint64_t>(d.N) * d.B * d.K;
+  const std::uint64_t node_values = static_cast<std::uint64_t>(ids != nullptr ? listed : d.N) * d.B * d.K;
   const std::uint64_t edge_values = static_cast<std::uint64_t>(d.E) * d.K * d.K;
   for (std::uint64_t tid = GridThread(); tid < node_values + edge_values; tid += GridStride()) {
     if (tid < node_values) {
       const std::uint64_t s = tid % d.K, nb = tid / d.K;
-      const std::uint64_t b = nb % d.B, n = nb / d.B;
-      if (skip != nullptr && skip[n] != 0) continue;
-      nodes[tid] = LogFactor(nodes_in[(b * d.N + n) * d.K + s], table);
+      const std::uint64_t b = nb % d.B, i = nb / d.B;
+      const std::uint64_t n = ids != nullptr ? ids[i] : i;
+      nodes[(n * d.B + b) * d.K + s] = LogFactor(nodes_in[(b * d.N + n) * d.K + s], table);
     } else {
       const std::uint64_t i = tid - node_values;
       edges[i] = LogFactor(edges_in[i], table);
@@ -817,10 +818,10 @@ __global__ void MaxInit(const In *nodes_in, const In *edges_in, double *nodes, d
 // x > 0 ? log x : -inf per value.
 template <typename In>
 __global__ void MaxInitQuad(const In *nodes_in, const In *edges_in, double *nodes, double *edges, Dims d,
-                            const std::uint8_t *skip) {
+                            const std::uint32_t *ids, std::uint32_t listed) {
   __shared__ LogF32Entry table[128];
   StageLogTable(table);
-  const std::uint64_t node_quads = static_cast<std::uint64_t>(d.N) * d.B * d.K / 4;
+  const std::uint64_t node_quads = static_cast<std::uint64_t>(ids != nullptr ? listed : d.N) * d.B * d.K / 4;
   const std::uint64_t edge_quads = static_cast<std::uint64_t>(d.E) * d.K * d.K / 4;
   const std::uint32_t kq = d.K / 4;
   for (std::uint64_t tid = GridThread(); tid < node_quads + edge_quads; tid += GridStride()) {
@@ -828,10 +829,10 @@ __global__ void MaxInitQuad(const In *nodes_in, const In *edges_in, double *node
     double *out;
     if (tid < node_quads) {
       const std::uint64_t q = tid % kq, nb = tid / kq;
-      const std::uint64_t b = nb % d.B, n = nb / d.B;
-      if (skip != nullptr && skip[n] != 0) continue;
+      const std::uint64_t b = nb % d.B, i = nb / d.B;
+      const std::uint64_t n = ids != nullptr ? ids[i] : i;
       x = LoadQuad(nodes_in + (b * d.N + n) * d.K + 4 * q);
-      out = nodes + 4 * tid;
+      out = nodes + (n * d.B + b) * d.K + 4 * q;
     } else {
       const std::uint64_t i = tid - node_quads;
       x = LoadQuad(edges_in + 4 * i);
@@ -882,14 +883,24 @@ __global__ void MaxCombine(DeviceOps ops, Dims d, std::uint32_t offset, std::uin
       buf.branches[(static_cast<std::uint64_t>(op.source) * d.B + b) * d.K + lane.s];
 }
 
+// buf.raw_leaves: round 0's absorption, the parent's first write, whose log
+// potential is taken here (the reduction table read from global memory).
 template <int G>
 __global__ void MaxAbsorb(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, MaxBuffers buf) {
   const Lane lane = LaneOf<G>();
   if (lane.item >= static_cast<std::uint64_t>(count) * d.B || lane.s >= d.K) return;
   const std::uint32_t b = static_cast<std::uint32_t>(lane.item % d.B);
   const DeviceAbsorb op = ops.absorbs[offset + lane.item / d.B];
-  buf.nodes[(static_cast<std::uint64_t>(op.parent) * d.B + b) * d.K + lane.s] +=
-      buf.branches[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
+  double *x = buf.nodes + (static_cast<std::uint64_t>(op.parent) * d.B + b) * d.K + lane.s;
+  double base;
+  if (buf.raw_leaves != nullptr) {
+    const std::uint64_t at = (static_cast<std::uint64_t>(b) * d.N + op.parent) * d.K + lane.s;
+    base = buf.raw_leaves_f32 ? LogFactor(static_cast<const float *>(buf.raw_leaves)[at], kLogTableDevice)
+                              : LogFactor(static_cast<const double *>(buf.raw_leaves)[at], nullptr);
+  } else {
+    base = *x;
+  }
+  *x = base + buf.branches[(static_cast<std::uint64_t>(op.branch) * d.B + b) * d.K + lane.s];
 }
 
 // src/inference.cc:614-639: candidate (L + mid) + R, strict '>'.
@@ -945,6 +956,26 @@ __device__ inline Quad<double> MaxQuad(const MaxBuffers &buf, const Dims &d, std
     return q;
   }
   return LoadQuad(MaxSlot(buf, d, slot, b) + at);
+}
+
+// The leaf's log quad at columns c0 .. c0 + 3 for a rake team of kT lanes
+// whose rows take kQ lanes each: from the node state, or (buf.raw_leaves,
+// the first rake level) logged here from the caller's potentials as MaxInit
+// logs them. The team's rows repeat the same quads, so lane l logs value
+// (l / kQ) % 4 of its quad and takes the other three from the lanes that
+// logged them.
+template <int K, int kQ, int kT>
+__device__ inline Quad<double> MaxLeafQuad(const MaxBuffers &buf, const Dims &d, std::uint32_t leaf, std::uint32_t b,
+                                           int c0, int l, const LogF32Entry *table) {
+  static_assert(kT / kQ >= 4, "four rows per team");
+  if (buf.raw_leaves == nullptr) return LoadQuad(buf.nodes + (static_cast<std::uint64_t>(leaf) * d.B + b) * K + c0);
+  const std::uint64_t at = (static_cast<std::uint64_t>(b) * d.N + leaf) * K + c0 + (l / kQ) % 4;
+  const double mine = buf.raw_leaves_f32 ? LogFactor(static_cast<const float *>(buf.raw_leaves)[at], table)
+                                         : LogFactor(static_cast<const double *>(buf.raw_leaves)[at], table);
+  Quad<double> v;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) v.v[i] = __shfl(mine, l % kQ + kQ * i, kT);
+  return v;
 }
 
 // Combines two (value, index) argmax partials of one row: the larger value,
@@ -1044,7 +1075,7 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
   const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
   const DeviceRake op = ops.rakes[offset + item / d.B];
   const int p = l / kQ, c0 = (l % kQ) * 4;
-  const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
+  const Quad<double> v = MaxLeafQuad<K, kQ, kLanes>(buf, d, op.leaf, b, c0, l, table);
   double best = kNegInf;  // the strict '>' scan from -inf: a row of -inf keeps its first column
   std::uint32_t arg = c0;
   // The sequential scan starts at column 0's candidate (src/inference.cc:
@@ -1129,7 +1160,7 @@ __global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::
   const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
   const DeviceRake op = ops.rakes[offset + item / d.B];
   const int p0 = l / kQ, c0 = (l % kQ) * 4;
-  const Quad<double> v = LoadQuad(buf.nodes + (static_cast<std::uint64_t>(op.leaf) * d.B + b) * K + c0);
+  const Quad<double> v = MaxLeafQuad<K, kQ, kT>(buf, d, op.leaf, b, c0, l, table);
   double best[kJ];
   std::uint32_t arg[kJ];
   bool nan_first[kJ];
@@ -1752,9 +1783,14 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
       return;
     }
   }
-  // Pieces are built only where the team kernel takes every piece level.
+  // Pieces are built only where the team kernel takes every piece level,
+  // raw leaves only where it takes the first rake level.
   if (primitive == kMaxPiecePrimitive) {
     Check(hipErrorInvalidValue, "max-product pieces outside the team kernels");
+    return;
+  }
+  if (primitive == 0 && buf.raw_leaves != nullptr) {
+    Check(hipErrorInvalidValue, "raw leaf potentials outside the team rake kernels");
     return;
   }
   const unsigned block = GroupBlock(G);
@@ -2189,20 +2225,23 @@ void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t cou
 }
 
 void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in, double *nodes,
-                   double *edges, const std::uint8_t *skip) {
-  const std::uint64_t total = static_cast<std::uint64_t>(l.dims.N) * l.dims.B * l.dims.K +
+                   double *edges, const NodeList *only) {
+  const std::uint32_t *ids = only != nullptr ? only->ids : nullptr;
+  const std::uint32_t listed = only != nullptr ? only->count : 0;
+  const std::uint64_t total = static_cast<std::uint64_t>(only != nullptr ? listed : l.dims.N) * l.dims.B * l.dims.K +
                               static_cast<std::uint64_t>(l.dims.E) * l.dims.K * l.dims.K;
+  if (total == 0) return;
   const auto aligned = [](const void *p) { return (reinterpret_cast<std::uintptr_t>(p) & 15u) == 0; };
   if (l.dims.K % 4 == 0 && aligned(nodes_in) && aligned(nodes) && (l.dims.E == 0 || (aligned(edges_in) && aligned(edges)))) {
     const unsigned grid = GridStrideBlocks(total / 4, kThreads);
     if (in_real == Real::kF32)
       MaxInitQuad<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
                                                             static_cast<const float *>(edges_in), nodes, edges, l.dims,
-                                                            skip);
+                                                            ids, listed);
     else
       MaxInitQuad<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(nodes_in),
                                                              static_cast<const double *>(edges_in), nodes, edges,
-                                                             l.dims, skip);
+                                                             l.dims, ids, listed);
     Check(hipGetLastError(), "max-product initialisation launch");
     return;
   }
@@ -2210,11 +2249,11 @@ void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const vo
   if (in_real == Real::kF32)
     MaxInit<float><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const float *>(nodes_in),
                                                       static_cast<const float *>(edges_in), nodes, edges, l.dims,
-                                                      skip);
+                                                      ids, listed);
   else
     MaxInit<double><<<grid, kThreads, 0, StreamOf(l)>>>(static_cast<const double *>(nodes_in),
                                                        static_cast<const double *>(edges_in), nodes, edges, l.dims,
-                                                       skip);
+                                                       ids, listed);
   Check(hipGetLastError(), "max-product initialisation launch");
 }
 

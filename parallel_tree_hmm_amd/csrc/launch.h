@@ -182,6 +182,13 @@ struct MaxBuffers {
   // log transform into `edges` does); null when edges come from `edges`.
   const void *raw_edges;
   int raw_edges_f32;
+  // Node potentials [B][N][K] in the workspace precision (16-byte aligned)
+  // for round 0's first rake level and absorption levels, whose leaves and
+  // parents MaxInit left unwritten: the team rake kernels log the leaf
+  // values on load, and the absorption logs the parent's (the same
+  // x > 0 ? log x : -inf); null when leaves and parents come from `nodes`.
+  const void *raw_leaves;
+  int raw_leaves_f32;
 };
 
 enum class Real { kF32 = 0, kF64 = 1 };
@@ -363,10 +370,11 @@ void LaunchWidenF32(const Launch &l, const float *in, double *out, std::size_t c
 void LaunchCastF64(const Launch &l, const double *in, void *out, std::size_t count);
 
 // ---- max-product (FP64) ----
-// Nodes with skip[n] set (null: none) are left unwritten.
+// With `only` (null: every node) just the listed nodes are written, for
+// every item.
 void LaunchMaxInit(const Launch &l, Real in_real, const void *nodes_in, const void *edges_in,
                    double *nodes, double *edges,
-                   const std::uint8_t *skip = nullptr);
+                   const NodeList *only = nullptr);
 void LaunchMaxLevel(const Launch &l, std::uint32_t primitive, std::uint32_t offset,
                     std::uint32_t count, const MaxBuffers &buf);
 void LaunchMaxFinish(const Launch &l, const MaxBuffers &buf, double *log_weight,

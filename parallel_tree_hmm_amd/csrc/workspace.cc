@@ -374,7 +374,7 @@ struct Workspace::Impl {
 
   MaxBuffers Max() const {
     return {m_nodes.as<double>(), m_edges.as<double>(), m_paths.as<double>(), m_branches.as<double>(),
-            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>(), nullptr, 0};
+            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>(), nullptr, 0, nullptr, 0};
   }
 };
 
@@ -926,14 +926,18 @@ void ReserveCommon(Impl &w, const Plan &plan, const void *identity, std::size_t 
       skip[plan.rakes[r].leaf] = 1;
     }
     // Round 0's absorbing parents are skipped too: with raw leaves the
-    // fused program's round 0 (its gather and partial-sum levels) forms
-    // their vectors from the caller's potentials when it first absorbs into
-    // them, and nothing reads them before.
+    // fused program's round 0 (its gather and partial-sum levels) and MAP's
+    // round-0 absorption levels form their vectors from the caller's
+    // potentials when they first absorb into them, and nothing reads them
+    // before. Each absorbs once in round 0 (else no raw leaves).
     for (std::size_t i = 0; leaves && i < plan.batches.size(); ++i) {
       const thmm_primitive_batch &b = plan.batches[i];
       if (i > 0 && b.primitive < plan.batches[i - 1].primitive) break;  // round 1 begins
       if (b.primitive == THMM_BRANCH_ABSORPTION)
-        for (std::uint32_t j = b.offset; j < b.offset + b.count; ++j) skip[plan.absorptions[j].parent] = 1;
+        for (std::uint32_t j = b.offset; leaves && j < b.offset + b.count; ++j) {
+          leaves = skip[plan.absorptions[j].parent] == 0;
+          skip[plan.absorptions[j].parent] = 1;
+        }
     }
     if (leaves) {
       std::vector<std::uint32_t> init;
@@ -1577,12 +1581,25 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
                              (reinterpret_cast<std::uintptr_t>(raw_edges) & 15u) == 0;
   Launch init = l;
   if (edges_on_load) init.dims.E = 0;
+  const bool pieces = !w.max_levels.empty() && w.tuning.compress_depth >= 2;
+  const std::vector<Level> &levels = pieces ? w.max_levels : w.levels;
+  // Round 0's leaves are logged on load by the team rake kernels of the
+  // first level (dense 16-byte aligned inputs at K = 8, 16 and 32, where that
+  // level stays on the team kernels; each lane logs one value of its quad),
+  // and round 0's parents by its absorption levels; MaxInit logs the other
+  // nodes.
+  const bool leaves_on_load =
+      src.categorical == nullptr && w.round0_rakes != 0 && (reinterpret_cast<std::uintptr_t>(call.nodes) & 15u) == 0 &&
+      !levels.empty() && levels.front().primitive == THMM_RAKE && levels.front().offset == 0 &&
+      levels.front().count == w.round0_rakes &&
+      (w.states == 8 || w.states == 16 ||
+       (w.states == 32 && static_cast<std::uint64_t>(w.round0_rakes) * batch * 32 * 32 / 4 < kMaxLaunchItems));
+  const NodeList inner = w.InitList();
   // Logs of the raw staged factors (bit-exact with the oracle's log of the
-  // same values), internal layout. The leaves are logged here too: a rake
-  // kernel logging its leaf on load would repeat each log across the K
-  // rows of its K*K/4 lanes.
+  // same values), internal layout.
   if (src.categorical == nullptr) {
-    LaunchMaxInit(init, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>());
+    LaunchMaxInit(init, w.real, call.nodes, call.edges, w.m_nodes.as<double>(), w.m_edges.as<double>(),
+                  leaves_on_load ? &inner : nullptr);
   } else {
     LaunchInitCategorical(l, w.real, true, w.observations_t.as<std::uint8_t>(), w.observation_index.as<std::uint32_t>(),
                           static_cast<std::uint32_t>(src.categorical->num_observation_nodes),
@@ -1600,9 +1617,17 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
     buf.raw_edges = raw_edges;
     buf.raw_edges_f32 = w.real == Real::kF32 ? 1 : 0;
   }
-  const bool pieces = !w.max_levels.empty() && w.tuning.compress_depth >= 2;
-  for (const Level &level : pieces ? w.max_levels : w.levels)
-    LaunchMaxLevel(l, level.primitive, level.offset, level.count, buf);
+  MaxBuffers raw = buf;
+  raw.raw_leaves = call.nodes;
+  raw.raw_leaves_f32 = w.real == Real::kF32 ? 1 : 0;
+  bool round0 = true;
+  for (std::size_t i = 0; i < levels.size(); ++i) {
+    const Level &level = levels[i];
+    round0 = round0 && (i == 0 || level.primitive >= levels[i - 1].primitive);
+    const bool from_raw = leaves_on_load && round0 &&
+                          (level.primitive == THMM_RAKE ? i == 0 : level.primitive == THMM_BRANCH_ABSORPTION);
+    LaunchMaxLevel(l, level.primitive, level.offset, level.count, from_raw ? raw : buf);
+  }
   LaunchMaxFinish(l, buf, call.log_z, call.assignment);
   for (std::size_t i = w.levels.size(); i-- > 0;)
     LaunchMaxExpandLevel(l, w.levels[i].primitive, w.levels[i].offset, w.levels[i].count, buf, call.assignment);
