@@ -549,7 +549,11 @@ thmm_status thmm_path_storage(const thmm_workspace *workspace, size_t *groups, s
  * 64, no fused forward program: the observations, emissions and root
  * potential), and dense partition and marginal calls on linear potentials
  * with the fused forward program (K = 8, 16, 32, 64; 16-byte aligned
- * buffers: round 0's rakes read the caller's potentials). Both settings
+ * buffers: round 0's rakes read the caller's potentials, and so do round
+ * 0's absorptions for their parents, which the initialisation also skips).
+ * Dense MAP calls at K = 8, 16 and 32 (16-byte aligned node potentials)
+ * likewise log round 0's leaves and parents where they first read them
+ * (1), or every node first (0). Both settings
  * give the same values bit for bit; a change drops a held forward pass.
  * Values outside these sets are invalid_argument. Setting them drops
  * captured graphs; a new gather_chunk or compress_depth rebuilds the fused

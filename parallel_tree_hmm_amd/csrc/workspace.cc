@@ -1589,7 +1589,8 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
   // and round 0's parents by its absorption levels; MaxInit logs the other
   // nodes.
   const bool leaves_on_load =
-      src.categorical == nullptr && w.round0_rakes != 0 && (reinterpret_cast<std::uintptr_t>(call.nodes) & 15u) == 0 &&
+      src.categorical == nullptr && w.tuning.input_leaves != 0 && w.round0_rakes != 0 &&
+      (reinterpret_cast<std::uintptr_t>(call.nodes) & 15u) == 0 &&
       !levels.empty() && levels.front().primitive == THMM_RAKE && levels.front().offset == 0 &&
       levels.front().count == w.round0_rakes &&
       (w.states == 8 || w.states == 16 ||

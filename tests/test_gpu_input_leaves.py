@@ -12,10 +12,10 @@ stars (many leaves of one parent), paths (no leaves but the last), recursive
 trees and single nodes; zero potentials, and leaves without support (device
 calls) and a round-0 parent without support; a star of 600 leaves whose
 partials take two reduction levels; batch 1 and 3; FP32 and FP64. A node
-array off 16-byte alignment
-takes the initialise-first path. MAP's log transform of the node potentials
-(quad kernel for aligned inputs, one-value kernel otherwise) gives the same
-states and weights either way."""
+array off 16-byte alignment takes the initialise-first path. MAP (K = 8,
+16, 32) logs round 0's leaves and parents where it first reads them under
+the same flag, and gives the same states and weights as logging every node
+first (quad kernel for aligned inputs, one-value kernel otherwise)."""
 
 from __future__ import annotations
 
@@ -142,37 +142,57 @@ def test_unaligned_inputs_fall_back(require_gpu):
 @pytest.mark.parametrize("k", [8, 16, 32])
 @pytest.mark.parametrize("batch", [1, 3])
 def test_map_log_transform_aligned_and_unaligned(require_gpu, k, precision, batch):
-    """MAP's log transform of the node potentials: 16-byte aligned inputs take
-    the quad kernel, others the one-value kernel; both log each value in
-    FP64 the same way, so states and log weights are equal bit for bit, and
-    the states equal the oracle's. (MAP keeps the separate log transform for
-    every node: a rake kernel logging its leaf on load would repeat each log
-    across the K rows of its lanes.)"""
+    """MAP's log transform of the node potentials: with 16-byte aligned
+    inputs and input_leaves = 1 (the default) round 0's leaves are logged by
+    the first rake level as it loads them and round 0's parents by their
+    absorption level, and the quad kernel logs the other nodes from a list;
+    with input_leaves = 0 the quad kernel logs every node, and unaligned
+    inputs take the one-value kernel for every node. All log each value in
+    FP64 the same way, so states and log weights are equal bit for bit, also
+    with a leaf and a round-0 parent without support, and the states equal
+    the oracle's."""
     import torch
 
     import oracle
 
     plan, nodes, edges = case(k, precision, batch, 7 * k + batch)
     n = plan.num_nodes
+    dead = nodes.copy()
+    leaves = np.setdiff1d(plan.edge_children, plan.edge_parents)
+    dead[0, leaves[:2], :] = 0.0
+    dead[0, 300, :] = 0.0  # the centre of star(70), absorbed in round 0
     dtype = torch.float32 if precision == "f32" else torch.float64
-    aligned = torch.from_numpy(nodes).cuda()
-    storage = torch.empty(nodes.size + 1, dtype=dtype, device="cuda")
-    shifted = storage[1:]
-    shifted.copy_(aligned.reshape(-1))
     de = torch.from_numpy(edges).cuda()
+    for inputs in (nodes, dead):
+        aligned = torch.from_numpy(inputs).cuda()
+        storage = torch.empty(inputs.size + 1, dtype=dtype, device="cuda")
+        shifted = storage[1:]
+        shifted.copy_(aligned.reshape(-1))
+        results = []
+        for dn, flag in ((aligned, 1), (aligned, 0), (shifted, 1)):
+            ws = thmm.Workspace()
+            ws.reserve(plan, k, batch, precision=precision,
+                       flags=thmm._abi.THMM_RESERVE_MAXIMUM | thmm._abi.THMM_RESERVE_DEVICE_RESIDENT)
+            ws.set_tuning(input_leaves=flag)
+            lw = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+            states = torch.empty(batch * n, dtype=torch.int32, device="cuda")
+            ws.maximum_device(dn.data_ptr(), de.data_ptr(), batch, lw.data_ptr(), states.data_ptr(),
+                              torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            results.append((lw.cpu().numpy(), states.cpu().numpy()))
+            ws.close()
+        for other in results[1:]:
+            for a, b in zip(results[0], other):
+                assert same(a, b), f"K={k} {precision} batch {batch}: MAP differs"
+    assert np.isneginf(results[0][0]).any()
+    lw0, expected = oracle.maximum(plan, k, nodes[0].astype(np.float64), edges.astype(np.float64))
     ws = thmm.Workspace()
     ws.reserve(plan, k, batch, precision=precision,
                flags=thmm._abi.THMM_RESERVE_MAXIMUM | thmm._abi.THMM_RESERVE_DEVICE_RESIDENT)
-    results = []
-    for dn in (aligned, shifted):
-        lw = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
-        states = torch.empty(batch * n, dtype=torch.int32, device="cuda")
-        ws.maximum_device(dn.data_ptr(), de.data_ptr(), batch, lw.data_ptr(), states.data_ptr(),
-                          torch.cuda.current_stream().cuda_stream)
-        torch.cuda.synchronize()
-        results.append((lw.cpu().numpy(), states.cpu().numpy()))
+    lw = torch.empty(batch * plan.roots.size, dtype=torch.float64, device="cuda")
+    states = torch.empty(batch * n, dtype=torch.int32, device="cuda")
+    ws.maximum_device(torch.from_numpy(nodes).cuda().data_ptr(), de.data_ptr(), batch, lw.data_ptr(),
+                      states.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
     ws.close()
-    for a, b in zip(results[0], results[1]):
-        assert same(a, b), f"K={k} {precision} batch {batch}: MAP differs"
-    lw0, expected = oracle.maximum(plan, k, nodes[0].astype(np.float64), edges.astype(np.float64))
-    assert np.array_equal(results[0][1].reshape(batch, n)[0].astype(np.uint32), expected)
+    assert np.array_equal(states.cpu().numpy().reshape(batch, n)[0].astype(np.uint32), expected)
