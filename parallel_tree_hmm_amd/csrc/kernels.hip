@@ -314,13 +314,21 @@ __global__ __launch_bounds__(256) void CompressForwardTeam(DeviceOps ops, Dims d
 // item), lane (p, quad q) owning entries 4q..4q+3 of row p. L diag(e), L, R
 // and the ratios Gr are staged in LDS; then lane (p, q) forms row p of S and
 // Gr on its quad, adj_L[p][4q..] and, as lane (m = p, q), adj_R[m][4q..].
-// The arithmetic is CompressReverse's, in the same order and element types
-// (FP64 ratios and sums), so the results are bitwise those of the generic
-// kernel; the middle adjoint sums each row of adj_R sequentially in LDS.
+// The FP64 form is CompressReverse's arithmetic, in the same order and
+// element types (FP64 ratios and sums), so its results are bitwise those of
+// the generic kernel; the middle adjoint sums each row of adj_R
+// sequentially in LDS.
+// FP32 (T = float): the ratios are taken with the hardware reciprocal and
+// the sums kept in FP32 when, for every lane of the block, each S entry is
+// 0 or at least 2^-100 and each path adjoint below 2^26 in magnitude (so a
+// ratio stays below 2^126) and every FP32 result is finite; otherwise
+// (block-uniform) the block takes the FP64 form. The FP32 values share the
+// FP64 arrays' LDS.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void CompressReverseTeam(DeviceOps ops, Dims d, std::uint32_t offset,
                                                            std::uint32_t count, LinearBuffers buf) {
   constexpr int kQ = K / 4, kLanes = K * kQ, kItems = 256 / kLanes;
+  constexpr bool kFloat = std::is_same<T, float>::value;
   __shared__ double weights[kItems][K];
   __shared__ T left[kItems][K * K];   // L[p][m] e[m]
   __shared__ T lraw[kItems][K * K];   // L
@@ -331,6 +339,8 @@ __global__ __launch_bounds__(256) void CompressReverseTeam(DeviceOps ops, Dims d
   const int p = l / kQ, c0 = (l % kQ) * 4;
   const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * kItems + t;
   const bool active = item < static_cast<std::uint64_t>(count) * d.B;
+  float *gr32 = reinterpret_cast<float *>(&gr[t][0]);
+  float *adjr32 = reinterpret_cast<float *>(&adjr[t][0]);
   DeviceCompress op{};
   std::uint32_t b = 0;
   std::uint64_t mid_at = 0, out_at = 0;
@@ -363,13 +373,76 @@ __global__ __launch_bounds__(256) void CompressReverseTeam(DeviceOps ops, Dims d
     e_row = top_w > kNegInf ? static_cast<T>(exp(weights[t][p] - top_w)) : T(0);
   }
   __syncthreads();
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+  bool fast = kFloat;
+  if constexpr (kFloat) {
+    bool ok = true;
+    if (active) {
+#pragma unroll
+      for (int m = 0; m < K; ++m) {
+        const T lm = left[t][p * K + m];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        ok = ok && !(acc[c] > 0.0f && acc[c] < 0x1p-100f) && fabsf(a.v[c]) < 0x1p26f;
+        gr32[p * K + c0 + c] =
+            (a.v[c] != 0.0f && acc[c] > 0.0f) ? a.v[c] * __builtin_amdgcn_rcpf(acc[c]) : 0.0f;
+      }
+    }
+    fast = __syncthreads_and(ok) != 0;
+    Quad<T> ql, qr;
+    float row_total = 0.0f;
+    bool finite = true;
+    if (fast && active) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int m = c0 + i;
+        float h = 0.0f;
+#pragma unroll
+        for (int c = 0; c < K; ++c) h += gr32[p * K + c] * right[t][m * K + c];
+        ql.v[i] = left[t][p * K + m] * h;
+      }
+      const int m = p;
+      float r[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int pp = 0; pp < K; ++pp) {
+        const float lp = lraw[t][pp * K + m];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) r[c] += lp * gr32[pp * K + c0 + c];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) qr.v[c] = right[t][m * K + c0 + c] * e_row * r[c];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) finite = finite && isfinite(ql.v[c]) && isfinite(qr.v[c]);
+    }
+    if (fast) fast = __syncthreads_and(finite) != 0;  // block-uniform
+    if (fast) {
+      if (active) {
+        StoreAdjointQuad<T>(SlotAdjointTarget<T>(buf, d, op.left, b), static_cast<std::uint64_t>(p) * K + c0, ql);
+        StoreAdjointQuad<T>(SlotAdjointTarget<T>(buf, d, op.right, b), static_cast<std::uint64_t>(p) * K + c0, qr);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) adjr32[p * K + c0 + c] = qr.v[c];
+      }
+      __syncthreads();
+      if (active && c0 == 0) {
+#pragma unroll
+        for (int c = 0; c < K; ++c) row_total += adjr32[p * K + c];
+        static_cast<T *>(buf.node_adj)[mid_at * K + p] = static_cast<T>(row_total);
+      }
+      return;
+    }
+  }
+  // FP64 form (every FP64 workspace; FP32 blocks outside the fast range).
   if (active) {
-    T acc[4] = {T(0), T(0), T(0), T(0)};
+    if constexpr (!kFloat) {
 #pragma unroll
-    for (int m = 0; m < K; ++m) {
-      const T lm = left[t][p * K + m];
+      for (int m = 0; m < K; ++m) {
+        const T lm = left[t][p * K + m];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
+        for (int c = 0; c < 4; ++c) acc[c] += lm * right[t][m * K + c0 + c];
+      }
     }
 #pragma unroll
     for (int c = 0; c < 4; ++c)
