@@ -1051,6 +1051,24 @@ __device__ inline Quad<double> MaxLeafQuad(const MaxBuffers &buf, const Dims &d,
   return v;
 }
 
+// The absorption of a rake whose branch is absorbed as it stands, in the
+// rake kernel: entry p of the parent's node vector plus the rake's value for
+// it (MaxAbsorb's arithmetic). With buf.raw_leaves (round 0) the parent's
+// log potential is taken here, as MaxAbsorb takes it there.
+__device__ inline void AbsorbDirect(const MaxBuffers &buf, const Dims &d, std::uint32_t parent, std::uint32_t b,
+                                    std::uint32_t K, std::uint32_t p, double value, const LogF32Entry *table) {
+  double *x = buf.nodes + (static_cast<std::uint64_t>(parent) * d.B + b) * K + p;
+  double base;
+  if (buf.raw_leaves != nullptr) {
+    const std::uint64_t at = (static_cast<std::uint64_t>(b) * d.N + parent) * K + p;
+    base = buf.raw_leaves_f32 ? LogFactor(static_cast<const float *>(buf.raw_leaves)[at], table)
+                              : LogFactor(static_cast<const double *>(buf.raw_leaves)[at], table);
+  } else {
+    base = *x;
+  }
+  *x = base + value;
+}
+
 // Combines two (value, index) argmax partials of one row: the larger value,
 // and on a tie the lower index, as an ascending strict '>' scan does.
 __device__ inline void ArgMaxMerge(double &best, std::uint32_t &arg, double other, std::uint32_t other_arg) {
@@ -1206,7 +1224,10 @@ __global__ __launch_bounds__(256) void MaxRakeTeam(DeviceOps ops, Dims d, std::u
     packed |= static_cast<std::uint32_t>(__shfl(arg, (l + i * kQ) % kLanes, kLanes)) << (8 * i);
   if (c0 == 0) {
     const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + p;
-    buf.branches[at] = best;
+    if (buf.rake_direct != nullptr && buf.rake_direct[offset + item / d.B] != 0)
+      AbsorbDirect(buf, d, op.parent, b, K, p, best, table);
+    else
+      buf.branches[at] = best;
     if (p % 4 == 0) *reinterpret_cast<std::uint32_t *>(buf.rake_choice + at) = packed;
   }
 }
@@ -1324,7 +1345,10 @@ __global__ __launch_bounds__(256) void MaxRakeSteps(DeviceOps ops, Dims d, std::
   for (int i = 1; i < 4; ++i) packed |= static_cast<std::uint32_t>(__shfl(row_arg, (l + i) % kT, kT)) << (8 * i);
   if (l < K) {
     const std::uint64_t at = (static_cast<std::uint64_t>(op.branch) * d.B + b) * K + l;
-    buf.branches[at] = row_best;
+    if (buf.rake_direct != nullptr && buf.rake_direct[offset + item / d.B] != 0)
+      AbsorbDirect(buf, d, op.parent, b, K, l, row_best, table);
+    else
+      buf.branches[at] = row_best;
     if (l % 4 == 0) *reinterpret_cast<std::uint32_t *>(buf.rake_choice + at) = packed;
   }
 }
@@ -1862,8 +1886,8 @@ void MaxLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offset, s
     Check(hipErrorInvalidValue, "max-product pieces outside the team kernels");
     return;
   }
-  if (primitive == 0 && buf.raw_leaves != nullptr) {
-    Check(hipErrorInvalidValue, "raw leaf potentials outside the team rake kernels");
+  if (primitive == 0 && (buf.raw_leaves != nullptr || buf.rake_direct != nullptr)) {
+    Check(hipErrorInvalidValue, "raw leaf potentials or direct absorptions outside the team rake kernels");
     return;
   }
   const unsigned block = GroupBlock(G);

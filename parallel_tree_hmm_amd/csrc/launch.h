@@ -189,6 +189,12 @@ struct MaxBuffers {
   // x > 0 ? log x : -inf); null when leaves and parents come from `nodes`.
   const void *raw_leaves;
   int raw_leaves_f32;
+  // Per rake of the plan: 1 when its branch is absorbed as it stands (its
+  // parent rakes no other child in that round). The team rake kernels then
+  // add the rake's values into the parent's node vector (the absorption's
+  // arithmetic) instead of writing the branch, and the absorption levels
+  // hold the other absorptions only. Null: every rake writes its branch.
+  const std::uint8_t *rake_direct;
 };
 
 enum class Real { kF32 = 0, kF64 = 1 };

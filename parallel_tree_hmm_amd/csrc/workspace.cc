@@ -159,6 +159,11 @@ struct Workspace::Impl {
   // plan's levels with each compression level replaced by the pieces rooted
   // there (empty when not built). compress_depth 1 runs the plain levels.
   DeviceBuffer max_pieces;
+  // MAP's direct absorptions (with the pieces program): per rake, whether
+  // its branch is absorbed as it stands, and the other absorptions, which
+  // the program's absorption levels index (rake_direct stays null when a
+  // rake level could leave the team kernels).
+  DeviceBuffer max_direct, max_absorbs;
   std::vector<Level> max_levels;
   // Log-potential calls: the edges that compressions read (the only ones
   // staged when the rake kernels read log edges in place), the largest rake
@@ -286,7 +291,7 @@ struct Workspace::Impl {
                             &s_branch_scale, &assignment, &failure, &m_nodes, &m_edges, &m_paths, &m_branches,
                             &rake_choice, &compress_choice, &log_weight, &chain_seed, &chain_rate, &chain_terms,
                             &chain_grad, &gather_rakes, &gather_chunks, &gather_segments, &gather_partials,
-                            &chains, &max_pieces, &expm_factors, &compress_edges})
+                            &chains, &max_pieces, &max_direct, &max_absorbs, &expm_factors, &compress_edges})
       b->Release();
     for (PinnedBuffer *b : {&host_nodes, &host_edges, &host_uniforms, &host_observations, &host_root,
                             &host_emissions, &host_values, &host_log_z, &host_assignment, &host_node_out,
@@ -374,7 +379,7 @@ struct Workspace::Impl {
 
   MaxBuffers Max() const {
     return {m_nodes.as<double>(), m_edges.as<double>(), m_paths.as<double>(), m_branches.as<double>(),
-            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>(), nullptr, 0, nullptr, 0};
+            rake_choice.as<std::uint8_t>(), compress_choice.as<std::uint8_t>(), nullptr, 0, nullptr, 0, nullptr};
   }
 };
 
@@ -787,6 +792,8 @@ void BuildPersistent(Impl &w, std::size_t batch) {
 void BuildMaxPieces(Impl &w, const Plan &plan, const std::vector<DeviceCompress> &compresses, std::size_t batch) {
   w.max_levels.clear();
   w.max_pieces.Release();
+  w.max_direct.Release();
+  w.max_absorbs.Release();
   const std::size_t K = w.states;
   if (!(w.flags & THMM_RESERVE_MAXIMUM) || (K != 8 && K != 16 && K != 32) || compresses.empty()) return;
   const std::uint32_t E = static_cast<std::uint32_t>(plan.num_edges);
@@ -805,9 +812,29 @@ void BuildMaxPieces(Impl &w, const Plan &plan, const std::vector<DeviceCompress>
       absorbed[c] = take ? 1 : 0;
     }
   }
+  // Direct absorptions: a rake whose branch no combination touches is
+  // absorbed by its own rake kernel; the absorption levels keep the others
+  // (their ops in max_absorbs). Only where every rake level stays on the
+  // team kernels.
+  std::vector<std::uint8_t> combined(plan.num_branches, 0);
+  for (const thmm_branch_combination &c : plan.combinations) combined[c.source] = combined[c.destination] = 1;
+  bool direct = true;
+  for (const thmm_primitive_batch &b : plan.batches)
+    if (b.primitive == THMM_RAKE && static_cast<std::uint64_t>(b.count) * batch * K * K / 4 >= kMaxLaunchItems)
+      direct = false;
+  std::vector<DeviceAbsorb> absorbs;
   std::vector<DeviceMaxPiece> pieces;
   std::vector<Level> levels;
   for (const thmm_primitive_batch &b : plan.batches) {
+    if (b.primitive == THMM_BRANCH_ABSORPTION && direct) {
+      const std::uint32_t first = static_cast<std::uint32_t>(absorbs.size());
+      for (std::uint32_t a = b.offset; a < b.offset + b.count; ++a)
+        if (combined[plan.absorptions[a].branch])
+          absorbs.push_back({static_cast<std::uint32_t>(plan.absorptions[a].parent), plan.absorptions[a].branch});
+      const std::uint32_t count = static_cast<std::uint32_t>(absorbs.size()) - first;
+      if (count != 0) levels.push_back({b.primitive, first, count});
+      continue;
+    }
     if (b.primitive != THMM_COMPRESSION) {
       levels.push_back({b.primitive, b.offset, b.count});
       continue;
@@ -826,6 +853,12 @@ void BuildMaxPieces(Impl &w, const Plan &plan, const std::vector<DeviceCompress>
     if (count != 0) levels.push_back({kMaxPiecePrimitive, first, count});
   }
   Upload(w.max_pieces, pieces, w.stream, "max-product piece upload");
+  if (direct) {
+    std::vector<std::uint8_t> flags(plan.rakes.size());
+    for (std::size_t r = 0; r < plan.rakes.size(); ++r) flags[r] = combined[plan.rakes[r].branch] ? 0 : 1;
+    Upload(w.max_direct, flags, w.stream, "max-product direct absorption upload");
+    Upload(w.max_absorbs, absorbs, w.stream, "max-product direct absorption upload");
+  }
   Hip(hipStreamSynchronize(w.stream), "max-product piece upload");
   w.max_levels = std::move(levels);
 }
@@ -1618,7 +1651,16 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
     buf.raw_edges = raw_edges;
     buf.raw_edges_f32 = w.real == Real::kF32 ? 1 : 0;
   }
-  MaxBuffers raw = buf;
+  // The pieces program's direct absorptions: its rake kernels absorb the
+  // branches no combination touches, and its absorption levels index the
+  // others in max_absorbs.
+  Launch program = l;
+  MaxBuffers forward = buf;
+  if (pieces && w.max_direct.ptr != nullptr) {
+    forward.rake_direct = w.max_direct.as<std::uint8_t>();
+    program.ops.absorbs = w.max_absorbs.as<DeviceAbsorb>();
+  }
+  MaxBuffers raw = forward;
   raw.raw_leaves = call.nodes;
   raw.raw_leaves_f32 = w.real == Real::kF32 ? 1 : 0;
   bool round0 = true;
@@ -1627,7 +1669,7 @@ void RunMaximum(Impl &w, std::size_t batch, const Source &src, const DenseCall &
     round0 = round0 && (i == 0 || level.primitive >= levels[i - 1].primitive);
     const bool from_raw = leaves_on_load && round0 &&
                           (level.primitive == THMM_RAKE ? i == 0 : level.primitive == THMM_BRANCH_ABSORPTION);
-    LaunchMaxLevel(l, level.primitive, level.offset, level.count, from_raw ? raw : buf);
+    LaunchMaxLevel(program, level.primitive, level.offset, level.count, from_raw ? raw : forward);
   }
   LaunchMaxFinish(l, buf, call.log_z, call.assignment);
   for (std::size_t i = w.levels.size(); i-- > 0;)
