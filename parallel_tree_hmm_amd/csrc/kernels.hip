@@ -569,6 +569,106 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
   if (lane.s < d.K) static_cast<T *>(buf.node_adj)[leaf_at * d.K + lane.s] = static_cast<T>(column);
 }
 
+// exp of the K = 4 leaf vector of categorical node n, item b, as
+// CategoricalLeaf forms it on the four lanes of a group and RakeReverse
+// exponentiates it (the maximum taken over the four entries here; it is
+// exact, so the order does not matter).
+template <typename T>
+__device__ inline void CategoricalLeafExp4(const CategoricalLeaves &c, const Dims &d, std::uint32_t n, std::uint32_t b,
+                                           T e[4]) {
+  const bool rooted = c.is_root[n] != 0;
+  const std::uint32_t o = c.index[n];
+  const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : c.obs[static_cast<std::uint64_t>(o) * d.B + b];
+  const T *root = static_cast<const T *>(c.root);
+  double x[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    T value;
+    if (o == 0xFFFFFFFFu) {
+      value = rooted ? root[s] : T(1);
+    } else {
+      const T em = category < c.categories
+                       ? static_cast<const T *>(c.emissions)[static_cast<std::uint64_t>(category) * 4 + s]
+                       : T(0);
+      value = (rooted ? root[s] : T(1)) * em;
+    }
+    x[s] = static_cast<double>(value);
+  }
+  if constexpr (std::is_same<T, float>::value) {
+    if (c.element_logs) {
+      float l[4];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) l[s] = ElementLog(static_cast<float>(x[s]));
+      const float top = Max(Max(l[0], l[1]), Max(l[2], l[3]));
+      const bool live = top > NegInf<float>();
+#pragma unroll
+      for (int s = 0; s < 4; ++s) e[s] = Exp(live ? l[s] - top : NegInf<float>());
+      return;
+    }
+  }
+  double l[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) l[s] = x[s] > 0.0 ? log(x[s]) : kNegInf;
+  const double top = Max(Max(l[0], l[1]), Max(l[2], l[3]));
+#pragma unroll
+  for (int s = 0; s < 4; ++s) e[s] = Exp(top > kNegInf && x[s] > 0.0 ? static_cast<T>(l[s] - top) : NegInf<T>());
+}
+
+// The rake adjoint at K = 4 with one lane per (rake, item): RakeReverse's
+// arithmetic in the same order (row sums in T, the ratio and the column sums
+// in FP64), the slot rows, leaf vector and adjoint read as quads and each
+// row of the slot adjoint stored as one quad, where the four-lane form
+// repeats the leaf exponentials on every lane and stores single values.
+template <typename T>
+__global__ __launch_bounds__(256) void RakeReverseK4(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                     LinearBuffers buf) {
+  constexpr int K = 4;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
+  const std::uint32_t r = offset + static_cast<std::uint32_t>(item / d.B);
+  const DeviceRake op = ops.rakes[r];
+  const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
+  T e[K];
+  if (r < buf.cat.rakes) {
+    CategoricalLeafExp4<T>(buf.cat, d, op.leaf, b, e);
+  } else {
+    const Quad<T> v = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K);
+#pragma unroll
+    for (int c = 0; c < K; ++c) e[c] = Exp(v.v[c]);
+  }
+  const T *P = Slot<T>(buf, d, op.slot, b);
+  Quad<T> row[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) row[p] = LoadQuad(P + p * K);
+  const Quad<T> a = LoadQuad(RakeAdjoint<T>(buf, d, op, b));
+  double g[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc += row[p].v[c] * e[c];
+    g[p] = (a.v[p] != T(0) && acc > T(0)) ? static_cast<double>(a.v[p]) / static_cast<double>(acc) : 0.0;
+  }
+  const AdjointTarget<T> out = SlotAdjointTarget<T>(buf, d, op.slot, b);
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    Quad<T> q;
+#pragma unroll
+    for (int c = 0; c < K; ++c) q.v[c] = static_cast<T>(g[p] * static_cast<double>(row[p].v[c] * e[c]));
+    StoreAdjointQuad<T, false>(out, static_cast<std::uint64_t>(p) * K, q);
+  }
+  Quad<T> column;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    double sum = 0.0;
+#pragma unroll
+    for (int p = 0; p < K; ++p) sum += g[p] * static_cast<double>(row[p].v[c] * e[c]);
+    column.v[c] = static_cast<T>(sum);
+  }
+  StoreQuad(static_cast<T *>(buf.node_adj) + leaf_at * K, column);
+}
+
 // adj(branch) = adj(parent) (src/inference.cc:466-475)
 template <typename T, int G>
 __global__ void AbsorbReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count, LinearBuffers buf) {
@@ -1791,7 +1891,13 @@ void ReverseLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
   const unsigned block = GroupBlock(G);
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
   switch (primitive) {
-    case 0: RakeReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 0:
+      if (G == 4 && l.dims.K == 4)
+        RakeReverseK4<T><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
+      else
+        RakeReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      break;
     case 1: CombineReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 2: AbsorbReverse<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 3: {
