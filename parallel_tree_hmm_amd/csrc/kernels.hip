@@ -575,7 +575,7 @@ __global__ void RakeReverse(DeviceOps ops, Dims d, std::uint32_t offset, std::ui
 // exact, so the order does not matter).
 template <typename T>
 __device__ inline void CategoricalLeafExp4(const CategoricalLeaves &c, const Dims &d, std::uint32_t n, std::uint32_t b,
-                                           T e[4]) {
+                                           T e[4], double *scale) {
   const bool rooted = c.is_root[n] != 0;
   const std::uint32_t o = c.index[n];
   const std::uint32_t category = o == 0xFFFFFFFFu ? 0 : c.obs[static_cast<std::uint64_t>(o) * d.B + b];
@@ -601,6 +601,7 @@ __device__ inline void CategoricalLeafExp4(const CategoricalLeaves &c, const Dim
       for (int s = 0; s < 4; ++s) l[s] = ElementLog(static_cast<float>(x[s]));
       const float top = Max(Max(l[0], l[1]), Max(l[2], l[3]));
       const bool live = top > NegInf<float>();
+      *scale = live ? static_cast<double>(top) : kNegInf;
 #pragma unroll
       for (int s = 0; s < 4; ++s) e[s] = Exp(live ? l[s] - top : NegInf<float>());
       return;
@@ -610,6 +611,7 @@ __device__ inline void CategoricalLeafExp4(const CategoricalLeaves &c, const Dim
 #pragma unroll
   for (int s = 0; s < 4; ++s) l[s] = x[s] > 0.0 ? log(x[s]) : kNegInf;
   const double top = Max(Max(l[0], l[1]), Max(l[2], l[3]));
+  *scale = top;
 #pragma unroll
   for (int s = 0; s < 4; ++s) e[s] = Exp(top > kNegInf && x[s] > 0.0 ? static_cast<T>(l[s] - top) : NegInf<T>());
 }
@@ -631,7 +633,8 @@ __global__ __launch_bounds__(256) void RakeReverseK4(DeviceOps ops, Dims d, std:
   const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
   T e[K];
   if (r < buf.cat.rakes) {
-    CategoricalLeafExp4<T>(buf.cat, d, op.leaf, b, e);
+    double unused;
+    CategoricalLeafExp4<T>(buf.cat, d, op.leaf, b, e, &unused);
   } else {
     const Quad<T> v = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K);
 #pragma unroll
@@ -667,6 +670,49 @@ __global__ __launch_bounds__(256) void RakeReverseK4(DeviceOps ops, Dims d, std:
     column.v[c] = static_cast<T>(sum);
   }
   StoreQuad(static_cast<T *>(buf.node_adj) + leaf_at * K, column);
+}
+
+// The rake forward at K = 4 with one lane per (rake, item): RakeForward's
+// arithmetic in the same order, the slot rows and the leaf vector read as
+// quads and the branch stored as one quad.
+template <typename T>
+__global__ __launch_bounds__(256) void RakeForwardK4(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                     LinearBuffers buf) {
+  constexpr int K = 4;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
+  const std::uint32_t r = offset + static_cast<std::uint32_t>(item / d.B);
+  const DeviceRake op = ops.rakes[r];
+  const std::uint64_t leaf_at = static_cast<std::uint64_t>(op.leaf) * d.B + b;
+  T e[K];
+  double leaf_scale;
+  if (r < buf.cat.rakes) {
+    CategoricalLeafExp4<T>(buf.cat, d, op.leaf, b, e, &leaf_scale);
+  } else {
+    const Quad<T> v = LoadQuad(static_cast<const T *>(buf.nodes) + leaf_at * K);
+#pragma unroll
+    for (int c = 0; c < K; ++c) e[c] = Exp(v.v[c]);
+    leaf_scale = buf.node_scale[leaf_at];
+  }
+  const T *P = Slot<T>(buf, d, op.slot, b);
+  double message[K];
+#pragma unroll
+  for (int p = 0; p < K; ++p) {
+    const Quad<T> row = LoadQuad(P + p * K);
+    T acc = T(0);
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc += row.v[c] * e[c];
+    message[p] = acc > T(0) ? RowScale(buf, d, op.slot, b, p) + log(static_cast<double>(acc)) : kNegInf;
+  }
+  const double top = Max(Max(message[0], message[1]), Max(message[2], message[3]));
+  const std::uint64_t at = static_cast<std::uint64_t>(op.branch) * d.B + b;
+  const bool live = top > kNegInf;
+  Quad<T> q;
+#pragma unroll
+  for (int p = 0; p < K; ++p) q.v[p] = live ? static_cast<T>(message[p] - top) : NegInf<T>();
+  StoreQuad(static_cast<T *>(buf.branches) + at * K, q);
+  buf.branch_scale[at] = live ? leaf_scale + top : kNegInf;
 }
 
 // adj(branch) = adj(parent) (src/inference.cc:466-475)
@@ -1877,7 +1923,13 @@ void ForwardLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
   const unsigned block = GroupBlock(G);
   const unsigned grid = BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B * G, block);
   switch (primitive) {
-    case 0: RakeForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 0:
+      if (G == 4 && l.dims.K == 4)
+        RakeForwardK4<T><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
+      else
+        RakeForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      break;
     case 1: CombineForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 2: AbsorbForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
     case 3: CompressForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
