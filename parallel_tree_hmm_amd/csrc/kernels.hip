@@ -185,6 +185,48 @@ __global__ void AbsorbForward(DeviceOps ops, Dims d, std::uint32_t offset, std::
                       static_cast<const T *>(buf.branches) + branch * d.K, buf.branch_scale[branch], lane.s, d.K);
 }
 
+// CombineForward (kAbsorb false) and AbsorbForward (true) at K = 4 with one
+// lane per (op, item): AddLogVectors' arithmetic, the vectors moved as
+// quads. The maximum over the four sums is fmax's, which is the same in any
+// order.
+template <typename T, bool kAbsorb>
+__global__ __launch_bounds__(256) void AddForwardK4(DeviceOps ops, Dims d, std::uint32_t offset, std::uint32_t count,
+                                                    LinearBuffers buf) {
+  constexpr int K = 4;
+  const std::uint64_t item = static_cast<std::uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (item >= static_cast<std::uint64_t>(count) * d.B) return;
+  const std::uint32_t b = static_cast<std::uint32_t>(item % d.B);
+  const std::uint32_t i = offset + static_cast<std::uint32_t>(item / d.B);
+  T *dst;
+  double *dst_scale;
+  std::uint64_t src;
+  if constexpr (kAbsorb) {
+    const DeviceAbsorb op = ops.absorbs[i];
+    const std::uint64_t node = static_cast<std::uint64_t>(op.parent) * d.B + b;
+    dst = static_cast<T *>(buf.nodes) + node * K;
+    dst_scale = buf.node_scale + node;
+    src = static_cast<std::uint64_t>(op.branch) * d.B + b;
+  } else {
+    const DeviceCombine op = ops.combines[i];
+    const std::uint64_t to = static_cast<std::uint64_t>(op.destination) * d.B + b;
+    dst = static_cast<T *>(buf.branches) + to * K;
+    dst_scale = buf.branch_scale + to;
+    src = static_cast<std::uint64_t>(op.source) * d.B + b;
+  }
+  const Quad<T> a = LoadQuad(dst);
+  const Quad<T> c = LoadQuad(static_cast<const T *>(buf.branches) + src * K);
+  T x[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) x[s] = a.v[s] + c.v[s];
+  const T top = Max(Max(x[0], x[1]), Max(x[2], x[3]));
+  const bool live = top > NegInf<T>();
+  Quad<T> q;
+#pragma unroll
+  for (int s = 0; s < K; ++s) q.v[s] = live ? x[s] - top : NegInf<T>();
+  StoreQuad(dst, q);
+  *dst_scale = live ? *dst_scale + buf.branch_scale[src] + static_cast<double>(top) : kNegInf;
+}
+
 // Middle weights of a compression: exp(mid[m] + r_m - W) with r_m the right
 // factor's row scales folded in and W their maximum (returned; -inf when no
 // middle state is supported).
@@ -1930,8 +1972,20 @@ void ForwardLevelT(const Launch &l, std::uint32_t primitive, std::uint32_t offse
       else
         RakeForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
       break;
-    case 1: CombineForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
-    case 2: AbsorbForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
+    case 1:
+      if (G == 4 && l.dims.K == 4)
+        AddForwardK4<T, false><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
+      else
+        CombineForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      break;
+    case 2:
+      if (G == 4 && l.dims.K == 4)
+        AddForwardK4<T, true><<<BlocksFor(static_cast<std::uint64_t>(count) * l.dims.B, 256), 256, 0, StreamOf(l)>>>(
+            l.ops, l.dims, offset, count, buf);
+      else
+        AbsorbForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf);
+      break;
     case 3: CompressForward<T, G><<<grid, block, 0, StreamOf(l)>>>(l.ops, l.dims, offset, count, buf); break;
   }
   Check(hipGetLastError(), "forward level launch");
